@@ -1,0 +1,198 @@
+"""Benchmark: Omniscape circuit solves per second on one MI355X.
+
+Workload (the headline configuration): a 4096 x 4096 float32 resistance map
+r = exp(2 z), z a seeded Gaussian random field with a power-law spectrum of
+exponent -3 (unit variance), 30 % NoData placed as seeded clustered blobs
+(Gaussian-smoothed noise, sigma 32 cells, below its 30 % quantile), sources
+1 / r, Omniscape radius 50, block 5, buffer 0, 8 neighbours, block-artifact
+correction on. One step is one whole Omniscape run over the map (every target
+window factorised, solved and refined to the residual bar, currents summed
+into the cumulative map); inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+prints one JSON line: circuits/s of the mixed-precision GPU path (FP32 MFMA
+fronts + FP64 refinement to a relative residual <= 1e-10), with the FP64 GPU
+path, the roofline of the dominant kernel and the CPU baseline (the FP64 CPU
+restatement of the reference, timed on a bounded sample of the same windows).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+MFMA_F32_PEAK = 157.3e12   # MI355X dense FP32 matrix, FLOP/s
+MFMA_F64_PEAK = 78.6e12
+HBM_PEAK = 8.0e12          # bytes/s
+
+
+def fractal_field(n: int, seed: int, exponent: float = -3.0) -> np.ndarray:
+    """Seeded Gaussian random field, power spectrum |k|^exponent, mean 0, variance 1."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((n, n))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.rfftfreq(n)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (exponent / 2)
+    amp[0, 0] = 0.0
+    z = np.fft.irfft2(np.fft.rfft2(white) * amp, s=(n, n))
+    return (z - z.mean()) / z.std()
+
+
+def nodata_blobs(n: int, seed: int, sigma: float, fraction: float) -> np.ndarray:
+    """Clustered NoData: Gaussian-smoothed white noise below its `fraction` quantile."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((n, n))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.rfftfreq(n)[None, :]
+    gauss = np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * (kx * kx + ky * ky))
+    smooth = np.fft.irfft2(np.fft.rfft2(white) * gauss, s=(n, n))
+    return smooth < np.quantile(smooth, fraction)
+
+
+def workload(n: int = 4096, seed: int = 103, nodata: float = 0.30, sigma: float = 32.0):
+    z = fractal_field(n, seed)
+    r = np.exp(2.0 * z).astype(np.float32).astype(np.float64)
+    r[nodata_blobs(n, seed + 1, sigma, nodata)] = np.nan
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s = np.where(np.isnan(r), 0.0, 1.0 / r)
+    return np.ascontiguousarray(r), np.ascontiguousarray(s)
+
+
+def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=True):
+    o = N.options(precision, tol, 30, 4096, timing)
+    st = N.SolveStats()
+    nt = ctypes.c_int64(0)
+    N.check(N.lib().lsc_circuit_omniscape_device(
+        ctypes.c_void_p(d_r), ctypes.c_void_p(d_s), ctypes.c_int64(nx), ctypes.c_int64(ny), radius, 0, block, 0, 0,
+        1, 0, ctypes.byref(o), ctypes.c_void_p(d_cur), None, ctypes.byref(nt), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def cpu_baseline(r, s, radius, block, seconds: float):
+    """The FP64 CPU oracle (a port of the reference's C++) on full-size windows of the same map."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    probe = 64
+    t0 = time.perf_counter()
+    O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-probe)
+    rate = probe / max(time.perf_counter() - t0, 1e-3)
+    sample = int(min(200000, max(256, rate * seconds)))
+    t0 = time.perf_counter()
+    out = O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-sample)
+    dt = time.perf_counter() - t0
+    return {"value": out["n_targets"] / dt, "unit": "circuits/s", "cores": threads, "kind": "port",
+            "sample": f"{out['n_targets']} unclipped radius-{radius} windows of the same map (FP64 multifrontal, "
+                      f"{threads} OpenMP threads, {dt:.1f} s, incl. the correction solves)",
+            "cpu": platform.processor() or platform.machine()}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--radius", type=int, default=50)
+    ap.add_argument("--block", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by row bands "
+                                   "of target windows, see DESIGN.md)"}))
+        return 2
+    import torch
+
+    from landschaft_amd import _native as N
+
+    n = a.size
+    t0 = time.perf_counter()
+    r, s = workload(n)
+    gen_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    tr = torch.from_numpy(r).to(dev)
+    ts = torch.from_numpy(s).to(dev)
+    tc = torch.empty((n, n), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    args = (N, tr.data_ptr(), ts.data_ptr(), tc.data_ptr(), n, n, a.radius, a.block)
+    for _ in range(a.warmup):
+        run_gpu(*args, "mixed", a.tol)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        stats.append(run_gpu(*args, "mixed", a.tol))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    st = stats[-1]
+    value = st["n_targets"] / dt
+    cum_mixed = tc.cpu().numpy()
+
+    plan = N.window_plan_stats(a.radius, 0, False, True)
+    kernels = {}
+    # factor: MFMA-bound; algorithmic flops = the plan's front flops per factorised window
+    f_flops = plan["factor_flops"] * st["n_factored"]
+    f_s = st["ms_factor"] / 1e3
+    kernels["k_factor"] = {"bound": "mfma", "achieved": f_flops / f_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
+                           "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"]}
+    # solve: HBM-bound; algorithmic bytes = L read once per pass (fp32) + 4 n-vectors of fp64
+    passes = st["n_factored"] * (1 + st["refine_mean"])
+    bytes_solve = passes * (plan["l_entries"] * 4 + 4 * plan["unknowns"] * 8) + \
+        st["n_factored"] * st["refine_mean"] * plan["l_entries"] * 4
+    s_s = st["ms_solve"] / 1e3
+    kernels["k_solve"] = {"bound": "hbm", "achieved": bytes_solve / s_s / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "launches": st["n_solve_launches"], "ms_total": st["ms_solve"]}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+    kd = kernels[dom]
+    roofline = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
+                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": None}
+
+    fp64 = None
+    if not a.no_fp64:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st64 = run_gpu(*args, "fp64", a.tol)
+        torch.cuda.synchronize()
+        d64 = time.perf_counter() - t0
+        cum64 = tc.cpu().numpy()
+        top = np.nanmax(np.abs(cum64))
+        fp64 = {"value": st64["n_targets"] / d64, "ms": d64 * 1e3, "resid_max": st64["resid_max"],
+                "cum_max_rel_diff_vs_mixed": float(np.nanmax(np.abs(cum_mixed - cum64)) / top)}
+    cpu = None if a.no_cpu else cpu_baseline(r, s, a.radius, a.block, a.cpu_seconds)
+    line = {
+        "metric": "circuit solves/sec at 4096^2 @ rel resid <=1e-10 (Omniscape windows, mixed-precision GPU)",
+        "value": round(value, 2), "unit": "circuits/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mfma fronts + f64 refinement", "data": "synthetic",
+        "config": {"workload": f"omniscape {n}x{n} fractal exp(2z) f32, 30% nodata blobs, radius {a.radius}, "
+                               f"block {a.block}, 8-neighbour, correction on",
+                   "radius": a.radius, "block_size": a.block, "tol": a.tol, "seed": 103},
+        "n_targets": st["n_targets"], "resid_max": st["resid_max"], "n_over_1e10": st["n_over_1e10"],
+        "refine_mean": round(st["refine_mean"], 3), "refine_max": st["refine_max"],
+        "phase_ms": {k[3:]: round(v, 2) for k, v in st.items() if k.startswith("ms_")},
+        "kernels": kernels, "roofline": roofline, "fp64_gpu": fp64, "cpu_baseline": cpu,
+        "gen_seconds": round(gen_s, 1),
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,112 @@
+/* landschaft_circuit.h - C-ABI of the MI355X circuit library (liblandschaft_circuit.so).
+ *
+ * The entry points replace the reference's pybind11 bindings of its circuit
+ * core (landschaft._core / landschaft._circuit_avx2, registered in
+ * src/cpp/circuit/python.cpp:527-559 and reached through
+ * src/landschaft/circuit/_backend.py:12-23). Arguments keep the binding's
+ * names, order and meaning; arrays are plain pointers to row-major
+ * (ny x nx) float64 rasters owned by the caller, outputs are written into
+ * caller-allocated buffers of known size. Every call returns a status:
+ *   0  success,
+ *   1  invalid argument (the binding's ValueError),
+ *   2  runtime failure (the binding's RuntimeError: a circuit that is not
+ *      positive definite, a HIP error),
+ * and lsc_last_error() gives the message of the calling thread's last failure.
+ * Computation runs on the current HIP device. No entry point touches the
+ * CPU solver: without a GPU, calls fail with status 2.
+ */
+#ifndef LANDSCHAFT_CIRCUIT_H
+#define LANDSCHAFT_CIRCUIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { LSC_PRECISION_MIXED = 0, LSC_PRECISION_FP64 = 1 };
+
+/* Solver options (NULL: the defaults). precision: LSC_PRECISION_MIXED (FP32
+ * MFMA fronts + FP64 iterative refinement, default) or LSC_PRECISION_FP64;
+ * tol: relative residual ||b - Ax|| / ||b|| at which refinement stops
+ * (default 1e-12); max_refine: refinement steps per circuit (default 30);
+ * batch: circuits per batch (default 4096); timing: 1 to time every phase
+ * with HIP events (adds synchronisations); stream: a hipStream_t, 0 for the
+ * null stream. */
+typedef struct lsc_solve_options {
+    int32_t precision;
+    int32_t max_refine;
+    double tol;
+    int64_t batch;
+    int32_t timing;
+    int32_t reserved;
+    void* stream;
+} lsc_solve_options;
+
+/* Per-run statistics: targets, circuits solved, refinement steps, the
+ * largest and mean relative FP64 residual of the solved circuits, and (with
+ * timing) milliseconds spent in each phase. */
+typedef struct lsc_solve_stats {
+    int64_t n_targets;
+    int64_t n_solved;
+    int64_t n_unconverged; /* circuits whose residual stayed above tol */
+    int64_t n_over_1e10;   /* circuits whose residual stayed above 1e-10 */
+    int32_t refine_max;
+    int32_t n_batches;        /* batches (launches of the factor kernel) */
+    int64_t n_solve_launches; /* launches of the triangular-solve kernel */
+    int64_t n_factored;       /* circuits factorised */
+    double refine_mean;
+    double resid_max;
+    double resid_mean;
+    double ms_fill, ms_flood, ms_factor, ms_solve, ms_residual, ms_currents, ms_accumulate;
+} lsc_solve_stats;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* lsc_last_error(void);
+
+/* Device name, compute units and HBM bytes of the current device; 0 or 2. */
+int lsc_device_info(char* name, int64_t name_len, int32_t* compute_units, int64_t* hbm_bytes);
+
+/* Omniscape's cumulative current (and flow potential) over moving windows.
+ * Replaces circuit_omniscape (src/cpp/circuit/python.cpp:90-137 ->
+ * circuit::omniscape, src/cpp/circuit/omniscape.cpp:394-608) without the
+ * conditional-connectivity arguments. resistance: NaN = NoData, values > 0;
+ * source: NaN = 0. current (and flow_potential_out when flow_potential != 0)
+ * receive ny x nx float64 maps; n_targets the number of targets. Host
+ * pointers. */
+int lsc_circuit_omniscape(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
+                          int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
+                          int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                          double* current, double* flow_potential_out, int64_t* n_targets, lsc_solve_stats* stats);
+
+/* The same with every array in device memory (HBM-resident inputs). */
+int lsc_circuit_omniscape_device(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
+                                 int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
+                                 int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                                 double* current, double* flow_potential_out, int64_t* n_targets,
+                                 lsc_solve_stats* stats);
+
+/* Omniscape's block-artifact correction factors, (2 (radius + buffer) + 1)^2
+ * float64 into out. Replaces circuit_omniscape_correction
+ * (src/cpp/circuit/python.cpp:139-153 -> omniscape.cpp:326-392). */
+int lsc_circuit_omniscape_correction(int radius, int buffer, int block_size, int four_neighbors, int64_t nx,
+                                     int64_t ny, const lsc_solve_options* options, double* out);
+
+/* Symbolic statistics of the plan the Omniscape windows share (disk: 1 for
+ * the circuit windows, 0 for the flow-potential windows): out = {unknowns,
+ * fronts, L entries, factor flops (sum np^3/3 + np^2 nu + np nu^2), largest
+ * front rows, levels}. No reference counterpart (the reference's figures of
+ * the same quantities are internal to frontal_analyze). */
+int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, double* out);
+
+/* Checks the MFMA operand layout the front kernels assume: out[256] receives
+ * the 16 x 16 product A (16x4) * B (4x16) of a = [i + 16 k], b = [k + 4 j]
+ * computed with v_mfma_f32_16x16x4f32 (precision 0) or the f64 MFMA (1),
+ * row-major. No reference counterpart (a self-test). */
+int lsc_probe_mfma(int precision, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LANDSCHAFT_CIRCUIT_H */

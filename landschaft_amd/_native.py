@@ -1,0 +1,107 @@
+"""ctypes binding of liblandschaft_circuit.so (the C-ABI in include/landschaft_circuit.h).
+
+The library is the only compute path: when it is missing this module raises,
+and no entry point falls back to a CPU solver.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "liblandschaft_circuit.so"
+
+PRECISION_MIXED = 0
+PRECISION_FP64 = 1
+
+# Every symbol declared in include/landschaft_circuit.h.
+EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_circuit_omniscape_device",
+           "lsc_circuit_omniscape_correction", "lsc_probe_mfma", "lsc_window_plan_stats", "lsc_circuit_advanced",
+           "lsc_circuit_advanced_device", "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise",
+           "lsc_circuitscape_multi_advanced", "lsc_effective_resistance_grad",
+           "lsc_effective_resistance_grad_device", "lsc_pair_voltages_device", "lsc_pair_adjoint_device")
+
+
+class SolveOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("max_refine", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("batch", ctypes.c_int64), ("timing", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("stream", ctypes.c_void_p)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("n_targets", ctypes.c_int64), ("n_solved", ctypes.c_int64), ("n_unconverged", ctypes.c_int64),
+                ("n_over_1e10", ctypes.c_int64), ("refine_max", ctypes.c_int32), ("n_batches", ctypes.c_int32),
+                ("n_solve_launches", ctypes.c_int64), ("n_factored", ctypes.c_int64), ("refine_mean", ctypes.c_double), ("resid_max", ctypes.c_double), ("resid_mean", ctypes.c_double),
+                ("ms_fill", ctypes.c_double), ("ms_flood", ctypes.c_double), ("ms_factor", ctypes.c_double),
+                ("ms_solve", ctypes.c_double), ("ms_residual", ctypes.c_double),
+                ("ms_currents", ctypes.c_double), ("ms_accumulate", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m landschaft_amd.build` "
+                              "(landschaft_amd has no CPU fallback)")
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _lib.lsc_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = lib().lsc_last_error().decode()
+    raise (ValueError if rc == 1 else RuntimeError)(msg)
+
+
+def options(precision="mixed", tol=None, max_refine=None, batch=None, timing=False, stream=None) -> SolveOptions:
+    o = SolveOptions()
+    o.precision = {"mixed": PRECISION_MIXED, "fp64": PRECISION_FP64}[precision]
+    o.tol = float(tol) if tol else 0.0
+    o.max_refine = int(max_refine) if max_refine else 0
+    o.batch = int(batch) if batch else 0
+    o.timing = 1 if timing else 0
+    o.stream = stream
+    return o
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ptr(a, t=ctypes.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int32(0)
+    hbm = ctypes.c_int64(0)
+    check(lib().lsc_device_info(name, ctypes.c_int64(64), ctypes.byref(cus), ctypes.byref(hbm)))
+    return {"arch": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
+
+
+def probe_mfma(precision: int = PRECISION_MIXED) -> np.ndarray:
+    out = np.empty(256)
+    check(lib().lsc_probe_mfma(int(precision), ptr(out)))
+    return out.reshape(16, 16)
+
+
+def window_plan_stats(radius: int, buffer: int = 0, four_neighbors: bool = False, disk: bool = True) -> dict:
+    out = np.empty(6)
+    check(lib().lsc_window_plan_stats(int(radius), int(buffer), int(bool(four_neighbors)), int(bool(disk)),
+                                      ptr(out)))
+    keys = ("unknowns", "fronts", "l_entries", "factor_flops", "max_front", "levels")
+    return dict(zip(keys, out.tolist()))

@@ -1,0 +1,86 @@
+// Error handling and device memory helpers of the circuit library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lsc {
+
+// std::invalid_argument maps to status 1 (ValueError in Python), every other
+// failure to status 2 (RuntimeError), as the reference's binding raises them
+// (src/cpp/circuit/python.cpp:67-70, grid.cpp:208).
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define LSC_HIP(x) ::lsc::hip_check((x), #x)
+
+// A device allocation owned by the host object.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(std::size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p, n = o.n;
+            o.p = nullptr, o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(std::size_t count) {
+        release();
+        if (count == 0) return;
+        LSC_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr, n = 0;
+    }
+    void upload(const T* h, std::size_t count, hipStream_t s) {
+        LSC_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        if (v.size() > n) alloc(v.size());
+        if (!v.empty()) upload(v.data(), v.size(), s);
+    }
+};
+
+template <typename T>
+DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
+    DevBuf<T> d(v.size() ? v.size() : 1);
+    if (!v.empty()) d.upload(v.data(), v.size(), s);
+    return d;
+}
+
+void set_error(const std::string& msg);
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        set_error(e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 2;
+    }
+}
+
+}  // namespace lsc

@@ -1,0 +1,574 @@
+// Omniscape on the window engine: host driver and C-ABI entry points.
+//
+// The reference runs Omniscape's targets in parallel CPU chunks, one sparse
+// factorisation per window in double precision
+// (src/cpp/circuit/omniscape.cpp:394-608). Here every window of a run, also
+// the ones clipped at the map edges, is the same W x W disk with the cells
+// off the map out of the circuit, so all windows share one symbolic plan and
+// a batch of thousands of windows is factorised by one persistent kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/landschaft_circuit.h"
+#include "common.hpp"
+#include "window.hpp"
+#include "window_kernels.hpp"
+
+namespace lsc {
+
+// Flow potential circuits: conductance 1 on every on-map cell of the window
+// (reference omniscape.cpp:127-166 uses a window of ones).
+__global__ void k_unit_conductance(Lattice Lt, long long k0, int B, double* g) {
+    const int k = blockIdx.x;
+    if (k >= B || k0 + k >= Lt.nlat) return;
+    const long long kk = k0 + k;
+    const long long tx = Lt.br + (kk / Lt.nly) * Lt.bs, ty = Lt.br + (kk % Lt.nly) * Lt.bs;
+    const int W = Lt.W, D = Lt.D;
+    double* gw = g + static_cast<long long>(k) * W * W;
+    for (int c = threadIdx.x; c < W * W; c += blockDim.x) {
+        const long long Y = ty - D + c / W, X = tx - D + c % W;
+        gw[c] = (Y >= 0 && Y < Lt.ny && X >= 0 && X < Lt.nx) ? 1.0 : -9999.0;
+    }
+}
+
+namespace {
+
+template <typename T>
+std::size_t factor_lds() {
+    return static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T);
+}
+template <typename T>
+std::size_t solve_lds() {
+    return static_cast<std::size_t>(kLbufEntries) * sizeof(T);
+}
+
+void reset(int* p, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+// Dynamic LDS beyond the default limit must be allowed per kernel.
+void allow_lds(const void* fn, std::size_t bytes) {
+    LSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
+int device_cus() {
+    int dev = 0;
+    LSC_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    LSC_HIP(hipGetDeviceProperties(&prop, dev));
+    return prop.multiProcessorCount;
+}
+
+}  // namespace
+
+void DevPlan::upload(Plan p, hipStream_t s) {
+    host = std::move(p);
+    const Plan& h = host;
+    const int H = static_cast<int>(h.lev_ptr.size()) - 1;
+    if (h.max_front + 1 > kBigTiles * 16)
+        throw std::invalid_argument("a front of " + std::to_string(h.max_front) +
+                                    " rows exceeds the window engine's LDS fronts (159 rows); use a smaller radius");
+    std::vector<int> lev_small(H, 0), chunk_lev(H + 1, 0), chunk_begin, chunk_end;
+    for (int lv = 0; lv < H; ++lv) {
+        int ns = 0;
+        for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
+            const int t = h.order[k];
+            if (h.npiv[t] + h.nupd[t] + 1 <= kSmallRows) ++ns;
+        }
+        lev_small[lv] = ns;
+        // solve chunks: consecutive fronts of the level whose L fits the LDS buffer
+        chunk_lev[lv] = static_cast<int>(chunk_begin.size());
+        long long acc = 0;
+        int start = h.lev_ptr[lv];
+        for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
+            const int t = h.order[k];
+            const long long np = h.npiv[t], m = np + h.nupd[t];
+            const long long sz = np * m - np * (np - 1) / 2;
+            if (sz > kLbufEntries) throw std::invalid_argument("a front's factor exceeds the solve kernel's LDS buffer");
+            if (acc + sz > kLbufEntries) {
+                chunk_begin.push_back(start);
+                chunk_end.push_back(k);
+                start = k;
+                acc = 0;
+            }
+            acc += sz;
+        }
+        if (h.lev_ptr[lv + 1] > start) {
+            chunk_begin.push_back(start);
+            chunk_end.push_back(h.lev_ptr[lv + 1]);
+        }
+    }
+    chunk_lev[H] = static_cast<int>(chunk_begin.size());
+    std::vector<long long> uposA(h.nfront, 0);
+    long long ua = 0;
+    for (int k = 0; k < h.nfront; ++k) {
+        const int t = h.order[k];
+        uposA[t] = ua;
+        ua += static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1);
+    }
+    ibufs.clear();
+    lbufs.clear();
+    auto I = [&](const std::vector<int>& v) {
+        ibufs.push_back(to_device(v, s));
+        return static_cast<const int*>(ibufs.back().p);
+    };
+    auto Lg = [&](const std::vector<long long>& v) {
+        lbufs.push_back(to_device(v, s));
+        return static_cast<const long long*>(lbufs.back().p);
+    };
+    ibufs.reserve(32);
+    lbufs.reserve(8);
+    std::vector<long long> lpos(h.lpos.begin(), h.lpos.end()), vpos(h.vpos.begin(), h.vpos.end());
+    d.n = h.n, d.nd = h.nd, d.W = static_cast<int>(h.bx), d.D = static_cast<int>(h.bx / 2), d.nfront = h.nfront;
+    d.H = H;
+    d.cell = I(h.cell), d.nbr = I(h.nbr), d.amap = I(h.amap);
+    d.first = I(h.first), d.npiv = I(h.npiv), d.nupd = I(h.nupd), d.upd = I(h.upd), d.cb = I(h.cb), d.ce = I(h.ce);
+    d.lpos = Lg(lpos), d.uposA = Lg(uposA), d.vpos = Lg(vpos);
+    d.kids = I(h.kids), d.rows = I(h.rows), d.relmap = I(h.relmap);
+    d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small), d.order = I(h.order);
+    d.chunk_lev = I(chunk_lev), d.chunk_begin = I(chunk_begin), d.chunk_end = I(chunk_end);
+    d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
+}
+
+void WindowBatchBuffers::ensure(int B_, int W2_, int n_, long long lsize_, int elem_) {
+    if (B_ <= B && W2_ == W2 && n_ == n && lsize_ == lsize && elem_ == elem) return;
+    B = B_, W2 = W2_, n = n_, lsize = lsize_, elem = elem_;
+    const std::size_t bw = static_cast<std::size_t>(B) * W2, bn = static_cast<std::size_t>(B) * n;
+    g.alloc(bw), src.alloc(bw), cur.alloc(bw), present.alloc(bw);
+    dg.alloc(bn), sc.alloc(bn), b.alloc(bn), x.alloc(bn), y.alloc(bn), z.alloc(bn), rs.alloc(bn);
+    rnorm.alloc(B), rprev.alloc(B);
+    active.alloc(B), fail.alloc(B), done.alloc(B), iters.alloc(B);
+    L.alloc(static_cast<std::size_t>(B) * lsize * elem);
+}
+
+void SlotBuffers::ensure(int slots_, long long usizeA, long long vsize, int elem) {
+    const std::size_t ub = static_cast<std::size_t>(slots_) * usizeA * elem;
+    const std::size_t vb = static_cast<std::size_t>(slots_) * vsize;
+    if (slots_ <= slots && U.n >= ub && V.n >= vb) return;
+    slots = slots_;
+    U.alloc(ub);
+    V.alloc(vb ? vb : 1);
+    if (!counter.p) counter.alloc(4);
+}
+
+// One batch of windows whose g, src and active flags are filled: flood,
+// factor, solve, refine. Returns per-window refinement statistics in `st`.
+template <typename T>
+struct WindowSolver {
+    hipStream_t s;
+    DevPlan plan;
+    WindowBatchBuffers wb;
+    SlotBuffers sb;
+    DevBuf<int> open;
+    int slots_factor = 0, slots_solve = 0;
+
+    explicit WindowSolver(hipStream_t st) : s(st) {}
+
+    void set_plan(Plan p, int B) {
+        plan.upload(std::move(p), s);
+        allow_lds(reinterpret_cast<const void*>(&k_factor<T>), factor_lds<T>());
+        allow_lds(reinterpret_cast<const void*>(&k_solve<T>), solve_lds<T>());
+        allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
+        const int cus = device_cus();
+        int occ_f = 0, occ_s = 0;
+        LSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(&k_factor<T>),
+                                                             kThreads, factor_lds<T>()));
+        LSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, reinterpret_cast<const void*>(&k_solve<T>),
+                                                             kThreads, solve_lds<T>()));
+        slots_factor = std::max(1, occ_f) * cus;
+        (void)hipGetLastError();
+        slots_solve = std::max(1, occ_s) * cus;
+        slots_factor = std::min(slots_factor, B);
+        slots_solve = std::min(slots_solve, B);
+        const int W = plan.d.W;
+        wb.ensure(B, W * W, plan.d.n, plan.d.lsize, sizeof(T));
+        sb.ensure(std::max(slots_factor, slots_solve), plan.d.usizeA, plan.d.vsize, sizeof(T));
+        if (!open.p) open.alloc(4);
+    }
+
+    void time_phase(double& acc, hipEvent_t a, hipEvent_t b, bool timing) {
+        if (!timing) return;
+        float ms = 0;
+        LSC_HIP(hipEventSynchronize(b));
+        LSC_HIP(hipEventElapsedTime(&ms, a, b));
+        acc += ms;
+    }
+
+    void solve(int nb, double tol, int maxit, IrStats& st, hipEvent_t* ev, bool timing) {
+        const DPlan& P = plan.d;
+        reset(wb.done.p, nb, s);
+        reset(wb.iters.p, nb, s);
+        reset(wb.fail.p, nb, s);
+        if (timing) LSC_HIP(hipEventRecord(ev[0], s));
+        hipLaunchKernelGGL(k_flood, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.src.p, wb.active.p,
+                           wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p,
+                           10 * std::numeric_limits<double>::epsilon());
+        if (timing) LSC_HIP(hipEventRecord(ev[1], s));
+        reset(sb.counter.p, 1, s);
+        hipLaunchKernelGGL(k_factor<T>, dim3(std::min(slots_factor, nb)), dim3(kThreads), factor_lds<T>(), s, P, nb,
+                           wb.g.p, wb.present.p, wb.sc.p, wb.rs.p, wb.active.p, reinterpret_cast<T*>(wb.L.p),
+                           wb.y.p, reinterpret_cast<T*>(sb.U.p), wb.fail.p, sb.counter.p);
+        if (timing) LSC_HIP(hipEventRecord(ev[2], s));
+        reset(sb.counter.p, 1, s);
+        hipLaunchKernelGGL(k_solve<T>, dim3(std::min(slots_solve, nb)), dim3(kThreads), solve_lds<T>(), s, P, nb, 0,
+                           wb.active.p, wb.done.p, reinterpret_cast<const T*>(wb.L.p), wb.rs.p, wb.y.p, wb.z.p,
+                           wb.sc.p, wb.x.p, sb.V.p, sb.counter.p);
+        if (timing) LSC_HIP(hipEventRecord(ev[3], s));
+        st.batches += 1;
+        st.solve_launches += 1;
+        time_phase(st.ms_flood, ev[0], ev[1], timing);
+        time_phase(st.ms_factor, ev[1], ev[2], timing);
+        time_phase(st.ms_solve, ev[2], ev[3], timing);
+        for (int it = 0;; ++it) {
+            reset(open.p, 1, s);
+            if (timing) LSC_HIP(hipEventRecord(ev[0], s));
+            hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.present.p, wb.dg.p,
+                               wb.sc.p, wb.b.p, wb.x.p, wb.rs.p, wb.active.p, wb.done.p, wb.rnorm.p, wb.rprev.p,
+                               wb.iters.p, tol, open.p);
+            if (timing) LSC_HIP(hipEventRecord(ev[1], s));
+            int h_open = 0;
+            LSC_HIP(hipMemcpyAsync(&h_open, open.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            LSC_HIP(hipStreamSynchronize(s));
+            time_phase(st.ms_residual, ev[0], ev[1], timing);
+            if (h_open == 0 || it >= maxit) break;
+            st.solve_launches += 1;
+            reset(sb.counter.p, 1, s);
+            if (timing) LSC_HIP(hipEventRecord(ev[0], s));
+            hipLaunchKernelGGL(k_solve<T>, dim3(std::min(slots_solve, nb)), dim3(kThreads), solve_lds<T>(), s, P,
+                               nb, 1, wb.active.p, wb.done.p, reinterpret_cast<const T*>(wb.L.p), wb.rs.p, wb.y.p,
+                               wb.z.p, wb.sc.p, wb.x.p, sb.V.p, sb.counter.p);
+            if (timing) LSC_HIP(hipEventRecord(ev[1], s));
+            time_phase(st.ms_solve, ev[0], ev[1], timing);
+        }
+        // statistics of the batch
+        std::vector<int> act(nb), fl(nb), its(nb), dn(nb);
+        std::vector<double> rn(nb);
+        LSC_HIP(hipMemcpyAsync(act.data(), wb.active.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(fl.data(), wb.fail.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(its.data(), wb.iters.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(dn.data(), wb.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(rn.data(), wb.rnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < nb; ++k) {
+            if (!act[k]) continue;
+            if (fl[k]) throw std::runtime_error("an Omniscape window is not positive definite");
+            ++st.solved;
+            st.iter_sum += its[k];
+            st.iter_max = std::max(st.iter_max, its[k]);
+            st.resid_max = std::max(st.resid_max, rn[k]);
+            st.resid_sum += rn[k];
+            if (!(rn[k] <= tol)) ++st.unconverged;
+            if (!(rn[k] <= 1e-10)) ++st.over_bar;
+        }
+    }
+};
+
+struct OmniArgs {
+    int radius, buffer, block, four, rc, correct, fp, precision;
+    double tol;
+    int maxit;
+    long long batch;
+    int timing;
+};
+
+namespace {
+
+Lattice make_lattice(long long nx, long long ny, const OmniArgs& a) {
+    Lattice L{};
+    L.nx = nx, L.ny = ny;
+    L.R = a.radius, L.D = a.radius + a.buffer, L.W = 2 * L.D + 1;
+    L.bs = a.block, L.br = (a.block - 1) / 2;
+    const long long nlx = nx >= a.block ? (nx - a.block) / a.block + 1 : 0;
+    const long long nly = ny >= a.block ? (ny - a.block) / a.block + 1 : 0;
+    L.nly = static_cast<int>(nly);
+    L.nlat = nlx * nly;
+    L.rc = a.rc;
+    return L;
+}
+
+// The correction circuits of omniscape_correction (reference
+// omniscape.cpp:326-392): conductance 1 within D, a block-source and a
+// null-source circuit; their currents give the factors.
+template <typename T>
+std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hipStream_t s, IrStats& st) {
+    const long long R = a.radius, D = a.radius + a.buffer, W = 2 * D + 1, bs = a.block, br = (bs - 1) / 2;
+    const long long n = W * W;
+    std::vector<double> g(n, -9999.0), null_src(n, 0.0), block_src(n, 0.0);
+    double n_sources = 0;
+    for (long long y = 0; y < W; ++y)
+        for (long long x = 0; x < W; ++x) {
+            const long long d2 = (y - D) * (y - D) + (x - D) * (x - D);
+            if (d2 <= D * D) g[y * W + x] = 1.0;
+            if (d2 <= R * R) {
+                null_src[y * W + x] = 1.0;
+                block_src[y * W + x] = 1.0;
+                n_sources += 1;
+            }
+        }
+    null_src[D * W + D] = 0.0;
+    for (auto& v : null_src)
+        if (v != 0.0) v = 1 / (n_sources - 1);
+    for (long long y = D - br; y <= std::min(D + br, ny - 1); ++y)
+        for (long long x = D - br; x <= std::min(D + br, nx - 1); ++x)
+            if (y >= 0 && y < W && x >= 0 && x < W) block_src[y * W + x] = 0.0;
+    double sum = 0.0;
+    for (long long x = 0; x < W; ++x)
+        for (long long y = 0; y < W; ++y)
+            if (block_src[y * W + x] > 0) sum += block_src[y * W + x];
+    const double amps = static_cast<double>(bs * bs);
+    for (auto& v : block_src)
+        if (v > 0) v = (v * amps) / sum;
+
+    WindowSolver<T> ws(s);
+    ws.set_plan(window_plan(static_cast<int>(D), true, a.four != 0), 2);
+    ws.wb.g.upload(g.data(), n, s);
+    LSC_HIP(hipMemcpyAsync(ws.wb.g.p + n, g.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    LSC_HIP(hipMemcpyAsync(ws.wb.src.p, block_src.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    LSC_HIP(hipMemcpyAsync(ws.wb.src.p + n, null_src.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    const int one[2] = {1, 1};
+    LSC_HIP(hipMemcpyAsync(ws.wb.active.p, one, sizeof(one), hipMemcpyHostToDevice, s));
+    hipEvent_t ev[4] = {};
+    IrStats cst;
+    ws.solve(2, std::min(a.tol, 1e-14), std::max(a.maxit, 30), cst, ev, false);
+    Lattice Lt{};
+    hipLaunchKernelGGL(k_currents, dim3(2), dim3(kThreads), static_cast<std::size_t>(n) * sizeof(double), s,
+                       ws.plan.d, Lt, 0ll, 2, ws.wb.g.p, ws.wb.present.p, ws.wb.x.p, ws.wb.active.p,
+                       static_cast<const double*>(nullptr), ws.wb.cur.p);
+    std::vector<double> cur(2 * n);
+    std::vector<int> act(2);
+    LSC_HIP(hipMemcpyAsync(cur.data(), ws.wb.cur.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipMemcpyAsync(act.data(), ws.wb.active.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipStreamSynchronize(s));
+    std::vector<double> bc(cur.begin(), cur.begin() + n), nc(cur.begin() + n, cur.end());
+    if (!act[0]) std::fill(bc.begin(), bc.end(), 0.0);
+    if (!act[1]) std::fill(nc.begin(), nc.end(), 0.0);
+    (void)st;
+    const long long T2 = 2 * D + bs;
+    std::vector<double> total(T2 * T2, 0.0);
+    for (long long i = 0; i < bs; ++i)
+        for (long long j = 0; j < bs; ++j)
+            for (long long y = 0; y < W; ++y)
+                for (long long x = 0; x < W; ++x) total[(i + y) * T2 + (j + x)] += nc[y * W + x];
+    double top = -std::numeric_limits<double>::infinity();
+    for (double v : bc) top = std::max(top, v);
+    const double floor = 1e-6 * top;
+    std::vector<double> corr(n);
+    for (long long y = 0; y < W; ++y)
+        for (long long x = 0; x < W; ++x) {
+            double b = bc[y * W + x], t = total[(y + br) * T2 + (x + br)];
+            if (b <= floor) b = 0.0;
+            if (b == 0.0) t = 0.0;
+            if (t == 0.0) t = 1.0;
+            if (b == 0.0) b = 1.0;
+            corr[y * W + x] = t / b;
+        }
+    return corr;
+}
+
+// One pass over the target lattice: `kind` 0 the landscape circuits, 1 the
+// flow potential's (conductance 1 on the whole clipped window).
+template <typename T>
+void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, const OmniArgs& a, int kind,
+                  const double* d_corr, double* d_map, unsigned long long* d_ntargets, hipStream_t s, IrStats& st) {
+    if (Lt.nlat == 0) return;
+    const long long B = std::min<long long>(a.batch > 0 ? a.batch : 4096, Lt.nlat);
+    WindowSolver<T> ws(s);
+    ws.set_plan(window_plan(Lt.D, kind == 0, a.four != 0), static_cast<int>(B));
+    hipEvent_t ev[4] = {};
+    const bool timing = a.timing != 0;
+    if (timing)
+        for (auto& e : ev) LSC_HIP(hipEventCreate(&e));
+    // kind 1 uses the same fill with every on-map cell of the window at conductance 1
+    Lattice L2 = Lt;
+    for (long long k0 = 0; k0 < Lt.nlat; k0 += B) {
+        const int nb = static_cast<int>(std::min<long long>(B, Lt.nlat - k0));
+        if (timing) LSC_HIP(hipEventRecord(ev[0], s));
+        hipLaunchKernelGGL(k_fill, dim3(nb), dim3(kThreads), 0, s, d_res, d_src, L2, k0, nb, ws.wb.g.p, ws.wb.src.p,
+                           ws.wb.active.p, d_ntargets);
+        if (kind == 1) {
+            // conductance 1 on every on-map cell of the box
+            hipLaunchKernelGGL(k_unit_conductance, dim3(nb), dim3(kThreads), 0, s, L2, k0, nb, ws.wb.g.p);
+        }
+        if (timing) {
+            LSC_HIP(hipEventRecord(ev[1], s));
+            ws.time_phase(st.ms_fill, ev[0], ev[1], true);
+        }
+        ws.solve(nb, a.tol, a.maxit, st, ev, timing);
+        if (timing) LSC_HIP(hipEventRecord(ev[0], s));
+        hipLaunchKernelGGL(k_currents, dim3(nb), dim3(kThreads), static_cast<std::size_t>(Lt.W) * Lt.W * sizeof(double),
+                           s, ws.plan.d, Lt, k0, nb, ws.wb.g.p, ws.wb.present.p, ws.wb.x.p, ws.wb.active.p, d_corr,
+                           ws.wb.cur.p);
+        if (timing) LSC_HIP(hipEventRecord(ev[1], s));
+        const long long ix0 = k0 / Lt.nly, ix1 = (k0 + nb - 1) / Lt.nly;
+        const long long X0 = std::max<long long>(0, Lt.br + ix0 * Lt.bs - Lt.D);
+        const long long X1 = std::min<long long>(Lt.nx - 1, Lt.br + ix1 * Lt.bs + Lt.D);
+        const long long ncol = X1 - X0 + 1;
+        const long long cells = ncol * Lt.ny;
+        hipLaunchKernelGGL(k_accumulate, dim3(static_cast<unsigned>((cells + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, s, Lt, k0, nb, X0, ncol, ws.wb.active.p, ws.wb.cur.p, d_map);
+        if (timing) {
+            LSC_HIP(hipEventRecord(ev[2], s));
+            ws.time_phase(st.ms_currents, ev[0], ev[1], true);
+            ws.time_phase(st.ms_accum, ev[1], ev[2], true);
+        }
+    }
+    LSC_HIP(hipStreamSynchronize(s));
+    if (timing)
+        for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+}  // namespace
+
+template <typename T>
+void omniscape_device(const double* d_res, const double* d_src, long long nx, long long ny, const OmniArgs& a,
+                      double* d_cur, double* d_fp, long long* n_targets, IrStats& st, hipStream_t s) {
+    if (a.radius < 0 || a.buffer < 0 || a.block < 1 || a.block % 2 == 0)
+        throw std::invalid_argument("radius and buffer must be >= 0 and block_size odd");
+    if (2 * (a.radius + a.buffer) + 1 > kMaxW)
+        throw std::invalid_argument("window side 2 (radius + buffer) + 1 exceeds " + std::to_string(kMaxW));
+    const Lattice Lt = make_lattice(nx, ny, a);
+    LSC_HIP(hipMemsetAsync(d_cur, 0, nx * ny * sizeof(double), s));
+    if (a.fp) LSC_HIP(hipMemsetAsync(d_fp, 0, nx * ny * sizeof(double), s));
+    DevBuf<unsigned long long> cnt(2);
+    LSC_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(unsigned long long), s));
+    DevBuf<double> corr;
+    const bool correct = a.correct && a.block > 1;
+    if (correct && Lt.nlat > 0) {
+        IrStats cst;
+        corr = to_device(correction<T>(a, nx, ny, s, cst), s);
+    }
+    lattice_pass<T>(d_res, d_src, Lt, a, 0, corr.p, d_cur, cnt.p, s, st);
+    if (a.fp) {
+        IrStats fst;
+        lattice_pass<T>(d_res, d_src, Lt, a, 1, corr.p, d_fp, cnt.p + 1, s, fst);
+    }
+    unsigned long long h = 0;
+    LSC_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipStreamSynchronize(s));
+    *n_targets = static_cast<long long>(h);
+}
+
+}  // namespace lsc
+
+namespace {
+
+lsc::OmniArgs omni_args(int radius, int buffer, int block_size, int four, int rc, int correct, int fp,
+                        const lsc_solve_options* opt) {
+    lsc::OmniArgs a{};
+    a.radius = radius, a.buffer = buffer, a.block = block_size, a.four = four, a.rc = rc, a.correct = correct,
+    a.fp = fp;
+    a.precision = opt ? opt->precision : LSC_PRECISION_MIXED;
+    a.tol = opt && opt->tol > 0 ? opt->tol : 1e-12;
+    a.maxit = opt && opt->max_refine > 0 ? opt->max_refine : 30;
+    a.batch = opt && opt->batch > 0 ? opt->batch : 4096;
+    a.timing = opt ? opt->timing : 0;
+    return a;
+}
+
+void fill_stats(const lsc::IrStats& st, long long n_targets, lsc_solve_stats* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->n_targets = n_targets;
+    out->n_solved = st.solved;
+    out->n_unconverged = st.unconverged;
+    out->n_over_1e10 = st.over_bar;
+    out->refine_max = st.iter_max;
+    out->n_batches = st.batches;
+    out->n_solve_launches = st.solve_launches;
+    out->n_factored = st.solved;
+    out->refine_mean = st.solved ? static_cast<double>(st.iter_sum) / st.solved : 0.0;
+    out->resid_max = st.resid_max;
+    out->resid_mean = st.solved ? st.resid_sum / st.solved : 0.0;
+    out->ms_fill = st.ms_fill, out->ms_flood = st.ms_flood, out->ms_factor = st.ms_factor;
+    out->ms_solve = st.ms_solve, out->ms_residual = st.ms_residual, out->ms_currents = st.ms_currents;
+    out->ms_accumulate = st.ms_accum;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsc_circuit_omniscape_device(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
+                                 int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
+                                 int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                                 double* current, double* flow_potential_out, int64_t* n_targets,
+                                 lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        const lsc::OmniArgs a = omni_args(radius, buffer, block_size, four_neighbors, resistance_is_conductance,
+                                          correct_artifacts, flow_potential, options);
+        hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
+        lsc::IrStats st;
+        long long nt = 0;
+        if (a.precision == LSC_PRECISION_FP64)
+            lsc::omniscape_device<double>(resistance, source, nx, ny, a, current, flow_potential_out, &nt, st, s);
+        else
+            lsc::omniscape_device<float>(resistance, source, nx, ny, a, current, flow_potential_out, &nt, st, s);
+        if (n_targets) *n_targets = nt;
+        fill_stats(st, nt, stats);
+    });
+}
+
+int lsc_circuit_omniscape(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
+                          int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
+                          int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                          double* current, double* flow_potential_out, int64_t* n_targets, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
+        const std::size_t n = static_cast<std::size_t>(nx) * ny;
+        hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
+        lsc::DevBuf<double> r(n ? n : 1), src(n ? n : 1), cur(n ? n : 1), fp(flow_potential && n ? n : 1);
+        if (n) {
+            r.upload(resistance, n, s);
+            src.upload(source, n, s);
+        }
+        int rc = lsc_circuit_omniscape_device(r.p, src.p, nx, ny, radius, buffer, block_size, four_neighbors,
+                                              resistance_is_conductance, correct_artifacts, flow_potential, options,
+                                              cur.p, fp.p, n_targets, stats);
+        if (rc != 0) throw lsc::HipError(lsc_last_error());
+        if (n) {
+            LSC_HIP(hipMemcpyAsync(current, cur.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            if (flow_potential)
+                LSC_HIP(hipMemcpyAsync(flow_potential_out, fp.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        LSC_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, double* out) {
+    return lsc::guarded([&] {
+        if (radius < 0 || buffer < 0) throw std::invalid_argument("radius and buffer must be >= 0");
+        const lsc::Plan p = lsc::window_plan(radius + buffer, disk != 0, four_neighbors != 0);
+        out[0] = p.n, out[1] = p.nfront, out[2] = static_cast<double>(p.lsize), out[3] = p.flops;
+        out[4] = p.max_front, out[5] = static_cast<double>(p.lev_ptr.size() - 1);
+    });
+}
+
+int lsc_circuit_omniscape_correction(int radius, int buffer, int block_size, int four_neighbors, int64_t nx,
+                                     int64_t ny, const lsc_solve_options* options, double* out) {
+    return lsc::guarded([&] {
+        lsc::OmniArgs a = omni_args(radius, buffer, block_size, four_neighbors, 0, 1, 0, options);
+        if (a.radius < 0 || a.buffer < 0 || a.block < 1 || a.block % 2 == 0)
+            throw std::invalid_argument("radius and buffer must be >= 0 and block_size odd");
+        const long long W = 2 * (static_cast<long long>(radius) + buffer) + 1;
+        if (a.block == 1) {
+            std::fill(out, out + W * W, 1.0);
+            return;
+        }
+        hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
+        lsc::IrStats st;
+        std::vector<double> c = a.precision == LSC_PRECISION_FP64 ? lsc::correction<double>(a, nx, ny, s, st)
+                                                                  : lsc::correction<float>(a, nx, ny, s, st);
+        std::copy(c.begin(), c.end(), out);
+    });
+}
+
+}  // extern "C"

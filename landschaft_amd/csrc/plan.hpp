@@ -1,0 +1,67 @@
+// Host-side symbolic plans of the multifrontal Cholesky, laid out for the GPU.
+//
+// A plan orders the unknowns of a set of grid cells by geometric nested
+// dissection (the scheme of the reference's TreeBuilder,
+// src/cpp/circuit/system.cpp:25-94: boxes shrunk to their cells, cut by the
+// middle row or column, leaves of <= 16 cells), builds every front's update
+// rows and its child -> parent row map (system.cpp:277-339), and then adds
+// what the device kernels need:
+//   * fronts grouped by height (0 = leaves), so one launch or one barrier
+//     phase factors every front of a level;
+//   * L stored level by level, each front's block as packed columns
+//     (column j holds rows j..m-1), so a level's factor is one contiguous
+//     stream of HBM;
+//   * per (unknown, neighbour direction): the front row of that matrix entry
+//     (-1 when the entry belongs to the neighbour's front);
+//   * offsets of the update matrices (nu x nu, column-major) and update
+//     vectors (nu) of every front.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace lsc {
+
+using i64 = std::int64_t;
+using i32 = std::int32_t;
+
+// Neighbours in increasing column-major index (the reference's kNeighbours8/4,
+// src/cpp/circuit/grid.hpp:29-31): this order fixes the diagonal sums.
+struct Offset {
+    int dy, dx;
+    bool diagonal;
+};
+constexpr Offset kNb8[8] = {{-1, -1, true}, {0, -1, false}, {1, -1, true}, {-1, 0, false},
+                            {1, 0, false},  {-1, 1, true},  {0, 1, false}, {1, 1, true}};
+constexpr Offset kNb4[4] = {{0, -1, false}, {-1, 0, false}, {1, 0, false}, {0, 1, false}};
+
+struct Plan {
+    i64 bx = 0, by = 0;   // box (row-major bx x by)
+    bool four = false;
+    int nd = 8;
+    i32 n = 0;            // unknowns
+    i32 nfront = 0;
+    // unknowns
+    std::vector<i32> cell;      // unknown -> box cell
+    std::vector<i32> unk_of;    // box cell -> unknown (-1)
+    std::vector<i32> nbr;       // [n*nd] neighbouring unknown (-1: none)
+    std::vector<i32> amap;      // [n*nd] front row of the entry (p, nbr) when nbr > p, else -1
+    // fronts, in post-order (children before parents)
+    std::vector<i32> first, npiv, nupd, upd, cb, ce, height;
+    std::vector<i64> lpos, upos, vpos;
+    std::vector<i32> kids, rows, relmap;
+    // levels: fronts of height h are order[lev_ptr[h] .. lev_ptr[h+1])
+    std::vector<i32> lev_ptr, order;
+    i64 lsize = 0, usize = 0, vsize = 0;
+    i32 max_front = 0;
+    double flops = 0;     // sum np^3/3 + np^2 nu + np nu^2
+};
+
+// The plan of the marked cells of a bx x by box.
+Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four);
+
+// An Omniscape window: W = 2D + 1 square, the cells within D of the centre
+// (every cell when disk is false), without the centre (the ground).
+Plan window_plan(int D, bool disk, bool four);
+
+}  // namespace lsc

@@ -1,0 +1,83 @@
+// The batched window engine: many circuits that share one symbolic plan (an
+// Omniscape window shape) solved together on one GPU. Host side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+namespace lsc {
+
+constexpr int kThreads = 256;      // 4 wavefronts per workgroup
+constexpr int kWaves = 4;
+constexpr int kSmallRows = 64;     // fronts of <= 64 rows (RHS row included): one wavefront each
+constexpr int kBigTiles = 10;      // fronts of <= 160 rows: the whole workgroup, tiles in LDS
+constexpr int kLbufEntries = 12288;  // solve kernels: L staged through LDS in chunks of this many entries
+constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
+
+// Device copy of a Plan plus the per-level chunking of the solve kernels.
+struct DPlan {
+    int n, nd, W, D, nfront, H;
+    const int *cell, *nbr, *amap;
+    const int *first, *npiv, *nupd, *upd, *cb, *ce;
+    const long long *lpos, *uposA, *vpos;
+    const int *kids, *rows, *relmap;
+    const int *lev_ptr, *lev_small, *order;
+    const int *chunk_lev, *chunk_begin, *chunk_end;  // solve chunks: level h's chunks chunk_lev[h]..chunk_lev[h+1]
+    long long lsize, usizeA, vsize;
+};
+
+struct DevPlan {
+    Plan host;
+    DPlan d{};
+    std::vector<DevBuf<int>> ibufs;
+    std::vector<DevBuf<long long>> lbufs;
+    void upload(Plan p, hipStream_t s);
+};
+
+// Per-window arrays of a batch (window slot k of B).
+struct WindowBatchBuffers {
+    int B = 0, W2 = 0, n = 0;
+    long long lsize = 0;
+    int elem = 4;  // bytes of an L entry
+    DevBuf<double> g, src, dg, sc, b, x, y, z, rs, cur, rnorm, rprev;
+    DevBuf<unsigned char> present;
+    DevBuf<int> active, fail, done, iters;
+    DevBuf<char> L;
+    void ensure(int B_, int W2_, int n_, long long lsize_, int elem_);
+};
+
+// Scratch of the persistent kernels (one slot per resident workgroup).
+struct SlotBuffers {
+    int slots = 0;
+    DevBuf<char> U;
+    DevBuf<double> V;
+    DevBuf<int> counter;
+    void ensure(int slots_, long long usizeA, long long vsize, int elem);
+};
+
+// Geometry of an Omniscape run on the target lattice (reference
+// omniscape.cpp:406-421): target (ix, iy) sits at (br + ix bs, br + iy bs),
+// lattice index ix * nly + iy.
+struct Lattice {
+    long long nx, ny;
+    int R, D, W, bs, br, nly;
+    long long nlat;
+    int rc;  // resistance_is_conductance
+};
+
+struct IrStats {
+    long long solved = 0, unconverged = 0, over_bar = 0, iter_sum = 0, solve_launches = 0;
+    int batches = 0;
+    int iter_max = 0;
+    double resid_max = 0, resid_sum = 0;
+    double ms_fill = 0, ms_flood = 0, ms_factor = 0, ms_solve = 0, ms_residual = 0, ms_currents = 0,
+           ms_accum = 0;
+};
+
+}  // namespace lsc

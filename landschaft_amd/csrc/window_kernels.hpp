@@ -1,0 +1,749 @@
+// Device kernels of the batched window engine (gfx950).
+//
+// One window = one circuit of the reference's solve_window
+// (src/cpp/circuit/omniscape.cpp:72-122): the cells of a W x W box within
+// radius D of the centre, grounded at the centre, NoData cells out of the
+// circuit (identity rows, system.cpp:466-469), only the ground's connected
+// component solved. Every window of a run shares one symbolic plan, so a
+// batch of windows is a uniform batch of sparse Cholesky factorisations.
+//
+// Pipeline per batch (one kernel each, so rocprof times each phase):
+//   k_fill      conductance and scaled sources of every window from the maps
+//   k_flood     the ground's component (row bit masks), Laplacian diagonal,
+//               Jacobi scale s = diag^-1/2, right-hand side
+//   k_factor    multifrontal LL^T of S A S (T = float: MFMA f32, or double),
+//               forward substitution fused (RHS row of every front)
+//   k_solve     level-by-level triangular solves, L staged through LDS
+//   k_residual  r = b - A x in FP64 from the conductances (never from L)
+//   k_currents  node currents (the reference's node_currents, grid.cpp:19-71),
+//               times the block-artifact correction
+//   k_accumulate  cumulative map, every cell summing its windows in a fixed
+//               order (deterministic: no atomics)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "front.hpp"
+#include "window.hpp"
+
+namespace lsc {
+
+namespace {
+
+__constant__ int c_dy8[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+__constant__ int c_dx8[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+__constant__ int c_dg8[8] = {1, 0, 1, 0, 0, 1, 0, 1};
+__constant__ int c_dy4[4] = {0, -1, 1, 0};
+__constant__ int c_dx4[4] = {-1, 0, 0, 1};
+
+__device__ __forceinline__ int ndy(int nd, int d) { return nd == 8 ? c_dy8[d] : c_dy4[d]; }
+__device__ __forceinline__ int ndx(int nd, int d) { return nd == 8 ? c_dx8[d] : c_dx4[d]; }
+__device__ __forceinline__ bool ndiag(int nd, int d) { return nd == 8 && c_dg8[d]; }
+
+// Circuitscape's edge weight (reference grid.hpp:38-40).
+__device__ __forceinline__ double edge_weight(double a, double b, bool diag) {
+    return diag ? (a + b) / 2.8284271247461903 : (a + b) / 2;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < kWaves; ++w) t += red[w];
+    return t;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = red[0];
+    for (int w = 1; w < kWaves; ++w) t = fmax(t, red[w]);
+    return t;
+}
+
+}  // namespace
+
+__device__ __forceinline__ void target_of(const Lattice& L, long long k, long long& tx, long long& ty) {
+    tx = L.br + (k / L.nly) * L.bs;
+    ty = L.br + (k % L.nly) * L.bs;
+}
+
+// Conductance within D (NoData -9999) and sources within R of every window,
+// the target block zeroed at D from the clipped window's corner and the
+// block's strength shared out (reference omniscape.cpp:406-421, 495-525).
+__global__ __launch_bounds__(kThreads) void k_fill(const double* __restrict__ res, const double* __restrict__ source,
+                                                   Lattice Lt, long long k0, int B, double* __restrict__ g,
+                                                   double* __restrict__ src, int* __restrict__ active,
+                                                   unsigned long long* __restrict__ n_targets) {
+    __shared__ double red[kWaves];
+    __shared__ double s_amps;
+    const int k = blockIdx.x;
+    if (k >= B) return;
+    const long long kk = k0 + k;
+    const int W = Lt.W, D = Lt.D, W2 = W * W;
+    double* gw = g + static_cast<long long>(k) * W2;
+    double* sw = src + static_cast<long long>(k) * W2;
+    if (kk >= Lt.nlat) {
+        if (threadIdx.x == 0) active[k] = 0;
+        return;
+    }
+    long long tx, ty;
+    target_of(Lt, kk, tx, ty);
+    if (threadIdx.x == 0) {
+        double amps = 0.0;  // column-major block sum, NaN skipped (omniscape.cpp:410-415)
+        for (long long xx = tx - Lt.br; xx <= tx + Lt.br; ++xx)
+            for (long long yy = ty - Lt.br; yy <= ty + Lt.br; ++yy) {
+                const double v = source[yy * Lt.nx + xx];
+                if (!isnan(v)) amps += v;
+            }
+        s_amps = amps;
+    }
+    __syncthreads();
+    const double amps = s_amps;
+    const long long oy = ty - D, ox = tx - D;
+    const long long y0c = oy > 0 ? oy : 0, x0c = ox > 0 ? ox : 0;
+    const long long zy1 = (D + Lt.br < Lt.ny - 1) ? D + Lt.br : Lt.ny - 1;
+    const long long zx1 = (D + Lt.br < Lt.nx - 1) ? D + Lt.br : Lt.nx - 1;
+    double part = 0.0;
+    for (int c = threadIdx.x; c < W2; c += kThreads) {
+        const int y = c / W, x = c - (c / W) * W;
+        const long long Y = oy + y, X = ox + x;
+        const bool on = Y >= 0 && Y < Lt.ny && X >= 0 && X < Lt.nx;
+        const long long d2 = static_cast<long long>(y - D) * (y - D) + static_cast<long long>(x - D) * (x - D);
+        double gv = -9999.0, sv = 0.0;
+        if (on) {
+            const double r = res[Y * Lt.nx + X];
+            if (d2 <= static_cast<long long>(D) * D && !isnan(r)) gv = Lt.rc ? r : 1 / r;
+            if (d2 <= static_cast<long long>(Lt.R) * Lt.R) {
+                sv = source[Y * Lt.nx + X];
+                if (isnan(sv)) sv = 0.0;
+            }
+            const long long yc = Y - y0c, xc = X - x0c;  // the clipped window's frame
+            if (yc >= D - Lt.br && yc <= zy1 && xc >= D - Lt.br && xc <= zx1) sv = 0.0;
+        }
+        gw[c] = gv;
+        sw[c] = sv;
+        if (sv > 0) part += sv;
+    }
+    const double sum = block_sum(part, red);
+    const bool target = amps > 0;
+    if (target) {
+        for (int c = threadIdx.x; c < W2; c += kThreads) {
+            const double v = sw[c];
+            if (v > 0) sw[c] = (v * amps) / sum;
+        }
+    }
+    if (threadIdx.x == 0) {
+        active[k] = target ? 1 : 0;
+        if (target) atomicAdd(n_targets, 1ull);
+    }
+}
+
+// The ground's component (cells with g > 0 joined to the centre), by row bit
+// masks: runs are filled along rows with one 128-bit add, then grown between
+// rows, until nothing changes. Then the Laplacian diagonal of every unknown
+// (edges to present cells in column-major neighbour order + the 10 eps
+// shift, reference system.cpp:471-485), its Jacobi scale and the RHS.
+__global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const double* __restrict__ g,
+                                                    const double* __restrict__ src, int* __restrict__ active,
+                                                    unsigned char* __restrict__ present, double* __restrict__ dg,
+                                                    double* __restrict__ sc, double* __restrict__ bvec,
+                                                    double* __restrict__ rs, double* __restrict__ x, double shift) {
+    __shared__ unsigned long long om[kMaxW + 1][2], pm[kMaxW + 1][2];
+    __shared__ double red[kWaves];
+    __shared__ int s_ok;
+    const int k = blockIdx.x;
+    if (k >= B || !active[k]) return;
+    const int W = P.W, W2 = W * W, D = P.D, n = P.n, nd = P.nd;
+    const double* gw = g + static_cast<long long>(k) * W2;
+    const double* sw = src + static_cast<long long>(k) * W2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ground = D * W + D;
+    if (threadIdx.x == 0) s_ok = gw[ground] > 0;
+    __syncthreads();
+    if (!s_ok) {
+        if (threadIdx.x == 0) active[k] = 0;
+        return;
+    }
+    // open masks
+    for (int y = wave; y < W; y += kWaves) {
+        for (int h = 0; h < 2; ++h) {
+            const int xx = h * 64 + lane;
+            const bool open = xx < W && gw[y * W + xx] > 0;
+            const unsigned long long m = __ballot(open);
+            if (lane == 0) om[y][h] = m;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        unsigned long long O[2][2], Pm[2][2];
+        for (int r = 0; r < 2; ++r) {
+            const int y = lane + 64 * r;
+            for (int h = 0; h < 2; ++h) {
+                O[r][h] = y < W ? om[y][h] : 0ull;
+                Pm[r][h] = 0ull;
+            }
+            if (y == D) Pm[r][D >> 6] = 1ull << (D & 63);
+        }
+        for (int it = 0; it < W2 + 2; ++it) {
+            // along rows, both directions
+            for (int r = 0; r < 2; ++r) {
+                const unsigned long long o0 = O[r][0], o1 = O[r][1];
+                unsigned long long s0 = Pm[r][0], s1 = Pm[r][1];
+                unsigned long long a0 = o0 + s0;
+                unsigned long long a1 = o1 + s1 + (a0 < o0 ? 1ull : 0ull);
+                const unsigned long long f0 = ((a0 ^ o0) & o0) | s0, f1 = ((a1 ^ o1) & o1) | s1;
+                // reversed rows for the other direction
+                const unsigned long long ro0 = __builtin_bitreverse64(o1), ro1 = __builtin_bitreverse64(o0);
+                const unsigned long long rs0 = __builtin_bitreverse64(s1), rs1 = __builtin_bitreverse64(s0);
+                const unsigned long long b0 = ro0 + rs0;
+                const unsigned long long b1 = ro1 + rs1 + (b0 < ro0 ? 1ull : 0ull);
+                const unsigned long long e0 = ((b0 ^ ro0) & ro0) | rs0, e1 = ((b1 ^ ro1) & ro1) | rs1;
+                Pm[r][0] = f0 | __builtin_bitreverse64(e1);
+                Pm[r][1] = f1 | __builtin_bitreverse64(e0);
+            }
+            for (int r = 0; r < 2; ++r) {
+                const int y = lane + 64 * r;
+                if (y < W) pm[y][0] = Pm[r][0], pm[y][1] = Pm[r][1];
+            }
+            wave_sync();
+            bool changed = false;
+            for (int r = 0; r < 2; ++r) {
+                const int y = lane + 64 * r;
+                if (y >= W) continue;
+                unsigned long long n0 = 0, n1 = 0;
+                if (y > 0) n0 |= pm[y - 1][0], n1 |= pm[y - 1][1];
+                if (y + 1 < W) n0 |= pm[y + 1][0], n1 |= pm[y + 1][1];
+                if (nd == 8) {
+                    const unsigned long long l0 = n0 << 1, l1 = (n1 << 1) | (n0 >> 63);
+                    const unsigned long long r0 = (n0 >> 1) | (n1 << 63), r1 = n1 >> 1;
+                    n0 |= l0 | r0;
+                    n1 |= l1 | r1;
+                }
+                const unsigned long long p0 = Pm[r][0] | (n0 & O[r][0]), p1 = Pm[r][1] | (n1 & O[r][1]);
+                changed = changed || p0 != Pm[r][0] || p1 != Pm[r][1];
+                Pm[r][0] = p0, Pm[r][1] = p1;
+            }
+            wave_sync();
+            if (!__any(changed)) break;
+        }
+        for (int r = 0; r < 2; ++r) {
+            const int y = lane + 64 * r;
+            if (y < W) pm[y][0] = Pm[r][0], pm[y][1] = Pm[r][1];
+        }
+    }
+    __syncthreads();
+    unsigned char* pw = present + static_cast<long long>(k) * W2;
+    double part = 0.0;
+    for (int c = threadIdx.x; c < W2; c += kThreads) {
+        const int y = c / W, xx = c - (c / W) * W;
+        const bool p = (pm[y][xx >> 6] >> (xx & 63)) & 1ull;
+        pw[c] = p;
+        if (p && c != ground) part += sw[c];
+    }
+    const double total = block_sum(part, red);
+    if (total == 0) {  // no source in the ground's component (omniscape.cpp:101)
+        if (threadIdx.x == 0) active[k] = 0;
+        return;
+    }
+    const long long off = static_cast<long long>(k) * n;
+    for (int p = threadIdx.x; p < n; p += kThreads) {
+        const int c = P.cell[p];
+        const int y = c / W, xx = c - (c / W) * W;
+        const bool pr = (pm[y][xx >> 6] >> (xx & 63)) & 1ull;
+        double d = 1.0, s = 1.0, b = 0.0;
+        if (pr) {
+            double sum = 0.0;
+            const double gc = gw[c];
+            for (int e = 0; e < nd; ++e) {
+                const int yy = y + ndy(nd, e), x2 = xx + ndx(nd, e);
+                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
+                if (!((pm[yy][x2 >> 6] >> (x2 & 63)) & 1ull)) continue;
+                sum += edge_weight(gc, gw[yy * W + x2], ndiag(nd, e));
+            }
+            d = sum + shift;
+            s = 1.0 / sqrt(d);
+            b = sw[c];
+        }
+        dg[off + p] = d;
+        sc[off + p] = s;
+        bvec[off + p] = b;
+        rs[off + p] = s * b;
+        x[off + p] = 0.0;
+    }
+}
+
+namespace {
+
+// One front: assembled (S A S entries, RHS row, children's update matrices),
+// factorised, written out (L packed by columns, y, the update matrix).
+template <typename T, int R, typename Sync>
+__device__ void process_front(const DPlan& P, int t, T* F, int tid, int nt, int wave, int nw, int lane, int* flag,
+                              Sync sync, const double* gw, const unsigned char* pw, const double* sw,
+                              const double* rsw, T* Lw, double* yw, T* U) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, ma = m + 1, first = P.first[t];
+    const int MT = (ma + 15) >> 4;
+    const int nz = ((MT * (MT + 1)) >> 1) * 256;
+    for (int e = tid; e < nz; e += nt) F[e] = T(0);
+    sync();
+    const int W = P.W, nd = P.nd;
+    for (int j = tid; j < np; j += nt) {
+        const int p = first + j, c = P.cell[p];
+        F[tix(j, j)] = T(1);
+        F[tix(m, j)] = static_cast<T>(rsw[p]);
+        if (pw[c]) {
+            const double sp = sw[p], gc = gw[c];
+            const int y = c / W, xx = c - (c / W) * W;
+            for (int d = 0; d < nd; ++d) {
+                const int a = P.amap[p * nd + d];
+                if (a < 0) continue;
+                const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
+                if (!pw[c2]) continue;
+                const int q = P.nbr[p * nd + d];
+                const double w = edge_weight(gc, gw[c2], ndiag(nd, d));
+                F[tix(a, j)] = static_cast<T>(-w * sp * sw[q]);
+            }
+        }
+    }
+    sync();
+    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c], u1 = u + 1;
+        const int* rel = P.relmap + P.upd[c];
+        const T* Uc = U + P.uposA[c];
+        for (int e = tid; e < u1 * u; e += nt) {
+            const int jj = e / u1, ii = e - jj * u1;
+            if (ii < jj) continue;
+            const int row = ii < u ? rel[ii] : m;
+            F[tix(row, rel[jj])] += Uc[jj * u1 + ii];
+        }
+        sync();
+    }
+    factor_front<T, R>(F, ma, np, wave, nw, lane, flag, sync);
+    T* Lt = Lw + P.lpos[t];
+    for (int e = tid; e < np * m; e += nt) {
+        const int j = e / m, i = e - j * m;
+        if (i < j) continue;
+        Lt[static_cast<long long>(j) * m - (static_cast<long long>(j) * (j - 1) >> 1) + (i - j)] = F[tix(i, j)];
+    }
+    for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
+    const int u1 = nu + 1;
+    T* Ut = U + P.uposA[t];
+    for (int e = tid; e < u1 * u1; e += nt) {
+        const int jj = e / u1, ii = e - jj * u1;
+        if (ii < jj) continue;
+        Ut[jj * u1 + ii] = F[tix(np + ii, np + jj)];
+    }
+    sync();
+}
+
+}  // namespace
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_factor(DPlan P, int B, const double* __restrict__ g,
+                                                     const unsigned char* __restrict__ present,
+                                                     const double* __restrict__ sc, const double* __restrict__ rs,
+                                                     const int* __restrict__ active, T* __restrict__ L,
+                                                     double* __restrict__ y, T* __restrict__ Uall,
+                                                     int* __restrict__ fail, int* __restrict__ counter) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* big = reinterpret_cast<T*>(smem_raw);
+    __shared__ int s_win;
+    __shared__ int s_flag[kWaves + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T* smallF = big + wave * 10 * 256;
+    T* U = Uall + static_cast<long long>(blockIdx.x) * P.usizeA;
+    const int W2 = P.W * P.W;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_win = atomicAdd(counter, 1);
+        if (threadIdx.x <= kWaves) s_flag[threadIdx.x] = 0;
+        __syncthreads();
+        const int k = s_win;
+        if (k >= B) break;
+        if (!active[k]) continue;
+        const double* gw = g + static_cast<long long>(k) * W2;
+        const unsigned char* pw = present + static_cast<long long>(k) * W2;
+        const double* sw = sc + static_cast<long long>(k) * P.n;
+        const double* rsw = rs + static_cast<long long>(k) * P.n;
+        double* yw = y + static_cast<long long>(k) * P.n;
+        T* Lw = L + static_cast<long long>(k) * P.lsize;
+        auto wsync = [] { wave_sync(); };
+        auto bsync = [] { __syncthreads(); };
+        for (int h = 0; h < P.H; ++h) {
+            const int base = P.lev_ptr[h], ns = P.lev_small[h], ne = P.lev_ptr[h + 1];
+            for (int idx = base + wave; idx < base + ns; idx += kWaves)
+                process_front<T, 1>(P, P.order[idx], smallF, lane, 64, 0, 1, lane, &s_flag[wave], wsync, gw, pw,
+                                    sw, rsw, Lw, yw, U);
+            __syncthreads();
+            for (int idx = base + ns; idx < ne; ++idx)
+                process_front<T, 3>(P, P.order[idx], big, threadIdx.x, kThreads, wave, kWaves, lane,
+                                    &s_flag[kWaves], bsync, gw, pw, sw, rsw, Lw, yw, U);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            int f = 0;
+            for (int w = 0; w <= kWaves; ++w) f |= s_flag[w];
+            fail[k] = f;
+        }
+    }
+}
+
+namespace {
+
+__device__ __forceinline__ long long colstart(int j, int m) {
+    return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
+}
+
+__device__ __forceinline__ double pick(const double (&v)[3], int slot) {
+    return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
+}
+
+// Forward substitution of front t (one wavefront): its vector assembled from
+// the pivots' RHS and the children's update vectors, then y = L11^-1 f and
+// the update vector f_upd - L21 y (reference system.cpp:974-989).
+template <typename T>
+__device__ void forward_front(const DPlan& P, int t, const T* Lc, double* fv, int lane, const double* rsw,
+                              double* yw, double* V) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    for (int i = lane; i < m; i += 64) fv[i] = i < np ? rsw[first + i] : 0.0;
+    wave_sync();
+    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c];
+        const int* rel = P.relmap + P.upd[c];
+        const double* Vc = V + P.vpos[c];
+        for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
+        wave_sync();
+    }
+    double f[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int i = lane + 64 * r;
+        f[r] = i < m ? fv[i] : 0.0;
+    }
+    for (int j = 0; j < np; ++j) {
+        const long long cs = colstart(j, m);
+        const int slot = j >> 6;
+        const double yj = bcast(pick(f, slot), j & 63) / static_cast<double>(Lc[cs]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int i = lane + 64 * r;
+            if (i > j && i < m) f[r] -= static_cast<double>(Lc[cs + i - j]) * yj;
+        }
+        if (lane == (j & 63)) {
+            if (slot == 0) f[0] = yj;
+            else if (slot == 1) f[1] = yj;
+            else f[2] = yj;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int i = lane + 64 * r;
+        if (i < np) yw[first + i] = f[r];
+        else if (i < m) V[P.vpos[t] + i - np] = f[r];
+    }
+    wave_sync();
+}
+
+// Back substitution of front t (one wavefront): z_piv = L11^-T (y - L21^T
+// z_upd) (reference system.cpp:992-1016).
+template <typename T>
+__device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, int lane, const double* yw,
+                               double* zw) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int* rows = P.rows + P.upd[t];
+    for (int ii = lane; ii < nu; ii += 64) xu[ii] = zw[rows[ii]];
+    wave_sync();
+    double tv[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int j = lane + 64 * r;
+        double acc = 0.0;
+        if (j < np) {
+            const T* col = Lc + colstart(j, m) + (np - j);
+            for (int ii = 0; ii < nu; ++ii) acc += static_cast<double>(col[ii]) * xu[ii];
+            acc = yw[first + j] - acc;
+        }
+        tv[r] = acc;
+    }
+    for (int j = np - 1; j >= 0; --j) {
+        const int slot = j >> 6;
+        const double xj = bcast(pick(tv, slot), j & 63) / static_cast<double>(Lc[colstart(j, m)]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int i = lane + 64 * r;
+            if (i < j) tv[r] -= static_cast<double>(Lc[colstart(i, m) + (j - i)]) * xj;
+        }
+        if (lane == (j & 63)) {
+            if (slot == 0) tv[0] = xj;
+            else if (slot == 1) tv[1] = xj;
+            else tv[2] = xj;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int i = lane + 64 * r;
+        if (i < np) zw[first + i] = tv[r];
+    }
+    wave_sync();
+}
+
+template <typename T>
+__device__ void stage_chunk(const DPlan& P, int c, const T* Lw, T* Lbuf, long long& base) {
+    const int t0 = P.order[P.chunk_begin[c]], t1 = P.order[P.chunk_end[c] - 1];
+    const int np1 = P.npiv[t1], m1 = np1 + P.nupd[t1];
+    base = P.lpos[t0];
+    const long long end = P.lpos[t1] + static_cast<long long>(np1) * m1 - ((static_cast<long long>(np1) * (np1 - 1)) >> 1);
+    const int cnt = static_cast<int>(end - base);
+    for (int e = threadIdx.x; e < cnt; e += kThreads) Lbuf[e] = Lw[base + e];
+    __syncthreads();
+}
+
+}  // namespace
+
+// Triangular solves of the windows not yet done. mode 0: back substitution
+// of the factor's fused forward result y, x = s z. mode 1: a refinement
+// step, forward and back substitution of rs, x += s z.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_solve(DPlan P, int B, int mode, const int* __restrict__ active,
+                                                    const int* __restrict__ done, const T* __restrict__ L,
+                                                    const double* __restrict__ rs, double* __restrict__ y,
+                                                    double* __restrict__ z, const double* __restrict__ sc,
+                                                    double* __restrict__ x, double* __restrict__ Vall,
+                                                    int* __restrict__ counter) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* Lbuf = reinterpret_cast<T*>(smem_raw);
+    __shared__ double vbuf[kWaves][192];
+    __shared__ int s_win;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* V = Vall + static_cast<long long>(blockIdx.x) * P.vsize;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_win = atomicAdd(counter, 1);
+        __syncthreads();
+        const int k = s_win;
+        if (k >= B) break;
+        if (!active[k] || done[k]) continue;
+        const long long off = static_cast<long long>(k) * P.n;
+        const T* Lw = L + static_cast<long long>(k) * P.lsize;
+        double* yw = y + off;
+        double* zw = z + off;
+        if (mode == 1) {
+            for (int h = 0; h < P.H; ++h)
+                for (int c = P.chunk_lev[h]; c < P.chunk_lev[h + 1]; ++c) {
+                    long long base;
+                    stage_chunk<T>(P, c, Lw, Lbuf, base);
+                    for (int idx = P.chunk_begin[c] + wave; idx < P.chunk_end[c]; idx += kWaves) {
+                        const int t = P.order[idx];
+                        forward_front<T>(P, t, Lbuf + (P.lpos[t] - base), vbuf[wave], lane, rs + off, yw, V);
+                    }
+                    __syncthreads();
+                }
+        }
+        for (int h = P.H - 1; h >= 0; --h)
+            for (int c = P.chunk_lev[h]; c < P.chunk_lev[h + 1]; ++c) {
+                long long base;
+                stage_chunk<T>(P, c, Lw, Lbuf, base);
+                for (int idx = P.chunk_begin[c] + wave; idx < P.chunk_end[c]; idx += kWaves) {
+                    const int t = P.order[idx];
+                    backward_front<T>(P, t, Lbuf + (P.lpos[t] - base), vbuf[wave], lane, yw, zw);
+                }
+                __syncthreads();
+            }
+        for (int p = threadIdx.x; p < P.n; p += kThreads) {
+            const double d = sc[off + p] * zw[p];
+            x[off + p] = mode == 0 ? d : x[off + p] + d;
+        }
+    }
+}
+
+// r = b - A x in FP64 from the conductances (the reference's matrix,
+// shifted, system.cpp:458-487); relative residual ||r|| / ||b||; the scaled
+// residual s r for the next refinement step. A window is done when it
+// reaches `tol` or stops improving.
+__global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, const double* __restrict__ g,
+                                                       const unsigned char* __restrict__ present,
+                                                       const double* __restrict__ dg, const double* __restrict__ sc,
+                                                       const double* __restrict__ bvec,
+                                                       const double* __restrict__ x, double* __restrict__ rs,
+                                                       const int* __restrict__ active, int* __restrict__ done,
+                                                       double* __restrict__ rnorm, double* __restrict__ rprev,
+                                                       int* __restrict__ iters, double tol, int* __restrict__ open) {
+    __shared__ double red[kWaves];
+    const int k = blockIdx.x;
+    if (k >= B || !active[k] || done[k]) return;
+    const int W = P.W, W2 = W * W, nd = P.nd, n = P.n;
+    const double* gw = g + static_cast<long long>(k) * W2;
+    const unsigned char* pw = present + static_cast<long long>(k) * W2;
+    const long long off = static_cast<long long>(k) * n;
+    double rr = 0.0, bb = 0.0;
+    for (int p = threadIdx.x; p < n; p += kThreads) {
+        const int c = P.cell[p];
+        double r = 0.0;
+        if (pw[c]) {
+            const int y = c / W, xx = c - (c / W) * W;
+            const double gc = gw[c];
+            double acc = dg[off + p] * x[off + p];
+            for (int d = 0; d < nd; ++d) {
+                const int q = P.nbr[p * nd + d];
+                if (q < 0) continue;
+                const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
+                if (!pw[c2]) continue;
+                acc -= edge_weight(gc, gw[c2], ndiag(nd, d)) * x[off + q];
+            }
+            const double b = bvec[off + p];
+            r = b - acc;
+            bb += b * b;
+        }
+        rr += r * r;
+        rs[off + p] = sc[off + p] * r;
+    }
+    rr = block_sum(rr, red);
+    bb = block_sum(bb, red);
+    if (threadIdx.x == 0) {
+        const double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+        const double prev = rprev[k];
+        rnorm[k] = rel;
+        rprev[k] = rel;
+        const bool stalled = iters[k] > 0 && rel > 0.5 * prev;
+        if (rel <= tol || stalled || !(rel == rel)) {
+            done[k] = 1;
+        } else {
+            iters[k] += 1;
+            atomicAdd(open, 1);
+        }
+    }
+}
+
+// Node currents of every window (reference grid.cpp:19-71), times the
+// correction factors of the clipped window (omniscape.cpp:553-571).
+__global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
+                                                       const double* __restrict__ g,
+                                                       const unsigned char* __restrict__ present,
+                                                       const double* __restrict__ x, const int* __restrict__ active,
+                                                       const double* __restrict__ corr, double* __restrict__ cur) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* v = reinterpret_cast<double*>(smem_raw);
+    __shared__ double red[kWaves];
+    const int k = blockIdx.x;
+    if (k >= B || !active[k]) return;
+    const int W = P.W, W2 = W * W, nd = P.nd, D = P.D;
+    const int lower = nd == 8 ? 4 : 2;
+    const double* gw = g + static_cast<long long>(k) * W2;
+    const unsigned char* pw = present + static_cast<long long>(k) * W2;
+    const long long off = static_cast<long long>(k) * P.n;
+    double* cw = cur + static_cast<long long>(k) * W2;
+    for (int c = threadIdx.x; c < W2; c += kThreads) v[c] = 0.0;
+    __syncthreads();
+    for (int p = threadIdx.x; p < P.n; p += kThreads) {
+        const int c = P.cell[p];
+        if (pw[c]) v[c] = x[off + p];
+    }
+    __syncthreads();
+    double mo = -INFINITY, mi = -INFINITY;
+    for (int q = threadIdx.x; q < W2; q += kThreads) {
+        if (!pw[q]) continue;
+        const int y = q / W, xx = q - (q / W) * W;
+        for (int d = lower; d < nd; ++d) {
+            const int yy = y + ndy(nd, d), x2 = xx + ndx(nd, d);
+            if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
+            const int r = yy * W + x2;
+            if (!pw[r]) continue;
+            const double b = edge_weight(gw[q], gw[r], ndiag(nd, d)) * (v[r] - v[q]);
+            mo = fmax(mo, b);
+            mi = fmax(mi, -b);
+        }
+    }
+    const double maxout = block_max(mo, red);
+    const double maxin = block_max(mi, red);
+    // correction window offsets (clipped windows keep Omniscape's slice of it)
+    long long tx = 0, ty = 0;
+    long long cx0 = 0, cy0 = 0, ox = 0, oy = 0, x0c = 0, y0c = 0;
+    if (corr) {
+        target_of(Lt, k0 + k, tx, ty);
+        oy = ty - D, ox = tx - D;
+        y0c = oy > 0 ? oy : 0, x0c = ox > 0 ? ox : 0;
+        const long long y1c = ty + D < Lt.ny - 1 ? ty + D : Lt.ny - 1;
+        const long long x1c = tx + D < Lt.nx - 1 ? tx + D : Lt.nx - 1;
+        const long long h = y1c - y0c + 1, w = x1c - x0c + 1;
+        cx0 = (tx < D && tx + D <= Lt.nx - 1) ? W - w : 0;
+        cy0 = (ty < D && ty + D <= Lt.ny - 1) ? W - h : 0;
+    }
+    for (int q = threadIdx.x; q < W2; q += kThreads) {
+        double out = 0.0;
+        if (pw[q]) {
+            const int y = q / W, xx = q - (q / W) * W;
+            double in = 0.0, outf = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                const int yy = y + ndy(nd, d), x2 = xx + ndx(nd, d);
+                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
+                const int r = yy * W + x2;
+                if (!pw[r]) continue;
+                const bool dgl = ndiag(nd, d);
+                const double b = d < lower ? -(edge_weight(gw[r], gw[q], dgl) * (v[q] - v[r]))
+                                           : edge_weight(gw[q], gw[r], dgl) * (v[r] - v[q]);
+                if (b > 0) {
+                    if (!(fabs(b / maxin) < 1e-8)) in += b;
+                } else if (b < 0) {
+                    if (!(fabs(b / maxout) < 1e-8)) outf -= b;
+                }
+            }
+            out = in > outf ? in : outf;
+            if (corr) {
+                const long long yc = oy + y - y0c, xc = ox + xx - x0c;
+                out *= corr[(cy0 + yc) * W + (cx0 + xc)];
+            }
+        }
+        cw[q] = out;
+    }
+}
+
+// Cumulative map: every map cell adds the currents of the batch's windows
+// that cover it, in lattice order (reference omniscape.cpp:581-605 sums in
+// target order; any fixed order keeps runs reproducible).
+__global__ __launch_bounds__(kThreads) void k_accumulate(Lattice Lt, long long k0, int B, long long X0, long long ncol,
+                                                         const int* __restrict__ active,
+                                                         const double* __restrict__ cur, double* __restrict__ map) {
+    const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+    if (e >= ncol * Lt.ny) return;
+    const long long Y = e / ncol, X = X0 + (e - (e / ncol) * ncol);
+    const int D = Lt.D, W = Lt.W, bs = Lt.bs, br = Lt.br;
+    const long long k1 = k0 + B;
+    // lattice columns / rows whose window covers (Y, X)
+    long long ixa = X - D - br, ixb = X + D - br;
+    ixa = ixa <= 0 ? 0 : (ixa + bs - 1) / bs;
+    ixb = ixb < 0 ? -1 : ixb / bs;
+    long long iya = Y - D - br, iyb = Y + D - br;
+    iya = iya <= 0 ? 0 : (iya + bs - 1) / bs;
+    iyb = iyb < 0 ? -1 : iyb / bs;
+    if (iyb > Lt.nly - 1) iyb = Lt.nly - 1;
+    const long long kfirst = k0 / Lt.nly, klast = (k1 - 1) / Lt.nly;
+    if (ixa < kfirst) ixa = kfirst;
+    if (ixb > klast) ixb = klast;
+    double sum = 0.0;
+    for (long long ix = ixa; ix <= ixb; ++ix)
+        for (long long iy = iya; iy <= iyb; ++iy) {
+            const long long kk = ix * Lt.nly + iy;
+            if (kk < k0 || kk >= k1) continue;
+            const int k = static_cast<int>(kk - k0);
+            if (!active[k]) continue;
+            const long long tx = br + ix * bs, ty = br + iy * bs;
+            const long long wy = Y - ty + D, wx = X - tx + D;
+            sum += cur[static_cast<long long>(k) * W * W + wy * W + wx];
+        }
+    if (sum != 0.0) map[Y * Lt.nx + X] += sum;
+}
+
+__global__ void k_reset(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace lsc

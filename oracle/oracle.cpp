@@ -716,7 +716,8 @@ struct Target {
 };
 
 // omniscape.cpp:394-608 (no conditions). `limit` > 0 solves only the first
-// `limit` targets of the sorted order (a bounded sample of a run).
+// `limit` targets of the sorted order, < 0 only the last -limit (a bounded
+// sample of a run).
 i64 omniscape(const double* res, const double* source, i64 nx, i64 ny, int radius, int buffer, int block, bool four,
               bool rc, bool correct_artifacts, bool fp, int threads, i64 limit, double* cur_out, double* fp_out) {
     if (radius < 0 || buffer < 0 || block < 1 || block % 2 == 0)
@@ -742,6 +743,9 @@ i64 omniscape(const double* res, const double* source, i64 nx, i64 ny, int radiu
         }
     std::stable_sort(targets.begin(), targets.end(), [](const Target& a, const Target& b) { return a.shape < b.shape; });
     if (limit > 0 && static_cast<i64>(targets.size()) > limit) targets.resize(limit);
+    // limit < 0: the last |limit| targets of the order (the unclipped windows,
+    // whose shape sorts last): a bounded sample of full-size circuits.
+    if (limit < 0 && static_cast<i64>(targets.size()) > -limit) targets.erase(targets.begin(), targets.end() + limit);
     const i64 nt = static_cast<i64>(targets.size());
     if (nt == 0) return 0;
     std::vector<double> corr;

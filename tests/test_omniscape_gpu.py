@@ -1,0 +1,85 @@
+"""Omniscape on the GPU against the oracle and the reference's stored outputs."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES
+from test_oracle import _omniscape_cases, _omniscape_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def backend():
+    from landschaft_amd.circuit import _backend
+
+    saved = (_backend.settings.precision, _backend.settings.tol)
+    _backend.settings.tol = 1e-14
+    yield _backend
+    _backend.settings.precision, _backend.settings.tol = saved
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mfma_operand_layout(precision):
+    from landschaft_amd import _native as N
+
+    got = N.probe_mfma(precision)
+    i = np.arange(16)[:, None]
+    j = np.arange(16)[None, :]
+    expect = sum((i + 16 * k) * (k + 4 * j) for k in range(4)).astype(float)
+    np.testing.assert_array_equal(got, expect)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("name", _omniscape_cases())
+def test_omniscape_matches_cholmod(backend, oracle, name, precision):
+    z = np.load(FIXTURES / "omniscape.npz")
+    res, s, args, o = _omniscape_inputs(z, name)
+    backend.settings.precision = precision
+    print(f"case {name} {precision}", flush=True)
+    out = backend.circuit_omniscape(res, s, args["radius"], args["buffer"], args["block_size"],
+                                    args["four_neighbors"], args["resistance_is_conductance"],
+                                    args["correct_artifacts"], args["flow_potential"])
+    ref = oracle.omniscape(res, s, **args)
+    assert out["n_targets"] == ref["n_targets"]
+    maps = {"cum_currmap": out["current"]}
+    if args["flow_potential"]:
+        maps["flow_potential"] = out["flow_potential"]
+    for k, v in maps.items():
+        key = f"{name}/cholmod/{k}"
+        if key not in z.files:
+            continue
+        v = v.copy()
+        if o.get("mask_nodata", True):
+            v[np.isnan(res)] = np.nan
+        chol = z[key]
+        top = np.nanmax(chol)
+        np.testing.assert_array_equal(np.isnan(v), np.isnan(chol))
+        np.testing.assert_array_equal(v == 0, chol == 0)
+        np.testing.assert_allclose(v, chol, rtol=1e-9, atol=1e-13 * top)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_omniscape_random_map_matches_oracle(backend, oracle, precision):
+    rng = np.random.default_rng(11)
+    r = np.exp(rng.normal(0.0, 1.0, (90, 120)))
+    r[rng.random(r.shape) < 0.08] = np.nan
+    s = np.where(np.isnan(r), 0.0, 1 / np.nan_to_num(r, nan=1.0))
+    backend.settings.precision = precision
+    for radius, block, four in [(12, 5, False), (9, 3, True), (20, 1, False)]:
+        print(f"random r={radius} b={block} four={four} {precision}", flush=True)
+        out = backend.circuit_omniscape(r, s, radius, 0, block, four, False, True, False)
+        ref = oracle.omniscape(r, s, radius, 0, block, four, False, True, False)
+        assert out["n_targets"] == ref["n_targets"]
+        top = ref["current"].max()
+        np.testing.assert_allclose(out["current"], ref["current"], rtol=1e-9, atol=1e-12 * top)
+
+
+def test_correction_matches_oracle(backend, oracle):
+    backend.settings.precision = "mixed"
+    for args in [(6, 2, 5, False, 50, 50), (10, 0, 3, True, 30, 40), (4, 1, 1, False, 50, 50)]:
+        got = backend.circuit_omniscape_correction(*args)
+        ref = oracle.omniscape_correction(*args)
+        np.testing.assert_allclose(got, ref, rtol=1e-9)
