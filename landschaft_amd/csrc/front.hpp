@@ -5,8 +5,8 @@
 // of F11, F21, F22: src/cpp/circuit/system.cpp:899-908) runs tile column by
 // tile column:
 //   * the panel (the 16 columns of one tile column, rows k0..m-1) is held in
-//     the registers of one wavefront, a row per lane and slot, and factored
-//     with lane broadcasts (v_readlane) - no LDS traffic inside the panel;
+//     registers, a row per lane, and factored with lane broadcasts
+//     (v_readlane) - no LDS traffic and no barrier inside the panel;
 //   * the trailing tiles are updated with v_mfma_f32_16x16x4f32 (or the f64
 //     MFMA), C(I,J) -= P_I P_J^T, spread over the waves of the team.
 // Fronts carry one extra row, the right-hand side: eliminating it with the
@@ -66,51 +66,41 @@ struct Mfma<double> {
 
 // The panel of tile column Kt (columns k0 = 16 Kt .. cend-1 with cend =
 // min(k0 + 16, m)): columns k0..kend-1 factorised (pivots), the others of
-// the tile column updated. One wavefront; row k0 + lane + 64 r in slot r.
-// Returns false (wave-uniform) when a pivot is not positive.
-template <typename T, int R>
-__device__ bool panel(T* F, int m, int k0, int kend, int lane) {
+// the tile column updated, in registers. Every wavefront of the team holds
+// the 16 rows of the diagonal block on lanes 0..15 and factors that block
+// redundantly, so it can solve its own 48 rows below (lanes 16..63 of wave w:
+// rows k0 + 16 + 48 w + lane - 16) without any exchange between waves.
+// Requires m - k0 - 16 <= 48 nw. Returns false when a pivot is not positive.
+template <typename T>
+__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane) {
     const int cend = min(k0 + 16, m);
-    T P[R][16];
+    const int i = lane < 16 ? k0 + lane : k0 + 16 + 48 * wave + (lane - 16);
+    const bool mine = i < m && (lane >= 16 || wave == 0);
+    T P[16];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = k0 + lane + 64 * r;
-#pragma unroll
-        for (int c = 0; c < 16; ++c)
-            P[r][c] = (i < m && k0 + c < cend && i >= k0 + c) ? F[tix(i, k0 + c)] : T(0);
-    }
+    for (int c = 0; c < 16; ++c) P[c] = (i < m && k0 + c < cend && i >= k0 + c) ? F[tix(i, k0 + c)] : T(0);
     bool ok = true;
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
         if (k0 + kk < kend) {
-            const T dkk = bcast(P[0][kk], kk);
+            const T dkk = bcast(P[kk], kk);
             ok = ok && (dkk > T(0));
             const T d = sqrt(dkk), inv = T(1) / d;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = k0 + lane + 64 * r;
-                if (i > k0 + kk) P[r][kk] *= inv;
-                else if (i == k0 + kk) P[r][kk] = d;
-            }
+            if (i > k0 + kk) P[kk] *= inv;
+            else if (i == k0 + kk) P[kk] = d;
 #pragma unroll
             for (int c = kk + 1; c < 16; ++c) {
                 if (k0 + c < cend) {
-                    const T ljk = bcast(P[0][kk], c);
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const int i = k0 + lane + 64 * r;
-                        if (i >= k0 + c) P[r][c] -= P[r][kk] * ljk;
-                    }
+                    const T ljk = bcast(P[kk], c);
+                    if (i >= k0 + c) P[c] -= P[kk] * ljk;
                 }
             }
         }
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = k0 + lane + 64 * r;
+    if (mine) {
 #pragma unroll
         for (int c = 0; c < 16; ++c)
-            if (i < m && k0 + c < cend && i >= k0 + c) F[tix(i, k0 + c)] = P[r][c];
+            if (k0 + c < cend && i >= k0 + c) F[tix(i, k0 + c)] = P[c];
     }
     return ok;
 }
@@ -151,17 +141,14 @@ __device__ __forceinline__ void tri_tile(int t, int& I, int& J) {
 
 // Partial Cholesky of the first np columns of a tiled front of m rows by a
 // team of `nw` wavefronts (this one is `wave`); `sync` orders the team. The
-// panel runs on wave 0; the result flag is returned to every wave through
-// `flag` (team-visible memory).
-template <typename T, int R, typename Sync>
+// result flag is returned to every wave through `flag` (team-visible memory).
+template <typename T, typename Sync>
 __device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, int* flag, Sync sync) {
     const int MT = (m + 15) >> 4;
     for (int Kt = 0; Kt * 16 < np; ++Kt) {
         const int k0 = Kt * 16, kend = min(k0 + 16, np);
-        if (wave == 0) {
-            const bool ok = panel<T, R>(F, m, k0, kend, lane);
-            if (!ok && lane == 0) *flag = 1;
-        }
+        const bool ok = panel<T>(F, m, k0, kend, wave, lane);
+        if (!ok && wave == 0 && lane == 0) *flag = 1;
         sync();
         const int Tn = MT - Kt - 1;
         const int ntiles = (Tn * (Tn + 1)) >> 1;

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -42,12 +43,24 @@ __global__ void k_unit_conductance(Lattice Lt, long long k0, int B, double* g) {
 namespace {
 
 template <typename T>
-std::size_t factor_lds() {
+std::size_t big_lds() {
     return static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T);
 }
 template <typename T>
-std::size_t solve_lds() {
-    return static_cast<std::size_t>(kLbufEntries) * sizeof(T);
+std::size_t small_lds(int stile) {
+    return static_cast<std::size_t>(kWaves) * stile * 256 * sizeof(T);
+}
+template <typename T>
+std::size_t solve_lds(int lmax) {
+    return 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);  // + the 16-byte tail
+}
+
+// Cells of the nested-dissection leaves of the window plans (LSC_ND_LEAF,
+// default 32: half the fronts of 16-cell leaves for 25 % more flops).
+int nd_leaf() {
+    const char* e = std::getenv("LSC_ND_LEAF");
+    const int v = e ? std::atoi(e) : 32;
+    return v >= 4 ? v : 32;
 }
 
 void reset(int* p, int n, hipStream_t s) {
@@ -76,38 +89,23 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     const int H = static_cast<int>(h.lev_ptr.size()) - 1;
     if (h.max_front + 1 > kBigTiles * 16)
         throw std::invalid_argument("a front of " + std::to_string(h.max_front) +
-                                    " rows exceeds the window engine's LDS fronts (159 rows); use a smaller radius");
-    std::vector<int> lev_small(H, 0), chunk_lev(H + 1, 0), chunk_begin, chunk_end;
-    for (int lv = 0; lv < H; ++lv) {
-        int ns = 0;
-        for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
-            const int t = h.order[k];
-            if (h.npiv[t] + h.nupd[t] + 1 <= kSmallRows) ++ns;
-        }
-        lev_small[lv] = ns;
-        // solve chunks: consecutive fronts of the level whose L fits the LDS buffer
-        chunk_lev[lv] = static_cast<int>(chunk_begin.size());
-        long long acc = 0;
-        int start = h.lev_ptr[lv];
+                                    " rows exceeds the LDS fronts of the batched engine (159 rows)");
+    lev_small_host.assign(H, 0);
+    lev_stile.assign(H, 1);
+    lev_lmax.assign(H, 1);
+    lev_mmax.assign(H, 1);
+    for (int lv = 0; lv < H; ++lv)
         for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
             const int t = h.order[k];
             const long long np = h.npiv[t], m = np + h.nupd[t];
-            const long long sz = np * m - np * (np - 1) / 2;
-            if (sz > kLbufEntries) throw std::invalid_argument("a front's factor exceeds the solve kernel's LDS buffer");
-            if (acc + sz > kLbufEntries) {
-                chunk_begin.push_back(start);
-                chunk_end.push_back(k);
-                start = k;
-                acc = 0;
+            if (m + 1 <= kSmallRows) {
+                ++lev_small_host[lv];
+                const int MT = static_cast<int>((m + 1 + 15) / 16);
+                lev_stile[lv] = std::max(lev_stile[lv], MT * (MT + 1) / 2);
             }
-            acc += sz;
+            lev_lmax[lv] = std::max(lev_lmax[lv], static_cast<int>(np * m - np * (np - 1) / 2));
+            lev_mmax[lv] = std::max(lev_mmax[lv], static_cast<int>(m));
         }
-        if (h.lev_ptr[lv + 1] > start) {
-            chunk_begin.push_back(start);
-            chunk_end.push_back(h.lev_ptr[lv + 1]);
-        }
-    }
-    chunk_lev[H] = static_cast<int>(chunk_begin.size());
     std::vector<long long> uposA(h.nfront, 0);
     long long ua = 0;
     for (int k = 0; k < h.nfront; ++k) {
@@ -117,6 +115,8 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     }
     ibufs.clear();
     lbufs.clear();
+    ibufs.reserve(32);
+    lbufs.reserve(8);
     auto I = [&](const std::vector<int>& v) {
         ibufs.push_back(to_device(v, s));
         return static_cast<const int*>(ibufs.back().p);
@@ -125,8 +125,6 @@ void DevPlan::upload(Plan p, hipStream_t s) {
         lbufs.push_back(to_device(v, s));
         return static_cast<const long long*>(lbufs.back().p);
     };
-    ibufs.reserve(32);
-    lbufs.reserve(8);
     std::vector<long long> lpos(h.lpos.begin(), h.lpos.end()), vpos(h.vpos.begin(), h.vpos.end());
     d.n = h.n, d.nd = h.nd, d.W = static_cast<int>(h.bx), d.D = static_cast<int>(h.bx / 2), d.nfront = h.nfront;
     d.H = H;
@@ -134,8 +132,9 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     d.first = I(h.first), d.npiv = I(h.npiv), d.nupd = I(h.nupd), d.upd = I(h.upd), d.cb = I(h.cb), d.ce = I(h.ce);
     d.lpos = Lg(lpos), d.uposA = Lg(uposA), d.vpos = Lg(vpos);
     d.kids = I(h.kids), d.rows = I(h.rows), d.relmap = I(h.relmap);
-    d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small), d.order = I(h.order);
-    d.chunk_lev = I(chunk_lev), d.chunk_begin = I(chunk_begin), d.chunk_end = I(chunk_end);
+    d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small_host), d.order = I(h.order);
+    d.fe_ptr = I(h.fe_ptr), d.fe_row = I(h.fe_row), d.fe_col = I(h.fe_col), d.fe_p = I(h.fe_p), d.fe_d = I(h.fe_d);
+    d.nent = static_cast<int>(h.fe_row.size());
     d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
 }
 
@@ -150,16 +149,6 @@ void WindowBatchBuffers::ensure(int B_, int W2_, int n_, long long lsize_, int e
     L.alloc(static_cast<std::size_t>(B) * lsize * elem);
 }
 
-void SlotBuffers::ensure(int slots_, long long usizeA, long long vsize, int elem) {
-    const std::size_t ub = static_cast<std::size_t>(slots_) * usizeA * elem;
-    const std::size_t vb = static_cast<std::size_t>(slots_) * vsize;
-    if (slots_ <= slots && U.n >= ub && V.n >= vb) return;
-    slots = slots_;
-    U.alloc(ub);
-    V.alloc(vb ? vb : 1);
-    if (!counter.p) counter.alloc(4);
-}
-
 // One batch of windows whose g, src and active flags are filled: flood,
 // factor, solve, refine. Returns per-window refinement statistics in `st`.
 template <typename T>
@@ -167,31 +156,31 @@ struct WindowSolver {
     hipStream_t s;
     DevPlan plan;
     WindowBatchBuffers wb;
-    SlotBuffers sb;
+    DevBuf<char> U, A;  // update matrices, assembled entries (per instance)
+    DevBuf<double> V;
     DevBuf<int> open;
-    int slots_factor = 0, slots_solve = 0;
+    int Bcap = 0;
 
     explicit WindowSolver(hipStream_t st) : s(st) {}
 
     void set_plan(Plan p, int B) {
         plan.upload(std::move(p), s);
-        allow_lds(reinterpret_cast<const void*>(&k_factor<T>), factor_lds<T>());
-        allow_lds(reinterpret_cast<const void*>(&k_solve<T>), solve_lds<T>());
+        int smax = 1, lmax = 1;
+        for (int v : plan.lev_stile) smax = std::max(smax, v);
+        for (int v : plan.lev_lmax) lmax = std::max(lmax, v);
+        allow_lds(reinterpret_cast<const void*>(&k_factor_small<T>), small_lds<T>(smax));
+        allow_lds(reinterpret_cast<const void*>(&k_factor_big<T>), big_lds<T>());
+        allow_lds(reinterpret_cast<const void*>(&k_forward_level<T, 1>), solve_lds<T>(lmax));
+        allow_lds(reinterpret_cast<const void*>(&k_backward_level<T, 1>), solve_lds<T>(lmax));
+        allow_lds(reinterpret_cast<const void*>(&k_forward_level<T, 3>), solve_lds<T>(lmax));
+        allow_lds(reinterpret_cast<const void*>(&k_backward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
-        const int cus = device_cus();
-        int occ_f = 0, occ_s = 0;
-        LSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(&k_factor<T>),
-                                                             kThreads, factor_lds<T>()));
-        LSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, reinterpret_cast<const void*>(&k_solve<T>),
-                                                             kThreads, solve_lds<T>()));
-        slots_factor = std::max(1, occ_f) * cus;
-        (void)hipGetLastError();
-        slots_solve = std::max(1, occ_s) * cus;
-        slots_factor = std::min(slots_factor, B);
-        slots_solve = std::min(slots_solve, B);
         const int W = plan.d.W;
+        Bcap = B;
         wb.ensure(B, W * W, plan.d.n, plan.d.lsize, sizeof(T));
-        sb.ensure(std::max(slots_factor, slots_solve), plan.d.usizeA, plan.d.vsize, sizeof(T));
+        U.alloc(static_cast<std::size_t>(B) * plan.d.usizeA * sizeof(T));
+        V.alloc(static_cast<std::size_t>(B) * (plan.d.vsize ? plan.d.vsize : 1));
+        A.alloc(static_cast<std::size_t>(B) * (plan.d.nent ? plan.d.nent : 1) * sizeof(T));
         if (!open.p) open.alloc(4);
     }
 
@@ -201,6 +190,51 @@ struct WindowSolver {
         LSC_HIP(hipEventSynchronize(b));
         LSC_HIP(hipEventElapsedTime(&ms, a, b));
         acc += ms;
+    }
+
+    static unsigned blocks(long long tasks, int per) { return static_cast<unsigned>((tasks + per - 1) / per); }
+
+    void factor(int nb) {
+        const DPlan& P = plan.d;
+        T* L = reinterpret_cast<T*>(wb.L.p);
+        T* Up = reinterpret_cast<T*>(U.p);
+        T* Ap = reinterpret_cast<T*>(A.p);
+        if (P.nent > 0)
+            hipLaunchKernelGGL(k_assemble<T>, dim3(blocks(P.nent, kThreads), nb), dim3(kThreads), 0, s, P, nb, wb.g.p,
+                               wb.present.p, wb.sc.p, wb.active.p, Ap);
+        for (int h = 0; h < P.H; ++h) {
+            const int ns = plan.lev_small_host[h];
+            const int nbig = plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h] - ns;
+            if (ns > 0)
+                hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kWaves)),
+                                   dim3(kThreads), small_lds<T>(plan.lev_stile[h]), s, P, h, nb, plan.lev_stile[h],
+                                   Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
+            if (nbig > 0)
+                hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
+                                   dim3(kThreads), big_lds<T>(), s, P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p,
+                                   Up, wb.fail.p);
+        }
+    }
+
+    void triangular(int nb, int mode) {
+        const DPlan& P = plan.d;
+        const T* L = reinterpret_cast<const T*>(wb.L.p);
+        auto grid = [&](int h) {
+            return dim3(static_cast<unsigned>(static_cast<long long>(nb) * (plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h])));
+        };
+        if (mode == 1)
+            for (int h = 0; h < P.H; ++h) {
+                auto kern = plan.lev_mmax[h] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
+                hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, wb.active.p,
+                                   wb.done.p, L, wb.rs.p, wb.y.p, V.p);
+            }
+        for (int h = P.H - 1; h >= 0; --h) {
+            auto kern = plan.lev_mmax[h] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
+            hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, wb.active.p,
+                               wb.done.p, L, wb.y.p, wb.z.p);
+        }
+        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, wb.active.p, wb.done.p,
+                           wb.sc.p, wb.z.p, wb.x.p);
     }
 
     void solve(int nb, double tol, int maxit, IrStats& st, hipEvent_t* ev, bool timing) {
@@ -213,15 +247,9 @@ struct WindowSolver {
                            wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p,
                            10 * std::numeric_limits<double>::epsilon());
         if (timing) LSC_HIP(hipEventRecord(ev[1], s));
-        reset(sb.counter.p, 1, s);
-        hipLaunchKernelGGL(k_factor<T>, dim3(std::min(slots_factor, nb)), dim3(kThreads), factor_lds<T>(), s, P, nb,
-                           wb.g.p, wb.present.p, wb.sc.p, wb.rs.p, wb.active.p, reinterpret_cast<T*>(wb.L.p),
-                           wb.y.p, reinterpret_cast<T*>(sb.U.p), wb.fail.p, sb.counter.p);
+        factor(nb);
         if (timing) LSC_HIP(hipEventRecord(ev[2], s));
-        reset(sb.counter.p, 1, s);
-        hipLaunchKernelGGL(k_solve<T>, dim3(std::min(slots_solve, nb)), dim3(kThreads), solve_lds<T>(), s, P, nb, 0,
-                           wb.active.p, wb.done.p, reinterpret_cast<const T*>(wb.L.p), wb.rs.p, wb.y.p, wb.z.p,
-                           wb.sc.p, wb.x.p, sb.V.p, sb.counter.p);
+        triangular(nb, 0);
         if (timing) LSC_HIP(hipEventRecord(ev[3], s));
         st.batches += 1;
         st.solve_launches += 1;
@@ -241,21 +269,17 @@ struct WindowSolver {
             time_phase(st.ms_residual, ev[0], ev[1], timing);
             if (h_open == 0 || it >= maxit) break;
             st.solve_launches += 1;
-            reset(sb.counter.p, 1, s);
             if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-            hipLaunchKernelGGL(k_solve<T>, dim3(std::min(slots_solve, nb)), dim3(kThreads), solve_lds<T>(), s, P,
-                               nb, 1, wb.active.p, wb.done.p, reinterpret_cast<const T*>(wb.L.p), wb.rs.p, wb.y.p,
-                               wb.z.p, wb.sc.p, wb.x.p, sb.V.p, sb.counter.p);
+            triangular(nb, 1);
             if (timing) LSC_HIP(hipEventRecord(ev[1], s));
             time_phase(st.ms_solve, ev[0], ev[1], timing);
         }
         // statistics of the batch
-        std::vector<int> act(nb), fl(nb), its(nb), dn(nb);
+        std::vector<int> act(nb), fl(nb), its(nb);
         std::vector<double> rn(nb);
         LSC_HIP(hipMemcpyAsync(act.data(), wb.active.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipMemcpyAsync(fl.data(), wb.fail.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipMemcpyAsync(its.data(), wb.iters.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        LSC_HIP(hipMemcpyAsync(dn.data(), wb.done.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipMemcpyAsync(rn.data(), wb.rnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < nb; ++k) {
@@ -380,9 +404,19 @@ template <typename T>
 void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, const OmniArgs& a, int kind,
                   const double* d_corr, double* d_map, unsigned long long* d_ntargets, hipStream_t s, IrStats& st) {
     if (Lt.nlat == 0) return;
-    const long long B = std::min<long long>(a.batch > 0 ? a.batch : 4096, Lt.nlat);
+    Plan plan = window_plan(Lt.D, kind == 0, a.four != 0, nd_leaf());
+    // per-window device memory: maps, vectors, L, update matrices and vectors
+    long long ua = 0;
+    for (int t = 0; t < plan.nfront; ++t) ua += static_cast<long long>(plan.nupd[t] + 1) * (plan.nupd[t] + 1);
+    const double per_window = static_cast<double>(Lt.W) * Lt.W * 33.0 + plan.n * 7.0 * 8.0 +
+                              static_cast<double>(plan.lsize + ua + plan.fe_row.size()) * sizeof(T) +
+                              plan.vsize * 8.0;
+    std::size_t free_b = 0, total_b = 0;
+    LSC_HIP(hipMemGetInfo(&free_b, &total_b));
+    const long long fit = std::max<long long>(1, static_cast<long long>(0.5 * static_cast<double>(free_b) / per_window));
+    const long long B = std::min({a.batch > 0 ? a.batch : 4096, Lt.nlat, fit});
     WindowSolver<T> ws(s);
-    ws.set_plan(window_plan(Lt.D, kind == 0, a.four != 0), static_cast<int>(B));
+    ws.set_plan(std::move(plan), static_cast<int>(B));
     hipEvent_t ev[4] = {};
     const bool timing = a.timing != 0;
     if (timing)

@@ -11,6 +11,7 @@ namespace {
 struct Dissection {
     const std::uint8_t* mark;
     i64 stride;
+    i64 leaf;
     std::vector<i32> cells;
     std::vector<i32> first, npiv;
     std::vector<std::vector<i32>> kids;
@@ -37,7 +38,7 @@ struct Dissection {
         if (bottom < top) return -1;
         r0 = top, r1 = bottom + 1, c0 = left, c1 = right + 1;
         const i64 h = r1 - r0, w = c1 - c0;
-        if (h * w <= 16) {
+        if (h * w <= leaf) {
             const i32 f = static_cast<i32>(cells.size());
             for (i64 r = r0; r < r1; ++r)
                 for (i64 c = c0; c < c1; ++c)
@@ -66,11 +67,11 @@ struct Dissection {
 
 }  // namespace
 
-Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four) {
+Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
     if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
     Plan p;
     p.bx = bx, p.by = by, p.four = four, p.nd = four ? 4 : 8;
-    Dissection d{mark, bx, {}, {}, {}, {}};
+    Dissection d{mark, bx, leaf, {}, {}, {}, {}};
     d.cut(0, by, 0, bx);
     p.cell = std::move(d.cells);
     p.n = static_cast<i32>(p.cell.size());
@@ -164,16 +165,31 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four) {
         const i64 np = p.npiv[t], m = np + p.nupd[t], nu = p.nupd[t];
         p.lpos[t] = l;
         l += np * m - np * (np - 1) / 2;
+        l = (l + 3) & ~i64(3);  // every block starts on 4 entries (16-byte loads)
         p.upos[t] = u;
         u += nu * nu;
         p.vpos[t] = v;
         v += nu;
     }
     p.lsize = l, p.usize = u, p.vsize = v;
+    // The off-diagonal entries of every front, front by front.
+    p.fe_ptr.assign(nf + 1, 0);
+    for (i32 t = 0; t < nf; ++t) {
+        for (i32 q = p.first[t]; q < p.first[t] + p.npiv[t]; ++q)
+            for (int e = 0; e < nd; ++e) {
+                const i32 a = p.amap[i64(q) * nd + e];
+                if (a < 0) continue;
+                p.fe_row.push_back(a);
+                p.fe_col.push_back(q - p.first[t]);
+                p.fe_p.push_back(q);
+                p.fe_d.push_back(e);
+            }
+        p.fe_ptr[t + 1] = static_cast<i32>(p.fe_row.size());
+    }
     return p;
 }
 
-Plan window_plan(int D, bool disk, bool four) {
+Plan window_plan(int D, bool disk, bool four, int leaf) {
     const i64 W = 2 * i64(D) + 1;
     std::vector<std::uint8_t> mark(W * W, 0);
     for (i64 y = 0; y < W; ++y)
@@ -182,7 +198,7 @@ Plan window_plan(int D, bool disk, bool four) {
             mark[y * W + x] = !disk || dy * dy + dx * dx <= i64(D) * D;
         }
     mark[D * W + D] = 0;
-    return make_plan(W, W, mark.data(), four);
+    return make_plan(W, W, mark.data(), four, leaf);
 }
 
 }  // namespace lsc

@@ -50,6 +50,11 @@ struct Plan {
     std::vector<i32> first, npiv, nupd, upd, cb, ce, height;
     std::vector<i64> lpos, upos, vpos;
     std::vector<i32> kids, rows, relmap;
+    // Off-diagonal entries assembled into each front: entries fe_ptr[t] ..
+    // fe_ptr[t+1] go to front-local (fe_row, fe_col); fe_p / fe_d name the
+    // unknown and neighbour direction of the entry (grids), fe_k the caller's
+    // entry index (graphs).
+    std::vector<i32> fe_ptr, fe_row, fe_col, fe_p, fe_d;
     // levels: fronts of height h are order[lev_ptr[h] .. lev_ptr[h+1])
     std::vector<i32> lev_ptr, order;
     i64 lsize = 0, usize = 0, vsize = 0;
@@ -57,11 +62,12 @@ struct Plan {
     double flops = 0;     // sum np^3/3 + np^2 nu + np nu^2
 };
 
-// The plan of the marked cells of a bx x by box.
-Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four);
+// The plan of the marked cells of a bx x by box; boxes of at most `leaf`
+// cells are not cut further (the reference cuts down to 16).
+Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf = 16);
 
 // An Omniscape window: W = 2D + 1 square, the cells within D of the centre
 // (every cell when disk is false), without the centre (the ground).
-Plan window_plan(int D, bool disk, bool four);
+Plan window_plan(int D, bool disk, bool four, int leaf = 16);
 
 }  // namespace lsc

@@ -17,7 +17,6 @@ constexpr int kThreads = 256;      // 4 wavefronts per workgroup
 constexpr int kWaves = 4;
 constexpr int kSmallRows = 64;     // fronts of <= 64 rows (RHS row included): one wavefront each
 constexpr int kBigTiles = 10;      // fronts of <= 160 rows: the whole workgroup, tiles in LDS
-constexpr int kLbufEntries = 12288;  // solve kernels: L staged through LDS in chunks of this many entries
 constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
 
 // Device copy of a Plan plus the per-level chunking of the solve kernels.
@@ -28,13 +27,16 @@ struct DPlan {
     const long long *lpos, *uposA, *vpos;
     const int *kids, *rows, *relmap;
     const int *lev_ptr, *lev_small, *order;
-    const int *chunk_lev, *chunk_begin, *chunk_end;  // solve chunks: level h's chunks chunk_lev[h]..chunk_lev[h+1]
+    const int *fe_ptr, *fe_row, *fe_col, *fe_p, *fe_d;  // entries assembled into each front
+    int nent;
     long long lsize, usizeA, vsize;
 };
 
 struct DevPlan {
     Plan host;
     DPlan d{};
+    std::vector<int> lev_small_host, lev_stile, lev_lmax, lev_mmax;  // per level: small fronts, their tiles,
+                                                                     // largest L block, largest front
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
     void upload(Plan p, hipStream_t s);
@@ -50,15 +52,6 @@ struct WindowBatchBuffers {
     DevBuf<int> active, fail, done, iters;
     DevBuf<char> L;
     void ensure(int B_, int W2_, int n_, long long lsize_, int elem_);
-};
-
-// Scratch of the persistent kernels (one slot per resident workgroup).
-struct SlotBuffers {
-    int slots = 0;
-    DevBuf<char> U;
-    DevBuf<double> V;
-    DevBuf<int> counter;
-    void ensure(int slots_, long long usizeA, long long vsize, int elem);
 };
 
 // Geometry of an Omniscape run on the target lattice (reference

@@ -279,120 +279,31 @@ __global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const double
     }
 }
 
-namespace {
-
-// One front: assembled (S A S entries, RHS row, children's update matrices),
-// factorised, written out (L packed by columns, y, the update matrix).
-template <typename T, int R, typename Sync>
-__device__ void process_front(const DPlan& P, int t, T* F, int tid, int nt, int wave, int nw, int lane, int* flag,
-                              Sync sync, const double* gw, const unsigned char* pw, const double* sw,
-                              const double* rsw, T* Lw, double* yw, T* U) {
-    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, ma = m + 1, first = P.first[t];
-    const int MT = (ma + 15) >> 4;
-    const int nz = ((MT * (MT + 1)) >> 1) * 256;
-    for (int e = tid; e < nz; e += nt) F[e] = T(0);
-    sync();
-    const int W = P.W, nd = P.nd;
-    for (int j = tid; j < np; j += nt) {
-        const int p = first + j, c = P.cell[p];
-        F[tix(j, j)] = T(1);
-        F[tix(m, j)] = static_cast<T>(rsw[p]);
-        if (pw[c]) {
-            const double sp = sw[p], gc = gw[c];
-            const int y = c / W, xx = c - (c / W) * W;
-            for (int d = 0; d < nd; ++d) {
-                const int a = P.amap[p * nd + d];
-                if (a < 0) continue;
-                const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
-                if (!pw[c2]) continue;
-                const int q = P.nbr[p * nd + d];
-                const double w = edge_weight(gc, gw[c2], ndiag(nd, d));
-                F[tix(a, j)] = static_cast<T>(-w * sp * sw[q]);
-            }
-        }
-    }
-    sync();
-    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
-        const int c = P.kids[ci];
-        const int u = P.nupd[c], u1 = u + 1;
-        const int* rel = P.relmap + P.upd[c];
-        const T* Uc = U + P.uposA[c];
-        for (int e = tid; e < u1 * u; e += nt) {
-            const int jj = e / u1, ii = e - jj * u1;
-            if (ii < jj) continue;
-            const int row = ii < u ? rel[ii] : m;
-            F[tix(row, rel[jj])] += Uc[jj * u1 + ii];
-        }
-        sync();
-    }
-    factor_front<T, R>(F, ma, np, wave, nw, lane, flag, sync);
-    T* Lt = Lw + P.lpos[t];
-    for (int e = tid; e < np * m; e += nt) {
-        const int j = e / m, i = e - j * m;
-        if (i < j) continue;
-        Lt[static_cast<long long>(j) * m - (static_cast<long long>(j) * (j - 1) >> 1) + (i - j)] = F[tix(i, j)];
-    }
-    for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
-    const int u1 = nu + 1;
-    T* Ut = U + P.uposA[t];
-    for (int e = tid; e < u1 * u1; e += nt) {
-        const int jj = e / u1, ii = e - jj * u1;
-        if (ii < jj) continue;
-        Ut[jj * u1 + ii] = F[tix(np + ii, np + jj)];
-    }
-    sync();
-}
-
-}  // namespace
-
+// The off-diagonal entries of S A S (S = diag^-1/2) of every window, in the
+// plan's front order: the "Laplacian assembly" of the reference's
+// CellSystem::assemble (system.cpp:458-487) with the Jacobi scaling folded in.
+// Entries touching a cell outside the circuit are 0 (identity rows).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_factor(DPlan P, int B, const double* __restrict__ g,
-                                                     const unsigned char* __restrict__ present,
-                                                     const double* __restrict__ sc, const double* __restrict__ rs,
-                                                     const int* __restrict__ active, T* __restrict__ L,
-                                                     double* __restrict__ y, T* __restrict__ Uall,
-                                                     int* __restrict__ fail, int* __restrict__ counter) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* big = reinterpret_cast<T*>(smem_raw);
-    __shared__ int s_win;
-    __shared__ int s_flag[kWaves + 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    T* smallF = big + wave * 10 * 256;
-    T* U = Uall + static_cast<long long>(blockIdx.x) * P.usizeA;
-    const int W2 = P.W * P.W;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_win = atomicAdd(counter, 1);
-        if (threadIdx.x <= kWaves) s_flag[threadIdx.x] = 0;
-        __syncthreads();
-        const int k = s_win;
-        if (k >= B) break;
-        if (!active[k]) continue;
+__global__ __launch_bounds__(kThreads) void k_assemble(DPlan P, int B, const double* __restrict__ g,
+                                                       const unsigned char* __restrict__ present,
+                                                       const double* __restrict__ sc, const int* __restrict__ active,
+                                                       T* __restrict__ aval) {
+    const int k = blockIdx.y;
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= B || e >= P.nent || !active[k]) return;
+    const int W = P.W, W2 = W * W, nd = P.nd;
+    const int p = P.fe_p[e], d = P.fe_d[e];
+    const int c = P.cell[p];
+    const int y = c / W, xx = c - (c / W) * W;
+    const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
+    const unsigned char* pw = present + static_cast<long long>(k) * W2;
+    double v = 0.0;
+    if (pw[c] && pw[c2]) {
         const double* gw = g + static_cast<long long>(k) * W2;
-        const unsigned char* pw = present + static_cast<long long>(k) * W2;
         const double* sw = sc + static_cast<long long>(k) * P.n;
-        const double* rsw = rs + static_cast<long long>(k) * P.n;
-        double* yw = y + static_cast<long long>(k) * P.n;
-        T* Lw = L + static_cast<long long>(k) * P.lsize;
-        auto wsync = [] { wave_sync(); };
-        auto bsync = [] { __syncthreads(); };
-        for (int h = 0; h < P.H; ++h) {
-            const int base = P.lev_ptr[h], ns = P.lev_small[h], ne = P.lev_ptr[h + 1];
-            for (int idx = base + wave; idx < base + ns; idx += kWaves)
-                process_front<T, 1>(P, P.order[idx], smallF, lane, 64, 0, 1, lane, &s_flag[wave], wsync, gw, pw,
-                                    sw, rsw, Lw, yw, U);
-            __syncthreads();
-            for (int idx = base + ns; idx < ne; ++idx)
-                process_front<T, 3>(P, P.order[idx], big, threadIdx.x, kThreads, wave, kWaves, lane,
-                                    &s_flag[kWaves], bsync, gw, pw, sw, rsw, Lw, yw, U);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            int f = 0;
-            for (int w = 0; w <= kWaves; ++w) f |= s_flag[w];
-            fail[k] = f;
-        }
+        v = -edge_weight(gw[c], gw[c2], ndiag(nd, d)) * sw[p] * sw[P.nbr[p * nd + d]];
     }
+    aval[static_cast<long long>(k) * P.nent + e] = static_cast<T>(v);
 }
 
 namespace {
@@ -401,14 +312,135 @@ __device__ __forceinline__ long long colstart(int j, int m) {
     return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
 }
 
+// One front: assembled (unit diagonal, the scaled entries, RHS row, the
+// children's update matrices), factorised, written out (L packed by columns,
+// y, the update matrix). Team of nw wavefronts (tid / nt over the team).
+template <typename T, typename Sync>
+__device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid, int nt, int wave, int nw, int lane,
+                              int* flag, Sync sync, const T* aval, const double* rsw, T* Lw, double* yw, T* U) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, ma = m + 1, first = P.first[t];
+    const int MT = (ma + 15) >> 4;
+    const int nz = ((MT * (MT + 1)) >> 1) * 256;
+    for (int e = tid; e < nz; e += nt) F[e] = T(0);
+    sync();
+    for (int j = tid; j < np; j += nt) {
+        F[tix(j, j)] = T(1);
+        F[tix(m, j)] = static_cast<T>(rsw[first + j]);
+    }
+    for (int e = P.fe_ptr[t] + tid; e < P.fe_ptr[t + 1]; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
+    sync();
+    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c], u1 = u + 1;
+        const int* rel = P.relmap + P.upd[c];
+        const T* Uc = U + P.uposA[c];
+        // the child's row map in LDS, its update matrix streamed with four
+        // independent loads in flight per lane
+        for (int i = tid; i < u; i += nt) relbuf[i] = rel[i];
+        if (tid == 0) relbuf[u] = m;
+        sync();
+        const int tot = u1 * u;
+        for (int base = tid; base < tot; base += 4 * nt) {
+            T val[4];
+            int row[4], col[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = base + q * nt;
+                const int jj = e / u1, ii = e - jj * u1;
+                const bool ok = e < tot && ii >= jj;
+                val[q] = ok ? Uc[e] : T(0);
+                row[q] = ok ? ii : -1;
+                col[q] = jj;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (row[q] >= 0) F[tix(relbuf[row[q]], relbuf[col[q]])] += val[q];
+        }
+        sync();
+    }
+    factor_front<T>(F, ma, np, wave, nw, lane, flag, sync);
+    T* Lt = Lw + P.lpos[t];
+    for (int j = wave; j < np; j += nw) {
+        T* col = Lt + colstart(j, m) - j;
+        for (int i = j + lane; i < m; i += 64) col[i] = F[tix(i, j)];
+    }
+    for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
+    const int u1 = nu + 1;
+    T* Ut = U + P.uposA[t];
+    for (int jj = wave; jj <= nu; jj += nw)
+        for (int ii = jj + lane; ii <= nu; ii += 64) Ut[jj * u1 + ii] = F[tix(np + ii, np + jj)];
+    sync();
+}
+
+}  // namespace
+
+// Level-synchronous batched factorisation: one launch per tree level and
+// front class, every (instance, front) pair of the level an independent task,
+// so thousands of fronts are in flight and their memory latencies overlap.
+// Small fronts (<= 64 rows with the RHS row): one wavefront each, its own LDS
+// region of `stile` tiles. Large fronts: one workgroup each, up to 55 tiles.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_factor_small(DPlan P, int h, int B, int stile,
+                                                           const T* __restrict__ aval, const double* __restrict__ rs,
+                                                           const int* __restrict__ active, T* __restrict__ L,
+                                                           double* __restrict__ y, T* __restrict__ Uall,
+                                                           int* __restrict__ fail) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_flag[kWaves];
+    __shared__ int s_rel[kWaves][kSmallRows + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ns = P.lev_small[h];
+    const long long task = static_cast<long long>(blockIdx.x) * kWaves + wave;
+    const int k = static_cast<int>(task / ns);
+    if (k >= B || !active[k]) return;
+    const int t = P.order[P.lev_ptr[h] + static_cast<int>(task - static_cast<long long>(k) * ns)];
+    T* F = reinterpret_cast<T*>(smem_raw) + wave * stile * 256;
+    if (lane == 0) s_flag[wave] = 0;
+    wave_sync();
+    auto wsync = [] { wave_sync(); };
+    process_front<T>(P, t, F, s_rel[wave], lane, 64, 0, 1, lane, &s_flag[wave], wsync,
+                     aval + static_cast<long long>(k) * P.nent,
+                     rs + static_cast<long long>(k) * P.n, L + static_cast<long long>(k) * P.lsize,
+                     y + static_cast<long long>(k) * P.n, Uall + static_cast<long long>(k) * P.usizeA);
+    if (lane == 0 && s_flag[wave]) atomicOr(&fail[k], 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, const T* __restrict__ aval,
+                                                         const double* __restrict__ rs, const int* __restrict__ active,
+                                                         T* __restrict__ L, double* __restrict__ y,
+                                                         T* __restrict__ Uall, int* __restrict__ fail) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_flag;
+    __shared__ int s_rel[kBigTiles * 16 + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = P.lev_ptr[h] + P.lev_small[h];
+    const int nb = P.lev_ptr[h + 1] - base;
+    const int k = blockIdx.x / nb;
+    if (k >= B || !active[k]) return;
+    const int t = P.order[base + (blockIdx.x - k * nb)];
+    if (threadIdx.x == 0) s_flag = 0;
+    __syncthreads();
+    auto bsync = [] { __syncthreads(); };
+    process_front<T>(P, t, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kThreads, wave, kWaves, lane, &s_flag,
+                     bsync,
+                     aval + static_cast<long long>(k) * P.nent, rs + static_cast<long long>(k) * P.n,
+                     L + static_cast<long long>(k) * P.lsize, y + static_cast<long long>(k) * P.n,
+                     Uall + static_cast<long long>(k) * P.usizeA);
+    if (threadIdx.x == 0 && s_flag) atomicOr(&fail[k], 1);
+}
+
+namespace {
+
 __device__ __forceinline__ double pick(const double (&v)[3], int slot) {
     return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
 }
 
-// Forward substitution of front t (one wavefront): its vector assembled from
-// the pivots' RHS and the children's update vectors, then y = L11^-1 f and
-// the update vector f_upd - L21 y (reference system.cpp:974-989).
-template <typename T>
+// Forward substitution of front t (one wavefront, L block Lc in LDS): its
+// vector assembled from the pivots' RHS and the children's update vectors,
+// then y = L11^-1 f and the update vector f_upd - L21 y (reference
+// system.cpp:974-989).
+template <typename T, int R>
 __device__ void forward_front(const DPlan& P, int t, const T* Lc, double* fv, int lane, const double* rsw,
                               double* yw, double* V) {
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
@@ -422,48 +454,65 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lc, double* fv, in
         for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
         wave_sync();
     }
-    double f[3];
+    double f[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         f[r] = i < m ? fv[i] : 0.0;
     }
-    for (int j = 0; j < np; ++j) {
-        const long long cs = colstart(j, m);
-        const int slot = j >> 6;
-        const double yj = bcast(pick(f, slot), j & 63) / static_cast<double>(Lc[cs]);
+    // 16 columns at a time: the lane's L entries and the reciprocal pivots
+    // in registers first, so the dependent chain is register-only.
+    for (int j0 = 0; j0 < np; j0 += 16) {
+        T Lr[R][16];
+        double dinv[16];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const int i = lane + 64 * r;
-            if (i > j && i < m) f[r] -= static_cast<double>(Lc[cs + i - j]) * yj;
+        for (int q = 0; q < 16; ++q) {
+            const int j = j0 + q;
+            const bool okj = j < np;
+            const long long cs = okj ? colstart(j, m) : 0;
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[cs]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                Lr[r][q] = (okj && i > j && i < m) ? Lc[cs + i - j] : T(0);
+            }
         }
-        if (lane == (j & 63)) {
-            if (slot == 0) f[0] = yj;
-            else if (slot == 1) f[1] = yj;
-            else f[2] = yj;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j0 + q;
+            if (j < np) {
+                const int slot = j >> 6;
+                const double yj = bcast(pick(f, slot), j & 63) * dinv[q];
+#pragma unroll
+                for (int r = 0; r < R; ++r) f[r] -= static_cast<double>(Lr[r][q]) * yj;
+                if (lane == (j & 63)) {
+                    if (slot == 0) f[0] = yj;
+                    else if (slot == 1) f[1] = yj;
+                    else f[2] = yj;
+                }
+            }
         }
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         if (i < np) yw[first + i] = f[r];
         else if (i < m) V[P.vpos[t] + i - np] = f[r];
     }
-    wave_sync();
 }
 
-// Back substitution of front t (one wavefront): z_piv = L11^-T (y - L21^T
-// z_upd) (reference system.cpp:992-1016).
-template <typename T>
+// Back substitution of front t (one wavefront, L block Lc in LDS): z_piv =
+// L11^-T (y - L21^T z_upd) (reference system.cpp:992-1016).
+template <typename T, int R>
 __device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, int lane, const double* yw,
                                double* zw) {
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     const int* rows = P.rows + P.upd[t];
     for (int ii = lane; ii < nu; ii += 64) xu[ii] = zw[rows[ii]];
     wave_sync();
-    double tv[3];
+    double tv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int j = lane + 64 * r;
         double acc = 0.0;
         if (j < np) {
@@ -473,94 +522,120 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, i
         }
         tv[r] = acc;
     }
-    for (int j = np - 1; j >= 0; --j) {
-        const int slot = j >> 6;
-        const double xj = bcast(pick(tv, slot), j & 63) / static_cast<double>(Lc[colstart(j, m)]);
+    // 16 pivots at a time from the last: L[j][i] of the lane's columns i and
+    // the reciprocal pivots in registers, then a register-only chain.
+    for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
+        T Lr[R][16];
+        double dinv[16];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const int i = lane + 64 * r;
-            if (i < j) tv[r] -= static_cast<double>(Lc[colstart(i, m) + (j - i)]) * xj;
+        for (int q = 0; q < 16; ++q) {
+            const int j = j1 - q;
+            const bool okj = j >= 0;
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[colstart(j, m)]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                Lr[r][q] = (okj && i < j) ? Lc[colstart(i, m) + (j - i)] : T(0);
+            }
         }
-        if (lane == (j & 63)) {
-            if (slot == 0) tv[0] = xj;
-            else if (slot == 1) tv[1] = xj;
-            else tv[2] = xj;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j1 - q;
+            if (j >= 0) {
+                const int slot = j >> 6;
+                const double xj = bcast(pick(tv, slot), j & 63) * dinv[q];
+#pragma unroll
+                for (int r = 0; r < R; ++r) tv[r] -= static_cast<double>(Lr[r][q]) * xj;
+                if (lane == (j & 63)) {
+                    if (slot == 0) tv[0] = xj;
+                    else if (slot == 1) tv[1] = xj;
+                    else tv[2] = xj;
+                }
+            }
         }
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         if (i < np) zw[first + i] = tv[r];
+    }
+}
+
+// The L block of front t copied from HBM to LDS in one coalesced burst.
+template <typename T>
+__device__ __forceinline__ void stage_block(const DPlan& P, int t, const T* Lw, T* Ls, int lane) {
+    const long long np = P.npiv[t], m = np + P.nupd[t];
+    const int sz = static_cast<int>(np * m - ((np * (np - 1)) >> 1));
+    // 16-byte loads, eight in flight per lane (blocks start on 16 bytes)
+    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
+    uint4* dst = reinterpret_cast<uint4*>(Ls);
+    const int nv = static_cast<int>((static_cast<long long>(sz) * sizeof(T) + 15) / 16);
+    for (int base = lane; base < nv; base += 8 * 64) {
+        uint4 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = base + q * 64;
+            if (e < nv) v[q] = src[e];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = base + q * 64;
+            if (e < nv) dst[e] = v[q];
+        }
     }
     wave_sync();
 }
 
-template <typename T>
-__device__ void stage_chunk(const DPlan& P, int c, const T* Lw, T* Lbuf, long long& base) {
-    const int t0 = P.order[P.chunk_begin[c]], t1 = P.order[P.chunk_end[c] - 1];
-    const int np1 = P.npiv[t1], m1 = np1 + P.nupd[t1];
-    base = P.lpos[t0];
-    const long long end = P.lpos[t1] + static_cast<long long>(np1) * m1 - ((static_cast<long long>(np1) * (np1 - 1)) >> 1);
-    const int cnt = static_cast<int>(end - base);
-    for (int e = threadIdx.x; e < cnt; e += kThreads) Lbuf[e] = Lw[base + e];
-    __syncthreads();
-}
-
 }  // namespace
 
-// Triangular solves of the windows not yet done. mode 0: back substitution
-// of the factor's fused forward result y, x = s z. mode 1: a refinement
-// step, forward and back substitution of rs, x += s z.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_solve(DPlan P, int B, int mode, const int* __restrict__ active,
-                                                    const int* __restrict__ done, const T* __restrict__ L,
-                                                    const double* __restrict__ rs, double* __restrict__ y,
-                                                    double* __restrict__ z, const double* __restrict__ sc,
-                                                    double* __restrict__ x, double* __restrict__ Vall,
-                                                    int* __restrict__ counter) {
+// Batched triangular solves, one launch per level: every (instance, front)
+// pair of the level is the task of a one-wavefront workgroup, which stages
+// the front's L block (contiguous in HBM) into LDS, then substitutes.
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, const int* __restrict__ active,
+                                                      const int* __restrict__ done, const T* __restrict__ L,
+                                                      const double* __restrict__ rs, double* __restrict__ y,
+                                                      double* __restrict__ Vall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* Lbuf = reinterpret_cast<T*>(smem_raw);
-    __shared__ double vbuf[kWaves][192];
-    __shared__ int s_win;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* V = Vall + static_cast<long long>(blockIdx.x) * P.vsize;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_win = atomicAdd(counter, 1);
-        __syncthreads();
-        const int k = s_win;
-        if (k >= B) break;
-        if (!active[k] || done[k]) continue;
-        const long long off = static_cast<long long>(k) * P.n;
-        const T* Lw = L + static_cast<long long>(k) * P.lsize;
-        double* yw = y + off;
-        double* zw = z + off;
-        if (mode == 1) {
-            for (int h = 0; h < P.H; ++h)
-                for (int c = P.chunk_lev[h]; c < P.chunk_lev[h + 1]; ++c) {
-                    long long base;
-                    stage_chunk<T>(P, c, Lw, Lbuf, base);
-                    for (int idx = P.chunk_begin[c] + wave; idx < P.chunk_end[c]; idx += kWaves) {
-                        const int t = P.order[idx];
-                        forward_front<T>(P, t, Lbuf + (P.lpos[t] - base), vbuf[wave], lane, rs + off, yw, V);
-                    }
-                    __syncthreads();
-                }
-        }
-        for (int h = P.H - 1; h >= 0; --h)
-            for (int c = P.chunk_lev[h]; c < P.chunk_lev[h + 1]; ++c) {
-                long long base;
-                stage_chunk<T>(P, c, Lw, Lbuf, base);
-                for (int idx = P.chunk_begin[c] + wave; idx < P.chunk_end[c]; idx += kWaves) {
-                    const int t = P.order[idx];
-                    backward_front<T>(P, t, Lbuf + (P.lpos[t] - base), vbuf[wave], lane, yw, zw);
-                }
-                __syncthreads();
-            }
-        for (int p = threadIdx.x; p < P.n; p += kThreads) {
-            const double d = sc[off + p] * zw[p];
-            x[off + p] = mode == 0 ? d : x[off + p] + d;
-        }
+    double* fv = reinterpret_cast<double*>(smem_raw);
+    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    const int lane = threadIdx.x;
+    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int k = blockIdx.x / nf;
+    if (k >= B || !active[k] || done[k]) return;
+    const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    const long long off = static_cast<long long>(k) * P.n;
+    stage_block<T>(P, t, L + static_cast<long long>(k) * P.lsize, Ls, lane);
+    forward_front<T, R>(P, t, Ls, fv, lane, rs + off, y + off, Vall + static_cast<long long>(k) * P.vsize);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, const int* __restrict__ active,
+                                                       const int* __restrict__ done, const T* __restrict__ L,
+                                                       const double* __restrict__ y, double* __restrict__ z) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* xu = reinterpret_cast<double*>(smem_raw);
+    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    const int lane = threadIdx.x;
+    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int k = blockIdx.x / nf;
+    if (k >= B || !active[k] || done[k]) return;
+    const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    const long long off = static_cast<long long>(k) * P.n;
+    stage_block<T>(P, t, L + static_cast<long long>(k) * P.lsize, Ls, lane);
+    backward_front<T, R>(P, t, Ls, xu, lane, y + off, z + off);
+}
+
+// x = s z (mode 0) or x += s z (mode 1) for the instances not done.
+__global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, const int* __restrict__ active,
+                                                       const int* __restrict__ done, const double* __restrict__ sc,
+                                                       const double* __restrict__ z, double* __restrict__ x) {
+    const int k = blockIdx.x;
+    if (k >= B || !active[k] || done[k]) return;
+    const long long off = static_cast<long long>(k) * n;
+    for (int p = threadIdx.x; p < n; p += kThreads) {
+        const double d = sc[off + p] * z[off + p];
+        x[off + p] = mode == 0 ? d : x[off + p] + d;
     }
 }
 
