@@ -87,11 +87,14 @@ def cpu_baseline(r, s, radius, block, seconds: float):
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    probe = 64
+    # marginal rate from two probes (the fixed cost: target list, correction solves)
     t0 = time.perf_counter()
-    O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-probe)
-    rate = probe / max(time.perf_counter() - t0, 1e-3)
-    sample = int(min(200000, max(256, rate * seconds)))
+    O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-64)
+    t1 = time.perf_counter()
+    O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-64 - 32 * threads)
+    t2 = time.perf_counter()
+    rate = 32 * threads / max((t2 - t1) - (t1 - t0), 1e-3)
+    sample = int(min(400000, max(256, rate * seconds)))
     t0 = time.perf_counter()
     out = O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-sample)
     dt = time.perf_counter() - t0
@@ -145,20 +148,24 @@ def main() -> int:
     value = st["n_targets"] / dt
     cum_mixed = tc.cpu().numpy()
 
+    # Algorithmic per-window figures of the reference's ordering (16-cell
+    # nested-dissection leaves): front flops sum np^3/3 + np^2 nu + np nu^2,
+    # and L as m x np column blocks in FP32 read once per substitution sweep.
     plan = N.window_plan_stats(a.radius, 0, False, True)
     kernels = {}
-    # factor: MFMA-bound; algorithmic flops = the plan's front flops per factorised window
     f_flops = plan["factor_flops"] * st["n_factored"]
     f_s = st["ms_factor"] / 1e3
     kernels["k_factor"] = {"bound": "mfma", "achieved": f_flops / f_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
-                           "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"]}
-    # solve: HBM-bound; algorithmic bytes = L read once per pass (fp32) + 4 n-vectors of fp64
-    passes = st["n_factored"] * (1 + st["refine_mean"])
-    bytes_solve = passes * (plan["l_entries"] * 4 + 4 * plan["unknowns"] * 8) + \
-        st["n_factored"] * st["refine_mean"] * plan["l_entries"] * 4
+                           "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"],
+                           "flops_per_circuit": plan["factor_flops"]}
+    # sweeps: the back substitution after the fused factor-forward, then a
+    # forward + back sweep per refinement step
+    sweeps = st["n_factored"] * (1 + 2 * st["refine_mean"])
+    l_bytes = plan["l_unpacked"] * 4
     s_s = st["ms_solve"] / 1e3
-    kernels["k_solve"] = {"bound": "hbm", "achieved": bytes_solve / s_s / 1e9, "peak": HBM_PEAK / 1e9,
-                          "unit": "GB/s", "launches": st["n_solve_launches"], "ms_total": st["ms_solve"]}
+    kernels["k_solve"] = {"bound": "hbm", "achieved": sweeps * l_bytes / s_s / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "launches": st["n_solve_launches"], "ms_total": st["ms_solve"],
+                          "bytes_per_sweep": l_bytes}
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     kd = kernels[dom]
     roofline = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],

@@ -92,11 +92,12 @@ int lsc_circuit_omniscape_device(const double* resistance, const double* source,
 int lsc_circuit_omniscape_correction(int radius, int buffer, int block_size, int four_neighbors, int64_t nx,
                                      int64_t ny, const lsc_solve_options* options, double* out);
 
-/* Symbolic statistics of the plan the Omniscape windows share (disk: 1 for
- * the circuit windows, 0 for the flow-potential windows): out = {unknowns,
- * fronts, L entries, factor flops (sum np^3/3 + np^2 nu + np nu^2), largest
- * front rows, levels}. No reference counterpart (the reference's figures of
- * the same quantities are internal to frontal_analyze). */
+/* Symbolic statistics of a window plan with the reference's ordering
+ * (16-cell nested-dissection leaves; disk: 1 for the circuit windows, 0 for
+ * the flow-potential windows): out[7] = {unknowns, fronts, packed L entries,
+ * factor flops (sum np^3/3 + np^2 nu + np nu^2), largest front rows, levels,
+ * L entries as m x np blocks}. No reference counterpart (the reference's
+ * figures of the same quantities are internal to frontal_analyze). */
 int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, double* out);
 
 /* Checks the MFMA operand layout the front kernels assume: out[256] receives

@@ -100,8 +100,8 @@ def probe_mfma(precision: int = PRECISION_MIXED) -> np.ndarray:
 
 
 def window_plan_stats(radius: int, buffer: int = 0, four_neighbors: bool = False, disk: bool = True) -> dict:
-    out = np.empty(6)
+    out = np.empty(7)
     check(lib().lsc_window_plan_stats(int(radius), int(buffer), int(bool(four_neighbors)), int(bool(disk)),
                                       ptr(out)))
-    keys = ("unknowns", "fronts", "l_entries", "factor_flops", "max_front", "levels")
+    keys = ("unknowns", "fronts", "l_entries", "factor_flops", "max_front", "levels", "l_unpacked")
     return dict(zip(keys, out.tolist()))

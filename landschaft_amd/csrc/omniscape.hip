@@ -581,8 +581,10 @@ int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, 
     return lsc::guarded([&] {
         if (radius < 0 || buffer < 0) throw std::invalid_argument("radius and buffer must be >= 0");
         const lsc::Plan p = lsc::window_plan(radius + buffer, disk != 0, four_neighbors != 0);
+        double unpacked = 0;
+        for (int t = 0; t < p.nfront; ++t) unpacked += static_cast<double>(p.npiv[t]) * (p.npiv[t] + p.nupd[t]);
         out[0] = p.n, out[1] = p.nfront, out[2] = static_cast<double>(p.lsize), out[3] = p.flops;
-        out[4] = p.max_front, out[5] = static_cast<double>(p.lev_ptr.size() - 1);
+        out[4] = p.max_front, out[5] = static_cast<double>(p.lev_ptr.size() - 1), out[6] = unpacked;
     });
 }
 
