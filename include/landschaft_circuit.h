@@ -92,6 +92,81 @@ int lsc_circuit_omniscape_device(const double* resistance, const double* source,
 int lsc_circuit_omniscape_correction(int radius, int buffer, int block_size, int four_neighbors, int64_t nx,
                                      int64_t ny, const lsc_solve_options* options, double* out);
 
+/* Circuitscape's advanced mode on a raster (sources, grounds: conductance to
+ * ground, +inf = direct). Replaces circuit_advanced (src/cpp/circuit/
+ * python.cpp:65-79 -> circuit::advanced, src/cpp/circuit/grid.cpp:73-210):
+ * current and voltage receive ny x nx maps (voltage NaN off the nodes). */
+int lsc_circuit_advanced(const double* conductance, const double* source, const double* ground, int64_t nx,
+                         int64_t ny, int four_neighbors, const lsc_solve_options* options, double* current,
+                         double* voltage, lsc_solve_stats* stats);
+
+/* Advanced mode on the raster node graph (short-circuit regions merged,
+ * average conductance or resistance, conflict policy 0 keepall, 1 rmvsrc,
+ * 2 rmvgnd, 3 rmvall). Replaces circuitscape_advanced (src/cpp/circuit/
+ * python.cpp:170-204 -> raster_graph + raster_advanced, circuitscape.cpp:
+ * 145-263, 393-468). polymap may be NULL. nodemap: ny x nx; voltage and
+ * current: per node (capacity nx * ny), n_nodes receives their count. */
+int lsc_circuitscape_advanced(const double* gmap, const int64_t* polymap, const double* source, const double* ground,
+                              int64_t nx, int64_t ny, int avg_resistances, int four_neighbors, int policy,
+                              const lsc_solve_options* options, int32_t* nodemap, double* voltage, double* current,
+                              int64_t* n_nodes, int32_t* solved, int32_t* n_components, lsc_solve_stats* stats);
+
+/* Receives a pair's maps (ny x nx; NULL when not asked for) during a pairwise run. */
+typedef void (*lsc_emit_fn)(int ci, int cj, const double* voltage, const double* current, void* ctx);
+
+/* Pairwise mode: effective resistances between focal points (their cells;
+ * -1: none), excluded pairs (npts x npts uint8), maps by `flags` bits
+ * 1 voltage maps, 2 current maps, 4 cumulative map, 8 max map, 16 focal
+ * currents zero, 32 null currents to nodata, 64 null voltages to nodata,
+ * 128 log10 current maps. Replaces circuitscape_pairwise (src/cpp/circuit/
+ * python.cpp:214-277 -> raster_pairwise, circuitscape.cpp:474-743, with its
+ * shortcut). resistances: npts x npts; cum / max: ny x nx or NULL. */
+int lsc_circuitscape_pairwise(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                              const int64_t* point_cells, int64_t npts, const uint8_t* excluded, int avg_resistances,
+                              int four_neighbors, int flags, lsc_emit_fn emit, void* ctx,
+                              const lsc_solve_options* options, double* resistances, double* cum, double* max,
+                              lsc_solve_stats* stats);
+
+/* Several advanced problems on one raster graph (the one-to-all driver).
+ * Problem p: source cells scells[sptr[p]..sptr[p+1]) with values svals,
+ * ground cells likewise, check cell check[p] (-1: every component).
+ * Replaces circuitscape_multi_advanced (src/cpp/circuit/python.cpp:404-482 ->
+ * raster_advanced_many, circuitscape.cpp:904-1083). Outputs per problem and
+ * cell (nprob x ny x nx): voltage, current, active (uint8); solved[p];
+ * nodemap (ny x nx). */
+int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                                    int avg_resistances, int four_neighbors, int policy, int64_t nprob,
+                                    const int64_t* sptr, const int64_t* scells, const double* svals,
+                                    const int64_t* gptr, const int64_t* gcells, const double* gvals,
+                                    const int64_t* check, const lsc_solve_options* options, int32_t* nodemap,
+                                    double* voltage, double* current, uint8_t* active, int32_t* solved,
+                                    lsc_solve_stats* stats);
+
+/* J = sum_k R_eff(src_k, gnd_k) on the grid of conductance 1 / r (cells with
+ * a finite r > 0, average conductance, 8 or 4 neighbours) and dJ/dr for every
+ * cell by the adjoint (lambda_k = v_k). No reference counterpart (new:
+ * resistance-surface calibration). R receives the npairs resistances, grad
+ * ny x nx. */
+int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
+                                  const int64_t* src_cells, const int64_t* gnd_cells, int64_t npairs,
+                                  const lsc_solve_options* options, double* J, double* R, double* grad,
+                                  lsc_solve_stats* stats);
+
+/* A factorised grid circuit kept for autograd (no reference counterpart):
+ * create factors the grid of conductance 1 / r (host resistance), every
+ * component grounded at its last node; solve maps k right-hand sides given
+ * per cell (device, k x ny x nx) to potentials per cell (device); adjoint
+ * solves L lambda = dJ/dV (device, k x ny x nx) with the same factor and
+ * writes dJ/dr (device, ny x nx) from the potentials V. */
+typedef struct lsc_grid_circuit lsc_grid_circuit;
+int lsc_grid_circuit_create(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
+                            const lsc_solve_options* options, lsc_grid_circuit** out);
+int lsc_grid_circuit_solve(lsc_grid_circuit* circuit, int k, const double* rhs_cells, double* out_cells);
+int lsc_grid_circuit_adjoint(lsc_grid_circuit* circuit, int k, const double* dJdV_cells, const double* V_cells,
+                             double* grad_r);
+int lsc_grid_circuit_stats(lsc_grid_circuit* circuit, lsc_solve_stats* stats);
+void lsc_grid_circuit_destroy(lsc_grid_circuit* circuit);
+
 /* Symbolic statistics of a window plan with the reference's ordering
  * (16-cell nested-dissection leaves; disk: 1 for the circuit windows, 0 for
  * the flow-potential windows): out[7] = {unknowns, fronts, packed L entries,

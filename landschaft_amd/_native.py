@@ -19,9 +19,9 @@ PRECISION_FP64 = 1
 # Every symbol declared in include/landschaft_circuit.h.
 EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_circuit_omniscape_device",
            "lsc_circuit_omniscape_correction", "lsc_probe_mfma", "lsc_window_plan_stats", "lsc_circuit_advanced",
-           "lsc_circuit_advanced_device", "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise",
-           "lsc_circuitscape_multi_advanced", "lsc_effective_resistance_grad",
-           "lsc_effective_resistance_grad_device", "lsc_pair_voltages_device", "lsc_pair_adjoint_device")
+           "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise", "lsc_circuitscape_multi_advanced",
+           "lsc_effective_resistance_grad", "lsc_grid_circuit_create", "lsc_grid_circuit_solve",
+           "lsc_grid_circuit_adjoint", "lsc_grid_circuit_stats", "lsc_grid_circuit_destroy")
 
 
 class SolveOptions(ctypes.Structure):

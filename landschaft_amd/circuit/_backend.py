@@ -85,3 +85,174 @@ def circuit_omniscape_correction(radius, buffer, block_size, four_neighbors, nx,
                                                      int(bool(four_neighbors)), ctypes.c_int64(nx),
                                                      ctypes.c_int64(ny), ctypes.byref(o), N.ptr(out)))
     return out
+
+
+_POLICIES = {"keepall": 0, "rmvsrc": 1, "rmvgnd": 2, "rmvall": 3}
+_PAIR_FLAGS = ("volt_maps", "cur_maps", "cum", "max", "focal_zero", "null_currents", "null_voltages",
+               "log_transform")
+_EMIT = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
+def circuit_advanced(conductance, source, ground, four_neighbors=False, n_threads=0):
+    """Circuitscape's advanced mode on a raster (python.cpp:65-79): node currents and voltages."""
+    g, s, gr = N.f64(conductance), N.f64(source), N.f64(ground)
+    if g.ndim != 2 or s.shape != g.shape or gr.shape != g.shape:
+        raise ValueError("expected conductance, source and ground rasters of the same shape")
+    ny, nx = g.shape
+    cur, volt = np.empty((ny, nx)), np.empty((ny, nx))
+    st = N.SolveStats()
+    o = _opts()
+    N.check(N.lib().lsc_circuit_advanced(N.ptr(g), N.ptr(s), N.ptr(gr), ctypes.c_int64(nx), ctypes.c_int64(ny),
+                                         int(bool(four_neighbors)), ctypes.byref(o), N.ptr(cur), N.ptr(volt),
+                                         ctypes.byref(st)))
+    _record(st)
+    return {"current": cur, "voltage": volt}
+
+
+def _poly(polymap, shape):
+    if polymap is None:
+        return None
+    p = np.ascontiguousarray(polymap, dtype=np.int64)
+    if p.shape != shape:
+        raise ValueError("the polygon raster must have the habitat's shape")
+    return p
+
+
+def _policy(policy):
+    if policy not in _POLICIES:
+        raise ValueError("remove_src_or_gnd must be keepall, rmvsrc, rmvgnd or rmvall")
+    return _POLICIES[policy]
+
+
+def circuitscape_advanced(gmap, polymap, source, ground, avg_resistances, four_neighbors, policy, n_threads=0):
+    """Advanced mode on the raster node graph (python.cpp:170-204): node map, per-node voltage and current."""
+    g, s, gr = N.f64(gmap), N.f64(source), N.f64(ground)
+    if g.ndim != 2 or s.shape != g.shape or gr.shape != g.shape:
+        raise ValueError("expected habitat, source and ground rasters of the same shape")
+    ny, nx = g.shape
+    poly = _poly(polymap, g.shape)
+    nm = np.empty((ny, nx), dtype=np.int32)
+    volt, cur = np.empty(nx * ny), np.empty(nx * ny)
+    nn, solved, ncomp = ctypes.c_int64(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    st = N.SolveStats()
+    o = _opts()
+    N.check(N.lib().lsc_circuitscape_advanced(N.ptr(g), N.ptr(poly, ctypes.c_int64), N.ptr(s), N.ptr(gr),
+                                              ctypes.c_int64(nx), ctypes.c_int64(ny), int(bool(avg_resistances)),
+                                              int(bool(four_neighbors)), _policy(policy), ctypes.byref(o),
+                                              N.ptr(nm, ctypes.c_int32), N.ptr(volt), N.ptr(cur), ctypes.byref(nn),
+                                              ctypes.byref(solved), ctypes.byref(ncomp), ctypes.byref(st)))
+    _record(st)
+    return {"nodemap": nm, "voltage": volt[:nn.value].copy(), "current": cur[:nn.value].copy(),
+            "solved": bool(solved.value), "n_components": ncomp.value}
+
+
+def circuitscape_pairwise(gmap, polymap, point_cells, excluded, avg_resistances, four_neighbors, options, emit=None,
+                          n_threads=0):
+    """Pairwise mode (python.cpp:214-277): resistances, cum / max maps; per-pair maps through `emit`."""
+    g = N.f64(gmap)
+    if g.ndim != 2:
+        raise ValueError("expected a 2-D habitat raster")
+    ny, nx = g.shape
+    if options.get("region_ids") is not None:
+        raise ValueError("focal regions are not supported by the GPU circuit path")
+    poly = _poly(polymap, g.shape)
+    pts = np.ascontiguousarray(point_cells, dtype=np.int64).ravel()
+    n = pts.size
+    ex = np.ascontiguousarray(excluded, dtype=np.uint8).reshape(n, n)
+    flags = sum(1 << k for k, f in enumerate(_PAIR_FLAGS) if options.get(f))
+    R = np.empty((n, n))
+    cum = np.empty((ny, nx)) if options.get("cum") else None
+    mx = np.empty((ny, nx)) if options.get("max") else None
+
+    def _cb(ci, cj, v, c, _ctx):
+        if emit is None:
+            return
+        vv = None if not v else np.ctypeslib.as_array(v, shape=(ny, nx)).copy()
+        cc = None if not c else np.ctypeslib.as_array(c, shape=(ny, nx)).copy()
+        emit(ci, cj, vv, cc)
+
+    cb = _EMIT(_cb)
+    st = N.SolveStats()
+    o = _opts()
+    N.check(N.lib().lsc_circuitscape_pairwise(N.ptr(g), N.ptr(poly, ctypes.c_int64), ctypes.c_int64(nx),
+                                              ctypes.c_int64(ny), N.ptr(pts, ctypes.c_int64), ctypes.c_int64(n),
+                                              N.ptr(ex, ctypes.c_uint8), int(bool(avg_resistances)),
+                                              int(bool(four_neighbors)), flags, cb, None, ctypes.byref(o), N.ptr(R),
+                                              N.ptr(cum), N.ptr(mx), ctypes.byref(st)))
+    _record(st)
+    out = {"resistances": R, "nodes": None}
+    if cum is not None:
+        out["cum"] = cum
+    if mx is not None:
+        out["max"] = mx
+    return out
+
+
+def circuitscape_multi_advanced(gmap, polymap, avg_resistances, four_neighbors, problems, policy, n_threads=0):
+    """Several advanced problems on one graph (python.cpp:404-482): per problem voltage, current, active, solved."""
+    g = N.f64(gmap)
+    ny, nx = g.shape
+    poly = _poly(polymap, g.shape)
+    npb = len(problems)
+
+    def cat(kc, kv):
+        ptr = np.zeros(npb + 1, dtype=np.int64)
+        for k, q in enumerate(problems):
+            ptr[k + 1] = ptr[k] + len(q[kc])
+        cells = np.ascontiguousarray(np.concatenate([np.asarray(q[kc], dtype=np.int64) for q in problems])
+                                     if npb else np.zeros(0, np.int64))
+        vals = np.ascontiguousarray(np.concatenate([np.asarray(q[kv], dtype=np.float64) for q in problems])
+                                    if npb else np.zeros(0))
+        return ptr, cells, vals
+
+    sp, sc, sv = cat("src_cells", "src_vals")
+    gp, gc, gv = cat("gnd_cells", "gnd_vals")
+    check = np.array([q["check_cell"] for q in problems], dtype=np.int64)
+    nm = np.empty((ny, nx), dtype=np.int32)
+    volt = np.empty((npb, ny, nx))
+    cur = np.empty((npb, ny, nx))
+    act = np.empty((npb, ny, nx), dtype=np.uint8)
+    solved = np.empty(npb, dtype=np.int32)
+    st = N.SolveStats()
+    o = _opts()
+    i64p = ctypes.c_int64
+    N.check(N.lib().lsc_circuitscape_multi_advanced(
+        N.ptr(g), N.ptr(poly, i64p), i64p(nx), i64p(ny), int(bool(avg_resistances)), int(bool(four_neighbors)),
+        _policy(policy), i64p(npb), N.ptr(sp, i64p), N.ptr(sc, i64p), N.ptr(sv), N.ptr(gp, i64p), N.ptr(gc, i64p),
+        N.ptr(gv), N.ptr(check, i64p), ctypes.byref(o), N.ptr(nm, ctypes.c_int32), N.ptr(volt), N.ptr(cur),
+        N.ptr(act, ctypes.c_uint8), N.ptr(solved, ctypes.c_int32), ctypes.byref(st)))
+    _record(st)
+    # per-node arrays, as the binding returns them (every node has a cell)
+    on = nm >= 0
+    n_nodes = int(nm.max()) + 1 if on.any() else 0
+    results = []
+    for p in range(npb):
+        v = np.zeros(n_nodes)
+        c = np.zeros(n_nodes)
+        a = np.zeros(n_nodes, dtype=bool)
+        v[nm[on]] = volt[p][on]
+        c[nm[on]] = cur[p][on]
+        a[nm[on]] = act[p][on].astype(bool)
+        results.append({"voltage": v, "current": c, "active": np.flatnonzero(a).astype(np.int32),
+                        "solved": bool(solved[p])})
+    return {"nodemap": nm, "results": results}
+
+
+def effective_resistance_grad(resistance, src_cells, gnd_cells, four_neighbors=False):
+    """J = sum of effective resistances of the pairs and dJ/dr per cell (the adjoint on the GPU)."""
+    r = N.f64(resistance)
+    ny, nx = r.shape
+    s = np.ascontiguousarray(src_cells, dtype=np.int64).ravel()
+    t = np.ascontiguousarray(gnd_cells, dtype=np.int64).ravel()
+    J = ctypes.c_double(0)
+    R = np.empty(s.size)
+    grad = np.empty((ny, nx))
+    st = N.SolveStats()
+    o = _opts()
+    N.check(N.lib().lsc_effective_resistance_grad(N.ptr(r), ctypes.c_int64(nx), ctypes.c_int64(ny),
+                                                  int(bool(four_neighbors)), N.ptr(s, ctypes.c_int64),
+                                                  N.ptr(t, ctypes.c_int64), ctypes.c_int64(s.size), ctypes.byref(o),
+                                                  ctypes.byref(J), N.ptr(R), N.ptr(grad), ctypes.byref(st)))
+    _record(st)
+    return J.value, R, grad

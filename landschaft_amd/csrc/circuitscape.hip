@@ -1,0 +1,937 @@
+// Circuitscape's raster modes on the graph engine, and the adjoint.
+//
+// Host side builds the node graph of a raster as the reference does
+// (raster_graph, src/cpp/circuit/circuitscape.cpp:145-263: column-major node
+// labels, short-circuit regions merged into one node, edges to the right,
+// below and on both right-hand diagonals, duplicates summed, components);
+// every solve, and the node currents, run on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/landschaft_circuit.h"
+#include "common.hpp"
+#include "system.hpp"
+
+namespace lsc {
+
+namespace {
+
+constexpr double kShift = 10 * std::numeric_limits<double>::epsilon();  // CHOLMOD's, circuitscape.cpp:28
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+unsigned nblk(long long n, int per) { return static_cast<unsigned>((n + per - 1) / per); }
+
+// Raster graph (reference circuitscape.hpp:22-33).
+struct RGraph {
+    i64 nx = 0, ny = 0;
+    i32 n = 0;
+    std::vector<i32> nodemap;   // cell -> node
+    std::vector<i64> node_cell; // node -> cell (single-cell nodes), -1 merged
+    std::vector<i64> ptr;
+    std::vector<i32> adj;
+    std::vector<double> w, degree;
+    std::vector<i32> comp;
+    i32 ncomp = 0;
+};
+
+// Circuitscape's cond_avg / weird_avg / res_avg / weirder_avg (circuitscape.cpp:31-34).
+double average(double x, double y, bool diag, bool res) {
+    if (res) return diag ? 1 / (1.4142135623730951 * (1 / x + 1 / y) / 2) : 1 / ((1 / x + 1 / y) / 2);
+    return diag ? (x + y) / 2.8284271247461903 : (x + y) / 2;
+}
+
+RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny, bool avg_res, bool four) {
+    RGraph g;
+    g.nx = nx, g.ny = ny;
+    const i64 n = nx * ny;
+    std::vector<i64> label(n, 0);
+    i64 next = 0;
+    for (i64 x = 0; x < nx; ++x)
+        for (i64 y = 0; y < ny; ++y)
+            if (gmap[y * nx + x] > 0) label[y * nx + x] = ++next;
+    if (poly) {
+        std::unordered_map<std::int64_t, i64> first;
+        for (i64 x = 0; x < nx; ++x)
+            for (i64 y = 0; y < ny; ++y) {
+                const i64 c = y * nx + x;
+                if (poly[c] != 0 && gmap[c] > 0) first.emplace(poly[c], label[c]);
+            }
+        for (i64 c = 0; c < n; ++c)
+            if (poly[c] != 0) {
+                auto it = first.find(poly[c]);
+                if (it != first.end()) label[c] = it->second;
+            }
+    }
+    // relabel by rank: labels are column-major positions, so rank order is
+    // the first-cell order
+    std::vector<i64> rank(next + 1, -1);
+    for (i64 c = 0; c < n; ++c)
+        if (label[c] != 0) rank[label[c]] = 0;
+    i32 cnt = 0;
+    for (i64 l = 1; l <= next; ++l)
+        if (rank[l] == 0) rank[l] = cnt++;
+    g.n = cnt;
+    g.nodemap.assign(n, -1);
+    std::vector<i32> cells_per(cnt, 0);
+    g.node_cell.assign(cnt, -1);
+    for (i64 c = 0; c < n; ++c) {
+        if (label[c] == 0) continue;
+        const i32 v = static_cast<i32>(rank[label[c]]);
+        g.nodemap[c] = v;
+        if (cells_per[v]++ == 0) g.node_cell[v] = c;
+    }
+    for (i32 v = 0; v < cnt; ++v)
+        if (cells_per[v] > 1) g.node_cell[v] = -1;
+    struct E {
+        i32 a, b;
+        double w;
+    };
+    std::vector<E> edges;
+    auto emit = [&](i64 c, i64 d, bool diag) {
+        const i32 a = g.nodemap[c], b = g.nodemap[d];
+        if (a < 0 || b < 0) return;
+        const double w = average(gmap[c], gmap[d], diag, avg_res);
+        if (a == b) return;
+        edges.push_back({a, b, w});
+        edges.push_back({b, a, w});
+    };
+    for (i64 x = 0; x < nx; ++x)
+        for (i64 y = 0; y < ny; ++y) {
+            const i64 c = y * nx + x;
+            if (g.nodemap[c] < 0) continue;
+            if (x + 1 < nx) emit(c, c + 1, false);
+            if (y + 1 < ny) emit(c, c + nx, false);
+            if (!four) {
+                if (y + 1 < ny && x + 1 < nx) emit(c, c + nx + 1, true);
+                if (y > 0 && x + 1 < nx) emit(c, c - nx + 1, true);
+            }
+        }
+    std::stable_sort(edges.begin(), edges.end(),
+                     [](const E& p, const E& q) { return std::tie(p.a, p.b) < std::tie(q.a, q.b); });
+    g.ptr.assign(cnt + 1, 0);
+    for (std::size_t k = 0; k < edges.size();) {
+        std::size_t e = k;
+        double w = edges[k].w;
+        for (++e; e < edges.size() && edges[e].a == edges[k].a && edges[e].b == edges[k].b; ++e) w += edges[e].w;
+        g.adj.push_back(edges[k].b);
+        g.w.push_back(w);
+        ++g.ptr[edges[k].a + 1];
+        k = e;
+    }
+    for (i32 v = 0; v < cnt; ++v) g.ptr[v + 1] += g.ptr[v];
+    g.degree.assign(cnt, 0.0);
+    for (i32 v = 0; v < cnt; ++v) {
+        double d = 0.0;
+        for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) d += g.w[k];
+        g.degree[v] = d;
+    }
+    g.comp.assign(cnt, -1);
+    std::vector<i32> st;
+    for (i32 u = 0; u < cnt; ++u) {
+        if (g.comp[u] >= 0) continue;
+        g.comp[u] = g.ncomp;
+        st.push_back(u);
+        while (!st.empty()) {
+            const i32 v = st.back();
+            st.pop_back();
+            for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) {
+                const i32 t = g.adj[k];
+                if (g.w[k] == 0 || g.comp[t] >= 0) continue;
+                g.comp[t] = g.ncomp;
+                st.push_back(t);
+            }
+        }
+        ++g.ncomp;
+    }
+    return g;
+}
+
+struct Members {
+    std::vector<i64> start;
+    std::vector<i32> mem;
+};
+Members members_of(const RGraph& g) {
+    Members m;
+    m.start.assign(g.ncomp + 1, 0);
+    for (i32 v = 0; v < g.n; ++v) ++m.start[g.comp[v] + 1];
+    for (i32 k = 0; k < g.ncomp; ++k) m.start[k + 1] += m.start[k];
+    m.mem.resize(g.n);
+    std::vector<i64> fill(m.start.begin(), m.start.end() - 1);
+    for (i32 v = 0; v < g.n; ++v) m.mem[fill[g.comp[v]]++] = v;
+    return m;
+}
+
+// The grounded Laplacian of one component without the dropped nodes, plus
+// `finite` on the diagonal and the shift (reference factor_component,
+// circuitscape.cpp:92-141). `nodes` receives the caller's node of every
+// unknown.
+template <class Drop>
+HostMatrix component_matrix(const RGraph& g, const i32* mem, i64 count, const double* finite, Drop drop,
+                            std::vector<i32>& nodes) {
+    HostMatrix M;
+    nodes.clear();
+    std::unordered_map<i32, i32> local;
+    i64 x0 = g.nx, x1 = -1, y0 = g.ny, y1 = -1;
+    for (i64 p = 0; p < count; ++p) {
+        const i32 v = mem[p];
+        if (drop(v)) continue;
+        local.emplace(v, static_cast<i32>(nodes.size()));
+        nodes.push_back(v);
+        const i64 c = g.node_cell[v];
+        if (c >= 0) {
+            x0 = std::min(x0, c % g.nx), x1 = std::max(x1, c % g.nx);
+            y0 = std::min(y0, c / g.nx), y1 = std::max(y1, c / g.nx);
+        }
+    }
+    M.n = static_cast<int>(nodes.size());
+    M.bx = x1 >= x0 ? x1 - x0 + 1 : 1;
+    M.by = y1 >= y0 ? y1 - y0 + 1 : 1;
+    M.cell.resize(nodes.size());
+    M.ptr.assign(nodes.size() + 1, 0);
+    M.diag.resize(nodes.size());
+    for (std::size_t i = 0; i < nodes.size(); ++i) {
+        const i32 v = nodes[i];
+        const i64 c = g.node_cell[v];
+        M.cell[i] = c >= 0 ? (c / g.nx - y0) * M.bx + (c % g.nx - x0) : -1;
+        for (i64 e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
+            auto it = local.find(g.adj[e]);
+            if (it == local.end()) continue;
+            M.adj.push_back(it->second);
+            M.off.push_back(-g.w[e]);
+        }
+        M.ptr[i + 1] = static_cast<i64>(M.adj.size());
+        M.diag[i] = (finite ? g.degree[v] + finite[v] : g.degree[v]) + kShift;
+    }
+    return M;
+}
+
+// ---- node currents on the GPU (reference graph_node_currents,
+// circuitscape.cpp:38-73; on a column-major raster graph it is also the
+// grid's node_currents, grid.cpp:19-71: an edge's current is computed from
+// either end with the opposite sign, exactly).
+
+struct DevGraph {
+    DevBuf<long long> ptr;
+    DevBuf<int> adj;
+    DevBuf<double> w;
+    void upload(const RGraph& g, hipStream_t s) {
+        std::vector<long long> p(g.ptr.begin(), g.ptr.end());
+        ptr = to_device(p, s);
+        adj = to_device(g.adj, s);
+        w = to_device(g.w.empty() ? std::vector<double>{0.0} : g.w, s);
+    }
+};
+
+__global__ void k_gc_max(int cnt, const int* __restrict__ mem, const long long* __restrict__ ptr,
+                         const int* __restrict__ adj, const double* __restrict__ w, const double* __restrict__ v,
+                         double* __restrict__ partial) {
+    __shared__ double red[2][4];
+    double mi = -INFINITY, mo = -INFINITY;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < cnt) {
+        const int c = mem[p];
+        for (long long e = ptr[c]; e < ptr[c + 1]; ++e) {
+            const int r = adj[e];
+            const double b = fabs(w[e]) * (v[r] - v[c]);
+            if (r < c) mi = fmax(mi, b);
+            else mo = fmax(mo, b);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mi = fmax(mi, __shfl_xor(mi, o));
+        mo = fmax(mo, __shfl_xor(mo, o));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[0][wave] = mi, red[1][wave] = mo;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+        partial[2 * blockIdx.x + 1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+    }
+}
+
+__global__ void k_gc_reduce(int nb, const double* __restrict__ partial, double* __restrict__ mx) {
+    __shared__ double red[2][4];
+    double mi = -INFINITY, mo = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) mi = fmax(mi, partial[2 * i]), mo = fmax(mo, partial[2 * i + 1]);
+    for (int o = 32; o > 0; o >>= 1) {
+        mi = fmax(mi, __shfl_xor(mi, o));
+        mo = fmax(mo, __shfl_xor(mo, o));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[0][wave] = mi, red[1][wave] = mo;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx[0] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+        mx[1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+    }
+}
+
+__global__ void k_gc_cur(int cnt, const int* __restrict__ mem, const long long* __restrict__ ptr,
+                         const int* __restrict__ adj, const double* __restrict__ w, const double* __restrict__ v,
+                         const double* __restrict__ finite, const double* __restrict__ mx, double* __restrict__ cur) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= cnt) return;
+    const double maxin = mx[0], maxout = mx[1];
+    const int c = mem[p];
+    double in = 0.0, out = 0.0;
+    for (long long e = ptr[c]; e < ptr[c + 1]; ++e) {
+        const int r = adj[e];
+        const double b = fabs(w[e]) * (v[r] - v[c]);
+        if (b > 0) {
+            if (!(fabs(b / maxin) < 1e-8)) in += b;
+        } else if (b < 0) {
+            if (!(fabs(b / maxout) < 1e-8)) out -= b;
+        }
+    }
+    if (finite) {
+        const double f = finite[c] * v[c];
+        if (f < 0) in -= f;
+        else if (f > 0) out += f;
+    }
+    cur[c] = in > out ? in : out;
+}
+
+// Node currents of one component (`d_mem`: its nodes, ascending) from
+// voltages `d_v` (by node) into `d_cur` (by node).
+void graph_currents(const DevGraph& G, const int* d_mem, int cnt, const double* d_v, const double* d_finite,
+                    double* d_cur, DevBuf<double>& scratch, hipStream_t s) {
+    if (cnt <= 0) return;
+    const unsigned nb = nblk(cnt, 256);
+    if (scratch.n < 2 * nb + 2) scratch.alloc(2 * nb + 2);
+    hipLaunchKernelGGL(k_gc_max, dim3(nb), dim3(256), 0, s, cnt, d_mem, G.ptr.p, G.adj.p, G.w.p, d_v, scratch.p);
+    hipLaunchKernelGGL(k_gc_reduce, dim3(1), dim3(256), 0, s, static_cast<int>(nb), scratch.p, scratch.p + 2 * nb);
+    hipLaunchKernelGGL(k_gc_cur, dim3(nb), dim3(256), 0, s, cnt, d_mem, G.ptr.p, G.adj.p, G.w.p, d_v, d_finite,
+                       scratch.p + 2 * nb, d_cur);
+}
+
+struct Resolved {
+    std::vector<double> src, gnd, finite;
+    bool any_finite = false;
+};
+
+// resolve_conflicts (reference circuitscape.cpp:881-900): 0 keepall,
+// 1 rmvsrc, 2 rmvgnd, 3 rmvall.
+Resolved resolve(std::vector<double> src, std::vector<double> gnd, int policy) {
+    Resolved r;
+    const std::size_t n = src.size();
+    r.finite.assign(n, 0.0);
+    for (std::size_t v = 0; v < n; ++v) {
+        r.finite[v] = gnd[v] < kInf ? gnd[v] : 0.0;
+        if (r.finite[v] != 0) r.any_finite = true;
+    }
+    for (std::size_t v = 0; v < n; ++v) {
+        if (!(src[v] != 0 && gnd[v] != 0)) continue;
+        if (policy == 1 || policy == 3) src[v] = 0.0;
+        else if (policy == 2) gnd[v] = 0.0;
+    }
+    for (std::size_t v = 0; v < n; ++v)
+        if (gnd[v] == kInf && src[v] > 0) gnd[v] = 0.0;
+    r.src = std::move(src);
+    r.gnd = std::move(gnd);
+    return r;
+}
+
+struct NodeSol {
+    std::vector<double> voltage, current;
+    std::vector<i32> active;
+    bool solved = false;
+};
+
+struct Ctx {
+    int precision;
+    double tol;
+    int maxit;
+    hipStream_t s;
+    IrStats st;
+};
+
+Ctx make_ctx(const lsc_solve_options* o) {
+    Ctx c{};
+    c.precision = o ? o->precision : LSC_PRECISION_MIXED;
+    c.tol = o && o->tol > 0 ? o->tol : 1e-12;
+    c.maxit = o && o->max_refine > 0 ? o->max_refine : 30;
+    c.s = o ? reinterpret_cast<hipStream_t>(o->stream) : nullptr;
+    return c;
+}
+
+// Advanced-mode problems on one graph, each factorised on its own (the
+// reference's raster_advanced / the per-problem branch of
+// raster_advanced_many, circuitscape.cpp:393-468, 971-993).
+std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std::vector<std::vector<double>>& srcs,
+                                   const std::vector<std::vector<double>>& gnds, const std::vector<i32>& only,
+                                   int policy, Ctx& cx) {
+    const std::size_t np = srcs.size();
+    const i32 n = g.n;
+    std::vector<Resolved> res(np);
+    std::vector<NodeSol> out(np);
+    for (std::size_t p = 0; p < np; ++p) {
+        res[p] = resolve(srcs[p], gnds[p], policy);
+        out[p].voltage.assign(n, 0.0);
+        out[p].current.assign(n, 0.0);
+        for (i32 v = 0; v < n; ++v)
+            if (res[p].src[v] != 0 || res[p].gnd[v] != 0) out[p].active.push_back(v);
+    }
+    const Members M = members_of(g);
+    DevBuf<double> dv(n ? n : 1), dcur(n ? n : 1), dfin(n ? n : 1), scratch;
+    DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
+    std::vector<i32> nodes;
+    for (std::size_t p = 0; p < np; ++p) {
+        const Resolved& r = res[p];
+        bool uploaded = false;
+        for (i32 k = 0; k < g.ncomp; ++k) {
+            if (only[p] == -2 || (only[p] >= 0 && g.comp[only[p]] != k)) continue;
+            double ss = 0, sg = 0;
+            for (i64 q = M.start[k]; q < M.start[k + 1]; ++q) ss += r.src[M.mem[q]], sg += r.gnd[M.mem[q]];
+            if (!(ss != 0 && sg != 0)) continue;
+            out[p].solved = true;
+            const i32* mem = M.mem.data() + M.start[k];
+            const i64 count = M.start[k + 1] - M.start[k];
+            HostMatrix A = component_matrix(g, mem, count, r.any_finite ? r.finite.data() : nullptr,
+                                            [&](i32 v) { return r.gnd[v] == kInf; }, nodes);
+            if (A.n > 0) {
+                SystemSolver S(cx.precision, cx.s);
+                S.factor(A);
+                std::vector<double> b(A.n), xv(A.n);
+                for (int i = 0; i < A.n; ++i) b[i] = r.src[nodes[i]];
+                S.solve(1, b.data(), xv.data(), cx.tol, cx.maxit, cx.st);
+                for (int i = 0; i < A.n; ++i) out[p].voltage[nodes[i]] = xv[i];
+            }
+            if (!uploaded) {
+                dfin.upload(r.finite.data(), n, cx.s);
+                uploaded = true;
+            }
+            dv.upload(out[p].voltage.data(), n, cx.s);
+            LSC_HIP(hipMemsetAsync(dcur.p, 0, n * sizeof(double), cx.s));
+            graph_currents(G, dmem.p + M.start[k], static_cast<int>(count), dv.p, r.any_finite ? dfin.p : nullptr,
+                           dcur.p, scratch, cx.s);
+            std::vector<double> cur(n);
+            LSC_HIP(hipMemcpyAsync(cur.data(), dcur.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+            LSC_HIP(hipStreamSynchronize(cx.s));
+            for (i64 q = 0; q < count; ++q) out[p].current[mem[q]] = cur[mem[q]];
+        }
+    }
+    return out;
+}
+
+std::vector<double> node_sums(const RGraph& g, const double* cells) {
+    std::vector<double> s(g.n, 0.0);
+    for (i64 x = 0; x < g.nx; ++x)
+        for (i64 y = 0; y < g.ny; ++y) {
+            const i64 c = y * g.nx + x;
+            const i32 v = g.nodemap[c];
+            if (v >= 0 && cells[c] != 0) s[v] += cells[c];
+        }
+    return s;
+}
+
+void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->n_solved = cx.st.solved;
+    out->n_unconverged = cx.st.unconverged;
+    out->n_over_1e10 = cx.st.over_bar;
+    out->refine_max = cx.st.iter_max;
+    out->n_solve_launches = cx.st.solve_launches;
+    out->refine_mean = cx.st.solved ? static_cast<double>(cx.st.iter_sum) / cx.st.solved : 0.0;
+    out->resid_max = cx.st.resid_max;
+    out->resid_mean = cx.st.solved ? cx.st.resid_sum / cx.st.solved : 0.0;
+}
+
+// ---- pairwise (reference raster_pairwise, circuitscape.cpp:474-743)
+
+struct PairOpts {
+    bool volt_maps, cur_maps, cum, max, focal_zero, null_currents, null_voltages, log_transform;
+    bool maps() const { return volt_maps || cur_maps || cum || max; }
+};
+
+void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std::vector<i32>& points,
+              const std::uint8_t* excluded, const PairOpts& o, lsc_emit_fn emit, void* ctx, double* R_out,
+              double* cum_out, double* max_out, Ctx& cx) {
+    const i32 np = static_cast<i32>(points.size());
+    const i64 ncell = g.nx * g.ny;
+    std::vector<double> Rm(static_cast<std::size_t>(np) * np, -1.0);
+    auto R = [&](i32 a, i32 b) -> double& { return Rm[static_cast<std::size_t>(a) * np + b]; };
+    bool any_ex = false;
+    for (i64 k = 0; k < static_cast<i64>(np) * np; ++k) any_ex = any_ex || excluded[k];
+    const bool shortcut = !o.maps() && !any_ex;
+    std::vector<double> voltmatrix, sres;
+    if (shortcut) voltmatrix.assign(static_cast<std::size_t>(np) * np, 0.0), sres.assign(static_cast<std::size_t>(np) * np, -1.0);
+    std::vector<double> cum, mx;
+    if (o.cum) cum.assign(ncell, 0.0);
+    if (o.max) mx.assign(ncell, -9999.0);
+    const Members M = members_of(g);
+    std::unordered_map<i32, std::vector<i32>> positions;
+    for (i32 i = 0; i < np; ++i)
+        if (points[i] >= 0) positions[points[i]].push_back(i);
+    DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
+    DevBuf<double> dv, dcur, scratch;
+    std::vector<i32> nodes;
+    for (i32 k = 0; k < g.ncomp; ++k) {
+        const i64 p0 = M.start[k], p1 = M.start[k + 1];
+        std::vector<i32> csub;
+        {
+            std::unordered_set<i32> seen;
+            for (i32 i = 0; i < np; ++i) {
+                const i32 v = points[i];
+                if (v >= 0 && g.comp[v] == k && seen.insert(v).second) csub.push_back(v);
+            }
+        }
+        if (csub.empty()) continue;
+        for (i32 v : csub) {
+            const auto& pos = positions[v];
+            for (std::size_t a = 0; a < pos.size(); ++a)
+                for (std::size_t b = a + 1; b < pos.size(); ++b) R(pos[a], pos[b]) = R(pos[b], pos[a]) = 0.0;
+        }
+        std::vector<std::pair<i32, i32>> jobs;
+        const std::size_t nsrc = shortcut ? std::min<std::size_t>(1, csub.size()) : csub.size();
+        for (std::size_t si = 0; si < nsrc; ++si)
+            for (std::size_t di = si + 1; di < csub.size(); ++di) {
+                bool any = false;
+                for (i32 ci : positions[csub[si]])
+                    for (i32 cj : positions[csub[di]]) any = any || !excluded[static_cast<std::size_t>(ci) * np + cj];
+                if (any) jobs.push_back({csub[si], csub[di]});
+            }
+        if (jobs.empty()) continue;
+        // one node grounded (the component's last): exact voltage differences
+        const i32 ref = M.mem[p1 - 1];
+        const i64 count = p1 - p0;
+        const i32* mb = M.mem.data() + p0;
+        HostMatrix A = component_matrix(g, mb, count, nullptr, [&](i32 v) { return v == ref; }, nodes);
+        SystemSolver S(cx.precision, cx.s);
+        S.factor(A);
+        std::unordered_map<i32, i32> uidx;
+        for (int i = 0; i < A.n; ++i) uidx.emplace(nodes[i], i);
+        // every job's voltages (relative to its source) by member
+        const std::size_t nj = jobs.size();
+        std::vector<double> B(nj * static_cast<std::size_t>(A.n), 0.0), X(nj * static_cast<std::size_t>(A.n), 0.0);
+        for (std::size_t j = 0; j < nj; ++j) {
+            auto a = uidx.find(jobs[j].first), b = uidx.find(jobs[j].second);
+            if (a != uidx.end()) B[j * A.n + a->second] = -1.0;
+            if (b != uidx.end()) B[j * A.n + b->second] = 1.0;
+        }
+        if (A.n > 0) S.solve(static_cast<int>(nj), B.data(), X.data(), cx.tol, cx.maxit, cx.st);
+        auto midx = [&](i32 v) { return static_cast<i32>(std::lower_bound(mb, mb + count, v) - mb); };
+        std::vector<double> vcol(count), vnode, cnode;
+        if (o.maps()) {
+            vnode.assign(g.n, 0.0), cnode.assign(g.n, 0.0);
+            dv.alloc(g.n ? g.n : 1), dcur.alloc(g.n ? g.n : 1);
+        }
+        for (std::size_t j = 0; j < nj; ++j) {
+            const auto& job = jobs[j];
+            for (i64 q = 0; q < count; ++q) {
+                auto it = uidx.find(mb[q]);
+                vcol[q] = it != uidx.end() ? X[j * A.n + it->second] : 0.0;
+            }
+            const double vs = vcol[midx(job.first)];
+            for (i64 q = 0; q < count; ++q) vcol[q] -= vs;
+            const double resistance = vcol[midx(job.second)] - vcol[midx(job.first)];
+            if (o.maps()) {
+                for (i64 q = 0; q < count; ++q) vnode[mb[q]] = vcol[q];
+                dv.upload(vnode.data(), g.n, cx.s);
+                graph_currents(G, dmem.p + p0, static_cast<int>(count), dv.p, nullptr, dcur.p, scratch, cx.s);
+                LSC_HIP(hipMemcpyAsync(cnode.data(), dcur.p, g.n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                LSC_HIP(hipStreamSynchronize(cx.s));
+            }
+            for (i32 ci : positions[job.first])
+                for (i32 cj : positions[job.second]) {
+                    if (excluded[static_cast<std::size_t>(ci) * np + cj]) continue;
+                    R(ci, cj) = R(cj, ci) = resistance;
+                    if (shortcut) {
+                        for (i32 i = 1; i < np; ++i) {
+                            const i32 v = points[i];
+                            if (v < 0 || g.comp[v] != k) continue;
+                            voltmatrix[static_cast<std::size_t>(i) * np + cj] = 1 - (vcol[midx(v)] / resistance);
+                        }
+                        continue;
+                    }
+                    if (!o.maps()) continue;
+                    std::vector<double> vmap, cmap(ncell, 0.0);
+                    if (o.volt_maps) {
+                        vmap.assign(ncell, 0.0);
+                        for (i64 c = 0; c < ncell; ++c) {
+                            const i32 v = g.nodemap[c];
+                            if (v >= 0 && g.comp[v] == k) vmap[c] = vnode[v];
+                            if (o.null_voltages && gmap[c] == 0) vmap[c] = -9999.0;
+                        }
+                    }
+                    for (i64 c = 0; c < ncell; ++c) {
+                        const i32 v = g.nodemap[c];
+                        if (v >= 0 && g.comp[v] == k) {
+                            cmap[c] = cnode[v];
+                            if (o.focal_zero && (v == job.first || v == job.second)) cmap[c] = 0.0;
+                        }
+                        if (o.log_transform) cmap[c] = cmap[c] > 0 ? std::log10(cmap[c]) : -9999.0;
+                        if (o.null_currents && gmap[c] == 0) cmap[c] = -9999.0;
+                    }
+                    if (o.cum)
+                        for (i64 c = 0; c < ncell; ++c) cum[c] += cmap[c];
+                    if (o.max)
+                        for (i64 c = 0; c < ncell; ++c) mx[c] = std::max(mx[c], cmap[c]);
+                    if (emit && (o.volt_maps || o.cur_maps))
+                        emit(ci, cj, o.volt_maps ? vmap.data() : nullptr, o.cur_maps ? cmap.data() : nullptr, ctx);
+                }
+        }
+        if (shortcut) {
+            // update_shortcut_resistances! (circuitscape.cpp:708-734)
+            const i32 anchor = positions[csub[0]].front();
+            auto Sr = [&](i32 a, i32 b) -> double& { return sres[static_cast<std::size_t>(a) * np + b]; };
+            constexpr double kInvalid = -777;
+            std::vector<std::uint8_t> check(np, 0);
+            for (i32 i = 0; i < np; ++i) check[i] = points[i] >= 0 && g.comp[points[i]] == k;
+            for (i32 x = 0; x < np; ++x) {
+                if (!check[x]) continue;
+                const double r1x = R(anchor, x);
+                if (r1x == -1) continue;
+                Sr(x, anchor) = Sr(anchor, x) = r1x;
+                for (i32 y = x; y < np; ++y) {
+                    if (!check[y]) continue;
+                    const double r12 = R(anchor, y);
+                    if (r12 == -1) continue;
+                    if (r1x != kInvalid) {
+                        Sr(anchor, y) = Sr(y, anchor) = r12;
+                        const double vx = voltmatrix[static_cast<std::size_t>(x) * np + y];
+                        const double r2x = 2 * r12 * vx + r1x - r12;
+                        if (Sr(y, x) != kInvalid) Sr(y, x) = Sr(x, y) = r2x;
+                    } else {
+                        for (i32 t = 0; t < np; ++t) Sr(x, t) = Sr(t, x) = kInvalid;
+                    }
+                }
+            }
+        }
+    }
+    if (shortcut) Rm = std::move(sres);
+    for (i32 i = 0; i < np; ++i) Rm[static_cast<std::size_t>(i) * np + i] = 0.0;
+    std::copy(Rm.begin(), Rm.end(), R_out);
+    if (o.cum && cum_out)
+        for (i64 c = 0; c < ncell; ++c) cum_out[c] = cum[c] < -9999 ? -9999 : cum[c];
+    if (o.max && max_out)
+        for (i64 c = 0; c < ncell; ++c) max_out[c] = mx[c] < -9999 ? -9999 : mx[c];
+}
+
+// ---- the adjoint of a grid circuit
+
+// dJ/dr per cell: every edge e = (a, b) adds -(sum_k (lam_a - lam_b)(v_a - v_b)) dw_e/dr_c at both ends,
+// w_e = (g_a + g_b) c_e, g = 1 / r, so dw_e/dr_c = -c_e / r_c^2.
+__global__ void k_adjoint_grad(long long nx, long long ny, int four, int npairs, const double* __restrict__ r,
+                               const double* __restrict__ V, const double* __restrict__ Lam,
+                               double* __restrict__ grad) {
+    const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long n = nx * ny;
+    if (c >= n) return;
+    const double rc = r[c];
+    if (!(rc > 0) || !isfinite(rc)) {
+        grad[c] = 0.0;
+        return;
+    }
+    const long long y = c / nx, x = c - y * nx;
+    const int nd = four ? 4 : 8;
+    const int dy8[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, dx8[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    const int dy4[4] = {0, -1, 1, 0}, dx4[4] = {-1, 0, 0, 1};
+    double acc = 0.0;
+    for (int d = 0; d < nd; ++d) {
+        const int dy = four ? dy4[d] : dy8[d], dx = four ? dx4[d] : dx8[d];
+        const long long yy = y + dy, xx = x + dx;
+        if (yy < 0 || yy >= ny || xx < 0 || xx >= nx) continue;
+        const long long q = yy * nx + xx;
+        const double rq = r[q];
+        if (!(rq > 0) || !isfinite(rq)) continue;
+        const double ce = (dy != 0 && dx != 0) ? 1.0 / 2.8284271247461903 : 0.5;
+        double s = 0.0;
+        for (int k = 0; k < npairs; ++k) {
+            const long long o = static_cast<long long>(k) * n;
+            s += (Lam[o + c] - Lam[o + q]) * (V[o + c] - V[o + q]);
+        }
+        acc += s * ce;
+    }
+    grad[c] = acc / (rc * rc);
+}
+
+}  // namespace
+
+// A grid circuit kept on the GPU for gradients: conductance 1 / r on the
+// cells with a finite r > 0 (8 or 4 neighbours, average conductance), every
+// component grounded at its last node, factorised once.
+struct GridCircuit {
+    Ctx cx;
+    RGraph g;
+    Members M;
+    std::vector<std::unique_ptr<SystemSolver>> sys;  // per component
+    std::vector<std::vector<i32>> nodes;             // unknown -> graph node, per component
+    DevBuf<double> r;                                // resistance (cells)
+    long long nx = 0, ny = 0;
+    int four = 0;
+};
+
+}  // namespace lsc
+
+namespace {
+
+using lsc::i32;
+using lsc::i64;
+
+// Solves potentials (by cell) of per-pair right-hand sides e_dst - e_src
+// (or arbitrary per-node vectors) on a GridCircuit.
+void grid_solve(lsc::GridCircuit& C, int k, const double* d_rhs_cells, double* d_out_cells) {
+    const i64 ncell = C.nx * C.ny;
+    std::vector<double> rhs(static_cast<std::size_t>(k) * ncell), out(static_cast<std::size_t>(k) * ncell, 0.0);
+    LSC_HIP(hipMemcpyAsync(rhs.data(), d_rhs_cells, rhs.size() * sizeof(double), hipMemcpyDeviceToHost, C.cx.s));
+    LSC_HIP(hipStreamSynchronize(C.cx.s));
+    for (std::size_t ci = 0; ci < C.sys.size(); ++ci) {
+        auto& S = *C.sys[ci];
+        const auto& nodes = C.nodes[ci];
+        const int nu = S.n();
+        if (nu == 0) continue;
+        std::vector<double> B(static_cast<std::size_t>(k) * nu), X(B.size());
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < nu; ++i) B[static_cast<std::size_t>(j) * nu + i] = rhs[j * ncell + C.g.node_cell[nodes[i]]];
+        S.solve(k, B.data(), X.data(), C.cx.tol, C.cx.maxit, C.cx.st);
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < nu; ++i) out[j * ncell + C.g.node_cell[nodes[i]]] = X[static_cast<std::size_t>(j) * nu + i];
+    }
+    LSC_HIP(hipMemcpyAsync(d_out_cells, out.data(), out.size() * sizeof(double), hipMemcpyHostToDevice, C.cx.s));
+    LSC_HIP(hipStreamSynchronize(C.cx.s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsc_circuit_advanced(const double* conductance, const double* source, const double* ground, int64_t nx,
+                         int64_t ny, int four_neighbors, const lsc_solve_options* options, double* current,
+                         double* voltage, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        const i64 n = nx * ny;
+        // cells with a conductance > 0 are nodes (grid.cpp:83-106)
+        std::vector<double> g(n);
+        for (i64 c = 0; c < n; ++c) g[c] = conductance[c] > 0 ? conductance[c] : 0.0;
+        lsc::RGraph G = lsc::raster_graph(g.data(), nullptr, nx, ny, false, four_neighbors != 0);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        std::vector<double> src(G.n, 0.0), gnd(G.n, 0.0);
+        for (i64 c = 0; c < n; ++c) {
+            const i32 v = G.nodemap[c];
+            if (v < 0) continue;
+            src[v] = source ? source[c] : 0.0;
+            gnd[v] = ground ? ground[c] : 0.0;
+        }
+        // a source on a ground is dropped (grid.cpp:110-123): the rmvsrc policy
+        auto sol = lsc::advanced_many(G, DG, {src}, {gnd}, {-1}, 1, cx);
+        for (i64 c = 0; c < n; ++c) {
+            const i32 v = G.nodemap[c];
+            current[c] = v >= 0 ? sol[0].current[v] : 0.0;
+            voltage[c] = v >= 0 ? sol[0].voltage[v] : std::numeric_limits<double>::quiet_NaN();
+        }
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_circuitscape_advanced(const double* gmap, const int64_t* polymap, const double* source, const double* ground,
+                              int64_t nx, int64_t ny, int avg_resistances, int four_neighbors, int policy,
+                              const lsc_solve_options* options, int32_t* nodemap, double* voltage, double* current,
+                              int64_t* n_nodes, int32_t* solved, int32_t* n_components, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (policy < 0 || policy > 3) throw std::invalid_argument("remove_src_or_gnd must be keepall, rmvsrc, rmvgnd or rmvall");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        auto sol = lsc::advanced_many(G, DG, {lsc::node_sums(G, source)}, {lsc::node_sums(G, ground)}, {-1}, policy, cx);
+        for (i64 c = 0; c < nx * ny; ++c) nodemap[c] = G.nodemap[c];
+        std::copy(sol[0].voltage.begin(), sol[0].voltage.end(), voltage);
+        std::copy(sol[0].current.begin(), sol[0].current.end(), current);
+        *n_nodes = G.n;
+        *solved = sol[0].solved;
+        *n_components = G.ncomp;
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_circuitscape_pairwise(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                              const int64_t* point_cells, int64_t npts, const uint8_t* excluded, int avg_resistances,
+                              int four_neighbors, int flags, lsc_emit_fn emit, void* ctx,
+                              const lsc_solve_options* options, double* resistances, double* cum, double* max,
+                              lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        std::vector<i32> nodes(npts);
+        for (int64_t i = 0; i < npts; ++i) {
+            const int64_t c = point_cells[i];
+            nodes[i] = c >= 0 && c < nx * ny ? G.nodemap[c] : -1;
+        }
+        lsc::PairOpts o{(flags & 1) != 0,  (flags & 2) != 0,  (flags & 4) != 0,  (flags & 8) != 0,
+                        (flags & 16) != 0, (flags & 32) != 0, (flags & 64) != 0, (flags & 128) != 0};
+        lsc::pairwise(G, DG, gmap, nodes, excluded, o, emit, ctx, resistances, cum, max, cx);
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                                    int avg_resistances, int four_neighbors, int policy, int64_t nprob,
+                                    const int64_t* sptr, const int64_t* scells, const double* svals,
+                                    const int64_t* gptr, const int64_t* gcells, const double* gvals,
+                                    const int64_t* check, const lsc_solve_options* options, int32_t* nodemap,
+                                    double* voltage, double* current, uint8_t* active, int32_t* solved,
+                                    lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (policy < 0 || policy > 3) throw std::invalid_argument("remove_src_or_gnd must be keepall, rmvsrc, rmvgnd or rmvall");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        // per-node sums in column-major cell order (python.cpp:437-450)
+        auto sums = [&](const int64_t* cells, const double* vals, int64_t a, int64_t b) {
+            std::vector<int64_t> order;
+            for (int64_t k = a; k < b; ++k) order.push_back(k);
+            std::stable_sort(order.begin(), order.end(), [&](int64_t p, int64_t q) {
+                return std::make_pair(cells[p] % nx, cells[p] / nx) < std::make_pair(cells[q] % nx, cells[q] / nx);
+            });
+            std::vector<double> s(G.n, 0.0);
+            for (int64_t k : order) {
+                if (cells[k] < 0 || cells[k] >= nx * ny) throw std::invalid_argument("source or ground cell out of range");
+                const i32 v = G.nodemap[cells[k]];
+                if (v >= 0 && vals[k] != 0) s[v] += vals[k];
+            }
+            return s;
+        };
+        std::vector<std::vector<double>> srcs, gnds;
+        std::vector<i32> only;
+        for (int64_t p = 0; p < nprob; ++p) {
+            srcs.push_back(sums(scells, svals, sptr[p], sptr[p + 1]));
+            gnds.push_back(sums(gcells, gvals, gptr[p], gptr[p + 1]));
+            i32 o = -1;
+            if (check[p] >= 0) {
+                o = G.nodemap[check[p]];
+                if (o < 0) o = -2;
+            }
+            only.push_back(o);
+        }
+        auto sols = lsc::advanced_many(G, DG, srcs, gnds, only, policy, cx);
+        const i64 n = nx * ny;
+        for (i64 c = 0; c < n; ++c) nodemap[c] = G.nodemap[c];
+        for (int64_t p = 0; p < nprob; ++p) {
+            std::vector<std::uint8_t> act(G.n, 0);
+            for (i32 v : sols[p].active) act[v] = 1;
+            for (i64 c = 0; c < n; ++c) {
+                const i32 v = G.nodemap[c];
+                voltage[p * n + c] = v >= 0 ? sols[p].voltage[v] : 0.0;
+                current[p * n + c] = v >= 0 ? sols[p].current[v] : 0.0;
+                active[p * n + c] = v >= 0 ? act[v] : 0;
+            }
+            solved[p] = sols[p].solved;
+        }
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_grid_circuit_create(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
+                            const lsc_solve_options* options, lsc_grid_circuit** out) {
+    return lsc::guarded([&] {
+        auto C = std::make_unique<lsc::GridCircuit>();
+        C->cx = lsc::make_ctx(options);
+        C->nx = nx, C->ny = ny, C->four = four_neighbors;
+        const i64 n = nx * ny;
+        std::vector<double> g(n);
+        for (i64 c = 0; c < n; ++c) g[c] = (resistance[c] > 0 && std::isfinite(resistance[c])) ? 1 / resistance[c] : 0.0;
+        C->g = lsc::raster_graph(g.data(), nullptr, nx, ny, false, four_neighbors != 0);
+        C->M = lsc::members_of(C->g);
+        for (i32 k = 0; k < C->g.ncomp; ++k) {
+            const i64 p0 = C->M.start[k], p1 = C->M.start[k + 1];
+            const i32 ref = C->M.mem[p1 - 1];
+            std::vector<i32> nodes;
+            lsc::HostMatrix A = lsc::component_matrix(C->g, C->M.mem.data() + p0, p1 - p0, nullptr,
+                                                      [&](i32 v) { return v == ref; }, nodes);
+            auto S = std::make_unique<lsc::SystemSolver>(C->cx.precision, C->cx.s);
+            S->factor(A);
+            C->sys.push_back(std::move(S));
+            C->nodes.push_back(std::move(nodes));
+        }
+        C->r.alloc(n ? n : 1);
+        if (n) C->r.upload(resistance, n, C->cx.s);
+        *out = reinterpret_cast<lsc_grid_circuit*>(C.release());
+    });
+}
+
+int lsc_grid_circuit_solve(lsc_grid_circuit* h, int k, const double* rhs_cells, double* out_cells) {
+    return lsc::guarded([&] { grid_solve(*reinterpret_cast<lsc::GridCircuit*>(h), k, rhs_cells, out_cells); });
+}
+
+int lsc_grid_circuit_adjoint(lsc_grid_circuit* h, int k, const double* dJdV_cells, const double* V_cells,
+                             double* grad_r) {
+    return lsc::guarded([&] {
+        auto& C = *reinterpret_cast<lsc::GridCircuit*>(h);
+        const i64 n = C.nx * C.ny;
+        lsc::DevBuf<double> lam(static_cast<std::size_t>(k) * n);
+        grid_solve(C, k, dJdV_cells, lam.p);
+        hipLaunchKernelGGL(lsc::k_adjoint_grad, dim3(lsc::nblk(n, 256)), dim3(256), 0, C.cx.s, C.nx, C.ny, C.four, k,
+                           C.r.p, V_cells, lam.p, grad_r);
+        LSC_HIP(hipStreamSynchronize(C.cx.s));
+    });
+}
+
+int lsc_grid_circuit_stats(lsc_grid_circuit* h, lsc_solve_stats* stats) {
+    return lsc::guarded([&] { lsc::fill_stats(reinterpret_cast<lsc::GridCircuit*>(h)->cx, stats); });
+}
+
+void lsc_grid_circuit_destroy(lsc_grid_circuit* h) { delete reinterpret_cast<lsc::GridCircuit*>(h); }
+
+int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
+                                  const int64_t* src_cells, const int64_t* gnd_cells, int64_t npairs,
+                                  const lsc_solve_options* options, double* J, double* R, double* grad,
+                                  lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        const i64 n = nx * ny;
+        lsc_grid_circuit* h = nullptr;
+        if (lsc_grid_circuit_create(resistance, nx, ny, four_neighbors, options, &h) != 0)
+            throw lsc::HipError(lsc_last_error());
+        std::unique_ptr<lsc::GridCircuit> C(reinterpret_cast<lsc::GridCircuit*>(h));
+        for (int64_t k = 0; k < npairs; ++k) {
+            if (src_cells[k] < 0 || src_cells[k] >= n || gnd_cells[k] < 0 || gnd_cells[k] >= n)
+                throw std::invalid_argument("pair cell out of range");
+            const i32 a = C->g.nodemap[src_cells[k]], b = C->g.nodemap[gnd_cells[k]];
+            if (a < 0 || b < 0 || C->g.comp[a] != C->g.comp[b])
+                throw std::invalid_argument("a pair is not in one connected component");
+        }
+        // dJ/dv of J = sum_k R_k, R_k = v_s - v_t for the RHS e_s - e_t
+        std::vector<double> rhs(static_cast<std::size_t>(npairs) * n, 0.0);
+        for (int64_t k = 0; k < npairs; ++k) {
+            rhs[k * n + src_cells[k]] += 1.0;
+            rhs[k * n + gnd_cells[k]] -= 1.0;
+        }
+        lsc::DevBuf<double> drhs(rhs.size() ? rhs.size() : 1), dV(rhs.size() ? rhs.size() : 1), dg(n ? n : 1);
+        drhs.upload(rhs.data(), rhs.size(), C->cx.s);
+        grid_solve(*C, static_cast<int>(npairs), drhs.p, dV.p);
+        std::vector<double> V(rhs.size());
+        LSC_HIP(hipMemcpyAsync(V.data(), dV.p, V.size() * sizeof(double), hipMemcpyDeviceToHost, C->cx.s));
+        LSC_HIP(hipStreamSynchronize(C->cx.s));
+        double tot = 0.0;
+        for (int64_t k = 0; k < npairs; ++k) {
+            R[k] = V[k * n + src_cells[k]] - V[k * n + gnd_cells[k]];
+            tot += R[k];
+        }
+        *J = tot;
+        // lambda_k = v_k (J is a sum of quadratic forms); the kernel takes both
+        hipLaunchKernelGGL(lsc::k_adjoint_grad, dim3(lsc::nblk(n, 256)), dim3(256), 0, C->cx.s, nx, ny,
+                           four_neighbors, static_cast<int>(npairs), C->r.p, dV.p, dV.p, dg.p);
+        LSC_HIP(hipMemcpyAsync(grad, dg.p, n * sizeof(double), hipMemcpyDeviceToHost, C->cx.s));
+        LSC_HIP(hipStreamSynchronize(C->cx.s));
+        lsc::fill_stats(C->cx, stats);
+    });
+}
+
+}  // extern "C"

@@ -69,13 +69,14 @@ struct Mfma<double> {
 // the tile column updated, in registers. Every wavefront of the team holds
 // the 16 rows of the diagonal block on lanes 0..15 and factors that block
 // redundantly, so it can solve its own 48 rows below (lanes 16..63 of wave w:
-// rows k0 + 16 + 48 w + lane - 16) without any exchange between waves.
-// Requires m - k0 - 16 <= 48 nw. Returns false when a pivot is not positive.
+// rows k0 + 16 + rowbase + 48 w + lane - 16) without any exchange between
+// waves (or workgroups: `rowbase` offsets a workgroup's rows). Requires
+// m - k0 - 16 <= rowbase + 48 nw. Returns false when a pivot is not positive.
 template <typename T>
-__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane) {
+__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase = 0) {
     const int cend = min(k0 + 16, m);
-    const int i = lane < 16 ? k0 + lane : k0 + 16 + 48 * wave + (lane - 16);
-    const bool mine = i < m && (lane >= 16 || wave == 0);
+    const int i = lane < 16 ? k0 + lane : k0 + 16 + rowbase + 48 * wave + (lane - 16);
+    const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0));
     T P[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) P[c] = (i < m && k0 + c < cend && i >= k0 + c) ? F[tix(i, k0 + c)] : T(0);

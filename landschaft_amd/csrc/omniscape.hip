@@ -83,17 +83,18 @@ int device_cus() {
 
 }  // namespace
 
-void DevPlan::upload(Plan p, hipStream_t s) {
+void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
     host = std::move(p);
     const Plan& h = host;
     const int H = static_cast<int>(h.lev_ptr.size()) - 1;
-    if (h.max_front + 1 > kBigTiles * 16)
+    if (!allow_huge && h.max_front + 1 > kBigTiles * 16)
         throw std::invalid_argument("a front of " + std::to_string(h.max_front) +
                                     " rows exceeds the LDS fronts of the batched engine (159 rows)");
     lev_small_host.assign(H, 0);
     lev_stile.assign(H, 1);
     lev_lmax.assign(H, 1);
     lev_mmax.assign(H, 1);
+    huge.clear();
     for (int lv = 0; lv < H; ++lv)
         for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
             const int t = h.order[k];
@@ -102,6 +103,10 @@ void DevPlan::upload(Plan p, hipStream_t s) {
                 ++lev_small_host[lv];
                 const int MT = static_cast<int>((m + 1 + 15) / 16);
                 lev_stile[lv] = std::max(lev_stile[lv], MT * (MT + 1) / 2);
+            }
+            if (m + 1 > kBigTiles * 16) {
+                huge.push_back(t);
+                continue;
             }
             lev_lmax[lv] = std::max(lev_lmax[lv], static_cast<int>(np * m - np * (np - 1) / 2));
             lev_mmax[lv] = std::max(lev_mmax[lv], static_cast<int>(m));
@@ -115,7 +120,7 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     }
     ibufs.clear();
     lbufs.clear();
-    ibufs.reserve(32);
+    ibufs.reserve(40);
     lbufs.reserve(8);
     auto I = [&](const std::vector<int>& v) {
         ibufs.push_back(to_device(v, s));
@@ -128,12 +133,16 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     std::vector<long long> lpos(h.lpos.begin(), h.lpos.end()), vpos(h.vpos.begin(), h.vpos.end());
     d.n = h.n, d.nd = h.nd, d.W = static_cast<int>(h.bx), d.D = static_cast<int>(h.bx / 2), d.nfront = h.nfront;
     d.H = H;
-    d.cell = I(h.cell), d.nbr = I(h.nbr), d.amap = I(h.amap);
+    d.cell = I(h.cell), d.nbr = I(h.nbr);
     d.first = I(h.first), d.npiv = I(h.npiv), d.nupd = I(h.nupd), d.upd = I(h.upd), d.cb = I(h.cb), d.ce = I(h.ce);
     d.lpos = Lg(lpos), d.uposA = Lg(uposA), d.vpos = Lg(vpos);
     d.kids = I(h.kids), d.rows = I(h.rows), d.relmap = I(h.relmap);
     d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small_host), d.order = I(h.order);
-    d.fe_ptr = I(h.fe_ptr), d.fe_row = I(h.fe_row), d.fe_col = I(h.fe_col), d.fe_p = I(h.fe_p), d.fe_d = I(h.fe_d);
+    d.fe_ptr = I(h.fe_ptr), d.fe_row = I(h.fe_row), d.fe_col = I(h.fe_col), d.fe_p = I(h.fe_p), d.fe_q = I(h.fe_q);
+    {
+        std::vector<long long> src(h.fe_src.begin(), h.fe_src.end());
+        d.fe_src = Lg(src);
+    }
     d.nent = static_cast<int>(h.fe_row.size());
     d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
 }
@@ -225,16 +234,16 @@ struct WindowSolver {
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {
                 auto kern = plan.lev_mmax[h] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
-                hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, wb.active.p,
-                                   wb.done.p, L, wb.rs.p, wb.y.p, V.p);
+                hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, P.lsize,
+                                   wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
             }
         for (int h = P.H - 1; h >= 0; --h) {
             auto kern = plan.lev_mmax[h] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
-            hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, wb.active.p,
-                               wb.done.p, L, wb.y.p, wb.z.p);
+            hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, P.lsize,
+                               wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
         }
-        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, wb.active.p, wb.done.p,
-                           wb.sc.p, wb.z.p, wb.x.p);
+        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, static_cast<long long>(P.n),
+                           wb.active.p, wb.done.p, wb.sc.p, wb.z.p, wb.x.p);
     }
 
     void solve(int nb, double tol, int maxit, IrStats& st, hipEvent_t* ev, bool timing) {

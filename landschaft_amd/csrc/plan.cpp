@@ -67,40 +67,21 @@ struct Dissection {
 
 }  // namespace
 
-Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
-    if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
-    Plan p;
-    p.bx = bx, p.by = by, p.four = four, p.nd = four ? 4 : 8;
-    Dissection d{mark, bx, leaf, {}, {}, {}, {}};
-    d.cut(0, by, 0, bx);
-    p.cell = std::move(d.cells);
-    p.n = static_cast<i32>(p.cell.size());
-    p.nfront = static_cast<i32>(d.first.size());
-    p.first = std::move(d.first);
-    p.npiv = std::move(d.npiv);
-    const i32 n = p.n, nd = p.nd;
-    p.unk_of.assign(bx * by, -1);
-    for (i32 k = 0; k < n; ++k) p.unk_of[p.cell[k]] = k;
-    const Offset* nb = four ? kNb4 : kNb8;
-    p.nbr.assign(i64(n) * nd, -1);
-    for (i32 k = 0; k < n; ++k) {
-        const i64 y = p.cell[k] / bx, x = p.cell[k] % bx;
-        for (int e = 0; e < nd; ++e) {
-            const i64 yy = y + nb[e].dy, xx = x + nb[e].dx;
-            if (yy < 0 || yy >= by || xx < 0 || xx >= bx) continue;
-            p.nbr[i64(k) * nd + e] = p.unk_of[yy * bx + xx];
-        }
-    }
-    // Fronts: update rows (pivot neighbours past the front, and the children's
-    // update rows past it), sorted; child rows mapped into the parent.
-    const i32 nf = p.nfront;
+namespace {
+
+// Fronts, levels and layouts of a plan whose unknowns (elimination order),
+// fronts (first, npiv, children) and neighbour lists (nptr / nidx: unknown ->
+// neighbouring unknowns, nsrc: the source id of each entry) are set.
+void finish_plan(Plan& p, const std::vector<std::vector<i32>>& kids, const std::vector<i64>& nptr,
+                 const std::vector<i32>& nidx, const std::vector<i64>& nsrc) {
+    const i32 n = p.n, nf = p.nfront;
     p.nupd.assign(nf, 0), p.upd.assign(nf, 0), p.cb.assign(nf, 0), p.ce.assign(nf, 0), p.height.assign(nf, 0);
-    p.amap.assign(p.nbr.size(), -1);
     std::vector<i32> stamp(n, -1), loc(n, -1), upd;
+    p.fe_ptr.assign(nf + 1, 0);
     for (i32 t = 0; t < nf; ++t) {
         p.cb[t] = static_cast<i32>(p.kids.size());
         i32 h = 0;
-        for (i32 c : d.kids[t]) {
+        for (i32 c : kids[t]) {
             p.kids.push_back(c);
             h = std::max(h, p.height[c] + 1);
         }
@@ -109,8 +90,8 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
         const i32 end = p.first[t] + p.npiv[t];
         upd.clear();
         for (i32 q = p.first[t]; q < end; ++q)
-            for (int e = 0; e < nd; ++e) {
-                const i32 r = p.nbr[i64(q) * nd + e];
+            for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) {
+                const i32 r = nidx[e];
                 if (r >= end && stamp[r] != t) stamp[r] = t, upd.push_back(r);
             }
         for (i32 ci = p.cb[t]; ci < p.ce[t]; ++ci) {
@@ -131,17 +112,25 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
             const i32 c = p.kids[ci];
             for (i32 i = 0; i < p.nupd[c]; ++i) p.relmap[p.upd[c] + i] = loc[p.rows[p.upd[c] + i]];
         }
+        // entries (q, r), r > q, of this front's pivots
         for (i32 q = p.first[t]; q < end; ++q)
-            for (int e = 0; e < nd; ++e) {
-                const i32 r = p.nbr[i64(q) * nd + e];
-                if (r > q) p.amap[i64(q) * nd + e] = loc[r];
+            for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) {
+                const i32 r = nidx[e];
+                if (r <= q) continue;
+                p.fe_row.push_back(loc[r]);
+                p.fe_col.push_back(q - p.first[t]);
+                p.fe_p.push_back(q);
+                p.fe_q.push_back(r);
+                p.fe_src.push_back(nsrc[e]);
             }
+        p.fe_ptr[t + 1] = static_cast<i32>(p.fe_row.size());
         const i32 m = p.npiv[t] + p.nupd[t];
         p.max_front = std::max(p.max_front, m);
         const double np = p.npiv[t], nu = p.nupd[t];
         p.flops += np * np * np / 3 + np * np * nu + np * nu * nu;
     }
-    // Levels by height; L laid out level by level (packed columns).
+    // Levels by height; within a level smaller fronts first (the kernels give
+    // small fronts a wavefront each and large ones a workgroup or more).
     i32 H = 0;
     for (i32 t = 0; t < nf; ++t) H = std::max(H, p.height[t] + 1);
     p.lev_ptr.assign(H + 1, 0);
@@ -152,12 +141,11 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
         std::vector<i32> fill(p.lev_ptr.begin(), p.lev_ptr.end() - 1);
         for (i32 t = 0; t < nf; ++t) p.order[fill[p.height[t]]++] = t;
     }
-    // Within a level, smaller fronts first (the kernels give small fronts a
-    // wavefront each and large ones the whole workgroup).
     for (i32 h = 0; h < H; ++h)
         std::stable_sort(p.order.begin() + p.lev_ptr[h], p.order.begin() + p.lev_ptr[h + 1], [&](i32 a, i32 b) {
             return p.npiv[a] + p.nupd[a] < p.npiv[b] + p.nupd[b];
         });
+    // L level by level, every block on 4 entries (16-byte loads).
     p.lpos.assign(nf, 0), p.upos.assign(nf, 0), p.vpos.assign(nf, 0);
     i64 l = 0, u = 0, v = 0;
     for (i32 k = 0; k < nf; ++k) {
@@ -165,27 +153,103 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
         const i64 np = p.npiv[t], m = np + p.nupd[t], nu = p.nupd[t];
         p.lpos[t] = l;
         l += np * m - np * (np - 1) / 2;
-        l = (l + 3) & ~i64(3);  // every block starts on 4 entries (16-byte loads)
+        l = (l + 3) & ~i64(3);
         p.upos[t] = u;
         u += nu * nu;
         p.vpos[t] = v;
         v += nu;
     }
     p.lsize = l, p.usize = u, p.vsize = v;
-    // The off-diagonal entries of every front, front by front.
-    p.fe_ptr.assign(nf + 1, 0);
-    for (i32 t = 0; t < nf; ++t) {
-        for (i32 q = p.first[t]; q < p.first[t] + p.npiv[t]; ++q)
-            for (int e = 0; e < nd; ++e) {
-                const i32 a = p.amap[i64(q) * nd + e];
-                if (a < 0) continue;
-                p.fe_row.push_back(a);
-                p.fe_col.push_back(q - p.first[t]);
-                p.fe_p.push_back(q);
-                p.fe_d.push_back(e);
-            }
-        p.fe_ptr[t + 1] = static_cast<i32>(p.fe_row.size());
+}
+
+}  // namespace
+
+Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
+    if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
+    Plan p;
+    p.bx = bx, p.by = by, p.four = four, p.nd = four ? 4 : 8;
+    Dissection d{mark, bx, leaf, {}, {}, {}, {}};
+    d.cut(0, by, 0, bx);
+    p.cell = std::move(d.cells);
+    p.n = static_cast<i32>(p.cell.size());
+    p.nfront = static_cast<i32>(d.first.size());
+    p.first = std::move(d.first);
+    p.npiv = std::move(d.npiv);
+    const i32 n = p.n, nd = p.nd;
+    p.unk_of.assign(bx * by, -1);
+    for (i32 k = 0; k < n; ++k) p.unk_of[p.cell[k]] = k;
+    const Offset* nb = four ? kNb4 : kNb8;
+    p.nbr.assign(i64(n) * nd, -1);
+    std::vector<i64> nptr(n + 1, 0), nsrc;
+    std::vector<i32> nidx;
+    for (i32 k = 0; k < n; ++k) {
+        const i64 y = p.cell[k] / bx, x = p.cell[k] % bx;
+        for (int e = 0; e < nd; ++e) {
+            const i64 yy = y + nb[e].dy, xx = x + nb[e].dx;
+            if (yy < 0 || yy >= by || xx < 0 || xx >= bx) continue;
+            const i32 q = p.unk_of[yy * bx + xx];
+            p.nbr[i64(k) * nd + e] = q;
+            if (q < 0) continue;
+            nidx.push_back(q);
+            nsrc.push_back(i64(k) * nd + e);
+        }
+        nptr[k + 1] = static_cast<i64>(nidx.size());
     }
+    finish_plan(p, d.kids, nptr, nidx, nsrc);
+    return p;
+}
+
+Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
+                const std::vector<i32>& adj, int leaf) {
+    if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
+    const i32 n = static_cast<i32>(cell.size());
+    Plan p;
+    p.bx = bx, p.by = by, p.nd = 0;
+    std::vector<std::uint8_t> mark(bx * by, 0);
+    std::vector<i32> node_of(bx * by, -1);
+    for (i32 c = 0; c < n; ++c)
+        if (cell[c] >= 0) {
+            if (cell[c] >= bx * by) throw std::invalid_argument("graph node outside its box");
+            mark[cell[c]] = 1;
+            node_of[cell[c]] = c;
+        }
+    Dissection d{mark.data(), bx, leaf, {}, {}, {}, {}};
+    const i32 root = d.cut(0, by, 0, bx);
+    p.cell = std::move(d.cells);
+    p.first = std::move(d.first);
+    p.npiv = std::move(d.npiv);
+    for (i32 b : p.cell) p.perm.push_back(node_of[b]);
+    const i32 first_merged = static_cast<i32>(p.perm.size());
+    for (i32 c = 0; c < n; ++c)
+        if (cell[c] < 0) {
+            p.perm.push_back(c);
+            p.cell.push_back(-1);
+        }
+    std::vector<std::vector<i32>> kids = std::move(d.kids);
+    if (static_cast<i32>(p.perm.size()) > first_merged) {
+        p.first.push_back(first_merged);
+        p.npiv.push_back(static_cast<i32>(p.perm.size()) - first_merged);
+        kids.push_back(root >= 0 ? std::vector<i32>{root} : std::vector<i32>{});
+    }
+    p.n = n;
+    p.nfront = static_cast<i32>(p.first.size());
+    p.iperm.assign(n, -1);
+    for (i32 k = 0; k < n; ++k) p.iperm[p.perm[k]] = k;
+    p.unk_of.assign(bx * by, -1);
+    for (i32 k = 0; k < n; ++k)
+        if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = k;
+    std::vector<i64> nptr(n + 1, 0), nsrc;
+    std::vector<i32> nidx;
+    for (i32 k = 0; k < n; ++k) {
+        const i32 c = p.perm[k];
+        for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
+            if (adj[e] < 0) continue;
+            nidx.push_back(p.iperm[adj[e]]);
+            nsrc.push_back(e);
+        }
+        nptr[k + 1] = static_cast<i64>(nidx.size());
+    }
+    finish_plan(p, kids, nptr, nidx, nsrc);
     return p;
 }
 

@@ -51,10 +51,13 @@ struct Plan {
     std::vector<i64> lpos, upos, vpos;
     std::vector<i32> kids, rows, relmap;
     // Off-diagonal entries assembled into each front: entries fe_ptr[t] ..
-    // fe_ptr[t+1] go to front-local (fe_row, fe_col); fe_p / fe_d name the
-    // unknown and neighbour direction of the entry (grids), fe_k the caller's
-    // entry index (graphs).
-    std::vector<i32> fe_ptr, fe_row, fe_col, fe_p, fe_d;
+    // fe_ptr[t+1] go to front-local (fe_row, fe_col); fe_p / fe_q are the
+    // unknowns of the entry, fe_src its source: p * nd + direction (grids) or
+    // the caller's adjacency entry (graphs).
+    std::vector<i32> fe_ptr, fe_row, fe_col, fe_p, fe_q;
+    std::vector<i64> fe_src;
+    // graphs: unknown -> caller's node, and back
+    std::vector<i32> perm, iperm;
     // levels: fronts of height h are order[lev_ptr[h] .. lev_ptr[h+1])
     std::vector<i32> lev_ptr, order;
     i64 lsize = 0, usize = 0, vsize = 0;
@@ -69,5 +72,15 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf = 1
 // An Omniscape window: W = 2D + 1 square, the cells within D of the centre
 // (every cell when disk is false), without the centre (the ground).
 Plan window_plan(int D, bool disk, bool four, int leaf = 16);
+
+// The plan of a graph of n nodes with geometry (the reference's
+// frontal_analyze_graph, src/cpp/circuit/system.cpp:373-440): node c is the
+// box cell cell[c] (row-major bx x by) or, when cell[c] < 0, a merged node
+// (a short-circuit region) eliminated last in one front at the root.
+// Adjacency in CSR: node c's neighbours adj[ptr[c] .. ptr[c+1]) (symmetric,
+// -1 ignored). Cell nodes that are grid neighbours may be the only edges
+// between cell nodes.
+Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
+                const std::vector<i32>& adj, int leaf = 16);
 
 }  // namespace lsc

@@ -1,0 +1,574 @@
+// The graph engine: one grounded Laplacian of any size factorised by the
+// batched multifrontal kernels (fronts of <= 159 rows in LDS) plus a
+// global-memory path for larger fronts, then solved for many right-hand
+// sides with FP64 iterative refinement against the FP64 CSR matrix.
+//
+// Large fronts (the top separators of big grids, the reference's
+// dense_front / factorize_large territory, src/cpp/circuit/system.cpp:517-846)
+// keep their 16 x 16 tiles in HBM. A panel of 16 columns is solved by as many
+// workgroups as it has row blocks of 192 (each factors the 16 x 16 diagonal
+// block redundantly, so they need no exchange), then the trailing tiles are
+// updated with MFMA, one tile per wavefront.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "system.hpp"
+#include "window_kernels.hpp"
+
+namespace lsc {
+
+namespace {
+
+void sys_reset(int* p, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+void sys_allow(const void* fn, std::size_t bytes) {
+    LSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
+unsigned nblocks(long long tasks, int per) { return static_cast<unsigned>((tasks + per - 1) / per); }
+
+// s[j] = diag^-1/2 of unknown j (elimination order).
+__global__ void k_g_scale(int n, const int* __restrict__ perm, const double* __restrict__ diag,
+                          double* __restrict__ sc) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) sc[j] = 1.0 / sqrt(diag[perm[j]]);
+}
+
+// The scaled entries of S A S in front order.
+template <typename T>
+__global__ void k_g_assemble(int nent, const int* __restrict__ fe_p, const int* __restrict__ fe_q,
+                             const long long* __restrict__ fe_src, const double* __restrict__ off,
+                             const double* __restrict__ sc, T* __restrict__ aval) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nent) aval[e] = static_cast<T>(off[fe_src[e]] * sc[fe_p[e]] * sc[fe_q[e]]);
+}
+
+// b (caller order, k x n) -> bp (elimination order) and the scaled RHS s b.
+__global__ void k_g_perm_in(int n, int k, const int* __restrict__ perm, const double* __restrict__ b,
+                            const double* __restrict__ sc, double* __restrict__ bp, double* __restrict__ rs,
+                            double* __restrict__ x) {
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= static_cast<long long>(n) * k) return;
+    const int inst = static_cast<int>(e / n), j = static_cast<int>(e - static_cast<long long>(inst) * n);
+    const double v = b[static_cast<long long>(inst) * n + perm[j]];
+    bp[e] = v;
+    rs[e] = sc[j] * v;
+    x[e] = 0.0;
+}
+
+__global__ void k_g_perm_out(int n, int k, const int* __restrict__ perm, const double* __restrict__ x,
+                             double* __restrict__ out) {
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= static_cast<long long>(n) * k) return;
+    const int inst = static_cast<int>(e / n), j = static_cast<int>(e - static_cast<long long>(inst) * n);
+    out[static_cast<long long>(inst) * n + perm[j]] = x[e];
+}
+
+// r = b - A x in FP64 from the CSR matrix; s r for the next step; per
+// right-hand side sums of r^2 and b^2.
+__global__ __launch_bounds__(256) void k_g_residual(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
+                                                    const long long* __restrict__ ptr, const int* __restrict__ adj,
+                                                    const double* __restrict__ diag, const double* __restrict__ off,
+                                                    const double* __restrict__ bp, const double* __restrict__ x,
+                                                    const double* __restrict__ sc, double* __restrict__ rs,
+                                                    const int* __restrict__ done, double* __restrict__ acc) {
+    __shared__ double red[2][4];
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long off0 = static_cast<long long>(inst) * n;
+    double rr = 0.0, bb = 0.0;
+    if (j < n) {
+        const int c = perm[j];
+        double a = diag[c] * x[off0 + j];
+        for (long long e = ptr[c]; e < ptr[c + 1]; ++e) {
+            const int q = adj[e];
+            if (q >= 0) a += off[e] * x[off0 + iperm[q]];
+        }
+        const double b = bp[off0 + j];
+        const double r = b - a;
+        rs[off0 + j] = sc[j] * r;
+        rr = r * r;
+        bb = b * b;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rr += __shfl_xor(rr, o);
+        bb += __shfl_xor(bb, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[0][wave] = rr, red[1][wave] = bb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const double b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        atomicAdd(&acc[2 * inst], a);
+        atomicAdd(&acc[2 * inst + 1], b);
+    }
+}
+
+__global__ void k_g_check(int k, double* __restrict__ acc, double* __restrict__ rnorm, double* __restrict__ rprev,
+                          int* __restrict__ iters, int* __restrict__ done, double tol, int* __restrict__ open) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k || done[i]) return;
+    const double rr = acc[2 * i], bb = acc[2 * i + 1];
+    acc[2 * i] = acc[2 * i + 1] = 0.0;
+    const double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+    const double prev = rprev[i];
+    rnorm[i] = rel;
+    rprev[i] = rel;
+    const bool stalled = iters[i] > 0 && rel > 0.5 * prev;
+    if (rel <= tol || stalled || !(rel == rel)) {
+        done[i] = 1;
+    } else {
+        iters[i] += 1;
+        atomicAdd(open, 1);
+    }
+}
+
+// ---- fronts of more than 159 rows, tiles in HBM
+
+template <typename T>
+__global__ void k_h_zero(T* __restrict__ F, long long count) {
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < count) F[e] = T(0);
+}
+
+template <typename T>
+__global__ void k_h_init(DPlan P, int t, T* __restrict__ F, const T* __restrict__ aval) {
+    const int np = P.npiv[t];
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int j = tid; j < np; j += nt) F[tix(j, j)] = T(1);
+    for (int e = P.fe_ptr[t] + tid; e < P.fe_ptr[t + 1]; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
+}
+
+template <typename T>
+__global__ void k_h_extend(DPlan P, int t, int c, T* __restrict__ F, const T* __restrict__ U) {
+    const int m = P.npiv[t] + P.nupd[t];
+    const int u = P.nupd[c], u1 = u + 1;
+    const int* rel = P.relmap + P.upd[c];
+    const T* Uc = U + P.uposA[c];
+    const long long tot = static_cast<long long>(u1) * u;
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int jj = static_cast<int>(e / u1), ii = static_cast<int>(e - static_cast<long long>(jj) * u1);
+    if (ii < jj) return;
+    F[tix(ii < u ? rel[ii] : m, rel[jj])] += Uc[e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_panel(int m, int k0, int kend, T* __restrict__ F, int* __restrict__ fail) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool ok = panel<T>(F, m, k0, kend, wave, lane, 192 * blockIdx.x);
+    if (!ok && lane == 0) atomicOr(fail, 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_trailing(int MT, int Kt, int k0, int kend, T* __restrict__ F) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Tn = MT - Kt - 1;
+    const long long ntiles = static_cast<long long>(Tn) * (Tn + 1) / 2;
+    const long long tt = static_cast<long long>(blockIdx.x) * 4 + wave;
+    if (tt >= ntiles) return;
+    int I, J;
+    tri_tile(static_cast<int>(tt), I, J);
+    tile_update<T>(F, Kt + 1 + I, Kt + 1 + J, Kt, k0, kend, lane);
+}
+
+template <typename T>
+__global__ void k_h_store(DPlan P, int t, const T* __restrict__ F, T* __restrict__ L, T* __restrict__ U) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, u1 = nu + 1;
+    const long long tid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long nt = static_cast<long long>(gridDim.x) * blockDim.x;
+    T* Lt = L + P.lpos[t];
+    for (long long e = tid; e < static_cast<long long>(np) * m; e += nt) {
+        const int j = static_cast<int>(e / m), i = static_cast<int>(e - static_cast<long long>(j) * m);
+        if (i < j) continue;
+        Lt[static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1) + (i - j)] = F[tix(i, j)];
+    }
+    T* Ut = U + P.uposA[t];
+    for (long long e = tid; e < static_cast<long long>(u1) * u1; e += nt) {
+        const int jj = static_cast<int>(e / u1), ii = static_cast<int>(e - static_cast<long long>(jj) * u1);
+        if (ii < jj) continue;
+        Ut[e] = F[tix(np + ii, np + jj)];
+    }
+}
+
+__device__ __forceinline__ long long hcol(int j, int m) {
+    return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
+}
+
+// Forward substitution of a large front for right-hand side blockIdx.y: its
+// vector in LDS, 16 pivots at a time (one wavefront solves the 16 x 16
+// triangle, the workgroup updates the rows below).
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_forward(DPlan P, int t, int n, long long vstride, const T* __restrict__ L,
+                                                   const double* __restrict__ rs, double* __restrict__ y,
+                                                   double* __restrict__ V, const int* __restrict__ done) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* fv = reinterpret_cast<double*>(smem_raw);
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const long long off = static_cast<long long>(inst) * n;
+    double* Vi = V + static_cast<long long>(inst) * vstride;
+    for (int i = threadIdx.x; i < m; i += 256) fv[i] = i < np ? rs[off + first + i] : 0.0;
+    __syncthreads();
+    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c];
+        const int* rel = P.relmap + P.upd[c];
+        for (int ii = threadIdx.x; ii < u; ii += 256) fv[rel[ii]] += Vi[P.vpos[c] + ii];
+        __syncthreads();
+    }
+    const T* Lt = L + P.lpos[t];
+    for (int j0 = 0; j0 < np; j0 += 16) {
+        const int jb = min(16, np - j0);
+        if (wave == 0) {
+            double f = lane < jb ? fv[j0 + lane] : 0.0;
+            for (int q = 0; q < jb; ++q) {
+                const double yq = bcast(f, q) / static_cast<double>(Lt[hcol(j0 + q, m)]);
+                if (lane > q && lane < jb) f -= static_cast<double>(Lt[hcol(j0 + q, m) + lane - q]) * yq;
+                if (lane == q) f = yq;
+            }
+            if (lane < jb) fv[j0 + lane] = f;
+        }
+        __syncthreads();
+        for (int i = j0 + jb + threadIdx.x; i < m; i += 256) {
+            double a = 0.0;
+            for (int q = 0; q < jb; ++q) a += static_cast<double>(Lt[hcol(j0 + q, m) + i - j0 - q]) * fv[j0 + q];
+            fv[i] -= a;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < m; i += 256) {
+        if (i < np) y[off + first + i] = fv[i];
+        else Vi[P.vpos[t] + i - np] = fv[i];
+    }
+}
+
+// Back substitution of a large front for right-hand side blockIdx.y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_backward(DPlan P, int t, int n, const T* __restrict__ L,
+                                                    const double* __restrict__ y, double* __restrict__ z,
+                                                    const int* __restrict__ done) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    double* xu = reinterpret_cast<double*>(smem_raw);
+    double* tv = xu + nu;
+    const long long off = static_cast<long long>(inst) * n;
+    const int* rows = P.rows + P.upd[t];
+    for (int ii = threadIdx.x; ii < nu; ii += 256) xu[ii] = z[off + rows[ii]];
+    __syncthreads();
+    const T* Lt = L + P.lpos[t];
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const T* col = Lt + hcol(j, m) + (np - j);
+        double a = 0.0;
+        for (int ii = 0; ii < nu; ++ii) a += static_cast<double>(col[ii]) * xu[ii];
+        tv[j] = y[off + first + j] - a;
+    }
+    __syncthreads();
+    for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
+        const int jlo = max(0, j1 - 15), jb = j1 - jlo + 1;
+        if (wave == 0) {
+            double v = lane < jb ? tv[jlo + lane] : 0.0;
+            for (int q = jb - 1; q >= 0; --q) {
+                const int j = jlo + q;
+                const double xq = bcast(v, q) / static_cast<double>(Lt[hcol(j, m)]);
+                if (lane < q) v -= static_cast<double>(Lt[hcol(jlo + lane, m) + (j - jlo - lane)]) * xq;
+                if (lane == q) v = xq;
+            }
+            if (lane < jb) tv[jlo + lane] = v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < jlo; i += 256) {
+            double a = 0.0;
+            const long long ci = hcol(i, m) - i;
+            for (int q = 0; q < jb; ++q) a += static_cast<double>(Lt[ci + jlo + q]) * tv[jlo + q];
+            tv[i] -= a;
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < np; j += 256) z[off + first + j] = tv[j];
+}
+
+}  // namespace
+
+struct SystemSolver::Impl {
+    int precision;
+    hipStream_t s;
+    DevPlan plan;
+    std::vector<std::vector<int>> lev_huge;  // huge fronts of each level
+    long long huge_tiles = 0;
+    int huge_rows = 0;
+    // matrix (caller order)
+    DevBuf<long long> ptr;
+    DevBuf<int> adj, perm, iperm;
+    DevBuf<double> diag, off, sc;
+    // factor
+    DevBuf<char> L, U, A, F;
+    DevBuf<double> rs0, y0;
+    DevBuf<int> one, zero, fail;
+    // solve scratch (grown on demand)
+    int kcap = 0;
+    DevBuf<double> bp, x, y, z, rs, V, acc, rnorm, rprev;
+    DevBuf<int> active, done, iters, open;
+
+    template <typename T>
+    void factor_t(const HostMatrix& m);
+    template <typename T>
+    void sweep_t(int k, int mode);
+    template <typename T>
+    void solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);
+};
+
+template <typename T>
+void SystemSolver::Impl::factor_t(const HostMatrix& m) {
+    Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
+    plan.upload(std::move(p), s, true);
+    const Plan& h = plan.host;
+    const DPlan& P = plan.d;
+    const int H = P.H, n = P.n;
+    lev_huge.assign(H, {});
+    huge_tiles = 0, huge_rows = 0;
+    for (int lv = 0; lv < H; ++lv)
+        for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
+            const int t = h.order[k];
+            const int ma = h.npiv[t] + h.nupd[t] + 1;
+            if (ma <= kBigTiles * 16) continue;
+            lev_huge[lv].push_back(t);
+            const long long MT = (ma + 15) / 16;
+            huge_tiles = std::max(huge_tiles, MT * (MT + 1) / 2);
+            huge_rows = std::max(huge_rows, ma);
+        }
+    std::vector<long long> ptrl(m.ptr.begin(), m.ptr.end());
+    ptr = to_device(ptrl, s);
+    adj = to_device(m.adj, s);
+    diag = to_device(m.diag, s);
+    off = to_device(m.off.empty() ? std::vector<double>{0.0} : m.off, s);
+    perm = to_device(h.perm, s);
+    iperm = to_device(h.iperm, s);
+    sc.alloc(n ? n : 1);
+    if (n) hipLaunchKernelGGL(k_g_scale, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, diag.p, sc.p);
+    L.alloc(static_cast<std::size_t>(P.lsize ? P.lsize : 1) * sizeof(T));
+    U.alloc(static_cast<std::size_t>(P.usizeA ? P.usizeA : 1) * sizeof(T));
+    A.alloc(static_cast<std::size_t>(P.nent ? P.nent : 1) * sizeof(T));
+    F.alloc(static_cast<std::size_t>(huge_tiles ? huge_tiles : 1) * 256 * sizeof(T));
+    rs0.alloc(n ? n : 1);
+    y0.alloc(n ? n : 1);
+    LSC_HIP(hipMemsetAsync(rs0.p, 0, (n ? n : 1) * sizeof(double), s));
+    one = to_device(std::vector<int>{1}, s);
+    zero = to_device(std::vector<int>{0}, s);
+    fail.alloc(1);
+    sys_reset(fail.p, 1, s);
+    T* Ap = reinterpret_cast<T*>(A.p);
+    T* Lp = reinterpret_cast<T*>(L.p);
+    T* Up = reinterpret_cast<T*>(U.p);
+    T* Fp = reinterpret_cast<T*>(F.p);
+    if (P.nent)
+        hipLaunchKernelGGL(k_g_assemble<T>, dim3(nblocks(P.nent, 256)), dim3(256), 0, s, P.nent, P.fe_p, P.fe_q,
+                           P.fe_src, off.p, sc.p, Ap);
+    int smax = 1;
+    for (int v : plan.lev_stile) smax = std::max(smax, v);
+    sys_allow(reinterpret_cast<const void*>(&k_factor_small<T>),
+              static_cast<std::size_t>(kWaves) * smax * 256 * sizeof(T));
+    sys_allow(reinterpret_cast<const void*>(&k_factor_big<T>),
+              static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T));
+    for (int lv = 0; lv < H; ++lv) {
+        const int ns = plan.lev_small_host[lv];
+        const int nbig = h.lev_ptr[lv + 1] - h.lev_ptr[lv] - ns;
+        if (ns > 0)
+            hipLaunchKernelGGL(k_factor_small<T>, dim3(nblocks(ns, kWaves)), dim3(kThreads),
+                               static_cast<std::size_t>(kWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
+                               plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
+        if (nbig > 0)
+            hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kThreads),
+                               static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T), s, P, lv,
+                               1, Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
+        for (int t : lev_huge[lv]) {
+            const int np = h.npiv[t], mm = np + h.nupd[t], ma = mm + 1;
+            const int MT = (ma + 15) / 16;
+            const long long tiles = static_cast<long long>(MT) * (MT + 1) / 2;
+            hipLaunchKernelGGL(k_h_zero<T>, dim3(nblocks(tiles * 256, 256)), dim3(256), 0, s, Fp, tiles * 256);
+            hipLaunchKernelGGL(k_h_init<T>, dim3(64), dim3(256), 0, s, P, t, Fp, Ap);
+            for (int ci = h.cb[t]; ci < h.ce[t]; ++ci) {
+                const int c = h.kids[ci];
+                const long long u = h.nupd[c];
+                if (u == 0) continue;
+                hipLaunchKernelGGL(k_h_extend<T>, dim3(nblocks((u + 1) * u, 256)), dim3(256), 0, s, P, t, c, Fp, Up);
+            }
+            for (int Kt = 0; Kt * 16 < np; ++Kt) {
+                const int k0 = Kt * 16, kend = std::min(k0 + 16, np);
+                const int below = std::max(0, ma - k0 - 16);
+                hipLaunchKernelGGL(k_h_panel<T>, dim3(std::max(1, (below + 191) / 192)), dim3(256), 0, s, ma, k0, kend,
+                                   Fp, fail.p);
+                const long long Tn = MT - Kt - 1;
+                const long long nt = Tn * (Tn + 1) / 2;
+                if (nt > 0)
+                    hipLaunchKernelGGL(k_h_trailing<T>, dim3(nblocks(nt, 4)), dim3(256), 0, s, MT, Kt, k0, kend, Fp);
+            }
+            const long long work = std::max<long long>(static_cast<long long>(np) * mm,
+                                                       static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1));
+            hipLaunchKernelGGL(k_h_store<T>, dim3(std::min<long long>(4096, nblocks(work, 256))), dim3(256), 0, s, P,
+                               t, Fp, Lp, Up);
+        }
+    }
+    int h_fail = 0;
+    LSC_HIP(hipMemcpyAsync(&h_fail, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipStreamSynchronize(s));
+    if (h_fail) throw std::runtime_error("a circuit component is not positive definite");
+    // the update matrices are no longer needed
+    U.release();
+    A.release();
+    F.release();
+}
+
+template <typename T>
+void SystemSolver::Impl::sweep_t(int k, int mode) {
+    const DPlan& P = plan.d;
+    const Plan& h = plan.host;
+    const T* Lp = reinterpret_cast<const T*>(L.p);
+    auto grid = [&](int lv) {
+        return dim3(static_cast<unsigned>(static_cast<long long>(k) * (h.lev_ptr[lv + 1] - h.lev_ptr[lv])));
+    };
+    auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
+    const std::size_t hl = static_cast<std::size_t>(huge_rows + 4) * sizeof(double);
+    if (mode == 1)
+        for (int lv = 0; lv < P.H; ++lv) {
+            auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
+            hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, rs.p, y.p,
+                               V.p);
+            for (int t : lev_huge[lv])
+                hipLaunchKernelGGL(k_h_forward<T>, dim3(1, k), dim3(256), hl, s, P, t, P.n, P.vsize, Lp, rs.p, y.p,
+                                   V.p, done.p);
+        }
+    for (int lv = P.H - 1; lv >= 0; --lv) {
+        for (int t : lev_huge[lv])
+            hipLaunchKernelGGL(k_h_backward<T>, dim3(1, k), dim3(256), hl, s, P, t, P.n, Lp, y.p, z.p, done.p);
+        auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
+        hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, y.p, z.p);
+    }
+    hipLaunchKernelGGL(k_update_x, dim3(k), dim3(kThreads), 0, s, P.n, k, 1, 0ll, active.p, done.p, sc.p, z.p, x.p);
+}
+
+template <typename T>
+void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
+    const DPlan& P = plan.d;
+    const int n = P.n;
+    if (k <= 0 || n == 0) return;
+    if (k > kcap) {
+        const std::size_t kn = static_cast<std::size_t>(k) * n;
+        bp.alloc(kn), x.alloc(kn), y.alloc(kn), z.alloc(kn), rs.alloc(kn);
+        V.alloc(static_cast<std::size_t>(k) * (P.vsize ? P.vsize : 1));
+        acc.alloc(2 * k), rnorm.alloc(k), rprev.alloc(k);
+        active.alloc(k), done.alloc(k), iters.alloc(k);
+        if (!open.p) open.alloc(1);
+        kcap = k;
+        std::vector<int> ones(k, 1);
+        active.upload(ones.data(), k, s);
+    }
+    int lmax = 1;
+    for (int v : plan.lev_lmax) lmax = std::max(lmax, v);
+    const std::size_t lds = 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);
+    sys_allow(reinterpret_cast<const void*>(&k_forward_level<T, 1>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_forward_level<T, 3>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 1>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 3>), lds);
+    if (huge_rows > 0) {
+        sys_allow(reinterpret_cast<const void*>(&k_h_forward<T>), static_cast<std::size_t>(huge_rows + 4) * 8);
+        sys_allow(reinterpret_cast<const void*>(&k_h_backward<T>), static_cast<std::size_t>(huge_rows + 4) * 8);
+    }
+    const long long kn = static_cast<long long>(k) * n;
+    sys_reset(done.p, k, s);
+    sys_reset(iters.p, k, s);
+    LSC_HIP(hipMemsetAsync(acc.p, 0, 2 * k * sizeof(double), s));
+    hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, x.p);
+    sweep_t<T>(k, 1);
+    st.solve_launches += 1;
+    for (int it = 0;; ++it) {
+        sys_reset(open.p, 1, s);
+        hipLaunchKernelGGL(k_g_residual, dim3(nblocks(n, 256), k), dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p,
+                           diag.p, off.p, bp.p, x.p, sc.p, rs.p, done.p, acc.p);
+        hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, acc.p, rnorm.p, rprev.p, iters.p,
+                           done.p, tol, open.p);
+        int h_open = 0;
+        LSC_HIP(hipMemcpyAsync(&h_open, open.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipStreamSynchronize(s));
+        if (h_open == 0 || it >= maxit) break;
+        st.solve_launches += 1;
+        sweep_t<T>(k, 1);
+    }
+    hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, x.p, d_x);
+    std::vector<int> its(k);
+    std::vector<double> rn(k);
+    LSC_HIP(hipMemcpyAsync(its.data(), iters.p, k * sizeof(int), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipMemcpyAsync(rn.data(), rnorm.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    LSC_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < k; ++i) {
+        ++st.solved;
+        st.iter_sum += its[i];
+        st.iter_max = std::max(st.iter_max, its[i]);
+        st.resid_max = std::max(st.resid_max, rn[i]);
+        st.resid_sum += rn[i];
+        if (!(rn[i] <= tol)) ++st.unconverged;
+        if (!(rn[i] <= 1e-10)) ++st.over_bar;
+    }
+}
+
+SystemSolver::SystemSolver(int precision, hipStream_t s) : impl_(new Impl) {
+    impl_->precision = precision;
+    impl_->s = s;
+}
+SystemSolver::~SystemSolver() { delete impl_; }
+
+void SystemSolver::factor(const HostMatrix& m) {
+    if (m.n <= 0) {
+        impl_->plan = DevPlan{};
+        return;
+    }
+    hipEvent_t a, b;
+    LSC_HIP(hipEventCreate(&a));
+    LSC_HIP(hipEventCreate(&b));
+    LSC_HIP(hipEventRecord(a, impl_->s));
+    if (impl_->precision == 1) impl_->factor_t<double>(m);
+    else impl_->factor_t<float>(m);
+    LSC_HIP(hipEventRecord(b, impl_->s));
+    LSC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    LSC_HIP(hipEventElapsedTime(&ms, a, b));
+    factor_ms += ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+}
+
+void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
+    if (impl_->plan.host.n == 0) return;
+    if (impl_->precision == 1) impl_->solve_t<double>(k, d_b, d_x, tol, maxit, st);
+    else impl_->solve_t<float>(k, d_b, d_x, tol, maxit, st);
+}
+
+void SystemSolver::solve(int k, const double* b, double* x, double tol, int maxit, IrStats& st) {
+    const int nn = impl_->plan.host.n;
+    if (nn == 0 || k <= 0) return;
+    const std::size_t kn = static_cast<std::size_t>(k) * nn;
+    DevBuf<double> db(kn), dx(kn);
+    db.upload(b, kn, impl_->s);
+    solve_device(k, db.p, dx.p, tol, maxit, st);
+    LSC_HIP(hipMemcpyAsync(x, dx.p, kn * sizeof(double), hipMemcpyDeviceToHost, impl_->s));
+    LSC_HIP(hipStreamSynchronize(impl_->s));
+}
+
+int SystemSolver::n() const { return impl_->plan.host.n; }
+const Plan& SystemSolver::plan() const { return impl_->plan.host; }
+
+}  // namespace lsc

@@ -1,0 +1,52 @@
+// One sparse SPD system (a grounded raster-graph Laplacian) factorised once
+// on the GPU and solved for any number of right-hand sides, with FP64
+// iterative refinement against the FP64 matrix. Host side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+#include "window.hpp"
+
+namespace lsc {
+
+// The matrix in the caller's node order: diagonal, and CSR off-diagonal
+// values (the negated edge conductances) over a symmetric adjacency.
+struct HostMatrix {
+    int n = 0;
+    i64 bx = 1, by = 1;
+    std::vector<i64> cell;   // node -> box cell (row-major bx x by), -1: merged node
+    std::vector<i64> ptr;    // CSR
+    std::vector<i32> adj;
+    std::vector<double> diag, off;
+};
+
+class SystemSolver {
+public:
+    SystemSolver(int precision, hipStream_t s);
+    ~SystemSolver();
+    SystemSolver(const SystemSolver&) = delete;
+    SystemSolver& operator=(const SystemSolver&) = delete;
+
+    // Symbolic analysis, upload and numeric factorisation of `m`.
+    void factor(const HostMatrix& m);
+    // X = A^-1 B for k right-hand sides in the caller's node order (device
+    // pointers, k x n row-major), refined to ||b - Ax|| / ||b|| <= tol.
+    void solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);
+    // The same with host arrays.
+    void solve(int k, const double* b, double* x, double tol, int maxit, IrStats& st);
+
+    int n() const;
+    const Plan& plan() const;
+    double factor_ms = 0;
+
+private:
+    struct Impl;
+    Impl* impl_;
+};
+
+}  // namespace lsc

@@ -22,12 +22,13 @@ constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 
 // Device copy of a Plan plus the per-level chunking of the solve kernels.
 struct DPlan {
     int n, nd, W, D, nfront, H;
-    const int *cell, *nbr, *amap;
+    const int *cell, *nbr;
     const int *first, *npiv, *nupd, *upd, *cb, *ce;
     const long long *lpos, *uposA, *vpos;
     const int *kids, *rows, *relmap;
     const int *lev_ptr, *lev_small, *order;
-    const int *fe_ptr, *fe_row, *fe_col, *fe_p, *fe_d;  // entries assembled into each front
+    const int *fe_ptr, *fe_row, *fe_col, *fe_p, *fe_q;  // entries assembled into each front
+    const long long* fe_src;
     int nent;
     long long lsize, usizeA, vsize;
 };
@@ -39,7 +40,8 @@ struct DevPlan {
                                                                      // largest L block, largest front
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
-    void upload(Plan p, hipStream_t s);
+    std::vector<int> huge;  // fronts of more than 159 rows (RHS row included), in level order
+    void upload(Plan p, hipStream_t s, bool allow_huge = false);
 };
 
 // Per-window arrays of a batch (window slot k of B).

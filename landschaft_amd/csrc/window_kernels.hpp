@@ -77,7 +77,7 @@ __device__ __forceinline__ void target_of(const Lattice& L, long long k, long lo
 // Conductance within D (NoData -9999) and sources within R of every window,
 // the target block zeroed at D from the clipped window's corner and the
 // block's strength shared out (reference omniscape.cpp:406-421, 495-525).
-__global__ __launch_bounds__(kThreads) void k_fill(const double* __restrict__ res, const double* __restrict__ source,
+static __global__ __launch_bounds__(kThreads) void k_fill(const double* __restrict__ res, const double* __restrict__ source,
                                                    Lattice Lt, long long k0, int B, double* __restrict__ g,
                                                    double* __restrict__ src, int* __restrict__ active,
                                                    unsigned long long* __restrict__ n_targets) {
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kThreads) void k_fill(const double* __restrict__ re
 // rows, until nothing changes. Then the Laplacian diagonal of every unknown
 // (edges to present cells in column-major neighbour order + the 10 eps
 // shift, reference system.cpp:471-485), its Jacobi scale and the RHS.
-__global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const double* __restrict__ g,
+static __global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const double* __restrict__ g,
                                                     const double* __restrict__ src, int* __restrict__ active,
                                                     unsigned char* __restrict__ present, double* __restrict__ dg,
                                                     double* __restrict__ sc, double* __restrict__ bvec,
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DPlan P, int B, const dou
     const int e = blockIdx.x * kThreads + threadIdx.x;
     if (k >= B || e >= P.nent || !active[k]) return;
     const int W = P.W, W2 = W * W, nd = P.nd;
-    const int p = P.fe_p[e], d = P.fe_d[e];
+    const int p = P.fe_p[e], d = static_cast<int>(P.fe_src[e] - static_cast<long long>(p) * nd);
     const int c = P.cell[p];
     const int y = c / W, xx = c - (c / W) * W;
     const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DPlan P, int B, const dou
     if (pw[c] && pw[c2]) {
         const double* gw = g + static_cast<long long>(k) * W2;
         const double* sw = sc + static_cast<long long>(k) * P.n;
-        v = -edge_weight(gw[c], gw[c2], ndiag(nd, d)) * sw[p] * sw[P.nbr[p * nd + d]];
+        v = -edge_weight(gw[c], gw[c2], ndiag(nd, d)) * sw[p] * sw[P.fe_q[e]];
     }
     aval[static_cast<long long>(k) * P.nent + e] = static_cast<T>(v);
 }
@@ -419,6 +419,7 @@ __global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, 
     const int k = blockIdx.x / nb;
     if (k >= B || !active[k]) return;
     const int t = P.order[base + (blockIdx.x - k * nb)];
+    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: the k_huge_* kernels
     if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
     auto bsync = [] { __syncthreads(); };
@@ -592,7 +593,8 @@ __device__ __forceinline__ void stage_block(const DPlan& P, int t, const T* Lw, 
 // pair of the level is the task of a one-wavefront workgroup, which stages
 // the front's L block (contiguous in HBM) into LDS, then substitutes.
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, const int* __restrict__ active,
+__global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, long long lstride,
+                                                      const int* __restrict__ active,
                                                       const int* __restrict__ done, const T* __restrict__ L,
                                                       const double* __restrict__ rs, double* __restrict__ y,
                                                       double* __restrict__ Vall) {
@@ -604,13 +606,15 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, con
     const int k = blockIdx.x / nf;
     if (k >= B || !active[k] || done[k]) return;
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
     const long long off = static_cast<long long>(k) * P.n;
-    stage_block<T>(P, t, L + static_cast<long long>(k) * P.lsize, Ls, lane);
+    stage_block<T>(P, t, L + static_cast<long long>(k) * lstride, Ls, lane);
     forward_front<T, R>(P, t, Ls, fv, lane, rs + off, y + off, Vall + static_cast<long long>(k) * P.vsize);
 }
 
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, const int* __restrict__ active,
+__global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, long long lstride,
+                                                       const int* __restrict__ active,
                                                        const int* __restrict__ done, const T* __restrict__ L,
                                                        const double* __restrict__ y, double* __restrict__ z) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -621,20 +625,24 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, co
     const int k = blockIdx.x / nf;
     if (k >= B || !active[k] || done[k]) return;
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_backward
     const long long off = static_cast<long long>(k) * P.n;
-    stage_block<T>(P, t, L + static_cast<long long>(k) * P.lsize, Ls, lane);
+    stage_block<T>(P, t, L + static_cast<long long>(k) * lstride, Ls, lane);
     backward_front<T, R>(P, t, Ls, xu, lane, y + off, z + off);
 }
 
 // x = s z (mode 0) or x += s z (mode 1) for the instances not done.
-__global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, const int* __restrict__ active,
-                                                       const int* __restrict__ done, const double* __restrict__ sc,
-                                                       const double* __restrict__ z, double* __restrict__ x) {
+static __global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, long long sc_stride,
+                                                              const int* __restrict__ active,
+                                                              const int* __restrict__ done,
+                                                              const double* __restrict__ sc,
+                                                              const double* __restrict__ z, double* __restrict__ x) {
     const int k = blockIdx.x;
     if (k >= B || !active[k] || done[k]) return;
     const long long off = static_cast<long long>(k) * n;
+    const double* sk = sc + static_cast<long long>(k) * sc_stride;
     for (int p = threadIdx.x; p < n; p += kThreads) {
-        const double d = sc[off + p] * z[off + p];
+        const double d = sk[p] * z[off + p];
         x[off + p] = mode == 0 ? d : x[off + p] + d;
     }
 }
@@ -643,7 +651,7 @@ __global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, c
 // shifted, system.cpp:458-487); relative residual ||r|| / ||b||; the scaled
 // residual s r for the next refinement step. A window is done when it
 // reaches `tol` or stops improving.
-__global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, const double* __restrict__ g,
+static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, const double* __restrict__ g,
                                                        const unsigned char* __restrict__ present,
                                                        const double* __restrict__ dg, const double* __restrict__ sc,
                                                        const double* __restrict__ bvec,
@@ -699,7 +707,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, const dou
 
 // Node currents of every window (reference grid.cpp:19-71), times the
 // correction factors of the clipped window (omniscape.cpp:553-571).
-__global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
+static __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
                                                        const double* __restrict__ g,
                                                        const unsigned char* __restrict__ present,
                                                        const double* __restrict__ x, const int* __restrict__ active,
@@ -783,7 +791,7 @@ __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice Lt, long
 // Cumulative map: every map cell adds the currents of the batch's windows
 // that cover it, in lattice order (reference omniscape.cpp:581-605 sums in
 // target order; any fixed order keeps runs reproducible).
-__global__ __launch_bounds__(kThreads) void k_accumulate(Lattice Lt, long long k0, int B, long long X0, long long ncol,
+static __global__ __launch_bounds__(kThreads) void k_accumulate(Lattice Lt, long long k0, int B, long long X0, long long ncol,
                                                          const int* __restrict__ active,
                                                          const double* __restrict__ cur, double* __restrict__ map) {
     const long long e = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
@@ -816,7 +824,7 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(Lattice Lt, long long k
     if (sum != 0.0) map[Y * Lt.nx + X] += sum;
 }
 
-__global__ void k_reset(int* p, int n) {
+static __global__ void k_reset(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }

@@ -1,0 +1,146 @@
+"""Circuitscape's raster modes on the GPU graph engine against the FP64 oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def backend():
+    from landschaft_amd.circuit import _backend
+
+    saved = (_backend.settings.precision, _backend.settings.tol)
+    _backend.settings.tol = 1e-14
+    yield _backend
+    _backend.settings.precision, _backend.settings.tol = saved
+
+
+def _advanced_cases():
+    z = np.load(FIXTURES / "advanced_inputs.npz")
+    return sorted({k.split("/")[0] for k in z.files})
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("name", _advanced_cases())
+def test_advanced_matches_cholmod(backend, name, precision):
+    inp = np.load(FIXTURES / "advanced_inputs.npz")
+    ref = np.load(FIXTURES / "advanced.npz")
+    backend.settings.precision = precision
+    print(f"advanced {name} {precision}", flush=True)
+    g = np.nan_to_num(inp[f"{name}/conductance"], nan=0.0)
+    s = np.nan_to_num(inp[f"{name}/source"], nan=0.0)
+    gr = np.nan_to_num(inp[f"{name}/ground"], nan=0.0)
+    r = backend.circuit_advanced(g, s, gr, bool(inp[f"{name}/four"]))
+    chol = ref[f"{name}/current_cholmod"]
+    top = chol.max()
+    np.testing.assert_array_equal(r["current"] != 0, chol != 0)
+    np.testing.assert_allclose(r["current"], chol, rtol=1e-9, atol=1e-12 * top)
+
+
+@pytest.mark.parametrize("shape,four", [((9, 11), False), ((60, 70), True), ((230, 210), False)])
+def test_advanced_random_matches_oracle(backend, oracle, shape, four):
+    # 230 x 210 has fronts beyond 159 rows: the HBM-tiled large-front path
+    rng = np.random.default_rng(sum(shape))
+    g = np.exp(rng.normal(0, 1, shape))
+    g[rng.random(shape) < 0.05] = 0.0
+    s = np.where(rng.random(shape) < 0.02, rng.uniform(0, 1, shape), 0.0)
+    gr = np.zeros(shape)
+    gr[shape[0] // 2, shape[1] // 3] = np.inf
+    gr[1, 1] = 0.5
+    backend.settings.precision = "mixed"
+    print(f"advanced random {shape} {four}", flush=True)
+    got = backend.circuit_advanced(g, s, gr, four)
+    ref = oracle.advanced(g, s, gr, four)
+    top = ref["current"].max()
+    np.testing.assert_allclose(got["current"], ref["current"], rtol=1e-9, atol=1e-12 * top)
+    np.testing.assert_allclose(got["voltage"], ref["voltage"], rtol=1e-9, atol=1e-12 * np.nanmax(np.abs(ref["voltage"])))
+
+
+@pytest.mark.parametrize("policy", ["keepall", "rmvsrc", "rmvgnd", "rmvall"])
+def test_circuitscape_advanced_polygons_and_policies(backend, oracle, policy):
+    rng = np.random.default_rng(5)
+    shape = (40, 50)
+    g = np.exp(rng.normal(0, 0.7, shape))
+    g[10:14, 5:30] = 0.0
+    poly = np.zeros(shape, dtype=np.int64)
+    poly[20:24, 20:26] = 3
+    poly[30:33, 40:45] = 7
+    src = np.zeros(shape)
+    src[2, 3], src[35, 44], src[21, 21] = 1.0, 2.0, 0.5
+    gnd = np.zeros(shape)
+    gnd[38, 2], gnd[21, 22], gnd[35, 44] = np.inf, 2.0, 1.0
+    backend.settings.precision = "mixed"
+    for avg in (False, True):
+        got = backend.circuitscape_advanced(g, poly, src, gnd, avg, False, policy)
+        ref = oracle.cs_advanced(g, poly, src, gnd, avg, False, policy)
+        np.testing.assert_array_equal(got["nodemap"], ref["nodemap"])
+        nm = got["nodemap"]
+        on = nm >= 0
+        top_v = np.abs(ref["voltage"]).max() or 1.0
+        np.testing.assert_allclose(got["voltage"][nm[on]], ref["voltage"][on], rtol=1e-9, atol=1e-12 * top_v)
+        top_c = np.abs(ref["current"]).max() or 1.0
+        np.testing.assert_allclose(got["current"][nm[on]], ref["current"][on], rtol=1e-9, atol=1e-12 * top_c)
+        assert got["solved"] == ref["solved"] and got["n_components"] == ref["n_components"]
+
+
+@pytest.mark.parametrize("maps", [False, True])
+def test_pairwise_matches_oracle(backend, oracle, maps):
+    rng = np.random.default_rng(9)
+    shape = (60, 80)
+    g = np.exp(rng.normal(0, 1, shape))
+    g[:, 40] = 0.0
+    g[30, 40] = 1.0  # one gap joins the halves
+    cells = np.array([5 * 80 + 5, 50 * 80 + 10, 20 * 80 + 70, 55 * 80 + 75, 5 * 80 + 5], dtype=np.int64)
+    ex = np.zeros((5, 5), dtype=np.uint8)
+    opts = {"cum": maps, "max": maps, "volt_maps": maps, "cur_maps": maps}
+    backend.settings.precision = "mixed"
+    got_maps, ref_maps = {}, {}
+    got = backend.circuitscape_pairwise(g, None, cells, ex, False, False, opts,
+                                        emit=lambda ci, cj, v, c: got_maps.__setitem__((ci, cj), (v, c)))
+    ref = oracle.cs_pairwise(g, None, cells, ex, False, False, opts,
+                             emit=lambda ci, cj, v, c: ref_maps.__setitem__((ci, cj), (v, c)))
+    np.testing.assert_allclose(got["resistances"], ref["resistances"], rtol=1e-9, atol=1e-12)
+    if maps:
+        for k in ("cum", "max"):
+            np.testing.assert_allclose(got[k], ref[k], rtol=1e-9, atol=1e-12 * np.abs(ref[k]).max())
+        assert got_maps.keys() == ref_maps.keys()
+        for key in ref_maps:
+            for a, b in zip(got_maps[key], ref_maps[key]):
+                np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * max(np.abs(b).max(), 1e-300))
+
+
+def test_pairwise_docstring_example(backend):
+    # the reference's circuitscape.py:621-636 (resistances on a 10 x 10 map)
+    r = np.ones((10, 10))
+    r[:, 5] = 10
+    cells = np.array([1 * 10 + 1, 8 * 10 + 1, 5 * 10 + 8], dtype=np.int64)
+    got = backend.circuitscape_pairwise(1 / r, None, cells, np.zeros((3, 3), np.uint8), False, False, {"cum": True})
+    np.testing.assert_allclose(got["resistances"][0, 1:], [0.8515, 0.8567], atol=5e-5)
+    np.testing.assert_allclose(got["resistances"][1, 2], 0.8409, atol=5e-5)
+
+
+def test_multi_advanced_matches_oracle(backend, oracle):
+    rng = np.random.default_rng(4)
+    shape = (30, 40)
+    g = np.exp(rng.normal(0, 1, shape))
+    pts = [3 * 40 + 3, 25 * 40 + 30, 10 * 40 + 35, 28 * 40 + 2]
+    problems = []
+    for i, c in enumerate(pts):  # all-to-one: point grounded, 1 A at the others
+        others = [p for p in pts if p != c]
+        problems.append({"src_cells": np.array(others), "src_vals": np.ones(len(others)),
+                         "gnd_cells": np.array([c]), "gnd_vals": np.array([np.inf]), "check_cell": c})
+    backend.settings.precision = "mixed"
+    got = backend.circuitscape_multi_advanced(g, None, False, False, problems, "rmvsrc")
+    ref = oracle.cs_multi_advanced(g, None, False, False, problems, "rmvsrc")
+    nm = got["nodemap"]
+    on = nm >= 0
+    for p, r in enumerate(got["results"]):
+        assert r["solved"] == ref["solved"][p]
+        np.testing.assert_allclose(r["voltage"][nm[on]], ref["voltage"][p][on], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(r["current"][nm[on]], ref["current"][p][on], rtol=1e-9, atol=1e-12)
+        assert set(r["active"]) == set(np.unique(nm[ref["active"][p] & on]))

@@ -177,3 +177,37 @@ def test_window_statistics(oracle):
     # The radius-50 window: 7,844 unknowns, 903 fronts, largest front 150 rows.
     st = oracle.window_stats(50)
     assert st["unknowns"] == 7844 and st["fronts"] == 903 and st["max_front"] == 150
+
+
+def test_pairwise_docstring_and_pseudo_inverse(oracle):
+    # circuitscape.py:621-636: a 10 x 10 map with a column of resistance 10
+    r = np.ones((10, 10))
+    r[:, 5] = 10
+    cells = np.array([11, 81, 58], dtype=np.int64)
+    got = oracle.cs_pairwise(1 / r, None, cells, np.zeros((3, 3), np.uint8), False, False, {"cum": True})
+    R = got["resistances"]
+    np.testing.assert_allclose([R[0, 1], R[0, 2], R[1, 2]], [0.8515, 0.8567, 0.8409], atol=5e-5)
+    np.testing.assert_allclose(R, R.T, rtol=1e-12)
+    # the full matrix against R_ij = P_ii + P_jj - 2 P_ij, P = (L + 1/n)^-1
+    g = 1 / r
+    n = g.size
+    L = np.zeros((n, n))
+    for y in range(10):
+        for x in range(10):
+            for dy, dx in ((0, 1), (1, 0), (1, 1), (1, -1)):
+                yy, xx = y + dy, x + dx
+                if 0 <= yy < 10 and 0 <= xx < 10:
+                    w = (g[y, x] + g[yy, xx]) / (2 * np.sqrt(2) if dy and dx else 2)
+                    a, b = y * 10 + x, yy * 10 + xx
+                    L[a, b] -= w
+                    L[b, a] -= w
+                    L[a, a] += w
+                    L[b, b] += w
+    P = np.linalg.inv(L + 1.0 / n)
+    for i in range(3):
+        for j in range(3):
+            s, t = cells[i], cells[j]
+            np.testing.assert_allclose(R[i, j], P[s, s] + P[t, t] - 2 * P[s, t], rtol=1e-10, atol=1e-14)
+    # the shortcut (no maps) and the explicit solves agree
+    full = oracle.cs_pairwise(1 / r, None, cells, np.zeros((3, 3), np.uint8), False, False, {"cur_maps": True})
+    np.testing.assert_allclose(full["resistances"], R, rtol=1e-11)
