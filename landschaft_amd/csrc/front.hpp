@@ -72,14 +72,19 @@ struct Mfma<double> {
 // rows k0 + 16 + rowbase + 48 w + lane - 16) without any exchange between
 // waves (or workgroups: `rowbase` offsets a workgroup's rows). Requires
 // m - k0 - 16 <= rowbase + 48 nw. Returns false when a pivot is not positive.
-template <typename T>
-__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase = 0) {
+// `sync` orders the team's loads of the diagonal block before wave 0 writes
+// it back. With `diag_out`, the factored diagonal block goes there instead
+// (16 x 16, column-major) and F keeps the unfactored one, for teams spread
+// over several workgroups that cannot be ordered inside one launch.
+template <typename T, typename Sync>
+__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase, T* diag_out, Sync sync) {
     const int cend = min(k0 + 16, m);
     const int i = lane < 16 ? k0 + lane : k0 + 16 + rowbase + 48 * wave + (lane - 16);
-    const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0));
+    const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0 && diag_out == nullptr));
     T P[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) P[c] = (i < m && k0 + c < cend && i >= k0 + c) ? F[tix(i, k0 + c)] : T(0);
+    sync();
     bool ok = true;
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
@@ -102,6 +107,10 @@ __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int row
 #pragma unroll
         for (int c = 0; c < 16; ++c)
             if (k0 + c < cend && i >= k0 + c) F[tix(i, k0 + c)] = P[c];
+    }
+    if (diag_out != nullptr && wave == 0 && lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) diag_out[(c << 4) + lane] = P[c];
     }
     return ok;
 }
@@ -148,7 +157,7 @@ __device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, in
     const int MT = (m + 15) >> 4;
     for (int Kt = 0; Kt * 16 < np; ++Kt) {
         const int k0 = Kt * 16, kend = min(k0 + 16, np);
-        const bool ok = panel<T>(F, m, k0, kend, wave, lane);
+        const bool ok = panel<T>(F, m, k0, kend, wave, lane, 0, static_cast<T*>(nullptr), sync);
         if (!ok && wave == 0 && lane == 0) *flag = 1;
         sync();
         const int Tn = MT - Kt - 1;

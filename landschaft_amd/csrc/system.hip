@@ -162,11 +162,21 @@ __global__ void k_h_extend(DPlan P, int t, int c, T* __restrict__ F, const T* __
     F[tix(ii < u ? rel[ii] : m, rel[jj])] += Uc[e];
 }
 
+// Every workgroup factors the (unfactored) diagonal block for its own rows;
+// workgroup 0 leaves the factored block in `diag`, which k_h_diag writes back
+// once the whole panel is done.
 template <typename T>
-__global__ __launch_bounds__(256) void k_h_panel(int m, int k0, int kend, T* __restrict__ F, int* __restrict__ fail) {
+__global__ __launch_bounds__(256) void k_h_panel(int m, int k0, int kend, T* __restrict__ F, T* __restrict__ diag,
+                                                 int* __restrict__ fail) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool ok = panel<T>(F, m, k0, kend, wave, lane, 192 * blockIdx.x);
+    const bool ok = panel<T>(F, m, k0, kend, wave, lane, 192 * blockIdx.x, blockIdx.x == 0 ? diag : nullptr,
+                             [] __device__() {});
     if (!ok && lane == 0) atomicOr(fail, 1);
+}
+
+template <typename T>
+__global__ void k_h_diag(int Kt, T* __restrict__ F, const T* __restrict__ diag) {
+    F[tile_index(Kt, Kt) * 256 + threadIdx.x] = diag[threadIdx.x];
 }
 
 template <typename T>
@@ -316,7 +326,7 @@ struct SystemSolver::Impl {
     DevBuf<int> adj, perm, iperm;
     DevBuf<double> diag, off, sc;
     // factor
-    DevBuf<char> L, U, A, F;
+    DevBuf<char> L, U, A, F, D;
     DevBuf<double> rs0, y0;
     DevBuf<int> one, zero, fail;
     // solve scratch (grown on demand)
@@ -364,6 +374,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
     U.alloc(static_cast<std::size_t>(P.usizeA ? P.usizeA : 1) * sizeof(T));
     A.alloc(static_cast<std::size_t>(P.nent ? P.nent : 1) * sizeof(T));
     F.alloc(static_cast<std::size_t>(huge_tiles ? huge_tiles : 1) * 256 * sizeof(T));
+    D.alloc(256 * sizeof(T));
     rs0.alloc(n ? n : 1);
     y0.alloc(n ? n : 1);
     LSC_HIP(hipMemsetAsync(rs0.p, 0, (n ? n : 1) * sizeof(double), s));
@@ -375,6 +386,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
     T* Lp = reinterpret_cast<T*>(L.p);
     T* Up = reinterpret_cast<T*>(U.p);
     T* Fp = reinterpret_cast<T*>(F.p);
+    T* Dp = reinterpret_cast<T*>(D.p);
     if (P.nent)
         hipLaunchKernelGGL(k_g_assemble<T>, dim3(nblocks(P.nent, 256)), dim3(256), 0, s, P.nent, P.fe_p, P.fe_q,
                            P.fe_src, off.p, sc.p, Ap);
@@ -411,7 +423,8 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
                 const int k0 = Kt * 16, kend = std::min(k0 + 16, np);
                 const int below = std::max(0, ma - k0 - 16);
                 hipLaunchKernelGGL(k_h_panel<T>, dim3(std::max(1, (below + 191) / 192)), dim3(256), 0, s, ma, k0, kend,
-                                   Fp, fail.p);
+                                   Fp, Dp, fail.p);
+                hipLaunchKernelGGL(k_h_diag<T>, dim3(1), dim3(256), 0, s, Kt, Fp, Dp);
                 const long long Tn = MT - Kt - 1;
                 const long long nt = Tn * (Tn + 1) / 2;
                 if (nt > 0)
@@ -431,6 +444,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
     U.release();
     A.release();
     F.release();
+    D.release();
 }
 
 template <typename T>
