@@ -52,6 +52,12 @@ def lib() -> ctypes.CDLL:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m landschaft_amd.build` "
                               "(landschaft_amd has no CPU fallback)")
+        # torch's HIP runtime first: a process that uses both then shares one
+        # runtime (torch cannot initialise the GPU after another runtime has)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(str(LIB_PATH))
         _lib.lsc_last_error.restype = ctypes.c_char_p
     return _lib
