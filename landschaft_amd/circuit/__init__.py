@@ -4,6 +4,12 @@ Mirrors ``landschaft.circuit`` (the reference's src/landschaft/circuit/__init__.
 """
 
 from ._backend import core, settings
+from .advanced import AdvancedResult, advanced
+from .circuitscape import (CircuitscapeResult, FocalPoints, IncludedPairs, circuitscape_advanced,
+                           circuitscape_one_to_all, circuitscape_pairwise, habitat_conductance)
 from .omniscape import omniscape, read_reclass_table, reclassify, source_from_resistance
 
-__all__ = ["core", "omniscape", "read_reclass_table", "reclassify", "settings", "source_from_resistance"]
+__all__ = ["AdvancedResult", "CircuitscapeResult", "FocalPoints", "IncludedPairs", "advanced",
+           "circuitscape_advanced", "circuitscape_one_to_all", "circuitscape_pairwise", "core",
+           "habitat_conductance", "omniscape", "read_reclass_table", "reclassify", "settings",
+           "source_from_resistance"]

@@ -365,60 +365,242 @@ Ctx make_ctx(const lsc_solve_options* o) {
     return c;
 }
 
-// Advanced-mode problems on one graph, each factorised on its own (the
-// reference's raster_advanced / the per-problem branch of
-// raster_advanced_many, circuitscape.cpp:393-468, 971-993).
+// corr(a, b) = sum over the neighbours e of special node a outside the
+// specials of w_ae Z_b[e], so that S = A_FF - A_FU A_UU^-1 A_UF = A_FF + corr
+// (A_UF = -w, Z = A_UU^-1 A_UF): one thread per (a, b).
+__global__ void k_schur(int nf, long long na, const long long* __restrict__ sptr, const int* __restrict__ sidx,
+                        const double* __restrict__ sw, const double* __restrict__ Z, double* __restrict__ S) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(nf) * nf) return;
+    const int a = static_cast<int>(t / nf), b = static_cast<int>(t - static_cast<long long>(a) * nf);
+    const double* Zb = Z + static_cast<long long>(b) * na;
+    double acc = 0.0;
+    for (long long k = sptr[a]; k < sptr[a + 1]; ++k) acc += sw[k] * Zb[sidx[k]];
+    S[t] = acc;
+}
+
+// B = A_UF, column f of nf: entry (f, e) = -w (Z holds B before the solves).
+__global__ void k_schur_b(long long cnt, long long na, const int* __restrict__ bf, const int* __restrict__ be,
+                          const double* __restrict__ bw, double* __restrict__ Z) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < cnt) Z[static_cast<long long>(bf[t]) * na + be[t]] = -bw[t];
+}
+
+__global__ void k_fill_d(double* __restrict__ p, long long n, double v) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// Voltages of the non-special nodes of one problem: v[nodes[e]] = -sum_f Z_f[e] y_f,
+// and the specials' own values.
+__global__ void k_schur_v(long long na, int nf, const double* __restrict__ Z, const double* __restrict__ yf,
+                          const int* __restrict__ nodes, double* __restrict__ v) {
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= na) return;
+    double t = 0.0;
+    for (int f = 0; f < nf; ++f) {
+        const double y = yf[f];
+        if (y != 0.0) t += Z[static_cast<long long>(f) * na + e] * y;
+    }
+    v[nodes[e]] = -t;
+}
+
+__global__ void k_scatter_d(int cnt, const int* __restrict__ idx, const double* __restrict__ val,
+                            double* __restrict__ v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < cnt) v[idx[t]] = val[t];
+}
+
+// Dense SPD solve A y = y in place (A row-major nu x nu, lower Cholesky).
+void dense_spd_solve(std::vector<double>& A, std::vector<double>& y, std::size_t nu) {
+    for (std::size_t j = 0; j < nu; ++j) {
+        double d = A[j * nu + j];
+        for (std::size_t q = 0; q < j; ++q) d -= A[j * nu + q] * A[j * nu + q];
+        if (!(d > 0)) throw std::runtime_error("a circuit component is not positive definite");
+        d = std::sqrt(d);
+        A[j * nu + j] = d;
+        for (std::size_t i = j + 1; i < nu; ++i) {
+            double t = A[i * nu + j];
+            for (std::size_t q = 0; q < j; ++q) t -= A[i * nu + q] * A[j * nu + q];
+            A[i * nu + j] = t / d;
+        }
+    }
+    for (std::size_t i = 0; i < nu; ++i) {
+        double t = y[i];
+        for (std::size_t q = 0; q < i; ++q) t -= A[i * nu + q] * y[q];
+        y[i] = t / A[i * nu + i];
+    }
+    for (std::size_t i = nu; i-- > 0;) {
+        double t = y[i];
+        for (std::size_t q = i + 1; q < nu; ++q) t -= A[q * nu + i] * y[q];
+        y[i] = t / A[i * nu + i];
+    }
+}
+
+// Advanced-mode problems on one graph (the reference's raster_advanced and
+// raster_advanced_many, circuitscape.cpp:393-468, 904-1083). Per component:
+// with two or more problems and at most 2048 special nodes (sources or
+// grounds of any problem), one factorisation without the specials and the
+// Schur complement on them, S = A_FF - A_FU A_UU^-1 A_UF, built from nf
+// solves; each problem then solves its free specials densely and gets the
+// rest as -Z y_F. Otherwise every problem is factorised on its own.
 std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std::vector<std::vector<double>>& srcs,
                                    const std::vector<std::vector<double>>& gnds, const std::vector<i32>& only,
-                                   int policy, Ctx& cx) {
+                                   int policy, bool allow_schur, Ctx& cx) {
     const std::size_t np = srcs.size();
     const i32 n = g.n;
     std::vector<Resolved> res(np);
+    std::vector<std::uint8_t> special(n, 0);
     std::vector<NodeSol> out(np);
     for (std::size_t p = 0; p < np; ++p) {
         res[p] = resolve(srcs[p], gnds[p], policy);
         out[p].voltage.assign(n, 0.0);
         out[p].current.assign(n, 0.0);
-        for (i32 v = 0; v < n; ++v)
+        for (i32 v = 0; v < n; ++v) {
             if (res[p].src[v] != 0 || res[p].gnd[v] != 0) out[p].active.push_back(v);
+            if (res[p].src[v] != 0 || res[p].gnd[v] != 0 || res[p].finite[v] != 0) special[v] = 1;
+        }
     }
     const Members M = members_of(g);
-    DevBuf<double> dv(n ? n : 1), dcur(n ? n : 1), dfin(n ? n : 1), scratch;
-    DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
-    std::vector<i32> nodes;
-    for (std::size_t p = 0; p < np; ++p) {
-        const Resolved& r = res[p];
-        bool uploaded = false;
+    std::vector<std::vector<std::size_t>> todo(g.ncomp);
+    for (std::size_t p = 0; p < np; ++p)
         for (i32 k = 0; k < g.ncomp; ++k) {
             if (only[p] == -2 || (only[p] >= 0 && g.comp[only[p]] != k)) continue;
             double ss = 0, sg = 0;
-            for (i64 q = M.start[k]; q < M.start[k + 1]; ++q) ss += r.src[M.mem[q]], sg += r.gnd[M.mem[q]];
-            if (!(ss != 0 && sg != 0)) continue;
-            out[p].solved = true;
-            const i32* mem = M.mem.data() + M.start[k];
-            const i64 count = M.start[k + 1] - M.start[k];
-            HostMatrix A = component_matrix(g, mem, count, r.any_finite ? r.finite.data() : nullptr,
-                                            [&](i32 v) { return r.gnd[v] == kInf; }, nodes);
-            if (A.n > 0) {
-                SystemSolver S(cx.precision, cx.s);
-                S.factor(A);
-                std::vector<double> b(A.n), xv(A.n);
-                for (int i = 0; i < A.n; ++i) b[i] = r.src[nodes[i]];
-                S.solve(1, b.data(), xv.data(), cx.tol, cx.maxit, cx.st);
-                for (int i = 0; i < A.n; ++i) out[p].voltage[nodes[i]] = xv[i];
+            for (i64 q = M.start[k]; q < M.start[k + 1]; ++q) ss += res[p].src[M.mem[q]], sg += res[p].gnd[M.mem[q]];
+            if (ss != 0 && sg != 0) {
+                todo[k].push_back(p);
+                out[p].solved = true;
             }
-            if (!uploaded) {
-                dfin.upload(r.finite.data(), n, cx.s);
-                uploaded = true;
+        }
+    DevBuf<double> dv(n ? n : 1), dcur(n ? n : 1), dfin(n ? n : 1), scratch;
+    DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
+    std::vector<i32> nodes;
+    std::vector<double> vh(n), ch(n);
+    // currents of problem p on component k from the voltages in dv; both copied out
+    auto finish = [&](std::size_t p, i32 k) {
+        const Resolved& r = res[p];
+        const i32* mem = M.mem.data() + M.start[k];
+        const i64 count = M.start[k + 1] - M.start[k];
+        if (r.any_finite) dfin.upload(r.finite.data(), n, cx.s);
+        LSC_HIP(hipMemsetAsync(dcur.p, 0, n * sizeof(double), cx.s));
+        graph_currents(G, dmem.p + M.start[k], static_cast<int>(count), dv.p, r.any_finite ? dfin.p : nullptr, dcur.p,
+                       scratch, cx.s);
+        LSC_HIP(hipMemcpyAsync(vh.data(), dv.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+        LSC_HIP(hipMemcpyAsync(ch.data(), dcur.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+        LSC_HIP(hipStreamSynchronize(cx.s));
+        for (i64 q = 0; q < count; ++q) {
+            out[p].voltage[mem[q]] = vh[mem[q]];
+            out[p].current[mem[q]] = ch[mem[q]];
+        }
+    };
+    for (i32 k = 0; k < g.ncomp; ++k) {
+        const auto& probs = todo[k];
+        if (probs.empty()) continue;
+        const i32* mem = M.mem.data() + M.start[k];
+        const i64 count = M.start[k + 1] - M.start[k];
+        std::vector<i32> F;
+        for (i64 q = 0; q < count; ++q)
+            if (special[mem[q]]) F.push_back(mem[q]);
+        const std::size_t nf = F.size();
+        if (!allow_schur || nf > 2048 || probs.size() < 2) {
+            for (std::size_t p : probs) {
+                const Resolved& r = res[p];
+                HostMatrix A = component_matrix(g, mem, count, r.any_finite ? r.finite.data() : nullptr,
+                                                [&](i32 v) { return r.gnd[v] == kInf; }, nodes);
+                std::vector<double> full(n, 0.0);
+                if (A.n > 0) {
+                    SystemSolver S(cx.precision, cx.s);
+                    S.factor(A);
+                    std::vector<double> b(A.n), xv(A.n);
+                    for (int i = 0; i < A.n; ++i) b[i] = r.src[nodes[i]];
+                    S.solve(1, b.data(), xv.data(), cx.tol, cx.maxit, cx.st);
+                    for (int i = 0; i < A.n; ++i) full[nodes[i]] = xv[i];
+                }
+                dv.upload(full.data(), n, cx.s);
+                finish(p, k);
             }
-            dv.upload(out[p].voltage.data(), n, cx.s);
-            LSC_HIP(hipMemsetAsync(dcur.p, 0, n * sizeof(double), cx.s));
-            graph_currents(G, dmem.p + M.start[k], static_cast<int>(count), dv.p, r.any_finite ? dfin.p : nullptr,
-                           dcur.p, scratch, cx.s);
-            std::vector<double> cur(n);
-            LSC_HIP(hipMemcpyAsync(cur.data(), dcur.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
-            LSC_HIP(hipStreamSynchronize(cx.s));
-            for (i64 q = 0; q < count; ++q) out[p].current[mem[q]] = cur[mem[q]];
+            continue;
+        }
+        HostMatrix A = component_matrix(g, mem, count, nullptr, [&](i32 v) { return special[v] != 0; }, nodes);
+        const long long na = A.n;
+        std::vector<i32> uidx(n, -1), fidx(n, -1);
+        for (long long e = 0; e < na; ++e) uidx[nodes[e]] = static_cast<i32>(e);
+        for (std::size_t f = 0; f < nf; ++f) fidx[F[f]] = static_cast<i32>(f);
+        // the couplings of every special to the rest, and S's special-special part
+        std::vector<long long> sptr(nf + 1, 0);
+        std::vector<int> sidx, bf;
+        std::vector<double> sw;
+        std::vector<double> S(nf * nf, 0.0);
+        for (std::size_t f = 0; f < nf; ++f) {
+            S[f * nf + f] = g.degree[F[f]];
+            for (i64 e = g.ptr[F[f]]; e < g.ptr[F[f] + 1]; ++e) {
+                const i32 t = g.adj[e];
+                if (uidx[t] >= 0) {
+                    sidx.push_back(uidx[t]);
+                    sw.push_back(g.w[e]);
+                    bf.push_back(static_cast<int>(f));
+                } else if (fidx[t] >= 0) {
+                    S[f * nf + fidx[t]] -= g.w[e];
+                }
+            }
+            sptr[f + 1] = static_cast<long long>(sidx.size());
+        }
+        DevBuf<double> Z(na > 0 ? static_cast<std::size_t>(na) * nf : 1);
+        if (na > 0) {
+            SystemSolver sys(cx.precision, cx.s);
+            sys.factor(A);
+            const long long tot = na * static_cast<long long>(nf);
+            hipLaunchKernelGGL(k_fill_d, dim3(nblk(tot, 256)), dim3(256), 0, cx.s, Z.p, tot, 0.0);
+            if (!sidx.empty()) {
+                DevBuf<int> dbf = to_device(bf, cx.s), dbe = to_device(sidx, cx.s);
+                DevBuf<double> dbw = to_device(sw, cx.s);
+                hipLaunchKernelGGL(k_schur_b, dim3(nblk(static_cast<long long>(sidx.size()), 256)), dim3(256), 0,
+                                   cx.s, static_cast<long long>(sidx.size()), na, dbf.p, dbe.p, dbw.p, Z.p);
+                // Z = A_UU^-1 B, a few right-hand sides at a time, in place
+                const int chunk = 16;
+                for (std::size_t f0 = 0; f0 < nf; f0 += chunk) {
+                    const int kc = static_cast<int>(std::min<std::size_t>(chunk, nf - f0));
+                    double* zc = Z.p + static_cast<long long>(f0) * na;
+                    sys.solve_device(kc, zc, zc, cx.tol, cx.maxit, cx.st);
+                }
+                DevBuf<long long> dsp = to_device(sptr, cx.s);
+                DevBuf<double> dS(nf * nf);
+                hipLaunchKernelGGL(k_schur, dim3(nblk(static_cast<long long>(nf) * nf, 256)), dim3(256), 0, cx.s,
+                                   static_cast<int>(nf), na, dsp.p, dbe.p, dbw.p, Z.p, dS.p);
+                std::vector<double> corr(nf * nf);
+                LSC_HIP(hipMemcpyAsync(corr.data(), dS.p, nf * nf * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                LSC_HIP(hipStreamSynchronize(cx.s));
+                for (std::size_t t = 0; t < nf * nf; ++t) S[t] += corr[t];  // - A_FU Z, A_FU = -w
+            }
+        }
+        for (std::size_t f = 0; f < nf; ++f) S[f * nf + f] += kShift;
+        DevBuf<int> dnodes = to_device(nodes.empty() ? std::vector<i32>{0} : nodes, cx.s);
+        DevBuf<int> dF = to_device(F, cx.s);
+        DevBuf<double> dy(nf), dyf(nf);
+        for (std::size_t p : probs) {
+            const Resolved& r = res[p];
+            std::vector<std::size_t> U;
+            for (std::size_t f = 0; f < nf; ++f)
+                if (r.gnd[F[f]] != kInf) U.push_back(f);
+            const std::size_t nu = U.size();
+            std::vector<double> Am(nu * nu), y(nu);
+            for (std::size_t a = 0; a < nu; ++a) {
+                for (std::size_t b = 0; b < nu; ++b) Am[a * nu + b] = S[U[a] * nf + U[b]];
+                if (r.any_finite) Am[a * nu + a] += r.finite[F[U[a]]];
+                y[a] = r.src[F[U[a]]];
+            }
+            dense_spd_solve(Am, y, nu);
+            std::vector<double> yf(nf, 0.0);
+            for (std::size_t a = 0; a < nu; ++a) yf[U[a]] = y[a];
+            dyf.upload(yf.data(), nf, cx.s);
+            hipLaunchKernelGGL(k_fill_d, dim3(nblk(n, 256)), dim3(256), 0, cx.s, dv.p, static_cast<long long>(n), 0.0);
+            if (na > 0)
+                hipLaunchKernelGGL(k_schur_v, dim3(nblk(na, 256)), dim3(256), 0, cx.s, na, static_cast<int>(nf), Z.p,
+                                   dyf.p, dnodes.p, dv.p);
+            hipLaunchKernelGGL(k_scatter_d, dim3(nblk(static_cast<long long>(nf), 256)), dim3(256), 0, cx.s,
+                               static_cast<int>(nf), dF.p, dyf.p, dv.p);
+            finish(p, k);
         }
     }
     return out;
@@ -728,7 +910,7 @@ int lsc_circuit_advanced(const double* conductance, const double* source, const 
             gnd[v] = ground ? ground[c] : 0.0;
         }
         // a source on a ground is dropped (grid.cpp:110-123): the rmvsrc policy
-        auto sol = lsc::advanced_many(G, DG, {src}, {gnd}, {-1}, 1, cx);
+        auto sol = lsc::advanced_many(G, DG, {src}, {gnd}, {-1}, 1, false, cx);
         for (i64 c = 0; c < n; ++c) {
             const i32 v = G.nodemap[c];
             current[c] = v >= 0 ? sol[0].current[v] : 0.0;
@@ -748,7 +930,8 @@ int lsc_circuitscape_advanced(const double* gmap, const int64_t* polymap, const 
         lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
         lsc::DevGraph DG;
         DG.upload(G, cx.s);
-        auto sol = lsc::advanced_many(G, DG, {lsc::node_sums(G, source)}, {lsc::node_sums(G, ground)}, {-1}, policy, cx);
+        auto sol = lsc::advanced_many(G, DG, {lsc::node_sums(G, source)}, {lsc::node_sums(G, ground)}, {-1}, policy, false,
+                                      cx);
         for (i64 c = 0; c < nx * ny; ++c) nodemap[c] = G.nodemap[c];
         std::copy(sol[0].voltage.begin(), sol[0].voltage.end(), voltage);
         std::copy(sol[0].current.begin(), sol[0].current.end(), current);
@@ -821,7 +1004,7 @@ int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, 
             }
             only.push_back(o);
         }
-        auto sols = lsc::advanced_many(G, DG, srcs, gnds, only, policy, cx);
+        auto sols = lsc::advanced_many(G, DG, srcs, gnds, only, policy, true, cx);
         const i64 n = nx * ny;
         for (i64 c = 0; c < n; ++c) nodemap[c] = G.nodemap[c];
         for (int64_t p = 0; p < nprob; ++p) {

@@ -144,3 +144,31 @@ def test_multi_advanced_matches_oracle(backend, oracle):
         np.testing.assert_allclose(r["voltage"][nm[on]], ref["voltage"][p][on], rtol=1e-9, atol=1e-12)
         np.testing.assert_allclose(r["current"][nm[on]], ref["current"][p][on], rtol=1e-9, atol=1e-12)
         assert set(r["active"]) == set(np.unique(nm[ref["active"][p] & on]))
+
+
+@pytest.mark.parametrize("policy", ["rmvgnd", "keepall"])
+def test_multi_advanced_one_to_all_finite_grounds(backend, oracle, policy):
+    # one-to-all problems with a finite ground too (the Schur path with a finite diagonal), polygons
+    rng = np.random.default_rng(12)
+    g = np.exp(rng.normal(0, 1, (45, 60)))
+    g[rng.random(g.shape) < 0.08] = 0.0
+    poly = np.zeros(g.shape, dtype=np.int64)
+    poly[30:33, 10:14] = 4
+    pts = [2 * 60 + 3, 40 * 60 + 55, 31 * 60 + 11, 44 * 60 + 2, 5 * 60 + 57]
+    problems = []
+    for c in pts:
+        others = [p for p in pts if p != c]
+        problems.append({"src_cells": np.array([c]), "src_vals": np.array([1.5]),
+                         "gnd_cells": np.array(others + [0]), "gnd_vals": np.array([np.inf] * len(others) + [0.3]),
+                         "check_cell": c})
+    backend.settings.precision = "mixed"
+    got = backend.circuitscape_multi_advanced(g, poly, False, False, problems, policy)
+    ref = oracle.cs_multi_advanced(g, poly, False, False, problems, policy, schur=False)
+    nm = got["nodemap"]
+    on = nm >= 0
+    for p, r in enumerate(got["results"]):
+        assert r["solved"] == ref["solved"][p]
+        top = np.abs(ref["voltage"][p]).max()
+        np.testing.assert_allclose(r["voltage"][nm[on]], ref["voltage"][p][on], rtol=1e-9, atol=1e-12 * top)
+        np.testing.assert_allclose(r["current"][nm[on]], ref["current"][p][on], rtol=1e-9,
+                                   atol=1e-12 * np.abs(ref["current"][p]).max())

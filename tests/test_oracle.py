@@ -211,3 +211,23 @@ def test_pairwise_docstring_and_pseudo_inverse(oracle):
     # the shortcut (no maps) and the explicit solves agree
     full = oracle.cs_pairwise(1 / r, None, cells, np.zeros((3, 3), np.uint8), False, False, {"cur_maps": True})
     np.testing.assert_allclose(full["resistances"], R, rtol=1e-11)
+
+
+def test_multi_advanced_schur_matches_per_problem_factors(oracle):
+    # the Schur complement on the special nodes against one factorisation per problem
+    rng = np.random.default_rng(12)
+    g = np.exp(rng.normal(0, 1, (25, 30)))
+    g[rng.random(g.shape) < 0.08] = 0.0
+    pts = [2 * 30 + 3, 20 * 30 + 25, 12 * 30 + 14, 23 * 30 + 2, 5 * 30 + 27]
+    problems = []
+    for c in pts:
+        others = [p for p in pts if p != c]
+        problems.append({"src_cells": np.array([c]), "src_vals": np.array([1.5]),
+                         "gnd_cells": np.array(others + [0]), "gnd_vals": np.array([np.inf] * len(others) + [0.3]),
+                         "check_cell": c})
+    for policy in ("rmvgnd", "keepall"):
+        a = oracle.cs_multi_advanced(g, None, False, False, problems, policy, schur=True)
+        b = oracle.cs_multi_advanced(g, None, False, False, problems, policy, schur=False)
+        np.testing.assert_array_equal(a["solved"], b["solved"])
+        np.testing.assert_allclose(a["voltage"], b["voltage"], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(a["current"], b["current"], rtol=1e-9, atol=1e-12)
