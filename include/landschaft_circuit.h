@@ -45,7 +45,9 @@ typedef struct lsc_solve_options {
 
 /* Per-run statistics: targets, circuits solved, refinement steps, the
  * largest and mean relative FP64 residual of the solved circuits, and (with
- * timing) milliseconds spent in each phase. */
+ * timing) milliseconds spent in each phase. The graph entry points (advanced,
+ * pairwise, multi-advanced, grid circuits) always report ms_fill (host
+ * symbolic analysis and upload), ms_factor and ms_solve (device). */
 typedef struct lsc_solve_stats {
     int64_t n_targets;
     int64_t n_solved;

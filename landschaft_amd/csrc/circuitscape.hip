@@ -356,6 +356,13 @@ struct Ctx {
     IrStats st;
 };
 
+// A solver's host analysis, factor and solve times into the call's totals.
+void tally(Ctx& cx, const SystemSolver& S) {
+    cx.st.ms_fill += S.analyze_ms;
+    cx.st.ms_factor += S.factor_ms;
+    cx.st.ms_solve += S.solve_ms;
+}
+
 Ctx make_ctx(const lsc_solve_options* o) {
     Ctx c{};
     c.precision = o ? o->precision : LSC_PRECISION_MIXED;
@@ -515,6 +522,7 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
                     std::vector<double> b(A.n), xv(A.n);
                     for (int i = 0; i < A.n; ++i) b[i] = r.src[nodes[i]];
                     S.solve(1, b.data(), xv.data(), cx.tol, cx.maxit, cx.st);
+                    tally(cx, S);
                     for (int i = 0; i < A.n; ++i) full[nodes[i]] = xv[i];
                 }
                 dv.upload(full.data(), n, cx.s);
@@ -568,6 +576,7 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
                 DevBuf<double> dS(nf * nf);
                 hipLaunchKernelGGL(k_schur, dim3(nblk(static_cast<long long>(nf) * nf, 256)), dim3(256), 0, cx.s,
                                    static_cast<int>(nf), na, dsp.p, dbe.p, dbw.p, Z.p, dS.p);
+                tally(cx, sys);
                 std::vector<double> corr(nf * nf);
                 LSC_HIP(hipMemcpyAsync(corr.data(), dS.p, nf * nf * sizeof(double), hipMemcpyDeviceToHost, cx.s));
                 LSC_HIP(hipStreamSynchronize(cx.s));
@@ -620,6 +629,9 @@ std::vector<double> node_sums(const RGraph& g, const double* cells) {
 void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
     if (!out) return;
     std::memset(out, 0, sizeof(*out));
+    out->ms_fill = cx.st.ms_fill;
+    out->ms_factor = cx.st.ms_factor;
+    out->ms_solve = cx.st.ms_solve;
     out->n_solved = cx.st.solved;
     out->n_unconverged = cx.st.unconverged;
     out->n_over_1e10 = cx.st.over_bar;
@@ -703,6 +715,7 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
             if (b != uidx.end()) B[j * A.n + b->second] = 1.0;
         }
         if (A.n > 0) S.solve(static_cast<int>(nj), B.data(), X.data(), cx.tol, cx.maxit, cx.st);
+        tally(cx, S);
         auto midx = [&](i32 v) { return static_cast<i32>(std::lower_bound(mb, mb + count, v) - mb); };
         std::vector<double> vcol(count), vnode, cnode;
         if (o.maps()) {
@@ -1068,7 +1081,12 @@ int lsc_grid_circuit_adjoint(lsc_grid_circuit* h, int k, const double* dJdV_cell
 }
 
 int lsc_grid_circuit_stats(lsc_grid_circuit* h, lsc_solve_stats* stats) {
-    return lsc::guarded([&] { lsc::fill_stats(reinterpret_cast<lsc::GridCircuit*>(h)->cx, stats); });
+    return lsc::guarded([&] {
+        auto& C = *reinterpret_cast<lsc::GridCircuit*>(h);
+        lsc::Ctx cx = C.cx;
+        for (const auto& S : C.sys) lsc::tally(cx, *S);
+        lsc::fill_stats(cx, stats);
+    });
 }
 
 void lsc_grid_circuit_destroy(lsc_grid_circuit* h) { delete reinterpret_cast<lsc::GridCircuit*>(h); }
@@ -1113,7 +1131,9 @@ int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t 
                            four_neighbors, static_cast<int>(npairs), C->r.p, dV.p, dV.p, dg.p);
         LSC_HIP(hipMemcpyAsync(grad, dg.p, n * sizeof(double), hipMemcpyDeviceToHost, C->cx.s));
         LSC_HIP(hipStreamSynchronize(C->cx.s));
-        lsc::fill_stats(C->cx, stats);
+        lsc::Ctx cx = C->cx;
+        for (const auto& S : C->sys) lsc::tally(cx, *S);
+        lsc::fill_stats(cx, stats);
     });
 }
 

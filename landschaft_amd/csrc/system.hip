@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <limits>
 #include <string>
@@ -335,7 +336,7 @@ struct SystemSolver::Impl {
     DevBuf<int> active, done, iters, open;
 
     template <typename T>
-    void factor_t(const HostMatrix& m);
+    void factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
     template <typename T>
     void sweep_t(int k, int mode);
     template <typename T>
@@ -343,7 +344,8 @@ struct SystemSolver::Impl {
 };
 
 template <typename T>
-void SystemSolver::Impl::factor_t(const HostMatrix& m) {
+void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms) {
+    const auto h0 = std::chrono::steady_clock::now();
     Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
     plan.upload(std::move(p), s, true);
     const Plan& h = plan.host;
@@ -382,6 +384,12 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
     zero = to_device(std::vector<int>{0}, s);
     fail.alloc(1);
     sys_reset(fail.p, 1, s);
+    LSC_HIP(hipStreamSynchronize(s));
+    analyze_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    hipEvent_t ev0, ev1;
+    LSC_HIP(hipEventCreate(&ev0));
+    LSC_HIP(hipEventCreate(&ev1));
+    LSC_HIP(hipEventRecord(ev0, s));
     T* Ap = reinterpret_cast<T*>(A.p);
     T* Lp = reinterpret_cast<T*>(L.p);
     T* Up = reinterpret_cast<T*>(U.p);
@@ -436,9 +444,15 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m) {
                                t, Fp, Lp, Up);
         }
     }
+    LSC_HIP(hipEventRecord(ev1, s));
     int h_fail = 0;
     LSC_HIP(hipMemcpyAsync(&h_fail, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
     LSC_HIP(hipStreamSynchronize(s));
+    float fms = 0;
+    LSC_HIP(hipEventElapsedTime(&fms, ev0, ev1));
+    factor_ms += fms;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
     if (h_fail) throw std::runtime_error("a circuit component is not positive definite");
     // the update matrices are no longer needed
     U.release();
@@ -550,25 +564,25 @@ void SystemSolver::factor(const HostMatrix& m) {
         impl_->plan = DevPlan{};
         return;
     }
-    hipEvent_t a, b;
-    LSC_HIP(hipEventCreate(&a));
-    LSC_HIP(hipEventCreate(&b));
-    LSC_HIP(hipEventRecord(a, impl_->s));
-    if (impl_->precision == 1) impl_->factor_t<double>(m);
-    else impl_->factor_t<float>(m);
-    LSC_HIP(hipEventRecord(b, impl_->s));
-    LSC_HIP(hipEventSynchronize(b));
-    float ms = 0;
-    LSC_HIP(hipEventElapsedTime(&ms, a, b));
-    factor_ms += ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (impl_->precision == 1) impl_->factor_t<double>(m, analyze_ms, factor_ms);
+    else impl_->factor_t<float>(m, analyze_ms, factor_ms);
 }
 
 void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
     if (impl_->plan.host.n == 0) return;
+    hipEvent_t a, b;
+    LSC_HIP(hipEventCreate(&a));
+    LSC_HIP(hipEventCreate(&b));
+    LSC_HIP(hipEventRecord(a, impl_->s));
     if (impl_->precision == 1) impl_->solve_t<double>(k, d_b, d_x, tol, maxit, st);
     else impl_->solve_t<float>(k, d_b, d_x, tol, maxit, st);
+    LSC_HIP(hipEventRecord(b, impl_->s));
+    LSC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    LSC_HIP(hipEventElapsedTime(&ms, a, b));
+    solve_ms += ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
 }
 
 void SystemSolver::solve(int k, const double* b, double* x, double tol, int maxit, IrStats& st) {

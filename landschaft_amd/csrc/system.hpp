@@ -42,7 +42,9 @@ public:
 
     int n() const;
     const Plan& plan() const;
-    double factor_ms = 0;
+    double analyze_ms = 0;  // host symbolic analysis and upload (wall clock)
+    double factor_ms = 0;   // numeric factorisation (device)
+    double solve_ms = 0;    // solves with refinement (device)
 
 private:
     struct Impl;
