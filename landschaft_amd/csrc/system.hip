@@ -72,63 +72,143 @@ __global__ void k_g_perm_out(int n, int k, const int* __restrict__ perm, const d
     out[static_cast<long long>(inst) * n + perm[j]] = x[e];
 }
 
-// r = b - A x in FP64 from the CSR matrix; s r for the next step; per
-// right-hand side sums of r^2 and b^2.
+// ---- preconditioned conjugate gradients in FP64, the factor as the
+// preconditioner M^-1 = S (L L^T)^-1 S. Every vector is in elimination
+// order, right-hand side blockIdx.y; dot products are summed per block and
+// then over blocks in a fixed order, so results do not depend on scheduling.
+
+// Sum of a and b over the 256 threads of a block into out[0], out[1].
+__device__ __forceinline__ void block_sum2(double a, double b, double* __restrict__ out) {
+    __shared__ double red[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[0][wave] = a, red[1][wave] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        out[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+// (A v)_j for the unknown j in elimination order, from the FP64 CSR matrix.
+__device__ __forceinline__ double spmv_row(int j, const int* __restrict__ perm, const int* __restrict__ iperm,
+                                           const long long* __restrict__ ptr, const int* __restrict__ adj,
+                                           const double* __restrict__ diag, const double* __restrict__ off,
+                                           const double* __restrict__ v) {
+    const int c = perm[j];
+    double a = diag[c] * v[j];
+    for (long long e = ptr[c]; e < ptr[c + 1]; ++e) a += off[e] * v[iperm[adj[e]]];
+    return a;
+}
+
+// r = b - A x; rs = s r (the preconditioner's input); partial r.r and b.b.
 __global__ __launch_bounds__(256) void k_g_residual(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
                                                     const long long* __restrict__ ptr, const int* __restrict__ adj,
                                                     const double* __restrict__ diag, const double* __restrict__ off,
                                                     const double* __restrict__ bp, const double* __restrict__ x,
                                                     const double* __restrict__ sc, double* __restrict__ rs,
-                                                    const int* __restrict__ done, double* __restrict__ acc) {
-    __shared__ double red[2][4];
+                                                    const int* __restrict__ done, double* __restrict__ part) {
     const int inst = blockIdx.y;
     if (done[inst]) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const long long off0 = static_cast<long long>(inst) * n;
+    const long long o = static_cast<long long>(inst) * n;
     double rr = 0.0, bb = 0.0;
     if (j < n) {
-        const int c = perm[j];
-        double a = diag[c] * x[off0 + j];
-        for (long long e = ptr[c]; e < ptr[c + 1]; ++e) {
-            const int q = adj[e];
-            if (q >= 0) a += off[e] * x[off0 + iperm[q]];
-        }
-        const double b = bp[off0 + j];
-        const double r = b - a;
-        rs[off0 + j] = sc[j] * r;
+        const double b = bp[o + j];
+        const double r = b - spmv_row(j, perm, iperm, ptr, adj, diag, off, x + o);
+        rs[o + j] = sc[j] * r;
         rr = r * r;
         bb = b * b;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        rr += __shfl_xor(rr, o);
-        bb += __shfl_xor(bb, o);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[0][wave] = rr, red[1][wave] = bb;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        const double b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-        atomicAdd(&acc[2 * inst], a);
-        atomicAdd(&acc[2 * inst + 1], b);
-    }
+    block_sum2(rr, bb, part + (static_cast<long long>(inst) * gridDim.x + blockIdx.x) * 2);
 }
 
-__global__ void k_g_check(int k, double* __restrict__ acc, double* __restrict__ rnorm, double* __restrict__ rprev,
-                          int* __restrict__ iters, int* __restrict__ done, double tol, int* __restrict__ open) {
+// q = A p; partial p.q.
+__global__ __launch_bounds__(256) void k_g_spmv(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
+                                                const long long* __restrict__ ptr, const int* __restrict__ adj,
+                                                const double* __restrict__ diag, const double* __restrict__ off,
+                                                const double* __restrict__ p, double* __restrict__ q,
+                                                const int* __restrict__ done, double* __restrict__ part) {
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long o = static_cast<long long>(inst) * n;
+    double pq = 0.0;
+    if (j < n) {
+        const double v = spmv_row(j, perm, iperm, ptr, adj, diag, off, p + o);
+        q[o + j] = v;
+        pq = p[o + j] * v;
+    }
+    block_sum2(pq, 0.0, part + (static_cast<long long>(inst) * gridDim.x + blockIdx.x) * 2);
+}
+
+// partial r.z = (s r).(L L^T)^-1 (s r), from rs and the sweep's output zz.
+__global__ __launch_bounds__(256) void k_g_dot(int n, const double* __restrict__ rs, const double* __restrict__ zz,
+                                               const int* __restrict__ done, double* __restrict__ part) {
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long o = static_cast<long long>(inst) * n;
+    const double d = j < n ? rs[o + j] * zz[o + j] : 0.0;
+    block_sum2(d, 0.0, part + (static_cast<long long>(inst) * gridDim.x + blockIdx.x) * 2);
+}
+
+// out[inst * 2 + t] = sum over the nb blocks of part, t = 0, 1, in a fixed order.
+__global__ __launch_bounds__(256) void k_g_sum(int nb, const double* __restrict__ part, const int* __restrict__ done,
+                                               double* __restrict__ out) {
+    const int inst = blockIdx.x;
+    if (done[inst]) return;
+    const double* pp = part + static_cast<long long>(inst) * nb * 2;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) a += pp[2 * i], b += pp[2 * i + 1];
+    block_sum2(a, b, out + 2 * inst);
+}
+
+// p = s zz + beta p, beta = rz_new / rz_old (0 on the first step).
+__global__ __launch_bounds__(256) void k_g_dir(int n, int first, const double* __restrict__ sc,
+                                               const double* __restrict__ zz, const double* __restrict__ rz_new,
+                                               const double* __restrict__ rz_old, double* __restrict__ p,
+                                               const int* __restrict__ done) {
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const long long o = static_cast<long long>(inst) * n;
+    const double beta = first ? 0.0 : rz_new[2 * inst] / rz_old[2 * inst];
+    p[o + j] = sc[j] * zz[o + j] + (first ? 0.0 : beta * p[o + j]);
+}
+
+// x += alpha p, alpha = rz / p.q.
+__global__ __launch_bounds__(256) void k_g_step(int n, const double* __restrict__ p, const double* __restrict__ rz,
+                                                const double* __restrict__ pq, double* __restrict__ x,
+                                                const int* __restrict__ done) {
+    const int inst = blockIdx.y;
+    if (done[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double den = pq[2 * inst];
+    const double alpha = den > 0 ? rz[2 * inst] / den : 0.0;
+    const long long o = static_cast<long long>(inst) * n;
+    x[o + j] += alpha * p[o + j];
+}
+
+// Relative residual of every open right-hand side: done at tol (or on a
+// breakdown); otherwise one more iteration.
+__global__ void k_g_check(int k, const double* __restrict__ sums, double* __restrict__ rnorm,
+                          int* __restrict__ iters, int* __restrict__ done, double tol, int first,
+                          int* __restrict__ open) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k || done[i]) return;
-    const double rr = acc[2 * i], bb = acc[2 * i + 1];
-    acc[2 * i] = acc[2 * i + 1] = 0.0;
+    const double rr = sums[2 * i], bb = sums[2 * i + 1];
     const double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
-    const double prev = rprev[i];
     rnorm[i] = rel;
-    rprev[i] = rel;
-    const bool stalled = iters[i] > 0 && rel > 0.5 * prev;
-    if (rel <= tol || stalled || !(rel == rel)) {
+    if (rel <= tol || !(rel == rel)) {
         done[i] = 1;
     } else {
-        iters[i] += 1;
+        if (!first) iters[i] += 1;
         atomicAdd(open, 1);
     }
 }
@@ -332,7 +412,7 @@ struct SystemSolver::Impl {
     DevBuf<int> one, zero, fail;
     // solve scratch (grown on demand)
     int kcap = 0;
-    DevBuf<double> bp, x, y, z, rs, V, acc, rnorm, rprev;
+    DevBuf<double> bp, x, y, z, rs, V, pv, qv, part, sums, rz, pq, rnorm;
     DevBuf<int> active, done, iters, open;
 
     template <typename T>
@@ -486,19 +566,26 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
         auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
         hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, y.p, z.p);
     }
-    hipLaunchKernelGGL(k_update_x, dim3(k), dim3(kThreads), 0, s, P.n, k, 1, 0ll, active.p, done.p, sc.p, z.p, x.p);
 }
 
+// X = A^-1 B by preconditioned conjugate gradients in FP64: the first
+// iterate is the direct solve with the factor, and every iteration applies
+// the factor once more, until ||b - A x|| / ||b|| <= tol (FP64, from the CSR
+// matrix) or `maxit` further iterations. Unlike plain refinement, this
+// converges when the FP32 factor of a badly conditioned Laplacian is only a
+// rough inverse.
 template <typename T>
 void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
     const DPlan& P = plan.d;
     const int n = P.n;
     if (k <= 0 || n == 0) return;
+    const unsigned nb = nblocks(n, 256);
     if (k > kcap) {
         const std::size_t kn = static_cast<std::size_t>(k) * n;
-        bp.alloc(kn), x.alloc(kn), y.alloc(kn), z.alloc(kn), rs.alloc(kn);
+        bp.alloc(kn), x.alloc(kn), y.alloc(kn), z.alloc(kn), rs.alloc(kn), pv.alloc(kn), qv.alloc(kn);
         V.alloc(static_cast<std::size_t>(k) * (P.vsize ? P.vsize : 1));
-        acc.alloc(2 * k), rnorm.alloc(k), rprev.alloc(k);
+        part.alloc(static_cast<std::size_t>(k) * nb * 2);
+        sums.alloc(2 * k), rz.alloc(4 * k), pq.alloc(2 * k), rnorm.alloc(k);
         active.alloc(k), done.alloc(k), iters.alloc(k);
         if (!open.p) open.alloc(1);
         kcap = k;
@@ -517,24 +604,36 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         sys_allow(reinterpret_cast<const void*>(&k_h_backward<T>), static_cast<std::size_t>(huge_rows + 4) * 8);
     }
     const long long kn = static_cast<long long>(k) * n;
+    const dim3 g2(nb, k);
     sys_reset(done.p, k, s);
     sys_reset(iters.p, k, s);
-    LSC_HIP(hipMemsetAsync(acc.p, 0, 2 * k * sizeof(double), s));
     hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, x.p);
-    sweep_t<T>(k, 1);
-    st.solve_launches += 1;
-    for (int it = 0;; ++it) {
+    auto residual_check = [&](int first) {
         sys_reset(open.p, 1, s);
-        hipLaunchKernelGGL(k_g_residual, dim3(nblocks(n, 256), k), dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p,
-                           diag.p, off.p, bp.p, x.p, sc.p, rs.p, done.p, acc.p);
-        hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, acc.p, rnorm.p, rprev.p, iters.p,
-                           done.p, tol, open.p);
+        hipLaunchKernelGGL(k_g_residual, g2, dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p, diag.p, off.p, bp.p,
+                           x.p, sc.p, rs.p, done.p, part.p);
+        hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, sums.p);
+        hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, sums.p, rnorm.p, iters.p, done.p,
+                           tol, first, open.p);
         int h_open = 0;
         LSC_HIP(hipMemcpyAsync(&h_open, open.p, sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipStreamSynchronize(s));
-        if (h_open == 0 || it >= maxit) break;
+        return h_open;
+    };
+    int h_open = residual_check(1);  // r = b: zero right-hand sides are done at once
+    for (int it = 0; h_open > 0 && it <= maxit; ++it) {
+        double* rz_new = rz.p + 2 * k * (it & 1);
+        double* rz_old = rz.p + 2 * k * ((it + 1) & 1);
+        sweep_t<T>(k, 1);  // z (scaled space) = (L L^T)^-1 s r
         st.solve_launches += 1;
-        sweep_t<T>(k, 1);
+        hipLaunchKernelGGL(k_g_dot, g2, dim3(256), 0, s, n, rs.p, z.p, done.p, part.p);
+        hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, rz_new);
+        hipLaunchKernelGGL(k_g_dir, g2, dim3(256), 0, s, n, it == 0 ? 1 : 0, sc.p, z.p, rz_new, rz_old, pv.p, done.p);
+        hipLaunchKernelGGL(k_g_spmv, g2, dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p, diag.p, off.p, pv.p, qv.p,
+                           done.p, part.p);
+        hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, pq.p);
+        hipLaunchKernelGGL(k_g_step, g2, dim3(256), 0, s, n, pv.p, rz_new, pq.p, x.p, done.p);
+        h_open = residual_check(0);
     }
     hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, x.p, d_x);
     std::vector<int> its(k);
