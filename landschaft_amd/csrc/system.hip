@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <string>
 #include <vector>
@@ -195,22 +197,48 @@ __global__ __launch_bounds__(256) void k_g_step(int n, const double* __restrict_
     x[o + j] += alpha * p[o + j];
 }
 
-// Relative residual of every open right-hand side: done at tol (or on a
-// breakdown); otherwise one more iteration.
+// Relative residual of every open right-hand side. The best iterate so far
+// is kept (keep[i] = 1: copy x); a right-hand side is done at tol, after
+// kStall iterations without a new best (the residual has reached what FP64
+// arithmetic can resolve for this matrix; beyond it CG only drifts), or on a
+// breakdown.
+constexpr int kStall = 4;
 __global__ void k_g_check(int k, const double* __restrict__ sums, double* __restrict__ rnorm,
-                          int* __restrict__ iters, int* __restrict__ done, double tol, int first,
-                          int* __restrict__ open) {
+                          int* __restrict__ iters, int* __restrict__ stall, int* __restrict__ keep,
+                          int* __restrict__ done, double tol, int first, int* __restrict__ open) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k || done[i]) return;
+    keep[i] = 0;
     const double rr = sums[2 * i], bb = sums[2 * i + 1];
     const double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
-    rnorm[i] = rel;
-    if (rel <= tol || !(rel == rel)) {
+    if (!(rel == rel)) {
+        done[i] = 1;
+        return;
+    }
+    if (first || rel < rnorm[i]) {
+        rnorm[i] = rel;
+        stall[i] = 0;
+        keep[i] = first ? 0 : 1;
+    } else {
+        stall[i] += 1;
+    }
+    if (rel <= tol || stall[i] >= kStall) {
         done[i] = 1;
     } else {
         if (!first) iters[i] += 1;
         atomicAdd(open, 1);
     }
+}
+
+// xb = x for the right-hand sides whose iterate is a new best.
+__global__ __launch_bounds__(256) void k_g_keep(int n, const int* __restrict__ keep, const double* __restrict__ x,
+                                                double* __restrict__ xb) {
+    const int inst = blockIdx.y;
+    if (!keep[inst]) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const long long o = static_cast<long long>(inst) * n;
+    xb[o + j] = x[o + j];
 }
 
 // ---- fronts of more than 159 rows, tiles in HBM
@@ -413,7 +441,8 @@ struct SystemSolver::Impl {
     // solve scratch (grown on demand)
     int kcap = 0;
     DevBuf<double> bp, x, y, z, rs, V, pv, qv, part, sums, rz, pq, rnorm;
-    DevBuf<int> active, done, iters, open;
+    DevBuf<double> xb;
+    DevBuf<int> active, done, iters, stall, keep, open;
 
     template <typename T>
     void factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
@@ -571,7 +600,8 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
 // X = A^-1 B by preconditioned conjugate gradients in FP64: the first
 // iterate is the direct solve with the factor, and every iteration applies
 // the factor once more, until ||b - A x|| / ||b|| <= tol (FP64, from the CSR
-// matrix) or `maxit` further iterations. Unlike plain refinement, this
+// matrix), `maxit` further iterations, or kStall iterations without a better
+// residual; the best iterate is returned. Unlike plain refinement, this
 // converges when the FP32 factor of a badly conditioned Laplacian is only a
 // rough inverse.
 template <typename T>
@@ -586,7 +616,8 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         V.alloc(static_cast<std::size_t>(k) * (P.vsize ? P.vsize : 1));
         part.alloc(static_cast<std::size_t>(k) * nb * 2);
         sums.alloc(2 * k), rz.alloc(4 * k), pq.alloc(2 * k), rnorm.alloc(k);
-        active.alloc(k), done.alloc(k), iters.alloc(k);
+        xb.alloc(kn);
+        active.alloc(k), done.alloc(k), iters.alloc(k), stall.alloc(k), keep.alloc(k);
         if (!open.p) open.alloc(1);
         kcap = k;
         std::vector<int> ones(k, 1);
@@ -608,17 +639,30 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     sys_reset(done.p, k, s);
     sys_reset(iters.p, k, s);
     hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, x.p);
+    hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, xb.p);
     auto residual_check = [&](int first) {
         sys_reset(open.p, 1, s);
         hipLaunchKernelGGL(k_g_residual, g2, dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p, diag.p, off.p, bp.p,
                            x.p, sc.p, rs.p, done.p, part.p);
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, sums.p);
-        hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, sums.p, rnorm.p, iters.p, done.p,
-                           tol, first, open.p);
+        hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, sums.p, rnorm.p, iters.p, stall.p,
+                           keep.p, done.p, tol, first, open.p);
+        if (!first) hipLaunchKernelGGL(k_g_keep, g2, dim3(256), 0, s, n, keep.p, x.p, xb.p);
         int h_open = 0;
         LSC_HIP(hipMemcpyAsync(&h_open, open.p, sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipStreamSynchronize(s));
         return h_open;
+    };
+    const bool trace = std::getenv("LSC_PCG_TRACE") != nullptr;
+    auto show = [&](int it, const double* rzp) {
+        if (!trace) return;
+        double hs[2], hr[2], hp[2], hn;
+        LSC_HIP(hipMemcpy(hs, sums.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        LSC_HIP(hipMemcpy(hr, rzp, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        LSC_HIP(hipMemcpy(hp, pq.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
+        LSC_HIP(hipMemcpy(&hn, rnorm.p, sizeof(double), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "pcg n=%d k=%d it=%d rel=%.3e rr=%.3e bb=%.3e rz=%.3e pq=%.3e\n", n, k, it, hn, hs[0],
+                     hs[1], hr[0], hp[0]);
     };
     int h_open = residual_check(1);  // r = b: zero right-hand sides are done at once
     for (int it = 0; h_open > 0 && it <= maxit; ++it) {
@@ -634,8 +678,9 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, pq.p);
         hipLaunchKernelGGL(k_g_step, g2, dim3(256), 0, s, n, pv.p, rz_new, pq.p, x.p, done.p);
         h_open = residual_check(0);
+        show(it, rz_new);
     }
-    hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, x.p, d_x);
+    hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, xb.p, d_x);
     std::vector<int> its(k);
     std::vector<double> rn(k);
     LSC_HIP(hipMemcpyAsync(its.data(), iters.p, k * sizeof(int), hipMemcpyDeviceToHost, s));
