@@ -241,7 +241,10 @@ __global__ __launch_bounds__(256) void k_g_keep(int n, const int* __restrict__ k
     xb[o + j] = x[o + j];
 }
 
-// ---- fronts of more than 159 rows, tiles in HBM
+// ---- fronts of more than 159 rows ("large"), tiles in HBM, every large
+// front of one tree level in the same launches: front f of the level is
+// t = ht[f], its tiles start at F + hoff[f] * 256, its diagonal-block
+// scratch at D + f * 256.
 
 template <typename T>
 __global__ void k_h_zero(T* __restrict__ F, long long count) {
@@ -250,58 +253,94 @@ __global__ void k_h_zero(T* __restrict__ F, long long count) {
 }
 
 template <typename T>
-__global__ void k_h_init(DPlan P, int t, T* __restrict__ F, const T* __restrict__ aval) {
+__global__ void k_h_init(DPlan P, const int* __restrict__ ht, const long long* __restrict__ hoff, T* __restrict__ F,
+                         const T* __restrict__ aval) {
+    const int f = blockIdx.y, t = ht[f];
+    T* Ft = F + hoff[f] * 256;
     const int np = P.npiv[t];
     const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-    for (int j = tid; j < np; j += nt) F[tix(j, j)] = T(1);
-    for (int e = P.fe_ptr[t] + tid; e < P.fe_ptr[t + 1]; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
+    for (int j = tid; j < np; j += nt) Ft[tix(j, j)] = T(1);
+    for (int e = P.fe_ptr[t] + tid; e < P.fe_ptr[t + 1]; e += nt) Ft[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
 }
 
+// Extend-add of the r-th child of every large front that has one (children
+// of one front go in separate launches, so no entry is written twice at
+// once). Workgroup b belongs to entry j of the child list with
+// cb[j] <= b < cb[j + 1].
 template <typename T>
-__global__ void k_h_extend(DPlan P, int t, int c, T* __restrict__ F, const T* __restrict__ U) {
+__global__ void k_h_extend(DPlan P, int nch, const int* __restrict__ chf, const int* __restrict__ chc,
+                           const long long* __restrict__ cb, const int* __restrict__ ht,
+                           const long long* __restrict__ hoff, T* __restrict__ F, const T* __restrict__ U) {
+    const long long b = blockIdx.x;
+    int lo = 0, hi = nch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cb[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const int f = chf[lo], c = chc[lo], t = ht[f];
     const int m = P.npiv[t] + P.nupd[t];
     const int u = P.nupd[c], u1 = u + 1;
-    const int* rel = P.relmap + P.upd[c];
-    const T* Uc = U + P.uposA[c];
-    const long long tot = static_cast<long long>(u1) * u;
-    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= tot) return;
+    const long long e = (b - cb[lo]) * blockDim.x + threadIdx.x;
+    if (e >= static_cast<long long>(u1) * u) return;
     const int jj = static_cast<int>(e / u1), ii = static_cast<int>(e - static_cast<long long>(jj) * u1);
     if (ii < jj) return;
-    F[tix(ii < u ? rel[ii] : m, rel[jj])] += Uc[e];
+    const int* rel = P.relmap + P.upd[c];
+    T* Ft = F + hoff[f] * 256;
+    Ft[tix(ii < u ? rel[ii] : m, rel[jj])] += U[P.uposA[c] + e];
 }
 
-// Every workgroup factors the (unfactored) diagonal block for its own rows;
-// workgroup 0 leaves the factored block in `diag`, which k_h_diag writes back
-// once the whole panel is done.
+// Panel step Kt of every large front still pivoting: workgroup (bx, f)
+// solves rows 192 bx.. of the panel below the diagonal block; every one
+// factors the (unfactored) diagonal block for itself, and workgroup 0 leaves
+// the factored block in D for k_h_diag, after the launch.
 template <typename T>
-__global__ __launch_bounds__(256) void k_h_panel(int m, int k0, int kend, T* __restrict__ F, T* __restrict__ diag,
-                                                 int* __restrict__ fail) {
+__global__ __launch_bounds__(256) void k_h_panel(DPlan P, int Kt, const int* __restrict__ ht,
+                                                 const long long* __restrict__ hoff, T* __restrict__ F,
+                                                 T* __restrict__ D, int* __restrict__ fail) {
+    const int f = blockIdx.y, t = ht[f];
+    const int np = P.npiv[t], ma = np + P.nupd[t] + 1;
+    const int k0 = 16 * Kt;
+    if (k0 >= np) return;
+    const int below = ma - k0 - 16;
+    if (blockIdx.x > 0 && static_cast<int>(192 * blockIdx.x) >= below) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool ok = panel<T>(F, m, k0, kend, wave, lane, 192 * blockIdx.x, blockIdx.x == 0 ? diag : nullptr,
-                             [] __device__() {});
+    const bool ok = panel<T>(F + hoff[f] * 256, ma, k0, min(k0 + 16, np), wave, lane, 192 * blockIdx.x,
+                             blockIdx.x == 0 ? D + static_cast<long long>(f) * 256 : nullptr, [] __device__() {});
     if (!ok && lane == 0) atomicOr(fail, 1);
 }
 
 template <typename T>
-__global__ void k_h_diag(int Kt, T* __restrict__ F, const T* __restrict__ diag) {
-    F[tile_index(Kt, Kt) * 256 + threadIdx.x] = diag[threadIdx.x];
+__global__ void k_h_diag(DPlan P, int Kt, const int* __restrict__ ht, const long long* __restrict__ hoff,
+                         T* __restrict__ F, const T* __restrict__ D) {
+    const int f = blockIdx.x, t = ht[f];
+    if (16 * Kt >= P.npiv[t]) return;
+    F[hoff[f] * 256 + tile_index(Kt, Kt) * 256 + threadIdx.x] = D[static_cast<long long>(f) * 256 + threadIdx.x];
 }
 
+// Trailing update after panel step Kt: one 16 x 16 tile per wavefront.
 template <typename T>
-__global__ __launch_bounds__(256) void k_h_trailing(int MT, int Kt, int k0, int kend, T* __restrict__ F) {
+__global__ __launch_bounds__(256) void k_h_trailing(DPlan P, int Kt, const int* __restrict__ ht,
+                                                    const long long* __restrict__ hoff, T* __restrict__ F) {
+    const int f = blockIdx.y, t = ht[f];
+    const int np = P.npiv[t], ma = np + P.nupd[t] + 1;
+    const int k0 = 16 * Kt;
+    if (k0 >= np) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Tn = MT - Kt - 1;
+    const int MT = (ma + 15) >> 4, Tn = MT - Kt - 1;
     const long long ntiles = static_cast<long long>(Tn) * (Tn + 1) / 2;
     const long long tt = static_cast<long long>(blockIdx.x) * 4 + wave;
     if (tt >= ntiles) return;
     int I, J;
     tri_tile(static_cast<int>(tt), I, J);
-    tile_update<T>(F, Kt + 1 + I, Kt + 1 + J, Kt, k0, kend, lane);
+    tile_update<T>(F + hoff[f] * 256, Kt + 1 + I, Kt + 1 + J, Kt, k0, min(k0 + 16, np), lane);
 }
 
 template <typename T>
-__global__ void k_h_store(DPlan P, int t, const T* __restrict__ F, T* __restrict__ L, T* __restrict__ U) {
+__global__ void k_h_store(DPlan P, const int* __restrict__ ht, const long long* __restrict__ hoff,
+                          const T* __restrict__ F, T* __restrict__ L, T* __restrict__ U) {
+    const int f = blockIdx.y, t = ht[f];
+    const T* Ft = F + hoff[f] * 256;
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, u1 = nu + 1;
     const long long tid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long nt = static_cast<long long>(gridDim.x) * blockDim.x;
@@ -309,31 +348,43 @@ __global__ void k_h_store(DPlan P, int t, const T* __restrict__ F, T* __restrict
     for (long long e = tid; e < static_cast<long long>(np) * m; e += nt) {
         const int j = static_cast<int>(e / m), i = static_cast<int>(e - static_cast<long long>(j) * m);
         if (i < j) continue;
-        Lt[static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1) + (i - j)] = F[tix(i, j)];
+        Lt[static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1) + (i - j)] = Ft[tix(i, j)];
     }
     T* Ut = U + P.uposA[t];
     for (long long e = tid; e < static_cast<long long>(u1) * u1; e += nt) {
         const int jj = static_cast<int>(e / u1), ii = static_cast<int>(e - static_cast<long long>(jj) * u1);
         if (ii < jj) continue;
-        Ut[e] = F[tix(np + ii, np + jj)];
+        Ut[e] = Ft[tix(np + ii, np + jj)];
     }
 }
 
+// Start of column j of a front's L block (column j holds rows j..m-1).
 __device__ __forceinline__ long long hcol(int j, int m) {
     return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
 }
 
-// Forward substitution of a large front for right-hand side blockIdx.y: its
-// vector in LDS, 16 pivots at a time (one wavefront solves the 16 x 16
-// triangle, the workgroup updates the rows below).
+// tri[c * 16 + r] = L(j0 + r, j0 + c) of the jb x jb diagonal block (0 above).
 template <typename T>
-__global__ __launch_bounds__(256) void k_h_forward(DPlan P, int t, int n, long long vstride, const T* __restrict__ L,
-                                                   const double* __restrict__ rs, double* __restrict__ y,
-                                                   double* __restrict__ V, const int* __restrict__ done) {
+__device__ __forceinline__ void stage_tri(const T* __restrict__ Lt, int m, int j0, int jb, double* __restrict__ tri) {
+    const int r = threadIdx.x & 15, c = threadIdx.x >> 4;
+    tri[threadIdx.x] = (r < jb && c < jb && r >= c) ? static_cast<double>(Lt[hcol(j0 + c, m) + (r - c)]) : 0.0;
+}
+
+// Forward substitution of the large fronts of one level: workgroup
+// (f, right-hand side). The vector lives in LDS; per block of 16 pivots the
+// diagonal block is staged in LDS, wavefront 0 solves it, and the workgroup
+// updates the rows below (coalesced along the columns of L).
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_forward(DPlan P, const int* __restrict__ ht, int n, long long vstride,
+                                                   const T* __restrict__ L, const double* __restrict__ rs,
+                                                   double* __restrict__ y, double* __restrict__ V,
+                                                   const int* __restrict__ done) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* fv = reinterpret_cast<double*>(smem_raw);
     const int inst = blockIdx.y;
     if (done[inst]) return;
+    const int t = ht[blockIdx.x];
+    double* tri = reinterpret_cast<double*>(smem_raw);
+    double* fv = tri + 256;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     const long long off = static_cast<long long>(inst) * n;
@@ -350,11 +401,13 @@ __global__ __launch_bounds__(256) void k_h_forward(DPlan P, int t, int n, long l
     const T* Lt = L + P.lpos[t];
     for (int j0 = 0; j0 < np; j0 += 16) {
         const int jb = min(16, np - j0);
+        stage_tri<T>(Lt, m, j0, jb, tri);
+        __syncthreads();
         if (wave == 0) {
             double f = lane < jb ? fv[j0 + lane] : 0.0;
             for (int q = 0; q < jb; ++q) {
-                const double yq = bcast(f, q) / static_cast<double>(Lt[hcol(j0 + q, m)]);
-                if (lane > q && lane < jb) f -= static_cast<double>(Lt[hcol(j0 + q, m) + lane - q]) * yq;
+                const double yq = bcast(f, q) / tri[q * 17];
+                if (lane > q && lane < jb) f -= tri[q * 16 + lane] * yq;
                 if (lane == q) f = yq;
             }
             if (lane < jb) fv[j0 + lane] = f;
@@ -373,53 +426,69 @@ __global__ __launch_bounds__(256) void k_h_forward(DPlan P, int t, int n, long l
     }
 }
 
-// Back substitution of a large front for right-hand side blockIdx.y.
+// Back substitution of the large fronts of one level, left-looking: per
+// block of 16 pivots (from the last) each column's dot product with the
+// solved part below is taken by one wavefront along the column (coalesced),
+// then wavefront 0 solves the staged diagonal block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_h_backward(DPlan P, int t, int n, const T* __restrict__ L,
-                                                    const double* __restrict__ y, double* __restrict__ z,
-                                                    const int* __restrict__ done) {
+__global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restrict__ ht, int n,
+                                                    const T* __restrict__ L, const double* __restrict__ y,
+                                                    double* __restrict__ z, const int* __restrict__ done) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int inst = blockIdx.y;
     if (done[inst]) return;
+    const int t = ht[blockIdx.x];
+    double* tri = reinterpret_cast<double*>(smem_raw);
+    double* dots = tri + 256;
+    double* fv = dots + 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    double* xu = reinterpret_cast<double*>(smem_raw);
-    double* tv = xu + nu;
     const long long off = static_cast<long long>(inst) * n;
     const int* rows = P.rows + P.upd[t];
-    for (int ii = threadIdx.x; ii < nu; ii += 256) xu[ii] = z[off + rows[ii]];
+    for (int i = threadIdx.x; i < m; i += 256) fv[i] = i < np ? y[off + first + i] : z[off + rows[i - np]];
     __syncthreads();
     const T* Lt = L + P.lpos[t];
-    for (int j = threadIdx.x; j < np; j += 256) {
-        const T* col = Lt + hcol(j, m) + (np - j);
-        double a = 0.0;
-        for (int ii = 0; ii < nu; ++ii) a += static_cast<double>(col[ii]) * xu[ii];
-        tv[j] = y[off + first + j] - a;
-    }
-    __syncthreads();
     for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
         const int jlo = max(0, j1 - 15), jb = j1 - jlo + 1;
+        for (int cc = wave; cc < jb; cc += 4) {
+            const int i = jlo + cc;
+            const long long base = hcol(i, m) - i;  // L(r, i) = Lt[base + r]
+            double a = 0.0;
+            for (int r = j1 + 1 + lane; r < m; r += 64) a += static_cast<double>(Lt[base + r]) * fv[r];
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            if (lane == 0) dots[cc] = a;
+        }
+        stage_tri<T>(Lt, m, jlo, jb, tri);
+        __syncthreads();
         if (wave == 0) {
-            double v = lane < jb ? tv[jlo + lane] : 0.0;
+            double v = lane < jb ? fv[jlo + lane] - dots[lane] : 0.0;
             for (int q = jb - 1; q >= 0; --q) {
-                const int j = jlo + q;
-                const double xq = bcast(v, q) / static_cast<double>(Lt[hcol(j, m)]);
-                if (lane < q) v -= static_cast<double>(Lt[hcol(jlo + lane, m) + (j - jlo - lane)]) * xq;
+                const double xq = bcast(v, q) / tri[q * 17];
+                if (lane < q) v -= tri[lane * 16 + q] * xq;
                 if (lane == q) v = xq;
             }
-            if (lane < jb) tv[jlo + lane] = v;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < jlo; i += 256) {
-            double a = 0.0;
-            const long long ci = hcol(i, m) - i;
-            for (int q = 0; q < jb; ++q) a += static_cast<double>(Lt[ci + jlo + q]) * tv[jlo + q];
-            tv[i] -= a;
+            if (lane < jb) fv[jlo + lane] = v;
         }
         __syncthreads();
     }
-    for (int j = threadIdx.x; j < np; j += 256) z[off + first + j] = tv[j];
+    for (int j = threadIdx.x; j < np; j += 256) z[off + first + j] = fv[j];
 }
+
+// The large fronts of one level, their tile arena and child lists.
+struct HugeLevel {
+    std::vector<int> t;                 // fronts
+    std::vector<long long> off;         // tile offsets in the arena
+    long long tiles = 0;                // arena size of the level
+    int maxKt = 0;                      // panel steps of the level's largest front
+    std::vector<int> rb, tb;            // per panel step: largest row-block and tile-block grids
+    std::vector<std::vector<int>> chf, chc;  // per child rank: (front index, child)
+    std::vector<std::vector<long long>> cbk; // per child rank: workgroup prefix
+    long long store_blocks = 1;
+    DevBuf<int> dt;
+    DevBuf<long long> doff;
+    std::vector<DevBuf<int>> dchf, dchc;
+    std::vector<DevBuf<long long>> dcb;
+};
 
 }  // namespace
 
@@ -427,9 +496,9 @@ struct SystemSolver::Impl {
     int precision;
     hipStream_t s;
     DevPlan plan;
-    std::vector<std::vector<int>> lev_huge;  // huge fronts of each level
-    long long huge_tiles = 0;
-    int huge_rows = 0;
+    std::vector<HugeLevel> lev_huge;  // large fronts of each level
+    long long huge_tiles = 0;          // largest arena of a level
+    int huge_rows = 0;                 // largest large front (rows, RHS row included)
     // matrix (caller order)
     DevBuf<long long> ptr;
     DevBuf<int> adj, perm, iperm;
@@ -460,18 +529,58 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     const Plan& h = plan.host;
     const DPlan& P = plan.d;
     const int H = P.H, n = P.n;
-    lev_huge.assign(H, {});
+    lev_huge.clear();
+    lev_huge.resize(H);
     huge_tiles = 0, huge_rows = 0;
-    for (int lv = 0; lv < H; ++lv)
+    for (int lv = 0; lv < H; ++lv) {
+        HugeLevel& hl = lev_huge[lv];
         for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
             const int t = h.order[k];
             const int ma = h.npiv[t] + h.nupd[t] + 1;
             if (ma <= kBigTiles * 16) continue;
-            lev_huge[lv].push_back(t);
             const long long MT = (ma + 15) / 16;
-            huge_tiles = std::max(huge_tiles, MT * (MT + 1) / 2);
+            hl.t.push_back(t);
+            hl.off.push_back(hl.tiles);
+            hl.tiles += MT * (MT + 1) / 2;
             huge_rows = std::max(huge_rows, ma);
         }
+        if (hl.t.empty()) continue;
+        huge_tiles = std::max(huge_tiles, hl.tiles);
+        for (int t : hl.t) hl.maxKt = std::max(hl.maxKt, (h.npiv[t] + 15) / 16);
+        hl.rb.assign(hl.maxKt, 1);
+        hl.tb.assign(hl.maxKt, 0);
+        long long work = 1;
+        for (std::size_t f = 0; f < hl.t.size(); ++f) {
+            const int t = hl.t[f];
+            const int np = h.npiv[t], ma = np + h.nupd[t] + 1, MT = (ma + 15) / 16;
+            for (int Kt = 0; Kt * 16 < np; ++Kt) {
+                hl.rb[Kt] = std::max(hl.rb[Kt], std::max(1, (ma - 16 * Kt - 16 + 191) / 192));
+                const long long Tn = MT - Kt - 1;
+                hl.tb[Kt] = std::max<long long>(hl.tb[Kt], (Tn * (Tn + 1) / 2 + 3) / 4);
+            }
+            work = std::max<long long>(work, std::max<long long>(static_cast<long long>(np) * (ma - 1),
+                                                                 static_cast<long long>(ma - np) * (ma - np)));
+            for (int ci = h.cb[t], r = 0; ci < h.ce[t]; ++ci) {
+                const int c = h.kids[ci];
+                const long long u = h.nupd[c];
+                if (u == 0) continue;
+                if (static_cast<int>(hl.chf.size()) <= r) hl.chf.emplace_back(), hl.chc.emplace_back(),
+                                                          hl.cbk.emplace_back(1, 0);
+                hl.chf[r].push_back(static_cast<int>(f));
+                hl.chc[r].push_back(c);
+                hl.cbk[r].push_back(hl.cbk[r].back() + ((u + 1) * u + 255) / 256);
+                ++r;
+            }
+        }
+        hl.store_blocks = std::min<long long>(1024, (work + 255) / 256);
+        hl.dt = to_device(hl.t, s);
+        hl.doff = to_device(hl.off, s);
+        for (std::size_t r = 0; r < hl.chf.size(); ++r) {
+            hl.dchf.push_back(to_device(hl.chf[r], s));
+            hl.dchc.push_back(to_device(hl.chc[r], s));
+            hl.dcb.push_back(to_device(hl.cbk[r], s));
+        }
+    }
     std::vector<long long> ptrl(m.ptr.begin(), m.ptr.end());
     ptr = to_device(ptrl, s);
     adj = to_device(m.adj, s);
@@ -485,7 +594,9 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     U.alloc(static_cast<std::size_t>(P.usizeA ? P.usizeA : 1) * sizeof(T));
     A.alloc(static_cast<std::size_t>(P.nent ? P.nent : 1) * sizeof(T));
     F.alloc(static_cast<std::size_t>(huge_tiles ? huge_tiles : 1) * 256 * sizeof(T));
-    D.alloc(256 * sizeof(T));
+    std::size_t dmax = 1;
+    for (const HugeLevel& hl : lev_huge) dmax = std::max(dmax, hl.t.size());
+    D.alloc(dmax * 256 * sizeof(T));
     rs0.alloc(n ? n : 1);
     y0.alloc(n ? n : 1);
     LSC_HIP(hipMemsetAsync(rs0.p, 0, (n ? n : 1) * sizeof(double), s));
@@ -524,34 +635,25 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
             hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kThreads),
                                static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T), s, P, lv,
                                1, Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
-        for (int t : lev_huge[lv]) {
-            const int np = h.npiv[t], mm = np + h.nupd[t], ma = mm + 1;
-            const int MT = (ma + 15) / 16;
-            const long long tiles = static_cast<long long>(MT) * (MT + 1) / 2;
-            hipLaunchKernelGGL(k_h_zero<T>, dim3(nblocks(tiles * 256, 256)), dim3(256), 0, s, Fp, tiles * 256);
-            hipLaunchKernelGGL(k_h_init<T>, dim3(64), dim3(256), 0, s, P, t, Fp, Ap);
-            for (int ci = h.cb[t]; ci < h.ce[t]; ++ci) {
-                const int c = h.kids[ci];
-                const long long u = h.nupd[c];
-                if (u == 0) continue;
-                hipLaunchKernelGGL(k_h_extend<T>, dim3(nblocks((u + 1) * u, 256)), dim3(256), 0, s, P, t, c, Fp, Up);
-            }
-            for (int Kt = 0; Kt * 16 < np; ++Kt) {
-                const int k0 = Kt * 16, kend = std::min(k0 + 16, np);
-                const int below = std::max(0, ma - k0 - 16);
-                hipLaunchKernelGGL(k_h_panel<T>, dim3(std::max(1, (below + 191) / 192)), dim3(256), 0, s, ma, k0, kend,
-                                   Fp, Dp, fail.p);
-                hipLaunchKernelGGL(k_h_diag<T>, dim3(1), dim3(256), 0, s, Kt, Fp, Dp);
-                const long long Tn = MT - Kt - 1;
-                const long long nt = Tn * (Tn + 1) / 2;
-                if (nt > 0)
-                    hipLaunchKernelGGL(k_h_trailing<T>, dim3(nblocks(nt, 4)), dim3(256), 0, s, MT, Kt, k0, kend, Fp);
-            }
-            const long long work = std::max<long long>(static_cast<long long>(np) * mm,
-                                                       static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1));
-            hipLaunchKernelGGL(k_h_store<T>, dim3(std::min<long long>(4096, nblocks(work, 256))), dim3(256), 0, s, P,
-                               t, Fp, Lp, Up);
+        HugeLevel& hl = lev_huge[lv];
+        if (hl.t.empty()) continue;
+        const unsigned nh = static_cast<unsigned>(hl.t.size());
+        hipLaunchKernelGGL(k_h_zero<T>, dim3(nblocks(hl.tiles * 256, 256)), dim3(256), 0, s, Fp, hl.tiles * 256);
+        hipLaunchKernelGGL(k_h_init<T>, dim3(64, nh), dim3(256), 0, s, P, hl.dt.p, hl.doff.p, Fp, Ap);
+        for (std::size_t r = 0; r < hl.chf.size(); ++r)
+            hipLaunchKernelGGL(k_h_extend<T>, dim3(static_cast<unsigned>(hl.cbk[r].back())), dim3(256), 0, s, P,
+                               static_cast<int>(hl.chf[r].size()), hl.dchf[r].p, hl.dchc[r].p, hl.dcb[r].p, hl.dt.p,
+                               hl.doff.p, Fp, Up);
+        for (int Kt = 0; Kt < hl.maxKt; ++Kt) {
+            hipLaunchKernelGGL(k_h_panel<T>, dim3(hl.rb[Kt], nh), dim3(256), 0, s, P, Kt, hl.dt.p, hl.doff.p, Fp, Dp,
+                               fail.p);
+            hipLaunchKernelGGL(k_h_diag<T>, dim3(nh), dim3(256), 0, s, P, Kt, hl.dt.p, hl.doff.p, Fp, Dp);
+            if (hl.tb[Kt] > 0)
+                hipLaunchKernelGGL(k_h_trailing<T>, dim3(hl.tb[Kt], nh), dim3(256), 0, s, P, Kt, hl.dt.p, hl.doff.p,
+                                   Fp);
         }
+        hipLaunchKernelGGL(k_h_store<T>, dim3(static_cast<unsigned>(hl.store_blocks), nh), dim3(256), 0, s, P,
+                           hl.dt.p, hl.doff.p, Fp, Lp, Up);
     }
     LSC_HIP(hipEventRecord(ev1, s));
     int h_fail = 0;
@@ -579,19 +681,22 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
         return dim3(static_cast<unsigned>(static_cast<long long>(k) * (h.lev_ptr[lv + 1] - h.lev_ptr[lv])));
     };
     auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
-    const std::size_t hl = static_cast<std::size_t>(huge_rows + 4) * sizeof(double);
+    const std::size_t hl = static_cast<std::size_t>(huge_rows + 272) * sizeof(double);
     if (mode == 1)
         for (int lv = 0; lv < P.H; ++lv) {
             auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
             hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, rs.p, y.p,
                                V.p);
-            for (int t : lev_huge[lv])
-                hipLaunchKernelGGL(k_h_forward<T>, dim3(1, k), dim3(256), hl, s, P, t, P.n, P.vsize, Lp, rs.p, y.p,
-                                   V.p, done.p);
+            const HugeLevel& H = lev_huge[lv];
+            if (!H.t.empty())
+                hipLaunchKernelGGL(k_h_forward<T>, dim3(static_cast<unsigned>(H.t.size()), k), dim3(256), hl, s, P,
+                                   H.dt.p, P.n, P.vsize, Lp, rs.p, y.p, V.p, done.p);
         }
     for (int lv = P.H - 1; lv >= 0; --lv) {
-        for (int t : lev_huge[lv])
-            hipLaunchKernelGGL(k_h_backward<T>, dim3(1, k), dim3(256), hl, s, P, t, P.n, Lp, y.p, z.p, done.p);
+        const HugeLevel& H = lev_huge[lv];
+        if (!H.t.empty())
+            hipLaunchKernelGGL(k_h_backward<T>, dim3(static_cast<unsigned>(H.t.size()), k), dim3(256), hl, s, P,
+                               H.dt.p, P.n, Lp, y.p, z.p, done.p);
         auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
         hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, y.p, z.p);
     }
@@ -631,8 +736,8 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 1>), lds);
     sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 3>), lds);
     if (huge_rows > 0) {
-        sys_allow(reinterpret_cast<const void*>(&k_h_forward<T>), static_cast<std::size_t>(huge_rows + 4) * 8);
-        sys_allow(reinterpret_cast<const void*>(&k_h_backward<T>), static_cast<std::size_t>(huge_rows + 4) * 8);
+        sys_allow(reinterpret_cast<const void*>(&k_h_forward<T>), static_cast<std::size_t>(huge_rows + 272) * 8);
+        sys_allow(reinterpret_cast<const void*>(&k_h_backward<T>), static_cast<std::size_t>(huge_rows + 272) * 8);
     }
     const long long kn = static_cast<long long>(k) * n;
     const dim3 g2(nb, k);
