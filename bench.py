@@ -104,6 +104,72 @@ def cpu_baseline(r, s, radius, block, seconds: float):
             "cpu": platform.processor() or platform.machine()}
 
 
+def _phase(st):
+    return {"ms_analysis": round(st["ms_fill"], 2), "ms_factor": round(st["ms_factor"], 2),
+            "ms_solve": round(st["ms_solve"], 2), "resid_max": st["resid_max"], "refine_max": st["refine_max"],
+            "refine_mean": round(st["refine_mean"], 2), "n_solves": st["n_solved"]}
+
+
+def pairwise_leg(B, n, four, npts, seed, amp, maps_too, cpu=False):
+    """Circuitscape pairwise on one n x n graph: mixed and FP64 GPU (and optionally the CPU oracle)."""
+    if amp is None:  # lognormal iid
+        r = np.exp(np.random.default_rng(seed).standard_normal((n, n)))
+    else:
+        r = np.exp(amp * fractal_field(n, seed)).astype(np.float32).astype(np.float64)
+    g = 1.0 / r
+    cells = np.random.default_rng(seed + 1000).choice(n * n, npts, replace=False).astype(np.int64)
+    ex = np.zeros((npts, npts), np.uint8)
+    out = {"workload": f"pairwise {n}x{n} {'4' if four else '8'}-neighbour, {npts} focal cells "
+                       f"({npts * (npts - 1) // 2} pairs), seed {seed}"}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        B.settings.precision = prec
+        t0 = time.perf_counter()
+        R = B.circuitscape_pairwise(g, None, cells, ex, False, four, {})["resistances"]
+        out[prec] = {"s_wall": round(time.perf_counter() - t0, 3), **_phase(B.last_stats)}
+        res[prec] = R
+    off = ~np.eye(npts, dtype=bool)
+    out["max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(res["mixed"] - res["fp64"])[off] / res["fp64"][off]))
+    if maps_too:
+        B.settings.precision = "mixed"
+        t0 = time.perf_counter()
+        Ra = B.circuitscape_pairwise(g, None, cells, ex, False, four, {"cum": True})["resistances"]
+        out["mixed_all_pairs_cum_map"] = {"s_wall": round(time.perf_counter() - t0, 3), **_phase(B.last_stats),
+                                          "max_rel_diff_vs_shortcut": float(
+                                              np.max(np.abs(Ra - res["mixed"])[off] / res["mixed"][off]))}
+    if cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        Rc = O.cs_pairwise(g, None, cells, ex, False, four, {"cum": True, "volt_maps": True})["resistances"]
+        out["cpu_oracle_fp64"] = {"s_wall": round(time.perf_counter() - t0, 3),
+                                  "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                  "max_rel_diff_gpu_mixed": float(
+                                      np.max(np.abs(res["mixed"] - Rc)[off] / Rc[off]))}
+    B.settings.precision = "mixed"
+    return out
+
+
+def adjoint_leg(B, n, npairs, seed):
+    """J = sum of effective resistances and dJ/dr on an n x n 8-neighbour lognormal surface."""
+    rng = np.random.default_rng(seed)
+    r = np.exp(rng.standard_normal((n, n))).astype(np.float32).astype(np.float64)
+    src = rng.choice(n * n, npairs, replace=False)
+    dst = rng.choice(n * n, npairs, replace=False)
+    out = {"workload": f"adjoint {n}x{n} 8-neighbour lognormal sigma 1, {npairs} pairs, seed {seed}"}
+    grads = {}
+    for prec in ("mixed", "fp64"):
+        B.settings.precision = prec
+        t0 = time.perf_counter()
+        J, R, grad = B.effective_resistance_grad(r, src, dst)
+        out[prec] = {"s_wall": round(time.perf_counter() - t0, 3), "J": J, **_phase(B.last_stats)}
+        grads[prec] = grad
+    out["grad_max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(grads["mixed"] - grads["fp64"]))
+                                                   / np.abs(grads["fp64"]).max())
+    B.settings.precision = "mixed"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +182,7 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the pairwise and adjoint graph workloads")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by row bands "
@@ -183,6 +250,14 @@ def main() -> int:
         fp64 = {"value": st64["n_targets"] / d64, "ms": d64 * 1e3, "resid_max": st64["resid_max"],
                 "cum_max_rel_diff_vs_mixed": float(np.nanmax(np.abs(cum_mixed - cum64)) / top)}
     cpu = None if a.no_cpu else cpu_baseline(r, s, a.radius, a.block, a.cpu_seconds)
+    graph = None
+    if not a.no_graph:
+        from landschaft_amd.circuit import _backend as B
+
+        B.settings.tol = 1e-12
+        graph = {"pairwise_256": pairwise_leg(B, 256, False, 4, 101, None, True, cpu=not a.no_cpu),
+                 "pairwise_2048": pairwise_leg(B, 2048, True, 16, 102, 2.0, True),
+                 "adjoint_1024": adjoint_leg(B, 1024, 32, 104)}
     line = {
         "metric": "circuit solves/sec at 4096^2 @ rel resid <=1e-10 (Omniscape windows, mixed-precision GPU)",
         "value": round(value, 2), "unit": "circuits/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -194,7 +269,7 @@ def main() -> int:
         "n_targets": st["n_targets"], "resid_max": st["resid_max"], "n_over_1e10": st["n_over_1e10"],
         "refine_mean": round(st["refine_mean"], 3), "refine_max": st["refine_max"],
         "phase_ms": {k[3:]: round(v, 2) for k, v in st.items() if k.startswith("ms_")},
-        "kernels": kernels, "roofline": roofline, "fp64_gpu": fp64, "cpu_baseline": cpu,
+        "kernels": kernels, "roofline": roofline, "fp64_gpu": fp64, "cpu_baseline": cpu, "graph": graph,
         "gen_seconds": round(gen_s, 1),
     }
     print(json.dumps(line))

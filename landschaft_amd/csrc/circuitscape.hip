@@ -50,7 +50,97 @@ double average(double x, double y, bool diag, bool res) {
     return diag ? (x + y) / 2.8284271247461903 : (x + y) / 2;
 }
 
+// Components of the graph over edges of nonzero weight (iterative DFS).
+void label_components(RGraph& g) {
+    g.comp.assign(g.n, -1);
+    g.ncomp = 0;
+    std::vector<i32> st;
+    for (i32 u = 0; u < g.n; ++u) {
+        if (g.comp[u] >= 0) continue;
+        g.comp[u] = g.ncomp;
+        st.push_back(u);
+        while (!st.empty()) {
+            const i32 v = st.back();
+            st.pop_back();
+            for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) {
+                const i32 t = g.adj[k];
+                if (g.w[k] == 0 || g.comp[t] >= 0) continue;
+                g.comp[t] = g.ncomp;
+                st.push_back(t);
+            }
+        }
+        ++g.ncomp;
+    }
+}
+
+// Without short-circuit regions every node is one cell and nodes are
+// numbered in column-major cell order, so a node's neighbours in ascending
+// node order are its cells' neighbours in column-major order: the CSR is
+// written directly, with the same entries as the sorted edge list (no pair
+// of cells is joined twice, and the edge average is symmetric in its two
+// arguments, bit for bit).
+RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool four) {
+    RGraph g;
+    g.nx = nx, g.ny = ny;
+    const i64 n = nx * ny;
+    g.nodemap.assign(n, -1);
+    i32 cnt = 0;
+    for (i64 x = 0; x < nx; ++x)
+        for (i64 y = 0; y < ny; ++y)
+            if (gmap[y * nx + x] > 0) g.nodemap[y * nx + x] = cnt++;
+    g.n = cnt;
+    g.node_cell.resize(cnt);
+    for (i64 x = 0; x < nx; ++x)
+        for (i64 y = 0; y < ny; ++y) {
+            const i32 v = g.nodemap[y * nx + x];
+            if (v >= 0) g.node_cell[v] = y * nx + x;
+        }
+    // neighbour offsets (dx, dy) in column-major order
+    static const int k8[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1}, {0, 1}, {1, -1}, {1, 0}, {1, 1}};
+    static const int k4[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
+    const int nd = four ? 4 : 8;
+    g.ptr.assign(static_cast<std::size_t>(cnt) + 1, 0);
+    for (i32 v = 0; v < cnt; ++v) {
+        const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
+        i64 deg = 0;
+        for (int d = 0; d < nd; ++d) {
+            const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
+            const i64 xx = x + dx, yy = y + dy;
+            if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && g.nodemap[yy * nx + xx] >= 0) ++deg;
+        }
+        g.ptr[v + 1] = g.ptr[v] + deg;
+    }
+    g.adj.resize(g.ptr[cnt]);
+    g.w.resize(g.ptr[cnt]);
+    g.degree.assign(cnt, 0.0);
+    for (i32 v = 0; v < cnt; ++v) {
+        const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
+        i64 k = g.ptr[v];
+        double dsum = 0.0;
+        for (int d = 0; d < nd; ++d) {
+            const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
+            const i64 xx = x + dx, yy = y + dy;
+            if (xx < 0 || xx >= nx || yy < 0 || yy >= ny) continue;
+            const i64 e = yy * nx + xx;
+            const i32 u = g.nodemap[e];
+            if (u < 0) continue;
+            const double w = average(gmap[c], gmap[e], dx != 0 && dy != 0, avg_res);
+            g.adj[k] = u;
+            g.w[k] = w;
+            dsum += w;
+            ++k;
+        }
+        g.degree[v] = dsum;
+    }
+    label_components(g);
+    return g;
+}
+
 RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny, bool avg_res, bool four) {
+    bool any_poly = false;
+    if (poly)
+        for (i64 c = 0; c < nx * ny && !any_poly; ++c) any_poly = poly[c] != 0;
+    if (!any_poly) return raster_graph_cells(gmap, nx, ny, avg_res, four);
     RGraph g;
     g.nx = nx, g.ny = ny;
     const i64 n = nx * ny;
@@ -135,24 +225,7 @@ RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny
         for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) d += g.w[k];
         g.degree[v] = d;
     }
-    g.comp.assign(cnt, -1);
-    std::vector<i32> st;
-    for (i32 u = 0; u < cnt; ++u) {
-        if (g.comp[u] >= 0) continue;
-        g.comp[u] = g.ncomp;
-        st.push_back(u);
-        while (!st.empty()) {
-            const i32 v = st.back();
-            st.pop_back();
-            for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) {
-                const i32 t = g.adj[k];
-                if (g.w[k] == 0 || g.comp[t] >= 0) continue;
-                g.comp[t] = g.ncomp;
-                st.push_back(t);
-            }
-        }
-        ++g.ncomp;
-    }
+    label_components(g);
     return g;
 }
 
@@ -180,12 +253,14 @@ HostMatrix component_matrix(const RGraph& g, const i32* mem, i64 count, const do
                             std::vector<i32>& nodes) {
     HostMatrix M;
     nodes.clear();
-    std::unordered_map<i32, i32> local;
+    // node -> unknown, -1 elsewhere; kept between calls and reset after use
+    static thread_local std::vector<i32> local;
+    if (local.size() < static_cast<std::size_t>(g.n)) local.assign(g.n, -1);
     i64 x0 = g.nx, x1 = -1, y0 = g.ny, y1 = -1;
     for (i64 p = 0; p < count; ++p) {
         const i32 v = mem[p];
         if (drop(v)) continue;
-        local.emplace(v, static_cast<i32>(nodes.size()));
+        local[v] = static_cast<i32>(nodes.size());
         nodes.push_back(v);
         const i64 c = g.node_cell[v];
         if (c >= 0) {
@@ -204,14 +279,15 @@ HostMatrix component_matrix(const RGraph& g, const i32* mem, i64 count, const do
         const i64 c = g.node_cell[v];
         M.cell[i] = c >= 0 ? (c / g.nx - y0) * M.bx + (c % g.nx - x0) : -1;
         for (i64 e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
-            auto it = local.find(g.adj[e]);
-            if (it == local.end()) continue;
-            M.adj.push_back(it->second);
+            const i32 u = local[g.adj[e]];
+            if (u < 0) continue;
+            M.adj.push_back(u);
             M.off.push_back(-g.w[e]);
         }
         M.ptr[i + 1] = static_cast<i64>(M.adj.size());
         M.diag[i] = (finite ? g.degree[v] + finite[v] : g.degree[v]) + kShift;
     }
+    for (i32 v : nodes) local[v] = -1;
     return M;
 }
 
@@ -704,15 +780,15 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         HostMatrix A = component_matrix(g, mb, count, nullptr, [&](i32 v) { return v == ref; }, nodes);
         SystemSolver S(cx.precision, cx.s);
         S.factor(A);
-        std::unordered_map<i32, i32> uidx;
-        for (int i = 0; i < A.n; ++i) uidx.emplace(nodes[i], i);
+        std::vector<i32> uidx(g.n, -1);  // node -> unknown (-1: the grounded node, other components)
+        for (int i = 0; i < A.n; ++i) uidx[nodes[i]] = i;
         // every job's voltages (relative to its source) by member
         const std::size_t nj = jobs.size();
         std::vector<double> B(nj * static_cast<std::size_t>(A.n), 0.0), X(nj * static_cast<std::size_t>(A.n), 0.0);
         for (std::size_t j = 0; j < nj; ++j) {
-            auto a = uidx.find(jobs[j].first), b = uidx.find(jobs[j].second);
-            if (a != uidx.end()) B[j * A.n + a->second] = -1.0;
-            if (b != uidx.end()) B[j * A.n + b->second] = 1.0;
+            const i32 a = uidx[jobs[j].first], b = uidx[jobs[j].second];
+            if (a >= 0) B[j * A.n + a] = -1.0;
+            if (b >= 0) B[j * A.n + b] = 1.0;
         }
         if (A.n > 0) S.solve(static_cast<int>(nj), B.data(), X.data(), cx.tol, cx.maxit, cx.st);
         tally(cx, S);
@@ -725,8 +801,8 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         for (std::size_t j = 0; j < nj; ++j) {
             const auto& job = jobs[j];
             for (i64 q = 0; q < count; ++q) {
-                auto it = uidx.find(mb[q]);
-                vcol[q] = it != uidx.end() ? X[j * A.n + it->second] : 0.0;
+                const i32 u = uidx[mb[q]];
+                vcol[q] = u >= 0 ? X[j * A.n + u] : 0.0;
             }
             const double vs = vcol[midx(job.first)];
             for (i64 q = 0; q < count; ++q) vcol[q] -= vs;
