@@ -440,14 +440,69 @@ __device__ __forceinline__ double pick(const double (&v)[3], int slot) {
 // Forward substitution of front t (one wavefront, L block Lc in LDS): its
 // vector assembled from the pivots' RHS and the children's update vectors,
 // then y = L11^-1 f and the update vector f_upd - L21 y (reference
-// system.cpp:974-989).
+// system.cpp:974-989). The L block, the pivots' RHS and the first two
+// children's update vectors are all requested from HBM before the first of
+// them is used, so their latencies overlap.
 template <typename T, int R>
-__device__ void forward_front(const DPlan& P, int t, const T* Lc, double* fv, int lane, const double* rsw,
+__device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double* fv, int lane, const double* rsw,
                               double* yw, double* V) {
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    for (int i = lane; i < m; i += 64) fv[i] = i < np ? rsw[first + i] : 0.0;
+    const int cb = P.cb[t], ce = P.ce[t];
+    // requests: L block, RHS, two children (one value per lane and slot)
+    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    uint4 lv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) lv[q] = src[e];
+    }
+    double pre[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        pre[r] = i < np ? rsw[first + i] : 0.0;
+    }
+    double cv[2][R];
+    int cr[2][R];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int ci = cb + k;
+        const int c = ci < ce ? P.kids[ci] : -1;
+        const int u = c >= 0 ? P.nupd[c] : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int ii = lane + 64 * r;
+            const bool ok = ii < u;
+            cr[k][r] = ok ? P.relmap[P.upd[c] + ii] : -1;
+            cv[k][r] = ok ? V[P.vpos[c] + ii] : 0.0;
+        }
+    }
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) dst[e] = lv[q];
+    }
+#pragma unroll 8
+    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < m) fv[i] = pre[r];
+    }
     wave_sync();
-    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+    // children's contributions: a child's rows map to distinct rows of the
+    // parent, but two children may share one, so they go one after the other
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (cr[k][r] >= 0) fv[cr[k][r]] += cv[k][r];
+        wave_sync();
+    }
+    for (int ci = cb + 2; ci < ce; ++ci) {
         const int c = P.kids[ci];
         const int u = P.nupd[c];
         const int* rel = P.relmap + P.upd[c];
@@ -503,13 +558,48 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lc, double* fv, in
 }
 
 // Back substitution of front t (one wavefront, L block Lc in LDS): z_piv =
-// L11^-T (y - L21^T z_upd) (reference system.cpp:992-1016).
+// L11^-T (y - L21^T z_upd) (reference system.cpp:992-1016). The L block,
+// the solved update rows and the pivots' y are requested together.
 template <typename T, int R>
-__device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, int lane, const double* yw,
+__device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double* xu, int lane, const double* yw,
                                double* zw) {
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     const int* rows = P.rows + P.upd[t];
-    for (int ii = lane; ii < nu; ii += 64) xu[ii] = zw[rows[ii]];
+    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    int ridx[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ii = lane + 64 * r;
+        ridx[r] = ii < nu ? rows[ii] : -1;
+    }
+    uint4 lv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) lv[q] = src[e];
+    }
+    double yv[R], zu[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + 64 * r;
+        yv[r] = j < np ? yw[first + j] : 0.0;
+        zu[r] = ridx[r] >= 0 ? zw[ridx[r]] : 0.0;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) dst[e] = lv[q];
+    }
+#pragma unroll 8
+    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ii = lane + 64 * r;
+        if (ii < nu) xu[ii] = zu[r];
+    }
     wave_sync();
     double tv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -519,7 +609,7 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, i
         if (j < np) {
             const T* col = Lc + colstart(j, m) + (np - j);
             for (int ii = 0; ii < nu; ++ii) acc += static_cast<double>(col[ii]) * xu[ii];
-            acc = yw[first + j] - acc;
+            acc = yv[r] - acc;
         }
         tv[r] = acc;
     }
@@ -562,31 +652,6 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lc, double* xu, i
     }
 }
 
-// The L block of front t copied from HBM to LDS in one coalesced burst.
-template <typename T>
-__device__ __forceinline__ void stage_block(const DPlan& P, int t, const T* Lw, T* Ls, int lane) {
-    const long long np = P.npiv[t], m = np + P.nupd[t];
-    const int sz = static_cast<int>(np * m - ((np * (np - 1)) >> 1));
-    // 16-byte loads, eight in flight per lane (blocks start on 16 bytes)
-    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
-    uint4* dst = reinterpret_cast<uint4*>(Ls);
-    const int nv = static_cast<int>((static_cast<long long>(sz) * sizeof(T) + 15) / 16);
-    for (int base = lane; base < nv; base += 8 * 64) {
-        uint4 v[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = base + q * 64;
-            if (e < nv) v[q] = src[e];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = base + q * 64;
-            if (e < nv) dst[e] = v[q];
-        }
-    }
-    wave_sync();
-}
-
 }  // namespace
 
 // Batched triangular solves, one launch per level: every (instance, front)
@@ -608,8 +673,8 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
     if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
     const long long off = static_cast<long long>(k) * P.n;
-    stage_block<T>(P, t, L + static_cast<long long>(k) * lstride, Ls, lane);
-    forward_front<T, R>(P, t, Ls, fv, lane, rs + off, y + off, Vall + static_cast<long long>(k) * P.vsize);
+    forward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, fv, lane, rs + off, y + off,
+                        Vall + static_cast<long long>(k) * P.vsize);
 }
 
 template <typename T, int R>
@@ -627,8 +692,7 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
     if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_backward
     const long long off = static_cast<long long>(k) * P.n;
-    stage_block<T>(P, t, L + static_cast<long long>(k) * lstride, Ls, lane);
-    backward_front<T, R>(P, t, Ls, xu, lane, y + off, z + off);
+    backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
 
 // x = s z (mode 0) or x += s z (mode 1) for the instances not done.
