@@ -155,7 +155,7 @@ void WindowBatchBuffers::ensure(int B_, int W2_, int n_, long long lsize_, int e
     dg.alloc(bn), sc.alloc(bn), b.alloc(bn), x.alloc(bn), y.alloc(bn), z.alloc(bn), rs.alloc(bn);
     rnorm.alloc(B), rprev.alloc(B);
     active.alloc(B), fail.alloc(B), done.alloc(B), iters.alloc(B);
-    L.alloc(static_cast<std::size_t>(B) * lsize * elem);
+    L.alloc((static_cast<std::size_t>(B) * lsize + 4) * elem);  // + 4: whole 16-byte loads at the end
 }
 
 // One batch of windows whose g, src and active flags are filled: flood,

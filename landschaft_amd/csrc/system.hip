@@ -590,7 +590,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     iperm = to_device(h.iperm, s);
     sc.alloc(n ? n : 1);
     if (n) hipLaunchKernelGGL(k_g_scale, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, diag.p, sc.p);
-    L.alloc(static_cast<std::size_t>(P.lsize ? P.lsize : 1) * sizeof(T));
+    L.alloc(static_cast<std::size_t>(P.lsize + 4) * sizeof(T));  // + 4: whole 16-byte loads at the end
     U.alloc(static_cast<std::size_t>(P.usizeA ? P.usizeA : 1) * sizeof(T));
     A.alloc(static_cast<std::size_t>(P.nent ? P.nent : 1) * sizeof(T));
     F.alloc(static_cast<std::size_t>(huge_tiles ? huge_tiles : 1) * 256 * sizeof(T));

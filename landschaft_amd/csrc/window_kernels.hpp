@@ -452,11 +452,16 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double*
     const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
     const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
     const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    // every lane loads (clamped index): a per-lane conditional load would
+    // put lv in private memory. A front without pivots has no L block and
+    // reads the start of the (non-empty) L array instead.
     uint4 lv[8];
+    const int last = nv > 0 ? nv - 1 : 0;
+    const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int e = lane + q * 64;
-        if (e < nv) lv[q] = src[e];
+        lv[q] = srcl[e < nv ? e : last];
     }
     double pre[R];
 #pragma unroll
@@ -574,11 +579,16 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
         const int ii = lane + 64 * r;
         ridx[r] = ii < nu ? rows[ii] : -1;
     }
+    // every lane loads (clamped index): a per-lane conditional load would
+    // put lv in private memory. A front without pivots has no L block and
+    // reads the start of the (non-empty) L array instead.
     uint4 lv[8];
+    const int last = nv > 0 ? nv - 1 : 0;
+    const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int e = lane + q * 64;
-        if (e < nv) lv[q] = src[e];
+        lv[q] = srcl[e < nv ? e : last];
     }
     double yv[R], zu[R];
 #pragma unroll
