@@ -92,6 +92,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
                                     " rows exceeds the LDS fronts of the batched engine (159 rows)");
     lev_small_host.assign(H, 0);
     lev_stile.assign(H, 1);
+    lev_btile.assign(H, 1);
     lev_lmax.assign(H, 1);
     lev_mmax.assign(H, 1);
     huge.clear();
@@ -108,6 +109,10 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
                 huge.push_back(t);
                 continue;
             }
+            if (m + 1 > kSmallRows) {
+                const int MT = static_cast<int>((m + 1 + 15) / 16);
+                lev_btile[lv] = std::max(lev_btile[lv], MT * (MT + 1) / 2);
+            }
             lev_lmax[lv] = std::max(lev_lmax[lv], static_cast<int>(np * m - np * (np - 1) / 2));
             lev_mmax[lv] = std::max(lev_mmax[lv], static_cast<int>(m));
         }
@@ -115,8 +120,8 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
     long long ua = 0;
     for (int k = 0; k < h.nfront; ++k) {
         const int t = h.order[k];
-        uposA[t] = ua;
-        ua += static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1);
+        uposA[t] = ua;  // 4-entry aligned: the extend-add reads 16-byte chunks
+        ua += (static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1) + 3) & ~3ll;
     }
     ibufs.clear();
     lbufs.clear();
@@ -220,8 +225,8 @@ struct WindowSolver {
                                    Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
             if (nbig > 0)
                 hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
-                                   dim3(kThreads), big_lds<T>(), s, P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p,
-                                   Up, wb.fail.p);
+                                   dim3(kThreads), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
+                                   P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
         }
     }
 
@@ -416,7 +421,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     Plan plan = window_plan(Lt.D, kind == 0, a.four != 0, nd_leaf());
     // per-window device memory: maps, vectors, L, update matrices and vectors
     long long ua = 0;
-    for (int t = 0; t < plan.nfront; ++t) ua += static_cast<long long>(plan.nupd[t] + 1) * (plan.nupd[t] + 1);
+    for (int t = 0; t < plan.nfront; ++t) ua += (static_cast<long long>(plan.nupd[t] + 1) * (plan.nupd[t] + 1) + 3) & ~3ll;
     const double per_window = static_cast<double>(Lt.W) * Lt.W * 33.0 + plan.n * 7.0 * 8.0 +
                               static_cast<double>(plan.lsize + ua + plan.fe_row.size()) * sizeof(T) +
                               plan.vsize * 8.0;

@@ -633,8 +633,8 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
                                plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
         if (nbig > 0)
             hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kThreads),
-                               static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T), s, P, lv,
-                               1, Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
+                               static_cast<std::size_t>(plan.lev_btile[lv]) * 256 * sizeof(T), s, P, lv, 1, Ap, rs0.p,
+                               one.p, Lp, y0.p, Up, fail.p);
         HugeLevel& hl = lev_huge[lv];
         if (hl.t.empty()) continue;
         const unsigned nh = static_cast<unsigned>(hl.t.size());

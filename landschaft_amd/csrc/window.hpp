@@ -36,8 +36,9 @@ struct DPlan {
 struct DevPlan {
     Plan host;
     DPlan d{};
-    std::vector<int> lev_small_host, lev_stile, lev_lmax, lev_mmax;  // per level: small fronts, their tiles,
-                                                                     // largest L block, largest front
+    std::vector<int> lev_small_host, lev_stile, lev_btile, lev_lmax, lev_mmax;  // per level: small fronts, their
+                                                                                // tiles, the big fronts' tiles,
+                                                                                // largest L block, largest front
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
     std::vector<int> huge;  // fronts of more than 159 rows (RHS row included), in level order

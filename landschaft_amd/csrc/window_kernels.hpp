@@ -312,6 +312,19 @@ __device__ __forceinline__ long long colstart(int j, int m) {
     return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
 }
 
+// Element r of a 16-byte chunk of T values.
+template <typename T>
+__device__ __forceinline__ T chunk_elem(const uint4& v, int r);
+template <>
+__device__ __forceinline__ float chunk_elem<float>(const uint4& v, int r) {
+    return __builtin_bit_cast(float, r == 0 ? v.x : r == 1 ? v.y : r == 2 ? v.z : v.w);
+}
+template <>
+__device__ __forceinline__ double chunk_elem<double>(const uint4& v, int r) {
+    const unsigned long long lo = r == 0 ? v.x : v.z, hi = r == 0 ? v.y : v.w;
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
 // One front: assembled (unit diagonal, the scaled entries, RHS row, the
 // children's update matrices), factorised, written out (L packed by columns,
 // y, the update matrix). Team of nw wavefronts (tid / nt over the team).
@@ -334,27 +347,36 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
         const int u = P.nupd[c], u1 = u + 1;
         const int* rel = P.relmap + P.upd[c];
         const T* Uc = U + P.uposA[c];
-        // the child's row map in LDS, its update matrix streamed with four
-        // independent loads in flight per lane
+        // the child's row map in LDS; its update matrix (columns 0..u-1,
+        // 16-byte aligned) streamed in 16-byte chunks, eight in flight per
+        // lane, every lane loading (clamped) so the chunks stay in registers
         for (int i = tid; i < u; i += nt) relbuf[i] = rel[i];
         if (tid == 0) relbuf[u] = m;
         sync();
+        constexpr int CH = 16 / static_cast<int>(sizeof(T));
         const int tot = u1 * u;
-        for (int base = tid; base < tot; base += 4 * nt) {
-            T val[4];
-            int row[4], col[4];
+        const int nch = (tot + CH - 1) / CH;
+        const uint4* Uv = reinterpret_cast<const uint4*>(Uc);
+        for (int base = tid; base < nch; base += 8 * nt) {
+            uint4 v[8];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int e = base + q * nt;
-                const int jj = e / u1, ii = e - jj * u1;
-                const bool ok = e < tot && ii >= jj;
-                val[q] = ok ? Uc[e] : T(0);
-                row[q] = ok ? ii : -1;
-                col[q] = jj;
+            for (int q = 0; q < 8; ++q) {
+                const int c = base + q * nt;
+                v[q] = Uv[c < nch ? c : nch - 1];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (row[q] >= 0) F[tix(relbuf[row[q]], relbuf[col[q]])] += val[q];
+            for (int q = 0; q < 8; ++q) {
+                const int c = base + q * nt;
+                if (c < nch) {
+                    const int e0 = c * CH;
+                    int jj = e0 / u1, ii = e0 - jj * u1;
+#pragma unroll
+                    for (int r = 0; r < CH; ++r) {
+                        if (e0 + r < tot && ii >= jj) F[tix(relbuf[ii], relbuf[jj])] += chunk_elem<T>(v[q], r);
+                        if (++ii == u1) ii = 0, ++jj;
+                    }
+                }
+            }
         }
         sync();
     }
