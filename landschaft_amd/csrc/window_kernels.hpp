@@ -334,13 +334,31 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, ma = m + 1, first = P.first[t];
     const int MT = (ma + 15) >> 4;
     const int nz = ((MT * (MT + 1)) >> 1) * 256;
+    // the first entries and the pivots' RHS requested before the tiles are
+    // cleared (clamped indices: every lane loads, the values stay in registers)
+    const int e0 = P.fe_ptr[t], e1 = P.fe_ptr[t + 1];
+    const int elast = e1 > e0 ? e1 - 1 : 0;  // entry arrays hold at least one element
+    int er[4], ec[4];
+    T ev[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = e0 + tid + q * nt;
+        const int eq = e < e1 ? e : elast;
+        er[q] = P.fe_row[eq];
+        ec[q] = P.fe_col[eq];
+        ev[q] = aval[eq];
+    }
+    const double rhs0 = np > 0 ? rsw[first + (tid < np ? tid : np - 1)] : 0.0;
     for (int e = tid; e < nz; e += nt) F[e] = T(0);
     sync();
     for (int j = tid; j < np; j += nt) {
         F[tix(j, j)] = T(1);
-        F[tix(m, j)] = static_cast<T>(rsw[first + j]);
+        F[tix(m, j)] = static_cast<T>(j == tid ? rhs0 : rsw[first + j]);
     }
-    for (int e = P.fe_ptr[t] + tid; e < P.fe_ptr[t + 1]; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (e0 + tid + q * nt < e1) F[tix(er[q], ec[q])] = ev[q];
+    for (int e = e0 + tid + 4 * nt; e < e1; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
     sync();
     for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
         const int c = P.kids[ci];
