@@ -234,8 +234,7 @@ struct WindowSolver {
         const DPlan& P = plan.d;
         const T* L = reinterpret_cast<const T*>(wb.L.p);
         auto grid = [&](int h) {
-            return dim3(static_cast<unsigned>(plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h]),
-                        static_cast<unsigned>((nb + kWin - 1) / kWin));
+            return dim3(static_cast<unsigned>(static_cast<long long>(nb) * (plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h])));
         };
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {

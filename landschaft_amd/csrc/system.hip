@@ -678,7 +678,7 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
     const Plan& h = plan.host;
     const T* Lp = reinterpret_cast<const T*>(L.p);
     auto grid = [&](int lv) {
-        return dim3(static_cast<unsigned>(h.lev_ptr[lv + 1] - h.lev_ptr[lv]), static_cast<unsigned>((k + kWin - 1) / kWin));
+        return dim3(static_cast<unsigned>(static_cast<long long>(k) * (h.lev_ptr[lv + 1] - h.lev_ptr[lv])));
     };
     auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
     const std::size_t hl = static_cast<std::size_t>(huge_rows + 272) * sizeof(double);

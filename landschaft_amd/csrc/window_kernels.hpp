@@ -477,28 +477,84 @@ __device__ __forceinline__ double pick(const double (&v)[3], int slot) {
     return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
 }
 
-// Level-synchronous triangular solves. Workgroup (f, g) is one wavefront
-// that solves front f of the level for the kWin instances g kWin.. of the
-// batch: the front's metadata and its children's row maps (the same for every
-// instance) are read once, and the next instance's L block, right-hand side
-// and update vectors are requested while the current one is solved. Every
-// lane issues every load (clamped indices), so the prefetched values stay in
-// registers.
-constexpr int kWin = 4;
-constexpr int kPre = 4;  // 16-byte chunks of L prefetched per lane
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-__device__ __forceinline__ int chunks_of(int np, int m) {
-    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
-    return static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
-}
-
-// y = L11^-1 f and the update vector f_upd - L21 y of one front, f in fv
-// (reference system.cpp:974-989); the L block in LDS (Lc).
+// Forward substitution of front t (one wavefront, L block Lc in LDS): its
+// vector assembled from the pivots' RHS and the children's update vectors,
+// then y = L11^-1 f and the update vector f_upd - L21 y (reference
+// system.cpp:974-989). The L block, the pivots' RHS and the first two
+// children's update vectors are all requested from HBM before the first of
+// them is used, so their latencies overlap.
 template <typename T, int R>
-__device__ __forceinline__ void forward_solve(const T* Lc, const double* fv, int np, int m, int lane, double* yw,
-                                              double* Vt) {
+__device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double* fv, int lane, const double* rsw,
+                              double* yw, double* V) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int cb = P.cb[t], ce = P.ce[t];
+    // requests: L block, RHS, two children (one value per lane and slot)
+    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    // every lane loads (clamped index): a per-lane conditional load would
+    // put lv in private memory. A front without pivots has no L block and
+    // reads the start of the (non-empty) L array instead.
+    uint4 lv[8];
+    const int last = nv > 0 ? nv - 1 : 0;
+    const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        lv[q] = srcl[e < nv ? e : last];
+    }
+    double pre[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        pre[r] = i < np ? rsw[first + i] : 0.0;
+    }
+    double cv[2][R];
+    int cr[2][R];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int ci = cb + k;
+        const int c = ci < ce ? P.kids[ci] : -1;
+        const int u = c >= 0 ? P.nupd[c] : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int ii = lane + 64 * r;
+            const bool ok = ii < u;
+            cr[k][r] = ok ? P.relmap[P.upd[c] + ii] : -1;
+            cv[k][r] = ok ? V[P.vpos[c] + ii] : 0.0;
+        }
+    }
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) dst[e] = lv[q];
+    }
+#pragma unroll 8
+    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < m) fv[i] = pre[r];
+    }
+    wave_sync();
+    // children's contributions: a child's rows map to distinct rows of the
+    // parent, but two children may share one, so they go one after the other
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (cr[k][r] >= 0) fv[cr[k][r]] += cv[k][r];
+        wave_sync();
+    }
+    for (int ci = cb + 2; ci < ce; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c];
+        const int* rel = P.relmap + P.upd[c];
+        const double* Vc = V + P.vpos[c];
+        for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
+        wave_sync();
+    }
     double f[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -506,16 +562,16 @@ __device__ __forceinline__ void forward_solve(const T* Lc, const double* fv, int
         f[r] = i < m ? fv[i] : 0.0;
     }
     // 16 columns at a time: the lane's L entries and the reciprocal pivots
-    // in registers first, so the dependent chain is register-only
+    // in registers first, so the dependent chain is register-only.
     for (int j0 = 0; j0 < np; j0 += 16) {
         T Lr[R][16];
-        T dinv[16];
+        double dinv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int j = j0 + q;
             const bool okj = j < np;
             const long long cs = okj ? colstart(j, m) : 0;
-            dinv[q] = okj ? T(1) / Lc[cs] : T(0);
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[cs]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
@@ -527,7 +583,7 @@ __device__ __forceinline__ void forward_solve(const T* Lc, const double* fv, int
             const int j = j0 + q;
             if (j < np) {
                 const int slot = j >> 6;
-                const double yj = bcast(pick(f, slot), j & 63) * static_cast<double>(dinv[q]);
+                const double yj = bcast(pick(f, slot), j & 63) * dinv[q];
 #pragma unroll
                 for (int r = 0; r < R; ++r) f[r] -= static_cast<double>(Lr[r][q]) * yj;
                 if (lane == (j & 63)) {
@@ -541,23 +597,82 @@ __device__ __forceinline__ void forward_solve(const T* Lc, const double* fv, int
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
-        if (i < np) yw[i] = f[r];
-        else if (i < m) Vt[i - np] = f[r];
+        if (i < np) yw[first + i] = f[r];
+        else if (i < m) V[P.vpos[t] + i - np] = f[r];
     }
 }
 
-// z_piv = L11^-T (y - L21^T z_upd) of one front (reference
-// system.cpp:992-1016); tv holds y - L21^T z_upd of the lane's pivots.
+// Back substitution of front t (one wavefront, L block Lc in LDS): z_piv =
+// L11^-T (y - L21^T z_upd) (reference system.cpp:992-1016). The L block,
+// the solved update rows and the pivots' y are requested together.
 template <typename T, int R>
-__device__ __forceinline__ void backward_solve(const T* Lc, double (&tv)[3], int np, int m, int lane, double* zw) {
+__device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double* xu, int lane, const double* yw,
+                               double* zw) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int* rows = P.rows + P.upd[t];
+    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    int ridx[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ii = lane + 64 * r;
+        ridx[r] = ii < nu ? rows[ii] : -1;
+    }
+    // every lane loads (clamped index): a per-lane conditional load would
+    // put lv in private memory. A front without pivots has no L block and
+    // reads the start of the (non-empty) L array instead.
+    uint4 lv[8];
+    const int last = nv > 0 ? nv - 1 : 0;
+    const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        lv[q] = srcl[e < nv ? e : last];
+    }
+    double yv[R], zu[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + 64 * r;
+        yv[r] = j < np ? yw[first + j] : 0.0;
+        zu[r] = ridx[r] >= 0 ? zw[ridx[r]] : 0.0;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = lane + q * 64;
+        if (e < nv) dst[e] = lv[q];
+    }
+#pragma unroll 8
+    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ii = lane + 64 * r;
+        if (ii < nu) xu[ii] = zu[r];
+    }
+    wave_sync();
+    double tv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + 64 * r;
+        double acc = 0.0;
+        if (j < np) {
+            const T* col = Lc + colstart(j, m) + (np - j);
+            for (int ii = 0; ii < nu; ++ii) acc += static_cast<double>(col[ii]) * xu[ii];
+            acc = yv[r] - acc;
+        }
+        tv[r] = acc;
+    }
+    // 16 pivots at a time from the last: L[j][i] of the lane's columns i and
+    // the reciprocal pivots in registers, then a register-only chain.
     for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
         T Lr[R][16];
-        T dinv[16];
+        double dinv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int j = j1 - q;
             const bool okj = j >= 0;
-            dinv[q] = okj ? T(1) / Lc[colstart(j, m)] : T(0);
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[colstart(j, m)]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
@@ -569,7 +684,7 @@ __device__ __forceinline__ void backward_solve(const T* Lc, double (&tv)[3], int
             const int j = j1 - q;
             if (j >= 0) {
                 const int slot = j >> 6;
-                const double xj = bcast(pick(tv, slot), j & 63) * static_cast<double>(dinv[q]);
+                const double xj = bcast(pick(tv, slot), j & 63) * dinv[q];
 #pragma unroll
                 for (int r = 0; r < R; ++r) tv[r] -= static_cast<double>(Lr[r][q]) * xj;
                 if (lane == (j & 63)) {
@@ -583,12 +698,15 @@ __device__ __forceinline__ void backward_solve(const T* Lc, double (&tv)[3], int
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
-        if (i < np) zw[i] = tv[r];
+        if (i < np) zw[first + i] = tv[r];
     }
 }
 
 }  // namespace
 
+// Batched triangular solves, one launch per level: every (instance, front)
+// pair of the level is the task of a one-wavefront workgroup, which stages
+// the front's L block (contiguous in HBM) into LDS, then substitutes.
 template <typename T, int R>
 __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, long long lstride,
                                                       const int* __restrict__ active,
@@ -597,110 +715,16 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
                                                       double* __restrict__ Vall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* fv = reinterpret_cast<double*>(smem_raw);
-    T* Lc = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
     const int lane = threadIdx.x;
-    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
-    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
-    const int n = P.n;
-    const int cb = P.cb[t], ce = P.ce[t];
-    const long long lpos = P.lpos[t], vt = P.vpos[t];
-    const int nv = chunks_of<T>(np, m);
-    const int nlast = nv > 0 ? nv - 1 : 0;
-    // the first two children: their rows in this front, the same for every instance
-    int cr[2][R];
-    long long cvp[2];
-#pragma unroll
-    for (int c2 = 0; c2 < 2; ++c2) {
-        const int c = cb + c2 < ce ? P.kids[cb + c2] : -1;
-        const int u = c >= 0 ? P.nupd[c] : 0;
-        cvp[c2] = c >= 0 ? P.vpos[c] : 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int ii = lane + 64 * r;
-            cr[c2][r] = ii < u ? P.relmap[P.upd[c] + ii] : -1;
-        }
-    }
-    const int k0 = blockIdx.y * kWin;
-    u32x4 lv[kPre];
-    double pre[R], cv[2][R];
-    // step w requests instance k0 + w and solves instance k0 + w - 1
-#pragma unroll
-    for (int w = 0; w <= kWin; ++w) {
-        u32x4 nlv[kPre];
-        double npre[R], ncv[2][R];
-        {
-            const int kn = k0 + w;
-            const int kc = kn < B ? kn : B - 1;
-            {
-                const u32x4* src = nv > 0 ? reinterpret_cast<const u32x4*>(L + static_cast<long long>(kc) * lstride + lpos)
-                                          : reinterpret_cast<const u32x4*>(L);
-#pragma unroll
-                for (int q = 0; q < kPre; ++q) {
-                    const int e = lane + q * 64;
-                    nlv[q] = src[e < nv ? e : nlast];
-                }
-            }
-            const double* rsw = rs + static_cast<long long>(kc) * n;
-            const double* Vn = Vall + static_cast<long long>(kc) * P.vsize;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = lane + 64 * r;
-                const int pi = first + (i < np ? i : 0);
-                npre[r] = rsw[pi < n ? pi : n - 1];
-#pragma unroll
-                for (int c2 = 0; c2 < 2; ++c2) ncv[c2][r] = Vn[cr[c2][r] >= 0 ? cvp[c2] + i : 0];
-            }
-        }
-        const int k = k0 + w - 1;
-        if (w > 0 && k < B && active[k] && !done[k]) {
-            double* V = Vall + static_cast<long long>(k) * P.vsize;
-            {
-                u32x4* dst = reinterpret_cast<u32x4*>(Lc);
-#pragma unroll
-                for (int q = 0; q < kPre; ++q) {
-                    const int e = lane + q * 64;
-                    if (e < nv) dst[e] = lv[q];
-                }
-                const u32x4* src = reinterpret_cast<const u32x4*>(L + static_cast<long long>(k) * lstride + lpos);
-#pragma unroll 8
-                for (int e = lane + kPre * 64; e < nv; e += 64) dst[e] = src[e];
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = lane + 64 * r;
-                if (i < m) fv[i] = i < np ? pre[r] : 0.0;
-            }
-            wave_sync();
-            // a child's rows map to distinct rows of this front, but two
-            // children may share one, so they go one after the other
-#pragma unroll
-            for (int c2 = 0; c2 < 2; ++c2) {
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (cr[c2][r] >= 0) fv[cr[c2][r]] += cv[c2][r];
-                wave_sync();
-            }
-            for (int ci = cb + 2; ci < ce; ++ci) {
-                const int c = P.kids[ci];
-                const int u = P.nupd[c];
-                const int* rel = P.relmap + P.upd[c];
-                const double* Vc = V + P.vpos[c];
-                for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
-                wave_sync();
-            }
-            forward_solve<T, R>(Lc, fv, np, m, lane, y + static_cast<long long>(k) * n + first, V + vt);
-            wave_sync();
-        }
-#pragma unroll
-        for (int q = 0; q < kPre; ++q) lv[q] = nlv[q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            pre[r] = npre[r];
-            cv[0][r] = ncv[0][r];
-            cv[1][r] = ncv[1][r];
-        }
-    }
+    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int k = blockIdx.x / nf;
+    if (k >= B || !active[k] || done[k]) return;
+    const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
+    const long long off = static_cast<long long>(k) * P.n;
+    forward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, fv, lane, rs + off, y + off,
+                        Vall + static_cast<long long>(k) * P.vsize);
 }
 
 template <typename T, int R>
@@ -710,92 +734,15 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
                                                        const double* __restrict__ y, double* __restrict__ z) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* xu = reinterpret_cast<double*>(smem_raw);
-    T* Lc = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
     const int lane = threadIdx.x;
-    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
-    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
-    const int n = P.n;
-    const long long lpos = P.lpos[t];
-    const int nv = chunks_of<T>(np, m);
-    const int nlast = nv > 0 ? nv - 1 : 0;
-    int ridx[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int ii = lane + 64 * r;
-        ridx[r] = ii < nu ? P.rows[P.upd[t] + ii] : -1;
-    }
-    const int k0 = blockIdx.y * kWin;
-    u32x4 lv[kPre];
-    double yv[R], zu[R];
-    // step w requests instance k0 + w and solves instance k0 + w - 1
-#pragma unroll
-    for (int w = 0; w <= kWin; ++w) {
-        u32x4 nlv[kPre];
-        double nyv[R], nzu[R];
-        {
-            const int kn = k0 + w;
-            const int kc = kn < B ? kn : B - 1;
-            {
-                const u32x4* src = nv > 0 ? reinterpret_cast<const u32x4*>(L + static_cast<long long>(kc) * lstride + lpos)
-                                          : reinterpret_cast<const u32x4*>(L);
-#pragma unroll
-                for (int q = 0; q < kPre; ++q) {
-                    const int e = lane + q * 64;
-                    nlv[q] = src[e < nv ? e : nlast];
-                }
-            }
-            const long long off = static_cast<long long>(kc) * n;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int j = lane + 64 * r;
-                const int pj = first + (j < np ? j : 0);
-                nyv[r] = y[off + (pj < n ? pj : n - 1)];
-                nzu[r] = z[off + (ridx[r] >= 0 ? ridx[r] : 0)];
-            }
-        }
-        const int k = k0 + w - 1;
-        if (w > 0 && k < B && active[k] && !done[k]) {
-            {
-                u32x4* dst = reinterpret_cast<u32x4*>(Lc);
-#pragma unroll
-                for (int q = 0; q < kPre; ++q) {
-                    const int e = lane + q * 64;
-                    if (e < nv) dst[e] = lv[q];
-                }
-                const u32x4* src = reinterpret_cast<const u32x4*>(L + static_cast<long long>(k) * lstride + lpos);
-#pragma unroll 8
-                for (int e = lane + kPre * 64; e < nv; e += 64) dst[e] = src[e];
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int ii = lane + 64 * r;
-                if (ii < nu) xu[ii] = zu[r];
-            }
-            wave_sync();
-            double tv[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int j = lane + 64 * r;
-                double acc = 0.0;
-                if (j < np) {
-                    const T* col = Lc + colstart(j, m) + (np - j);
-                    for (int ii = 0; ii < nu; ++ii) acc += static_cast<double>(col[ii]) * xu[ii];
-                    acc = yv[r] - acc;
-                }
-                tv[r] = acc;
-            }
-            backward_solve<T, R>(Lc, tv, np, m, lane, z + static_cast<long long>(k) * n + first);
-            wave_sync();
-        }
-#pragma unroll
-        for (int q = 0; q < kPre; ++q) lv[q] = nlv[q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            yv[r] = nyv[r];
-            zu[r] = nzu[r];
-        }
-    }
+    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int k = blockIdx.x / nf;
+    if (k >= B || !active[k] || done[k]) return;
+    const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_backward
+    const long long off = static_cast<long long>(k) * P.n;
+    backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
 
 // x = s z (mode 0) or x += s z (mode 1) for the instances not done.
