@@ -144,6 +144,24 @@ int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, 
                                     double* voltage, double* current, uint8_t* active, int32_t* solved,
                                     lsc_solve_stats* stats);
 
+/* The same problems reduced on the device to what run_onetoall keeps when
+ * no per-point map is asked for (src/landschaft/circuit/circuitscape.py:
+ * 826-848): every problem's node currents (0 at its sources and grounds with
+ * focal_zero) summed into cum and maximised into max per cell, in problem
+ * order. cum (ny x nx) and max (ny x nx or NULL) are read and updated in
+ * place, so calls over several polygon maps continue one sum. Per problem:
+ * the voltage at the check cell's node (0 if not solved there) and solved[p].
+ * Replaces, for that case, the per-problem outputs of
+ * circuitscape_multi_advanced (src/cpp/circuit/python.cpp:404-482); one
+ * problem's maps never leave the GPU. */
+int lsc_circuitscape_multi_advanced_sum(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                                        int avg_resistances, int four_neighbors, int policy, int64_t nprob,
+                                        const int64_t* sptr, const int64_t* scells, const double* svals,
+                                        const int64_t* gptr, const int64_t* gcells, const double* gvals,
+                                        const int64_t* check, int focal_zero, const lsc_solve_options* options,
+                                        int32_t* nodemap, double* cum, double* max, double* check_voltage,
+                                        int32_t* solved, lsc_solve_stats* stats);
+
 /* J = sum_k R_eff(src_k, gnd_k) on the grid of conductance 1 / r (cells with
  * a finite r > 0, average conductance, 8 or 4 neighbours) and dJ/dr for every
  * cell by the adjoint (lambda_k = v_k). No reference counterpart (new:

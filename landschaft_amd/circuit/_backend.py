@@ -239,6 +239,55 @@ def circuitscape_multi_advanced(gmap, polymap, avg_resistances, four_neighbors, 
     return {"nodemap": nm, "results": results}
 
 
+def _problem_arrays(problems):
+    npb = len(problems)
+
+    def cat(kc, kv):
+        ptr = np.zeros(npb + 1, dtype=np.int64)
+        for k, q in enumerate(problems):
+            ptr[k + 1] = ptr[k] + len(q[kc])
+        cells = np.ascontiguousarray(np.concatenate([np.asarray(q[kc], dtype=np.int64) for q in problems])
+                                     if npb else np.zeros(0, np.int64))
+        vals = np.ascontiguousarray(np.concatenate([np.asarray(q[kv], dtype=np.float64) for q in problems])
+                                    if npb else np.zeros(0))
+        return ptr, cells, vals
+
+    sp, sc, sv = cat("src_cells", "src_vals")
+    gp, gc, gv = cat("gnd_cells", "gnd_vals")
+    check = np.array([q["check_cell"] for q in problems], dtype=np.int64)
+    return sp, sc, sv, gp, gc, gv, check
+
+
+def circuitscape_multi_advanced_sum(gmap, polymap, avg_resistances, four_neighbors, problems, policy, cum, max_map,
+                                    focal_zero=False, n_threads=0):
+    """The problems of circuitscape_multi_advanced reduced to the cum / max current maps (updated in place,
+    ny x nx f64; max_map may be None), as run_onetoall sums them (circuitscape.py:826-848), plus each
+    problem's voltage at its check cell and whether it was solved. The per-problem maps stay on the GPU."""
+    g = N.f64(gmap)
+    ny, nx = g.shape
+    poly = _poly(polymap, g.shape)
+    npb = len(problems)
+    if cum.shape != g.shape or cum.dtype != np.float64 or not cum.flags.c_contiguous:
+        raise ValueError("cum must be a C-contiguous float64 raster of the habitat's shape")
+    if max_map is not None and (max_map.shape != g.shape or max_map.dtype != np.float64
+                                or not max_map.flags.c_contiguous):
+        raise ValueError("max_map must be a C-contiguous float64 raster of the habitat's shape")
+    sp, sc, sv, gp, gc, gv, check = _problem_arrays(problems)
+    nm = np.empty((ny, nx), dtype=np.int32)
+    vcheck = np.zeros(npb)
+    solved = np.zeros(npb, dtype=np.int32)
+    st = N.SolveStats()
+    o = _opts()
+    i64p = ctypes.c_int64
+    N.check(N.lib().lsc_circuitscape_multi_advanced_sum(
+        N.ptr(g), N.ptr(poly, i64p), i64p(nx), i64p(ny), int(bool(avg_resistances)), int(bool(four_neighbors)),
+        _policy(policy), i64p(npb), N.ptr(sp, i64p), N.ptr(sc, i64p), N.ptr(sv), N.ptr(gp, i64p), N.ptr(gc, i64p),
+        N.ptr(gv), N.ptr(check, i64p), int(bool(focal_zero)), ctypes.byref(o), N.ptr(nm, ctypes.c_int32),
+        N.ptr(cum), N.ptr(max_map), N.ptr(vcheck), N.ptr(solved, ctypes.c_int32), ctypes.byref(st)))
+    _record(st)
+    return {"nodemap": nm, "check_voltage": vcheck, "solved": solved.astype(bool)}
+
+
 def effective_resistance_grad(resistance, src_cells, gnd_cells, four_neighbors=False):
     """J = sum of effective resistances of the pairs and dJ/dr per cell (the adjoint on the GPU)."""
     r = N.f64(resistance)

@@ -360,8 +360,25 @@ def _one_to_all(g, poly, points: FocalPoints, strengths, pairs, one_to_all: bool
             groups[-1][1].append((i, q))
         else:
             groups.append((q["poly"], [(i, q)]))
+    if not (opts["volt_maps"] or opts["cur_maps"]):
+        # only the cum / max maps and the point voltages: reduced on the GPU,
+        # in problem order, into running maps
+        cum_run = cum if cum is not None else np.zeros((ny, nx))
+        for pm_, items in groups:
+            out = core().circuitscape_multi_advanced_sum(
+                np.ascontiguousarray(g), np.ascontiguousarray(pm_, dtype=np.int64), bool(avg), bool(four),
+                [q for _, q in items], policy, cum_run, mx, bool(opts["focal_zero"]), int(n_threads))
+            for (i, q), v, ok in zip(items, out["check_voltage"], out["solved"]):
+                if not ok:
+                    res[i] = -1
+                elif not one_to_all:
+                    res[i] = 0
+                else:
+                    val = v / q["strength"]
+                    res[i] = -1 if val == 0 else val
+        todo = []
     solved = {}
-    for pm_, items in groups:
+    for pm_, items in groups if todo else []:
         out = core().circuitscape_multi_advanced(np.ascontiguousarray(g), np.ascontiguousarray(pm_, dtype=np.int64),
                                                  bool(avg), bool(four), [q for _, q in items], policy,
                                                  int(n_threads))

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -391,30 +392,63 @@ void graph_currents(const DevGraph& G, const int* d_mem, int cnt, const double* 
                        scratch.p + 2 * nb, d_cur);
 }
 
-struct Resolved {
-    std::vector<double> src, gnd, finite;
-    bool any_finite = false;
+// One advanced problem: the nodes with a nonzero source or ground (ascending)
+// and their per-node sums.
+struct Problem {
+    std::vector<i32> node;
+    std::vector<double> src, gnd;
 };
 
-// resolve_conflicts (reference circuitscape.cpp:881-900): 0 keepall,
-// 1 rmvsrc, 2 rmvgnd, 3 rmvall.
-Resolved resolve(std::vector<double> src, std::vector<double> gnd, int policy) {
+// From dense per-node arrays.
+Problem problem_from_dense(const std::vector<double>& src, const std::vector<double>& gnd) {
+    Problem p;
+    for (std::size_t v = 0; v < src.size(); ++v)
+        if (src[v] != 0 || gnd[v] != 0) {
+            p.node.push_back(static_cast<i32>(v));
+            p.src.push_back(src[v]);
+            p.gnd.push_back(gnd[v]);
+        }
+    return p;
+}
+
+// The problem's values after resolve_conflicts (reference
+// circuitscape.cpp:881-900; 0 keepall, 1 rmvsrc, 2 rmvgnd, 3 rmvall): finite
+// grounds stay on the diagonal as given (before the policy), the policy on
+// nodes that are both, and a direct ground under a positive source dropped.
+// Nodes whose source and ground are both zero play no part, so the sparse
+// form is exact.
+struct Resolved {
+    std::vector<i32> node;
+    std::vector<double> src, gnd, finite;
+    bool any_finite = false;
+    double at(const std::vector<double>& a, i32 v) const {
+        auto it = std::lower_bound(node.begin(), node.end(), v);
+        return it != node.end() && *it == v ? a[it - node.begin()] : 0.0;
+    }
+    std::vector<double> dense(const std::vector<double>& a, i32 n) const {
+        std::vector<double> d(n, 0.0);
+        for (std::size_t i = 0; i < node.size(); ++i) d[node[i]] = a[i];
+        return d;
+    }
+};
+
+Resolved resolve(const Problem& pb, int policy) {
     Resolved r;
-    const std::size_t n = src.size();
-    r.finite.assign(n, 0.0);
-    for (std::size_t v = 0; v < n; ++v) {
-        r.finite[v] = gnd[v] < kInf ? gnd[v] : 0.0;
-        if (r.finite[v] != 0) r.any_finite = true;
+    r.node = pb.node;
+    r.src = pb.src;
+    r.gnd = pb.gnd;
+    r.finite.assign(pb.node.size(), 0.0);
+    for (std::size_t i = 0; i < r.node.size(); ++i) {
+        r.finite[i] = r.gnd[i] < kInf ? r.gnd[i] : 0.0;
+        if (r.finite[i] != 0) r.any_finite = true;
     }
-    for (std::size_t v = 0; v < n; ++v) {
-        if (!(src[v] != 0 && gnd[v] != 0)) continue;
-        if (policy == 1 || policy == 3) src[v] = 0.0;
-        else if (policy == 2) gnd[v] = 0.0;
+    for (std::size_t i = 0; i < r.node.size(); ++i) {
+        if (!(r.src[i] != 0 && r.gnd[i] != 0)) continue;
+        if (policy == 1 || policy == 3) r.src[i] = 0.0;
+        else if (policy == 2) r.gnd[i] = 0.0;
     }
-    for (std::size_t v = 0; v < n; ++v)
-        if (gnd[v] == kInf && src[v] > 0) gnd[v] = 0.0;
-    r.src = std::move(src);
-    r.gnd = std::move(gnd);
+    for (std::size_t i = 0; i < r.node.size(); ++i)
+        if (r.gnd[i] == kInf && r.src[i] > 0) r.gnd[i] = 0.0;
     return r;
 }
 
@@ -520,6 +554,13 @@ void dense_spd_solve(std::vector<double>& A, std::vector<double>& y, std::size_t
     }
 }
 
+// Receives every solved (problem, component): the full per-node voltage and
+// current vectors on the device (valid on the component's members).
+struct Sink {
+    virtual ~Sink() = default;
+    virtual void component(std::size_t p, i32 k, const Resolved& r, const double* d_v, const double* d_cur) = 0;
+};
+
 // Advanced-mode problems on one graph (the reference's raster_advanced and
 // raster_advanced_many, circuitscape.cpp:393-468, 904-1083). Per component:
 // with two or more problems and at most 2048 special nodes (sources or
@@ -527,76 +568,76 @@ void dense_spd_solve(std::vector<double>& A, std::vector<double>& y, std::size_t
 // Schur complement on them, S = A_FF - A_FU A_UU^-1 A_UF, built from nf
 // solves; each problem then solves its free specials densely and gets the
 // rest as -Z y_F. Otherwise every problem is factorised on its own.
-std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std::vector<std::vector<double>>& srcs,
-                                   const std::vector<std::vector<double>>& gnds, const std::vector<i32>& only,
-                                   int policy, bool allow_schur, Ctx& cx) {
-    const std::size_t np = srcs.size();
+// Components go in order, and within one its problems in order. Returns
+// whether each problem had a component solved.
+std::vector<char> advanced_many(const RGraph& g, const DevGraph& G, const std::vector<Problem>& probs,
+                                const std::vector<i32>& only, int policy, bool allow_schur, Ctx& cx, Sink& sink) {
+    const std::size_t np = probs.size();
     const i32 n = g.n;
     std::vector<Resolved> res(np);
     std::vector<std::uint8_t> special(n, 0);
-    std::vector<NodeSol> out(np);
     for (std::size_t p = 0; p < np; ++p) {
-        res[p] = resolve(srcs[p], gnds[p], policy);
-        out[p].voltage.assign(n, 0.0);
-        out[p].current.assign(n, 0.0);
-        for (i32 v = 0; v < n; ++v) {
-            if (res[p].src[v] != 0 || res[p].gnd[v] != 0) out[p].active.push_back(v);
-            if (res[p].src[v] != 0 || res[p].gnd[v] != 0 || res[p].finite[v] != 0) special[v] = 1;
-        }
+        res[p] = resolve(probs[p], policy);
+        for (std::size_t i = 0; i < res[p].node.size(); ++i)
+            if (res[p].src[i] != 0 || res[p].gnd[i] != 0 || res[p].finite[i] != 0) special[res[p].node[i]] = 1;
     }
     const Members M = members_of(g);
+    // the problems to solve in every component (advanced_kernel's tests):
+    // source and ground sums over the component's nodes, in node order
     std::vector<std::vector<std::size_t>> todo(g.ncomp);
-    for (std::size_t p = 0; p < np; ++p)
+    std::vector<char> solved(np, 0);
+    for (std::size_t p = 0; p < np; ++p) {
+        std::vector<double> ss(g.ncomp, 0.0), sg(g.ncomp, 0.0);
+        const Resolved& r = res[p];
+        for (std::size_t i = 0; i < r.node.size(); ++i) {
+            const i32 k = g.comp[r.node[i]];
+            ss[k] += r.src[i];
+            sg[k] += r.gnd[i];
+        }
         for (i32 k = 0; k < g.ncomp; ++k) {
             if (only[p] == -2 || (only[p] >= 0 && g.comp[only[p]] != k)) continue;
-            double ss = 0, sg = 0;
-            for (i64 q = M.start[k]; q < M.start[k + 1]; ++q) ss += res[p].src[M.mem[q]], sg += res[p].gnd[M.mem[q]];
-            if (ss != 0 && sg != 0) {
+            if (ss[k] != 0 && sg[k] != 0) {
                 todo[k].push_back(p);
-                out[p].solved = true;
+                solved[p] = 1;
             }
         }
+    }
     DevBuf<double> dv(n ? n : 1), dcur(n ? n : 1), dfin(n ? n : 1), scratch;
     DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
     std::vector<i32> nodes;
-    std::vector<double> vh(n), ch(n);
-    // currents of problem p on component k from the voltages in dv; both copied out
+    // currents of problem p on component k from the voltages in dv
     auto finish = [&](std::size_t p, i32 k) {
         const Resolved& r = res[p];
-        const i32* mem = M.mem.data() + M.start[k];
         const i64 count = M.start[k + 1] - M.start[k];
-        if (r.any_finite) dfin.upload(r.finite.data(), n, cx.s);
-        LSC_HIP(hipMemsetAsync(dcur.p, 0, n * sizeof(double), cx.s));
+        if (r.any_finite) {
+            const std::vector<double> fin = r.dense(r.finite, n);
+            dfin.upload(fin.data(), n, cx.s);
+        }
         graph_currents(G, dmem.p + M.start[k], static_cast<int>(count), dv.p, r.any_finite ? dfin.p : nullptr, dcur.p,
                        scratch, cx.s);
-        LSC_HIP(hipMemcpyAsync(vh.data(), dv.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
-        LSC_HIP(hipMemcpyAsync(ch.data(), dcur.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
-        LSC_HIP(hipStreamSynchronize(cx.s));
-        for (i64 q = 0; q < count; ++q) {
-            out[p].voltage[mem[q]] = vh[mem[q]];
-            out[p].current[mem[q]] = ch[mem[q]];
-        }
+        sink.component(p, k, r, dv.p, dcur.p);
     };
     for (i32 k = 0; k < g.ncomp; ++k) {
-        const auto& probs = todo[k];
-        if (probs.empty()) continue;
+        const auto& pk = todo[k];
+        if (pk.empty()) continue;
         const i32* mem = M.mem.data() + M.start[k];
         const i64 count = M.start[k + 1] - M.start[k];
         std::vector<i32> F;
         for (i64 q = 0; q < count; ++q)
             if (special[mem[q]]) F.push_back(mem[q]);
         const std::size_t nf = F.size();
-        if (!allow_schur || nf > 2048 || probs.size() < 2) {
-            for (std::size_t p : probs) {
+        if (!allow_schur || nf > 2048 || pk.size() < 2) {
+            for (std::size_t p : pk) {
                 const Resolved& r = res[p];
-                HostMatrix A = component_matrix(g, mem, count, r.any_finite ? r.finite.data() : nullptr,
-                                                [&](i32 v) { return r.gnd[v] == kInf; }, nodes);
+                const std::vector<double> fin = r.any_finite ? r.dense(r.finite, n) : std::vector<double>{};
+                HostMatrix A = component_matrix(g, mem, count, r.any_finite ? fin.data() : nullptr,
+                                                [&](i32 v) { return r.at(r.gnd, v) == kInf; }, nodes);
                 std::vector<double> full(n, 0.0);
                 if (A.n > 0) {
                     SystemSolver S(cx.precision, cx.s);
                     S.factor(A);
                     std::vector<double> b(A.n), xv(A.n);
-                    for (int i = 0; i < A.n; ++i) b[i] = r.src[nodes[i]];
+                    for (int i = 0; i < A.n; ++i) b[i] = r.at(r.src, nodes[i]);
                     S.solve(1, b.data(), xv.data(), cx.tol, cx.maxit, cx.st);
                     tally(cx, S);
                     for (int i = 0; i < A.n; ++i) full[nodes[i]] = xv[i];
@@ -630,10 +671,15 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
             }
             sptr[f + 1] = static_cast<long long>(sidx.size());
         }
+        uidx.clear();
+        uidx.shrink_to_fit();
+        fidx.clear();
+        fidx.shrink_to_fit();
         DevBuf<double> Z(na > 0 ? static_cast<std::size_t>(na) * nf : 1);
         if (na > 0) {
             SystemSolver sys(cx.precision, cx.s);
             sys.factor(A);
+            A = HostMatrix{};
             const long long tot = na * static_cast<long long>(nf);
             hipLaunchKernelGGL(k_fill_d, dim3(nblk(tot, 256)), dim3(256), 0, cx.s, Z.p, tot, 0.0);
             if (!sidx.empty()) {
@@ -648,11 +694,11 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
                     double* zc = Z.p + static_cast<long long>(f0) * na;
                     sys.solve_device(kc, zc, zc, cx.tol, cx.maxit, cx.st);
                 }
+                tally(cx, sys);
                 DevBuf<long long> dsp = to_device(sptr, cx.s);
                 DevBuf<double> dS(nf * nf);
                 hipLaunchKernelGGL(k_schur, dim3(nblk(static_cast<long long>(nf) * nf, 256)), dim3(256), 0, cx.s,
                                    static_cast<int>(nf), na, dsp.p, dbe.p, dbw.p, Z.p, dS.p);
-                tally(cx, sys);
                 std::vector<double> corr(nf * nf);
                 LSC_HIP(hipMemcpyAsync(corr.data(), dS.p, nf * nf * sizeof(double), hipMemcpyDeviceToHost, cx.s));
                 LSC_HIP(hipStreamSynchronize(cx.s));
@@ -662,18 +708,18 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
         for (std::size_t f = 0; f < nf; ++f) S[f * nf + f] += kShift;
         DevBuf<int> dnodes = to_device(nodes.empty() ? std::vector<i32>{0} : nodes, cx.s);
         DevBuf<int> dF = to_device(F, cx.s);
-        DevBuf<double> dy(nf), dyf(nf);
-        for (std::size_t p : probs) {
+        DevBuf<double> dyf(nf);
+        for (std::size_t p : pk) {
             const Resolved& r = res[p];
             std::vector<std::size_t> U;
             for (std::size_t f = 0; f < nf; ++f)
-                if (r.gnd[F[f]] != kInf) U.push_back(f);
+                if (r.at(r.gnd, F[f]) != kInf) U.push_back(f);
             const std::size_t nu = U.size();
             std::vector<double> Am(nu * nu), y(nu);
             for (std::size_t a = 0; a < nu; ++a) {
                 for (std::size_t b = 0; b < nu; ++b) Am[a * nu + b] = S[U[a] * nf + U[b]];
-                if (r.any_finite) Am[a * nu + a] += r.finite[F[U[a]]];
-                y[a] = r.src[F[U[a]]];
+                if (r.any_finite) Am[a * nu + a] += r.at(r.finite, F[U[a]]);
+                y[a] = r.at(r.src, F[U[a]]);
             }
             dense_spd_solve(Am, y, nu);
             std::vector<double> yf(nf, 0.0);
@@ -688,8 +734,170 @@ std::vector<NodeSol> advanced_many(const RGraph& g, const DevGraph& G, const std
             finish(p, k);
         }
     }
-    return out;
+    return solved;
 }
+
+// Per-problem node voltages and currents on the host (the binding's outputs).
+struct NodeSink : Sink {
+    const RGraph& g;
+    const Members& M;
+    hipStream_t s;
+    std::vector<NodeSol> out;
+    std::vector<double> vh, ch;
+    NodeSink(const RGraph& g_, const Members& M_, std::size_t np, hipStream_t s_)
+        : g(g_), M(M_), s(s_), out(np), vh(g_.n), ch(g_.n) {
+        for (auto& o : out) o.voltage.assign(g.n, 0.0), o.current.assign(g.n, 0.0);
+    }
+    void component(std::size_t p, i32 k, const Resolved&, const double* d_v, const double* d_cur) override {
+        const i32* mem = M.mem.data() + M.start[k];
+        const i64 count = M.start[k + 1] - M.start[k];
+        LSC_HIP(hipMemcpyAsync(vh.data(), d_v, g.n * sizeof(double), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(ch.data(), d_cur, g.n * sizeof(double), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipStreamSynchronize(s));
+        for (i64 q = 0; q < count; ++q) {
+            out[p].voltage[mem[q]] = vh[mem[q]];
+            out[p].current[mem[q]] = ch[mem[q]];
+        }
+    }
+};
+
+std::vector<NodeSol> advanced_nodes(const RGraph& g, const DevGraph& G, const std::vector<Problem>& probs,
+                                    const std::vector<i32>& only, int policy, bool allow_schur, Ctx& cx) {
+    const Members M = members_of(g);
+    NodeSink sink(g, M, probs.size(), cx.s);
+    const std::vector<char> solved = advanced_many(g, G, probs, only, policy, allow_schur, cx, sink);
+    for (std::size_t p = 0; p < probs.size(); ++p) {
+        sink.out[p].solved = solved[p] != 0;
+        const Resolved r = resolve(probs[p], policy);
+        for (std::size_t i = 0; i < r.node.size(); ++i)
+            if (r.src[i] != 0 || r.gnd[i] != 0) sink.out[p].active.push_back(r.node[i]);
+    }
+    return std::move(sink.out);
+}
+
+// Problems given as cell lists: per-node sums in column-major cell order
+// (python.cpp:437-450); the check cell restricts a problem to its component
+// (-1: every component; a check cell without a node: none).
+void problems_from_cells(const RGraph& g, i64 nprob, const std::int64_t* sptr, const std::int64_t* scells,
+                         const double* svals, const std::int64_t* gptr, const std::int64_t* gcells,
+                         const double* gvals, const std::int64_t* check, std::vector<Problem>& probs,
+                         std::vector<i32>& only) {
+    const i64 ncell = g.nx * g.ny;
+    probs.assign(nprob, Problem{});
+    only.assign(nprob, -1);
+    for (i64 p = 0; p < nprob; ++p) {
+        // (node, column-major position, value, is-ground)
+        std::vector<std::tuple<i32, i64, int, double>> e;
+        auto add = [&](const std::int64_t* cells, const double* vals, i64 a, i64 b, int kind) {
+            for (i64 k = a; k < b; ++k) {
+                const std::int64_t c = cells[k];
+                if (c < 0 || c >= ncell) throw std::invalid_argument("source or ground cell out of range");
+                const i32 v = g.nodemap[c];
+                if (v >= 0 && vals[k] != 0) e.emplace_back(v, (c % g.nx) * g.ny + c / g.nx, kind, vals[k]);
+            }
+        };
+        add(scells, svals, sptr[p], sptr[p + 1], 0);
+        add(gcells, gvals, gptr[p], gptr[p + 1], 1);
+        std::stable_sort(e.begin(), e.end(), [](const auto& x, const auto& y) {
+            return std::make_pair(std::get<0>(x), std::get<1>(x)) < std::make_pair(std::get<0>(y), std::get<1>(y));
+        });
+        Problem& pb = probs[p];
+        for (const auto& [v, pos, kind, val] : e) {
+            (void)pos;
+            if (pb.node.empty() || pb.node.back() != v) {
+                pb.node.push_back(v);
+                pb.src.push_back(0.0);
+                pb.gnd.push_back(0.0);
+            }
+            (kind == 0 ? pb.src : pb.gnd).back() += val;
+        }
+        if (check[p] >= 0) {
+            if (check[p] >= ncell) throw std::invalid_argument("check cell out of range");
+            const i32 o = g.nodemap[check[p]];
+            only[p] = o >= 0 ? o : -2;
+        }
+    }
+}
+
+// cum[c] += cur[node of c], mx[c] = max(mx[c], cur[node of c]) over the cells
+// of one component.
+__global__ void k_cell_acc(long long cnt, const int* __restrict__ cells, const int* __restrict__ nodemap,
+                           const double* __restrict__ cur, double* __restrict__ cum, double* __restrict__ mx) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const int c = cells[i];
+    const double x = cur[nodemap[c]];
+    cum[c] += x;
+    if (mx) mx[c] = fmax(mx[c], x);
+}
+
+__global__ void k_zero_at(int cnt, const int* __restrict__ idx, double* __restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) v[idx[i]] = 0.0;
+}
+
+__global__ void k_max0(long long n, double* __restrict__ mx) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) mx[i] = fmax(mx[i], 0.0);
+}
+
+// The cumulative and maximum current maps over the problems, summed on the
+// device per cell in problem order (as run_onetoall sums its per-point maps,
+// circuitscape.py:826-848), and each problem's voltage at its check node.
+struct SumSink : Sink {
+    const RGraph& g;
+    hipStream_t s;
+    bool focal_zero;
+    std::vector<i64> cstart;     // cells of component k: cells[cstart[k] .. cstart[k + 1])
+    DevBuf<int> dcells, dnodemap, dz;
+    DevBuf<double> cum, mx;
+    bool with_max;
+    std::vector<i32> check_node;
+    std::vector<double> vcheck;
+    SumSink(const RGraph& g_, hipStream_t s_, bool focal_zero_, const double* cum_in, const double* max_in,
+            std::vector<i32> check_node_)
+        : g(g_), s(s_), focal_zero(focal_zero_), with_max(max_in != nullptr), check_node(std::move(check_node_)),
+          vcheck(check_node.size(), 0.0) {
+        const i64 ncell = g.nx * g.ny;
+        cstart.assign(g.ncomp + 1, 0);
+        for (i64 c = 0; c < ncell; ++c)
+            if (g.nodemap[c] >= 0) ++cstart[g.comp[g.nodemap[c]] + 1];
+        for (i32 k = 0; k < g.ncomp; ++k) cstart[k + 1] += cstart[k];
+        std::vector<int> cells(cstart[g.ncomp] > 0 ? cstart[g.ncomp] : 1);
+        std::vector<i64> fill(cstart.begin(), cstart.end() - 1);
+        for (i64 c = 0; c < ncell; ++c)
+            if (g.nodemap[c] >= 0) cells[fill[g.comp[g.nodemap[c]]]++] = static_cast<int>(c);
+        dcells = to_device(cells, s);
+        dnodemap = to_device(g.nodemap, s);
+        cum.alloc(ncell ? ncell : 1);
+        if (ncell) cum.upload(cum_in, ncell, s);
+        if (with_max) {
+            mx.alloc(ncell ? ncell : 1);
+            if (ncell) mx.upload(max_in, ncell, s);
+        }
+    }
+    void component(std::size_t p, i32 k, const Resolved& r, const double* d_v, const double* d_cur) override {
+        double* cur = const_cast<double*>(d_cur);
+        if (focal_zero) {
+            std::vector<int> act;
+            for (std::size_t i = 0; i < r.node.size(); ++i)
+                if ((r.src[i] != 0 || r.gnd[i] != 0) && g.comp[r.node[i]] == k) act.push_back(r.node[i]);
+            if (!act.empty()) {
+                dz = to_device(act, s);
+                hipLaunchKernelGGL(k_zero_at, dim3(nblk(static_cast<long long>(act.size()), 256)), dim3(256), 0, s,
+                                   static_cast<int>(act.size()), dz.p, cur);
+            }
+        }
+        const long long cnt = cstart[k + 1] - cstart[k];
+        if (cnt > 0)
+            hipLaunchKernelGGL(k_cell_acc, dim3(nblk(cnt, 256)), dim3(256), 0, s, cnt, dcells.p + cstart[k], dnodemap.p,
+                               d_cur, cum.p, with_max ? mx.p : nullptr);
+        const i32 v = check_node[p];
+        if (v >= 0 && g.comp[v] == k)
+            LSC_HIP(hipMemcpyAsync(&vcheck[p], d_v + v, sizeof(double), hipMemcpyDeviceToHost, s));
+        if (focal_zero) LSC_HIP(hipStreamSynchronize(s));  // dz is reused
+    }
+};
 
 std::vector<double> node_sums(const RGraph& g, const double* cells) {
     std::vector<double> s(g.n, 0.0);
@@ -999,7 +1207,7 @@ int lsc_circuit_advanced(const double* conductance, const double* source, const 
             gnd[v] = ground ? ground[c] : 0.0;
         }
         // a source on a ground is dropped (grid.cpp:110-123): the rmvsrc policy
-        auto sol = lsc::advanced_many(G, DG, {src}, {gnd}, {-1}, 1, false, cx);
+        auto sol = lsc::advanced_nodes(G, DG, {lsc::problem_from_dense(src, gnd)}, {-1}, 1, false, cx);
         for (i64 c = 0; c < n; ++c) {
             const i32 v = G.nodemap[c];
             current[c] = v >= 0 ? sol[0].current[v] : 0.0;
@@ -1019,8 +1227,9 @@ int lsc_circuitscape_advanced(const double* gmap, const int64_t* polymap, const 
         lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
         lsc::DevGraph DG;
         DG.upload(G, cx.s);
-        auto sol = lsc::advanced_many(G, DG, {lsc::node_sums(G, source)}, {lsc::node_sums(G, ground)}, {-1}, policy, false,
-                                      cx);
+        auto sol = lsc::advanced_nodes(
+            G, DG, {lsc::problem_from_dense(lsc::node_sums(G, source), lsc::node_sums(G, ground))}, {-1}, policy, false,
+            cx);
         for (i64 c = 0; c < nx * ny; ++c) nodemap[c] = G.nodemap[c];
         std::copy(sol[0].voltage.begin(), sol[0].voltage.end(), voltage);
         std::copy(sol[0].current.begin(), sol[0].current.end(), current);
@@ -1066,34 +1275,10 @@ int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, 
         lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
         lsc::DevGraph DG;
         DG.upload(G, cx.s);
-        // per-node sums in column-major cell order (python.cpp:437-450)
-        auto sums = [&](const int64_t* cells, const double* vals, int64_t a, int64_t b) {
-            std::vector<int64_t> order;
-            for (int64_t k = a; k < b; ++k) order.push_back(k);
-            std::stable_sort(order.begin(), order.end(), [&](int64_t p, int64_t q) {
-                return std::make_pair(cells[p] % nx, cells[p] / nx) < std::make_pair(cells[q] % nx, cells[q] / nx);
-            });
-            std::vector<double> s(G.n, 0.0);
-            for (int64_t k : order) {
-                if (cells[k] < 0 || cells[k] >= nx * ny) throw std::invalid_argument("source or ground cell out of range");
-                const i32 v = G.nodemap[cells[k]];
-                if (v >= 0 && vals[k] != 0) s[v] += vals[k];
-            }
-            return s;
-        };
-        std::vector<std::vector<double>> srcs, gnds;
+        std::vector<lsc::Problem> probs;
         std::vector<i32> only;
-        for (int64_t p = 0; p < nprob; ++p) {
-            srcs.push_back(sums(scells, svals, sptr[p], sptr[p + 1]));
-            gnds.push_back(sums(gcells, gvals, gptr[p], gptr[p + 1]));
-            i32 o = -1;
-            if (check[p] >= 0) {
-                o = G.nodemap[check[p]];
-                if (o < 0) o = -2;
-            }
-            only.push_back(o);
-        }
-        auto sols = lsc::advanced_many(G, DG, srcs, gnds, only, policy, true, cx);
+        lsc::problems_from_cells(G, nprob, sptr, scells, svals, gptr, gcells, gvals, check, probs, only);
+        auto sols = lsc::advanced_nodes(G, DG, probs, only, policy, true, cx);
         const i64 n = nx * ny;
         for (i64 c = 0; c < n; ++c) nodemap[c] = G.nodemap[c];
         for (int64_t p = 0; p < nprob; ++p) {
@@ -1106,6 +1291,46 @@ int lsc_circuitscape_multi_advanced(const double* gmap, const int64_t* polymap, 
                 active[p * n + c] = v >= 0 ? act[v] : 0;
             }
             solved[p] = sols[p].solved;
+        }
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_circuitscape_multi_advanced_sum(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                                        int avg_resistances, int four_neighbors, int policy, int64_t nprob,
+                                        const int64_t* sptr, const int64_t* scells, const double* svals,
+                                        const int64_t* gptr, const int64_t* gcells, const double* gvals,
+                                        const int64_t* check, int focal_zero, const lsc_solve_options* options,
+                                        int32_t* nodemap, double* cum, double* max, double* check_voltage,
+                                        int32_t* solved, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (policy < 0 || policy > 3) throw std::invalid_argument("remove_src_or_gnd must be keepall, rmvsrc, rmvgnd or rmvall");
+        if (!cum) throw std::invalid_argument("the cumulative map is required");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        const auto h0 = std::chrono::steady_clock::now();
+        lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        std::vector<lsc::Problem> probs;
+        std::vector<i32> only;
+        lsc::problems_from_cells(G, nprob, sptr, scells, svals, gptr, gcells, gvals, check, probs, only);
+        std::vector<i32> cnode(nprob, -1);
+        for (int64_t p = 0; p < nprob; ++p) cnode[p] = only[p] >= 0 ? only[p] : -1;
+        lsc::SumSink sink(G, cx.s, focal_zero != 0, cum, max, cnode);
+        cx.st.ms_fill += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+        const std::vector<char> ok = lsc::advanced_many(G, DG, probs, only, policy, true, cx, sink);
+        const i64 n = nx * ny;
+        if (max && nprob > 0 && n > 0)
+            hipLaunchKernelGGL(lsc::k_max0, dim3(lsc::nblk(n, 256)), dim3(256), 0, cx.s, n, sink.mx.p);
+        if (n > 0) {
+            LSC_HIP(hipMemcpyAsync(cum, sink.cum.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+            if (max) LSC_HIP(hipMemcpyAsync(max, sink.mx.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+        }
+        LSC_HIP(hipStreamSynchronize(cx.s));
+        for (i64 c = 0; c < n; ++c) nodemap[c] = G.nodemap[c];
+        for (int64_t p = 0; p < nprob; ++p) {
+            check_voltage[p] = sink.vcheck[p];
+            solved[p] = ok[p];
         }
         lsc::fill_stats(cx, stats);
     });

@@ -172,3 +172,40 @@ def test_multi_advanced_one_to_all_finite_grounds(backend, oracle, policy):
         np.testing.assert_allclose(r["voltage"][nm[on]], ref["voltage"][p][on], rtol=1e-9, atol=1e-12 * top)
         np.testing.assert_allclose(r["current"][nm[on]], ref["current"][p][on], rtol=1e-9,
                                    atol=1e-12 * np.abs(ref["current"][p]).max())
+
+
+@pytest.mark.parametrize("focal_zero", [False, True])
+def test_multi_advanced_sum_matches_per_problem_maps(backend, focal_zero):
+    # the device reduction against the per-problem outputs summed on the host, in problem order
+    rng = np.random.default_rng(21)
+    g = np.exp(rng.normal(0, 1, (50, 64)))
+    g[rng.random(g.shape) < 0.1] = 0.0
+    g[:, 30] = 0.0  # two components
+    pts = [3 * 64 + 3, 45 * 64 + 50, 20 * 64 + 10, 40 * 64 + 5, 10 * 64 + 60, 25 * 64 + 40]
+    problems = []
+    for c in pts:
+        others = [p for p in pts if p != c]
+        problems.append({"src_cells": np.array(others), "src_vals": np.ones(len(others)),
+                         "gnd_cells": np.array([c]), "gnd_vals": np.array([np.inf]), "check_cell": c})
+    backend.settings.precision = "mixed"
+    full = backend.circuitscape_multi_advanced(g, None, False, False, problems, "rmvsrc")
+    nm = full["nodemap"]
+    on = nm >= 0
+    cum0 = np.zeros(g.shape)
+    mx0 = np.full(g.shape, -9999.0)
+    for r in full["results"]:
+        cur = np.zeros(g.shape)
+        cur[on] = r["current"][nm[on]]
+        if focal_zero:
+            cur[np.isin(nm, r["active"])] = 0.0
+        cum0 += cur
+        np.maximum(mx0, cur, out=mx0)
+    cum = np.zeros(g.shape)
+    mx = np.full(g.shape, -9999.0)
+    out = backend.circuitscape_multi_advanced_sum(g, None, False, False, problems, "rmvsrc", cum, mx, focal_zero)
+    np.testing.assert_array_equal(out["nodemap"], nm)
+    np.testing.assert_array_equal(out["solved"], [r["solved"] for r in full["results"]])
+    np.testing.assert_array_equal(cum, cum0)
+    np.testing.assert_array_equal(mx, mx0)
+    for p, (c, r) in enumerate(zip(pts, full["results"])):
+        assert out["check_voltage"][p] == r["voltage"][nm.flat[c]]

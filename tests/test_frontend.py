@@ -44,6 +44,24 @@ class OracleCore:
     def circuitscape_pairwise(self, g, poly, cells, ex, avg, four, opts, emit=None, n_threads=0):
         return self.o.cs_pairwise(g, poly, cells, ex, avg, four, opts, emit)
 
+    def circuitscape_multi_advanced_sum(self, g, poly, avg, four, problems, policy, cum, max_map, focal_zero=False,
+                                        n_threads=0):
+        out = self.circuitscape_multi_advanced(g, poly, avg, four, problems, policy)
+        nm = out["nodemap"]
+        on = nm >= 0
+        vcheck = np.zeros(len(problems))
+        for p, (q, r) in enumerate(zip(problems, out["results"])):
+            cur = np.zeros(nm.shape)
+            cur[on] = r["current"][nm[on]]
+            if focal_zero:
+                cur[np.isin(nm, r["active"])] = 0.0
+            cum += cur
+            if max_map is not None:
+                np.maximum(max_map, cur, out=max_map)
+            node = nm.flat[q["check_cell"]]
+            vcheck[p] = r["voltage"][node] if node >= 0 else 0.0
+        return {"nodemap": nm, "check_voltage": vcheck, "solved": np.array([r["solved"] for r in out["results"]])}
+
     def circuitscape_multi_advanced(self, g, poly, avg, four, problems, policy, n_threads=0):
         r = self.o.cs_multi_advanced(g, poly, avg, four, problems, policy)
         nm = r["nodemap"]
@@ -148,3 +166,19 @@ def test_all_to_one_kirchhoff(api):
     for pid, cell in ((1, (1, 1)), (2, (8, 1)), (3, (5, 8))):
         assert out.maps[("voltage", pid)][cell] == 0.0
         np.testing.assert_allclose(out.maps[("current", pid)][cell], 2.0, rtol=1e-9)
+
+
+def test_one_to_all_maps_and_sums_agree(api):
+    # the per-point maps path and the reduced (cum / max only) path of run_onetoall
+    r, pts = _column_map()
+    r = r * np.exp(np.random.default_rng(3).normal(0, 0.5, r.shape))
+    for all_to_one in (False, True):
+        a = api.circuitscape_one_to_all(r, pts, all_to_one=all_to_one, cum_map=True, max_map=True,
+                                        set_focal_node_currents_to_zero=True)
+        b = api.circuitscape_one_to_all(r, pts, all_to_one=all_to_one, current_maps=True, max_map=True,
+                                        set_focal_node_currents_to_zero=True)
+        np.testing.assert_array_equal(a.resistances, b.resistances)
+        np.testing.assert_allclose(a.cum_current, b.cum_current, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(a.max_current, b.max_current, rtol=1e-13, atol=1e-15)
+        total = sum(v for k, v in b.maps.items() if k[0] == "current")
+        np.testing.assert_allclose(b.cum_current, total, rtol=1e-13, atol=1e-15)
