@@ -2,6 +2,10 @@
 // layout self-test.
 #include <hip/hip_runtime.h>
 
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+
 #include <cstring>
 #include <string>
 
@@ -10,6 +14,18 @@
 #include "front.hpp"
 
 namespace lsc {
+
+void progress(const char* fmt, ...) {
+    static const bool on = std::getenv("LSC_PROGRESS") != nullptr;
+    if (!on) return;
+    std::va_list ap;
+    va_start(ap, fmt);
+    std::fputs("[lsc] ", stderr);
+    std::vfprintf(stderr, fmt, ap);
+    std::fputc('\n', stderr);
+    std::fflush(stderr);
+    va_end(ap);
+}
 
 namespace {
 thread_local std::string g_error;

@@ -647,6 +647,8 @@ std::vector<char> advanced_many(const RGraph& g, const DevGraph& G, const std::v
             }
             continue;
         }
+        progress("component %d: %lld nodes, %zu special, %zu problems (Schur)", k, static_cast<long long>(count), nf,
+                 pk.size());
         HostMatrix A = component_matrix(g, mem, count, nullptr, [&](i32 v) { return special[v] != 0; }, nodes);
         const long long na = A.n;
         std::vector<i32> uidx(n, -1), fidx(n, -1);
@@ -722,6 +724,7 @@ std::vector<char> advanced_many(const RGraph& g, const DevGraph& G, const std::v
                 y[a] = r.at(r.src, F[U[a]]);
             }
             dense_spd_solve(Am, y, nu);
+            progress("problem %zu of component %d", p, k);
             std::vector<double> yf(nf, 0.0);
             for (std::size_t a = 0; a < nu; ++a) yf[U[a]] = y[a];
             dyf.upload(yf.data(), nf, cx.s);

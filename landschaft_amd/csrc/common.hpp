@@ -69,6 +69,9 @@ DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
 
 void set_error(const std::string& msg);
 
+// One line on stderr per stage of a long graph solve when LSC_PROGRESS is set.
+void progress(const char* fmt, ...);
+
 template <class F>
 int guarded(F&& f) {
     try {

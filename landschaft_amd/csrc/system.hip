@@ -524,7 +524,10 @@ struct SystemSolver::Impl {
 template <typename T>
 void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms) {
     const auto h0 = std::chrono::steady_clock::now();
+    progress("analysis of %d unknowns", m.n);
     Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
+    progress("plan: %d fronts, L %lld entries, largest front %d", p.nfront, static_cast<long long>(p.lsize),
+             p.max_front);
     plan.upload(std::move(p), s, true);
     const Plan& h = plan.host;
     const DPlan& P = plan.d;
@@ -656,6 +659,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
                            hl.dt.p, hl.doff.p, Fp, Lp, Up);
     }
     LSC_HIP(hipEventRecord(ev1, s));
+    progress("factor launched");
     int h_fail = 0;
     LSC_HIP(hipMemcpyAsync(&h_fail, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
     LSC_HIP(hipStreamSynchronize(s));
@@ -784,6 +788,7 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         hipLaunchKernelGGL(k_g_step, g2, dim3(256), 0, s, n, pv.p, rz_new, pq.p, x.p, done.p);
         h_open = residual_check(0);
         show(it, rz_new);
+        progress("solve of %d right-hand sides: iteration %d, %d open", k, it + 1, h_open);
     }
     hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, xb.p, d_x);
     std::vector<int> its(k);
