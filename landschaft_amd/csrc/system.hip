@@ -370,32 +370,39 @@ __device__ __forceinline__ void stage_tri(const T* __restrict__ Lt, int m, int j
     tri[threadIdx.x] = (r < jb && c < jb && r >= c) ? static_cast<double>(Lt[hcol(j0 + c, m) + (r - c)]) : 0.0;
 }
 
-// Forward substitution of the large fronts of one level: workgroup
-// (f, right-hand side). The vector lives in LDS; per block of 16 pivots the
-// diagonal block is staged in LDS, wavefront 0 solves it, and the workgroup
+// Forward substitution of the large fronts of one level for RB right-hand
+// sides at once: workgroup (f, g) holds the vectors of right-hand sides
+// g RB .. in LDS side by side, so every L column block is read once for all
+// of them. Per block of 16 pivots the diagonal block is staged in LDS, the
+// wavefronts solve it for one right-hand side each, and the workgroup
 // updates the rows below (coalesced along the columns of L).
-template <typename T>
-__global__ __launch_bounds__(256) void k_h_forward(DPlan P, const int* __restrict__ ht, int n, long long vstride,
-                                                   const T* __restrict__ L, const double* __restrict__ rs,
-                                                   double* __restrict__ y, double* __restrict__ V,
-                                                   const int* __restrict__ done) {
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void k_h_forward(DPlan P, const int* __restrict__ ht, int n, int k,
+                                                   long long vstride, const T* __restrict__ L,
+                                                   const double* __restrict__ rs, double* __restrict__ y,
+                                                   double* __restrict__ V, const int* __restrict__ done) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int inst = blockIdx.y;
-    if (done[inst]) return;
     const int t = ht[blockIdx.x];
+    const int g0 = blockIdx.y * RB;
     double* tri = reinterpret_cast<double*>(smem_raw);
-    double* fv = tri + 256;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    const long long off = static_cast<long long>(inst) * n;
-    double* Vi = V + static_cast<long long>(inst) * vstride;
-    for (int i = threadIdx.x; i < m; i += 256) fv[i] = i < np ? rs[off + first + i] : 0.0;
+    double* fv = tri + 256;  // fv[r * m + i]
+    for (int r = 0; r < RB; ++r) {
+        const int inst = g0 + r < k ? g0 + r : k - 1;
+        const long long off = static_cast<long long>(inst) * n;
+        for (int i = threadIdx.x; i < m; i += 256) fv[r * m + i] = i < np ? rs[off + first + i] : 0.0;
+    }
     __syncthreads();
     for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
         const int c = P.kids[ci];
         const int u = P.nupd[c];
         const int* rel = P.relmap + P.upd[c];
-        for (int ii = threadIdx.x; ii < u; ii += 256) fv[rel[ii]] += Vi[P.vpos[c] + ii];
+        for (int r = 0; r < RB; ++r) {
+            const int inst = g0 + r < k ? g0 + r : k - 1;
+            const double* Vi = V + static_cast<long long>(inst) * vstride + P.vpos[c];
+            for (int ii = threadIdx.x; ii < u; ii += 256) fv[r * m + rel[ii]] += Vi[ii];
+        }
         __syncthreads();
     }
     const T* Lt = L + P.lpos[t];
@@ -403,49 +410,67 @@ __global__ __launch_bounds__(256) void k_h_forward(DPlan P, const int* __restric
         const int jb = min(16, np - j0);
         stage_tri<T>(Lt, m, j0, jb, tri);
         __syncthreads();
-        if (wave == 0) {
-            double f = lane < jb ? fv[j0 + lane] : 0.0;
+        for (int r = wave; r < RB; r += 4) {
+            double f = lane < jb ? fv[r * m + j0 + lane] : 0.0;
             for (int q = 0; q < jb; ++q) {
                 const double yq = bcast(f, q) / tri[q * 17];
                 if (lane > q && lane < jb) f -= tri[q * 16 + lane] * yq;
                 if (lane == q) f = yq;
             }
-            if (lane < jb) fv[j0 + lane] = f;
+            if (lane < jb) fv[r * m + j0 + lane] = f;
         }
         __syncthreads();
         for (int i = j0 + jb + threadIdx.x; i < m; i += 256) {
-            double a = 0.0;
-            for (int q = 0; q < jb; ++q) a += static_cast<double>(Lt[hcol(j0 + q, m) + i - j0 - q]) * fv[j0 + q];
-            fv[i] -= a;
+            double Lq[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                Lq[q] = q < jb ? static_cast<double>(Lt[hcol(j0 + q, m) + i - j0 - q]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                double a = 0.0;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) a += Lq[q] * fv[r * m + j0 + (q < jb ? q : 0)];
+                fv[r * m + i] -= a;
+            }
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < m; i += 256) {
-        if (i < np) y[off + first + i] = fv[i];
-        else Vi[P.vpos[t] + i - np] = fv[i];
+    for (int r = 0; r < RB; ++r) {
+        const int inst = g0 + r;
+        if (inst >= k || done[inst]) continue;
+        const long long off = static_cast<long long>(inst) * n;
+        double* Vi = V + static_cast<long long>(inst) * vstride;
+        for (int i = threadIdx.x; i < m; i += 256) {
+            if (i < np) y[off + first + i] = fv[r * m + i];
+            else Vi[P.vpos[t] + i - np] = fv[r * m + i];
+        }
     }
 }
 
-// Back substitution of the large fronts of one level, left-looking: per
-// block of 16 pivots (from the last) each column's dot product with the
-// solved part below is taken by one wavefront along the column (coalesced),
-// then wavefront 0 solves the staged diagonal block.
-template <typename T>
-__global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restrict__ ht, int n,
+// Back substitution of the large fronts of one level for RB right-hand sides
+// at once, left-looking: per block of 16 pivots (from the last) each
+// column's dot products with the solved part below are taken by one
+// wavefront along the column (coalesced, one L read for all right-hand
+// sides), then the wavefronts solve the staged diagonal block.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restrict__ ht, int n, int k,
                                                     const T* __restrict__ L, const double* __restrict__ y,
                                                     double* __restrict__ z, const int* __restrict__ done) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int inst = blockIdx.y;
-    if (done[inst]) return;
     const int t = ht[blockIdx.x];
+    const int g0 = blockIdx.y * RB;
     double* tri = reinterpret_cast<double*>(smem_raw);
-    double* dots = tri + 256;
-    double* fv = dots + 16;
+    double* dots = tri + 256;       // dots[r * 16 + cc]
+    double* fv = dots + 16 * RB;    // fv[r * m + i]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    const long long off = static_cast<long long>(inst) * n;
     const int* rows = P.rows + P.upd[t];
-    for (int i = threadIdx.x; i < m; i += 256) fv[i] = i < np ? y[off + first + i] : z[off + rows[i - np]];
+    for (int r = 0; r < RB; ++r) {
+        const int inst = g0 + r < k ? g0 + r : k - 1;
+        const long long off = static_cast<long long>(inst) * n;
+        for (int i = threadIdx.x; i < m; i += 256)
+            fv[r * m + i] = i < np ? y[off + first + i] : z[off + rows[i - np]];
+    }
     __syncthreads();
     const T* Lt = L + P.lpos[t];
     for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
@@ -453,25 +478,40 @@ __global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restri
         for (int cc = wave; cc < jb; cc += 4) {
             const int i = jlo + cc;
             const long long base = hcol(i, m) - i;  // L(r, i) = Lt[base + r]
-            double a = 0.0;
-            for (int r = j1 + 1 + lane; r < m; r += 64) a += static_cast<double>(Lt[base + r]) * fv[r];
-            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
-            if (lane == 0) dots[cc] = a;
+            double a[RB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) a[r] = 0.0;
+            for (int row = j1 + 1 + lane; row < m; row += 64) {
+                const double l = static_cast<double>(Lt[base + row]);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) a[r] += l * fv[r * m + row];
+            }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                double v = a[r];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                if (lane == 0) dots[r * 16 + cc] = v;
+            }
         }
         stage_tri<T>(Lt, m, jlo, jb, tri);
         __syncthreads();
-        if (wave == 0) {
-            double v = lane < jb ? fv[jlo + lane] - dots[lane] : 0.0;
+        for (int r = wave; r < RB; r += 4) {
+            double v = lane < jb ? fv[r * m + jlo + lane] - dots[r * 16 + lane] : 0.0;
             for (int q = jb - 1; q >= 0; --q) {
                 const double xq = bcast(v, q) / tri[q * 17];
                 if (lane < q) v -= tri[lane * 16 + q] * xq;
                 if (lane == q) v = xq;
             }
-            if (lane < jb) fv[jlo + lane] = v;
+            if (lane < jb) fv[r * m + jlo + lane] = v;
         }
         __syncthreads();
     }
-    for (int j = threadIdx.x; j < np; j += 256) z[off + first + j] = fv[j];
+    for (int r = 0; r < RB; ++r) {
+        const int inst = g0 + r;
+        if (inst >= k || done[inst]) continue;
+        const long long off = static_cast<long long>(inst) * n;
+        for (int j = threadIdx.x; j < np; j += 256) z[off + first + j] = fv[r * m + j];
+    }
 }
 
 // The large fronts of one level, their tile arena and child lists.
@@ -480,6 +520,7 @@ struct HugeLevel {
     std::vector<long long> off;         // tile offsets in the arena
     long long tiles = 0;                // arena size of the level
     int maxKt = 0;                      // panel steps of the level's largest front
+    int mmax = 0;                       // rows of the level's largest front (RHS row excluded)
     std::vector<int> rb, tb;            // per panel step: largest row-block and tile-block grids
     std::vector<std::vector<int>> chf, chc;  // per child rank: (front index, child)
     std::vector<std::vector<long long>> cbk; // per child rank: workgroup prefix
@@ -518,6 +559,10 @@ struct SystemSolver::Impl {
     template <typename T>
     void sweep_t(int k, int mode);
     template <typename T>
+    void huge_forward(const HugeLevel& H, int k);
+    template <typename T>
+    void huge_backward(const HugeLevel& H, int k);
+    template <typename T>
     void solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);
 };
 
@@ -549,7 +594,10 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
         }
         if (hl.t.empty()) continue;
         huge_tiles = std::max(huge_tiles, hl.tiles);
-        for (int t : hl.t) hl.maxKt = std::max(hl.maxKt, (h.npiv[t] + 15) / 16);
+        for (int t : hl.t) {
+            hl.maxKt = std::max(hl.maxKt, (h.npiv[t] + 15) / 16);
+            hl.mmax = std::max(hl.mmax, h.npiv[t] + h.nupd[t]);
+        }
         hl.rb.assign(hl.maxKt, 1);
         hl.tb.assign(hl.maxKt, 0);
         long long work = 1;
@@ -676,33 +724,76 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     D.release();
 }
 
+// Right-hand sides per workgroup of a level's large-front solves: as many as
+// their vectors fit in 120 KB of LDS, at most 8.
+inline int huge_rb(int mmax, int k) {
+    const int fit = static_cast<int>((120 * 1024 / 8 - 272) / std::max(mmax, 1));
+    int rb = 1;
+    while (rb * 2 <= std::min(fit, 8) && rb < k) rb *= 2;
+    return rb;
+}
+
+template <typename T, int RB>
+void huge_forward_rb(const DPlan& P, const HugeLevel& H, int k, const T* L, const double* rs, double* y, double* V,
+                     const int* done, hipStream_t s) {
+    const std::size_t lds = (256 + static_cast<std::size_t>(RB) * H.mmax) * sizeof(double);
+    sys_allow(reinterpret_cast<const void*>(&k_h_forward<T, RB>), lds);
+    hipLaunchKernelGGL((k_h_forward<T, RB>), dim3(static_cast<unsigned>(H.t.size()), (k + RB - 1) / RB), dim3(256),
+                       lds, s, P, H.dt.p, P.n, k, P.vsize, L, rs, y, V, done);
+}
+
+template <typename T, int RB>
+void huge_backward_rb(const DPlan& P, const HugeLevel& H, int k, const T* L, const double* y, double* z,
+                      const int* done, hipStream_t s) {
+    const std::size_t lds = (256 + 16 * RB + static_cast<std::size_t>(RB) * H.mmax) * sizeof(double);
+    sys_allow(reinterpret_cast<const void*>(&k_h_backward<T, RB>), lds);
+    hipLaunchKernelGGL((k_h_backward<T, RB>), dim3(static_cast<unsigned>(H.t.size()), (k + RB - 1) / RB), dim3(256),
+                       lds, s, P, H.dt.p, P.n, k, L, y, z, done);
+}
+
+template <typename T>
+void SystemSolver::Impl::huge_forward(const HugeLevel& H, int k) {
+    if (H.t.empty()) return;
+    const T* Lp = reinterpret_cast<const T*>(L.p);
+    switch (huge_rb(H.mmax, k)) {
+        case 8: huge_forward_rb<T, 8>(plan.d, H, k, Lp, rs.p, y.p, V.p, done.p, s); break;
+        case 4: huge_forward_rb<T, 4>(plan.d, H, k, Lp, rs.p, y.p, V.p, done.p, s); break;
+        case 2: huge_forward_rb<T, 2>(plan.d, H, k, Lp, rs.p, y.p, V.p, done.p, s); break;
+        default: huge_forward_rb<T, 1>(plan.d, H, k, Lp, rs.p, y.p, V.p, done.p, s); break;
+    }
+}
+
+template <typename T>
+void SystemSolver::Impl::huge_backward(const HugeLevel& H, int k) {
+    if (H.t.empty()) return;
+    const T* Lp = reinterpret_cast<const T*>(L.p);
+    switch (huge_rb(H.mmax, k)) {
+        case 8: huge_backward_rb<T, 8>(plan.d, H, k, Lp, y.p, z.p, done.p, s); break;
+        case 4: huge_backward_rb<T, 4>(plan.d, H, k, Lp, y.p, z.p, done.p, s); break;
+        case 2: huge_backward_rb<T, 2>(plan.d, H, k, Lp, y.p, z.p, done.p, s); break;
+        default: huge_backward_rb<T, 1>(plan.d, H, k, Lp, y.p, z.p, done.p, s); break;
+    }
+}
+
 template <typename T>
 void SystemSolver::Impl::sweep_t(int k, int mode) {
     const DPlan& P = plan.d;
     const Plan& h = plan.host;
     const T* Lp = reinterpret_cast<const T*>(L.p);
-    auto grid = [&](int lv) {
-        return dim3(static_cast<unsigned>(static_cast<long long>(k) * (h.lev_ptr[lv + 1] - h.lev_ptr[lv])));
-    };
     auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
-    const std::size_t hl = static_cast<std::size_t>(huge_rows + 272) * sizeof(double);
+    auto rgrid = [&](int lv) {
+        return dim3(static_cast<unsigned>(h.lev_ptr[lv + 1] - h.lev_ptr[lv]), static_cast<unsigned>((k + kRhs - 1) / kRhs));
+    };
     if (mode == 1)
         for (int lv = 0; lv < P.H; ++lv) {
-            auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
-            hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, rs.p, y.p,
-                               V.p);
-            const HugeLevel& H = lev_huge[lv];
-            if (!H.t.empty())
-                hipLaunchKernelGGL(k_h_forward<T>, dim3(static_cast<unsigned>(H.t.size()), k), dim3(256), hl, s, P,
-                                   H.dt.p, P.n, P.vsize, Lp, rs.p, y.p, V.p, done.p);
+            auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_rhs<T, 1> : k_forward_rhs<T, 3>;
+            hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, rs.p, y.p, V.p);
+            huge_forward<T>(lev_huge[lv], k);
         }
     for (int lv = P.H - 1; lv >= 0; --lv) {
-        const HugeLevel& H = lev_huge[lv];
-        if (!H.t.empty())
-            hipLaunchKernelGGL(k_h_backward<T>, dim3(static_cast<unsigned>(H.t.size()), k), dim3(256), hl, s, P,
-                               H.dt.p, P.n, Lp, y.p, z.p, done.p);
-        auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
-        hipLaunchKernelGGL(kern, grid(lv), dim3(64), lds(lv), s, P, lv, k, 0ll, active.p, done.p, Lp, y.p, z.p);
+        huge_backward<T>(lev_huge[lv], k);
+        auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_rhs<T, 1> : k_backward_rhs<T, 3>;
+        hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, y.p, z.p);
     }
 }
 
@@ -735,14 +826,10 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     int lmax = 1;
     for (int v : plan.lev_lmax) lmax = std::max(lmax, v);
     const std::size_t lds = 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);
-    sys_allow(reinterpret_cast<const void*>(&k_forward_level<T, 1>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_forward_level<T, 3>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 1>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_backward_level<T, 3>), lds);
-    if (huge_rows > 0) {
-        sys_allow(reinterpret_cast<const void*>(&k_h_forward<T>), static_cast<std::size_t>(huge_rows + 272) * 8);
-        sys_allow(reinterpret_cast<const void*>(&k_h_backward<T>), static_cast<std::size_t>(huge_rows + 272) * 8);
-    }
+    sys_allow(reinterpret_cast<const void*>(&k_forward_rhs<T, 1>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_forward_rhs<T, 3>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 1>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 3>), lds);
     const long long kn = static_cast<long long>(k) * n;
     const dim3 g2(nb, k);
     sys_reset(done.p, k, s);

@@ -745,6 +745,182 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
 
+// Solves of one system for many right-hand sides (graphs: one L shared by
+// every instance): workgroup (f, g) is one wavefront that stages front f's L
+// block in LDS once and solves right-hand sides g kRhs.. against it, so L is
+// read once per kRhs right-hand sides instead of once each.
+constexpr int kRhs = 16;
+
+template <typename T, int R>
+__device__ __forceinline__ void forward_compute(const T* Lc, const double* fv, int np, int m, int lane, double* yw,
+                                                double* Vt) {
+    double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        f[r] = i < m ? fv[i] : 0.0;
+    }
+    for (int j0 = 0; j0 < np; j0 += 16) {
+        T Lr[R][16];
+        double dinv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j0 + q;
+            const bool okj = j < np;
+            const long long cs = okj ? colstart(j, m) : 0;
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[cs]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                Lr[r][q] = (okj && i > j && i < m) ? Lc[cs + i - j] : T(0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j0 + q;
+            if (j < np) {
+                const int slot = j >> 6;
+                const double yj = bcast(pick(f, slot), j & 63) * dinv[q];
+#pragma unroll
+                for (int r = 0; r < R; ++r) f[r] -= static_cast<double>(Lr[r][q]) * yj;
+                if (lane == (j & 63)) {
+                    if (slot == 0) f[0] = yj;
+                    else if (slot == 1) f[1] = yj;
+                    else f[2] = yj;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < np) yw[i] = f[r];
+        else if (i < m) Vt[i - np] = f[r];
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void backward_compute(const T* Lc, const double* xu, const double* ywf, int np, int nu,
+                                                 int m, int lane, double* zw) {
+    double tv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = lane + 64 * r;
+        double acc = 0.0;
+        if (j < np) {
+            const T* col = Lc + colstart(j, m) + (np - j);
+            for (int ii = 0; ii < nu; ++ii) acc += static_cast<double>(col[ii]) * xu[ii];
+            acc = ywf[j] - acc;
+        }
+        tv[r] = acc;
+    }
+    for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
+        T Lr[R][16];
+        double dinv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j1 - q;
+            const bool okj = j >= 0;
+            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[colstart(j, m)]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                Lr[r][q] = (okj && i < j) ? Lc[colstart(i, m) + (j - i)] : T(0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j1 - q;
+            if (j >= 0) {
+                const int slot = j >> 6;
+                const double xj = bcast(pick(tv, slot), j & 63) * dinv[q];
+#pragma unroll
+                for (int r = 0; r < R; ++r) tv[r] -= static_cast<double>(Lr[r][q]) * xj;
+                if (lane == (j & 63)) {
+                    if (slot == 0) tv[0] = xj;
+                    else if (slot == 1) tv[1] = xj;
+                    else tv[2] = xj;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < np) zw[i] = tv[r];
+    }
+}
+
+// The L block of front t into LDS (16-byte chunks; blocks start on 16 bytes).
+template <typename T>
+__device__ __forceinline__ void stage_L(const T* L, long long lpos, int np, int m, int lane, T* Lc) {
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(L + lpos);
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll 8
+    for (int e = lane; e < nv; e += 64) dst[e] = src[e];
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_forward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+                                                    const T* __restrict__ L, const double* __restrict__ rs,
+                                                    double* __restrict__ y, double* __restrict__ Vall) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* fv = reinterpret_cast<double*>(smem_raw);
+    T* Lc = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    const int lane = threadIdx.x;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
+    const int cb = P.cb[t], ce = P.ce[t];
+    stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = blockIdx.y * kRhs + rr;
+        if (inst >= k) break;
+        if (done[inst]) continue;
+        const double* rsw = rs + static_cast<long long>(inst) * P.n;
+        double* V = Vall + static_cast<long long>(inst) * P.vsize;
+        for (int i = lane; i < m; i += 64) fv[i] = i < np ? rsw[first + i] : 0.0;
+        wave_sync();
+        for (int ci = cb; ci < ce; ++ci) {
+            const int c = P.kids[ci];
+            const int u = P.nupd[c];
+            const int* rel = P.relmap + P.upd[c];
+            const double* Vc = V + P.vpos[c];
+            for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
+            wave_sync();
+        }
+        forward_compute<T, R>(Lc, fv, np, m, lane, y + static_cast<long long>(inst) * P.n + first, V + P.vpos[t]);
+        wave_sync();
+    }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_backward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+                                                     const T* __restrict__ L, const double* __restrict__ y,
+                                                     double* __restrict__ z) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* xu = reinterpret_cast<double*>(smem_raw);
+    T* Lc = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    const int lane = threadIdx.x;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
+    const int* rows = P.rows + P.upd[t];
+    stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = blockIdx.y * kRhs + rr;
+        if (inst >= k) break;
+        if (done[inst]) continue;
+        const long long off = static_cast<long long>(inst) * P.n;
+        for (int ii = lane; ii < nu; ii += 64) xu[ii] = z[off + rows[ii]];
+        wave_sync();
+        backward_compute<T, R>(Lc, xu, y + off + first, np, nu, m, lane, z + off + first);
+        wave_sync();
+    }
+}
+
 // x = s z (mode 0) or x += s z (mode 1) for the instances not done.
 static __global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, long long sc_stride,
                                                               const int* __restrict__ active,
