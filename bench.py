@@ -72,8 +72,11 @@ def workload(n: int = 4096, seed: int = 103, nodata: float = 0.30, sigma: float 
     return np.ascontiguousarray(r), np.ascontiguousarray(s)
 
 
+BATCH = 4096  # windows per batch
+
+
 def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=True):
-    o = N.options(precision, tol, 30, 4096, timing)
+    o = N.options(precision, tol, 30, BATCH, timing)
     st = N.SolveStats()
     nt = ctypes.c_int64(0)
     N.check(N.lib().lsc_circuit_omniscape_device(
@@ -183,6 +186,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the pairwise and adjoint graph workloads")
+    ap.add_argument("--batch", type=int, default=4096, help="Omniscape windows per batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by row bands "
@@ -191,6 +195,9 @@ def main() -> int:
     import torch
 
     from landschaft_amd import _native as N
+
+    global BATCH
+    BATCH = a.batch
 
     n = a.size
     t0 = time.perf_counter()

@@ -5,7 +5,8 @@
 // (src/cpp/circuit/omniscape.cpp:394-608). Here every window of a run, also
 // the ones clipped at the map edges, is the same W x W disk with the cells
 // off the map out of the circuit, so all windows share one symbolic plan and
-// a batch of thousands of windows is factorised by one persistent kernel.
+// a batch of thousands of windows is factorised level by level, every front
+// of a level in one launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
