@@ -72,7 +72,7 @@ def workload(n: int = 4096, seed: int = 103, nodata: float = 0.30, sigma: float 
     return np.ascontiguousarray(r), np.ascontiguousarray(s)
 
 
-BATCH = 4096  # windows per batch
+BATCH = 16384  # windows per batch
 
 
 def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=True):
@@ -186,7 +186,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the pairwise and adjoint graph workloads")
-    ap.add_argument("--batch", type=int, default=4096, help="Omniscape windows per batch")
+    ap.add_argument("--batch", type=int, default=16384, help="Omniscape windows per batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by row bands "

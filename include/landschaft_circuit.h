@@ -30,9 +30,9 @@ enum { LSC_PRECISION_MIXED = 0, LSC_PRECISION_FP64 = 1 };
  * MFMA fronts + FP64 iterative refinement, default) or LSC_PRECISION_FP64;
  * tol: relative residual ||b - Ax|| / ||b|| at which refinement stops
  * (default 1e-12); max_refine: refinement steps per circuit (default 30);
- * batch: circuits per batch (default 4096); timing: 1 to time every phase
- * with HIP events (adds synchronisations); stream: a hipStream_t, 0 for the
- * null stream. */
+ * batch: circuits per batch (default 16384, fewer when HBM is short);
+ * timing: 1 to time every phase with HIP events (adds synchronisations);
+ * stream: a hipStream_t, 0 for the null stream. */
 typedef struct lsc_solve_options {
     int32_t precision;
     int32_t max_refine;

@@ -27,7 +27,7 @@ class SolverSettings:
     precision: str = "mixed"
     tol: float = 1e-12
     max_refine: int = 30
-    batch: int = 4096
+    batch: int = 16384
 
 
 settings = SolverSettings()

@@ -121,8 +121,11 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
     long long ua = 0;
     for (int k = 0; k < h.nfront; ++k) {
         const int t = h.order[k];
-        uposA[t] = ua;  // 4-entry aligned: the extend-add reads 16-byte chunks
-        ua += (static_cast<long long>(h.nupd[t] + 1) * (h.nupd[t] + 1) + 3) & ~3ll;
+        // the lower triangle of the (nu+1)^2 update matrix packed by columns,
+        // 4-entry aligned: the extend-add reads 16-byte chunks
+        uposA[t] = ua;
+        const long long u1 = h.nupd[t] + 1;
+        ua += (u1 * (u1 + 1) / 2 + 3) & ~3ll;
     }
     ibufs.clear();
     lbufs.clear();
@@ -422,14 +425,17 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     Plan plan = window_plan(Lt.D, kind == 0, a.four != 0, nd_leaf());
     // per-window device memory: maps, vectors, L, update matrices and vectors
     long long ua = 0;
-    for (int t = 0; t < plan.nfront; ++t) ua += (static_cast<long long>(plan.nupd[t] + 1) * (plan.nupd[t] + 1) + 3) & ~3ll;
+    for (int t = 0; t < plan.nfront; ++t) {
+        const long long u1 = plan.nupd[t] + 1;
+        ua += (u1 * (u1 + 1) / 2 + 3) & ~3ll;
+    }
     const double per_window = static_cast<double>(Lt.W) * Lt.W * 33.0 + plan.n * 7.0 * 8.0 +
                               static_cast<double>(plan.lsize + ua + plan.fe_row.size()) * sizeof(T) +
                               plan.vsize * 8.0;
     std::size_t free_b = 0, total_b = 0;
     LSC_HIP(hipMemGetInfo(&free_b, &total_b));
     const long long fit = std::max<long long>(1, static_cast<long long>(0.5 * static_cast<double>(free_b) / per_window));
-    const long long B = std::min({a.batch > 0 ? a.batch : 4096, Lt.nlat, fit});
+    const long long B = std::min({a.batch > 0 ? a.batch : 16384, Lt.nlat, fit});
     WindowSolver<T> ws(s);
     ws.set_plan(std::move(plan), static_cast<int>(B));
     hipEvent_t ev[4] = {};
@@ -518,7 +524,7 @@ lsc::OmniArgs omni_args(int radius, int buffer, int block_size, int four, int rc
     a.precision = opt ? opt->precision : LSC_PRECISION_MIXED;
     a.tol = opt && opt->tol > 0 ? opt->tol : 1e-12;
     a.maxit = opt && opt->max_refine > 0 ? opt->max_refine : 30;
-    a.batch = opt && opt->batch > 0 ? opt->batch : 4096;
+    a.batch = opt && opt->batch > 0 ? opt->batch : 16384;
     a.timing = opt ? opt->timing : 0;
     return a;
 }

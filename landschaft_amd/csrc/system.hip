@@ -287,7 +287,7 @@ __global__ void k_h_extend(DPlan P, int nch, const int* __restrict__ chf, const 
     if (ii < jj) return;
     const int* rel = P.relmap + P.upd[c];
     T* Ft = F + hoff[f] * 256;
-    Ft[tix(ii < u ? rel[ii] : m, rel[jj])] += U[P.uposA[c] + e];
+    Ft[tix(ii < u ? rel[ii] : m, rel[jj])] += U[P.uposA[c] + colstart(jj, u1) + (ii - jj)];
 }
 
 // Panel step Kt of every large front still pivoting: workgroup (bx, f)
@@ -350,11 +350,11 @@ __global__ void k_h_store(DPlan P, const int* __restrict__ ht, const long long* 
         if (i < j) continue;
         Lt[static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1) + (i - j)] = Ft[tix(i, j)];
     }
-    T* Ut = U + P.uposA[t];
+    T* Ut = U + P.uposA[t];  // the lower triangle packed by columns
     for (long long e = tid; e < static_cast<long long>(u1) * u1; e += nt) {
         const int jj = static_cast<int>(e / u1), ii = static_cast<int>(e - static_cast<long long>(jj) * u1);
         if (ii < jj) continue;
-        Ut[e] = Ft[tix(np + ii, np + jj)];
+        Ut[colstart(jj, u1) + (ii - jj)] = Ft[tix(np + ii, np + jj)];
     }
 }
 

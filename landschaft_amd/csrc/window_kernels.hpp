@@ -312,6 +312,17 @@ __device__ __forceinline__ long long colstart(int j, int m) {
     return static_cast<long long>(j) * m - ((static_cast<long long>(j) * (j - 1)) >> 1);
 }
 
+// Column of entry p of a lower triangle of order m packed by columns: the
+// largest j with colstart(j, m) <= p.
+__device__ __forceinline__ int packed_col(long long p, int m) {
+    const double b = 2.0 * m + 1.0;
+    int j = static_cast<int>((b - sqrt(b * b - 8.0 * static_cast<double>(p))) * 0.5);
+    if (j < 0) j = 0;
+    while (j > 0 && colstart(j, m) > p) --j;
+    while (j + 1 < m && colstart(j + 1, m) <= p) ++j;
+    return j;
+}
+
 // Element r of a 16-byte chunk of T values.
 template <typename T>
 __device__ __forceinline__ T chunk_elem(const uint4& v, int r);
@@ -372,7 +383,7 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
         if (tid == 0) relbuf[u] = m;
         sync();
         constexpr int CH = 16 / static_cast<int>(sizeof(T));
-        const int tot = u1 * u;
+        const int tot = static_cast<int>(colstart(u, u1));  // columns 0..u-1 of the packed triangle
         const int nch = (tot + CH - 1) / CH;
         const uint4* Uv = reinterpret_cast<const uint4*>(Uc);
         for (int base = tid; base < nch; base += 8 * nt) {
@@ -387,11 +398,12 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
                 const int c = base + q * nt;
                 if (c < nch) {
                     const int e0 = c * CH;
-                    int jj = e0 / u1, ii = e0 - jj * u1;
+                    int jj = packed_col(e0, u1);
+                    int ii = jj + static_cast<int>(e0 - colstart(jj, u1));
 #pragma unroll
                     for (int r = 0; r < CH; ++r) {
-                        if (e0 + r < tot && ii >= jj) F[tix(relbuf[ii], relbuf[jj])] += chunk_elem<T>(v[q], r);
-                        if (++ii == u1) ii = 0, ++jj;
+                        if (e0 + r < tot) F[tix(relbuf[ii], relbuf[jj])] += chunk_elem<T>(v[q], r);
+                        if (++ii == u1) ++jj, ii = jj;
                     }
                 }
             }
@@ -407,8 +419,10 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
     for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
     const int u1 = nu + 1;
     T* Ut = U + P.uposA[t];
-    for (int jj = wave; jj <= nu; jj += nw)
-        for (int ii = jj + lane; ii <= nu; ii += 64) Ut[jj * u1 + ii] = F[tix(np + ii, np + jj)];
+    for (int jj = wave; jj <= nu; jj += nw) {
+        T* col = Ut + colstart(jj, u1) - jj;
+        for (int ii = jj + lane; ii <= nu; ii += 64) col[ii] = F[tix(np + ii, np + jj)];
+    }
     sync();
 }
 
