@@ -81,7 +81,7 @@ def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=Tr
     nt = ctypes.c_int64(0)
     N.check(N.lib().lsc_circuit_omniscape_device(
         ctypes.c_void_p(d_r), ctypes.c_void_p(d_s), ctypes.c_int64(nx), ctypes.c_int64(ny), radius, 0, block, 0, 0,
-        1, 0, ctypes.byref(o), ctypes.c_void_p(d_cur), None, ctypes.byref(nt), ctypes.byref(st)))
+        1, 0, None, ctypes.byref(o), ctypes.c_void_p(d_cur), None, ctypes.byref(nt), ctypes.byref(st)))
     return st.as_dict()
 
 

@@ -43,6 +43,19 @@ typedef struct lsc_solve_options {
     void* stream;
 } lsc_solve_options;
 
+/* One condition of Omniscape's conditional connectivity (reference
+ * src/cpp/circuit/omniscape.hpp:28-33): a source cell keeps its amps when its
+ * present value v matches the target block's value b of the future map
+ * (median for "within", mode for "equal"): "within" lower <= v - b <= upper,
+ * "equal" v == b. NaN present values always keep theirs. */
+typedef struct lsc_omniscape_condition {
+    const double* present; /* ny x nx; NULL: no condition */
+    const double* future;  /* ny x nx; NULL: the present map */
+    int32_t equal;         /* 1: "equal", 0: "within" */
+    int32_t reserved;
+    double lower, upper;
+} lsc_omniscape_condition;
+
 /* Per-run statistics: targets, circuits solved, refinement steps, the
  * largest and mean relative FP64 residual of the solved circuits, and (with
  * timing) milliseconds spent in each phase. The graph entry points (advanced,
@@ -71,20 +84,23 @@ int lsc_device_info(char* name, int64_t name_len, int32_t* compute_units, int64_
 
 /* Omniscape's cumulative current (and flow potential) over moving windows.
  * Replaces circuit_omniscape (src/cpp/circuit/python.cpp:90-137 ->
- * circuit::omniscape, src/cpp/circuit/omniscape.cpp:394-608) without the
- * conditional-connectivity arguments. resistance: NaN = NoData, values > 0;
- * source: NaN = 0. current (and flow_potential_out when flow_potential != 0)
- * receive ny x nx float64 maps; n_targets the number of targets. Host
- * pointers. */
+ * circuit::omniscape, src/cpp/circuit/omniscape.cpp:394-608). resistance:
+ * NaN = NoData, values > 0; source: NaN = 0. conditions: NULL or two
+ * conditions (the binding's condition1 / condition2, present = NULL for
+ * none). current (and flow_potential_out when flow_potential != 0) receive
+ * ny x nx float64 maps; n_targets the number of targets. Host pointers. */
 int lsc_circuit_omniscape(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
                           int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
-                          int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                          int correct_artifacts, int flow_potential,
+                          const lsc_omniscape_condition* conditions, const lsc_solve_options* options,
                           double* current, double* flow_potential_out, int64_t* n_targets, lsc_solve_stats* stats);
 
-/* The same with every array in device memory (HBM-resident inputs). */
+/* The same with every array in device memory (HBM-resident inputs), the
+ * condition maps included. */
 int lsc_circuit_omniscape_device(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
                                  int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
-                                 int correct_artifacts, int flow_potential, const lsc_solve_options* options,
+                                 int correct_artifacts, int flow_potential,
+                                 const lsc_omniscape_condition* conditions, const lsc_solve_options* options,
                                  double* current, double* flow_potential_out, int64_t* n_targets,
                                  lsc_solve_stats* stats);
 

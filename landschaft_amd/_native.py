@@ -31,6 +31,12 @@ class SolveOptions(ctypes.Structure):
                 ("stream", ctypes.c_void_p)]
 
 
+class OmniscapeCondition(ctypes.Structure):
+    _fields_ = [("present", ctypes.POINTER(ctypes.c_double)), ("future", ctypes.POINTER(ctypes.c_double)),
+                ("equal", ctypes.c_int32), ("reserved", ctypes.c_int32), ("lower", ctypes.c_double),
+                ("upper", ctypes.c_double)]
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("n_targets", ctypes.c_int64), ("n_solved", ctypes.c_int64), ("n_unconverged", ctypes.c_int64),
                 ("n_over_1e10", ctypes.c_int64), ("refine_max", ctypes.c_int32), ("n_batches", ctypes.c_int32),

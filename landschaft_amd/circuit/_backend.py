@@ -50,15 +50,37 @@ def _record(stats: N.SolveStats):
 
 def circuit_omniscape(resistance, source, radius, buffer, block_size, four_neighbors, resistance_is_conductance,
                       correct_artifacts, flow_potential, n_threads=0, conditions=None, memory_limit=0.0):
-    """Omniscape's cumulative current (python.cpp:90-137). ``n_threads`` and
-    ``memory_limit`` size the reference's CPU threads and are accepted and
-    ignored; conditional connectivity is not available on this path."""
-    if conditions and any(v is not None for k, v in conditions.items() if k.startswith("condition")):
-        raise ValueError("conditional connectivity is not supported by the GPU circuit path")
+    """Omniscape's cumulative current (python.cpp:90-137), conditions as the binding's dict
+    (condition1, condition1_future, comparison1, condition1_lower, condition1_upper; the same for 2).
+    ``n_threads`` and ``memory_limit`` size the reference's CPU threads and are accepted and ignored."""
     r, s = N.f64(resistance), N.f64(source)
     if r.ndim != 2 or s.shape != r.shape:
         raise ValueError("expected resistance and source rasters of the same shape")
     ny, nx = r.shape
+    conds = None
+    keep = []
+    if conditions and conditions.get("condition1") is not None:
+        conds = (N.OmniscapeCondition * 2)()
+        for i, k in enumerate((1, 2)):
+            present = conditions.get(f"condition{k}")
+            if present is None:
+                continue
+            cmp = conditions.get(f"comparison{k}", "within")
+            if cmp not in ("within", "equal"):
+                raise ValueError("comparisons must be 'within' or 'equal'")
+            p = N.f64(present)
+            if p.shape != r.shape:
+                raise ValueError(f"condition{k} must have the resistance's shape")
+            f = conditions.get(f"condition{k}_future")
+            f = None if f is None else N.f64(f)
+            if f is not None and f.shape != r.shape:
+                raise ValueError(f"condition{k}_future must have the resistance's shape")
+            keep += [p, f]
+            conds[i].present = N.ptr(p)
+            conds[i].future = N.ptr(f)
+            conds[i].equal = 1 if cmp == "equal" else 0
+            conds[i].lower = float(conditions[f"condition{k}_lower"])
+            conds[i].upper = float(conditions[f"condition{k}_upper"])
     cur = np.empty((ny, nx))
     fp = np.empty((ny, nx)) if flow_potential else None
     nt = ctypes.c_int64(0)
@@ -67,7 +89,7 @@ def circuit_omniscape(resistance, source, radius, buffer, block_size, four_neigh
     N.check(N.lib().lsc_circuit_omniscape(N.ptr(r), N.ptr(s), ctypes.c_int64(nx), ctypes.c_int64(ny), int(radius),
                                           int(buffer), int(block_size), int(bool(four_neighbors)),
                                           int(bool(resistance_is_conductance)), int(bool(correct_artifacts)),
-                                          int(bool(flow_potential)), ctypes.byref(o), N.ptr(cur), N.ptr(fp),
+                                          int(bool(flow_potential)), conds, ctypes.byref(o), N.ptr(cur), N.ptr(fp),
                                           ctypes.byref(nt), ctypes.byref(st)))
     _record(st)
     out = {"current": cur, "n_targets": nt.value, "threads": 1}

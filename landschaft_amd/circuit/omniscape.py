@@ -105,6 +105,17 @@ def source_from_resistance(resistance, *, r_cutoff: float = np.inf, resistance_i
     return np.where(np.isnan(s) | (s < 1 / float(r_cutoff)), 0.0, s)
 
 
+def _condition(value, shape, name, dtype):
+    if value is None:
+        return None
+    v, _ = _grid(value, name)
+    if v.shape != shape:
+        raise ValueError(f"{name} must have the resistance's shape")
+    if dtype is np.float32:
+        v = v.astype(np.float32).astype(np.float64)
+    return np.ascontiguousarray(v)
+
+
 def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source_threshold: float = 0.0,
               r_cutoff: float = np.inf, resistance_is_conductance: bool = False, four_neighbors: bool = False,
               flow_potential: bool = False, normalized_current: bool = False, correct_artifacts: bool = True,
@@ -120,9 +131,11 @@ def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source
     ``block_size`` cells with a positive source sum is a target, its window of
     ``radius + buffer`` is solved with the target grounded, and the windows'
     currents add up to ``cum_currmap`` (plus ``flow_potential`` and
-    ``normalized_cum_currmap`` when asked for). Windows are solved on the GPU
-    in batches; conditional connectivity (``condition1``...) is not
-    available on this path and raises ``ValueError``.
+    ``normalized_cum_currmap`` when asked for). With ``condition1`` (and
+    ``condition2``), a window's sources whose condition does not match the
+    target block's (future) value are dropped after scaling: ``within`` keeps
+    lower <= condition - median(block) <= upper, ``equal`` keeps condition ==
+    mode(block). Windows are solved on the GPU in batches.
     """
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {PRECISIONS}")
@@ -161,12 +174,37 @@ def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source
         if finite.size and finite.min() <= 0:
             raise ValueError("the reclassified resistance must be > 0")
     r = np.ascontiguousarray(r)
-    if any(c is not None for c in (condition1, condition2, condition1_future, condition2_future)):
-        raise ValueError("conditional connectivity is not supported by the GPU circuit path")
+    # conditional connectivity (omniscape.py:344-368 of the reference)
+    conditions = {}
+    if condition2 is not None and condition1 is None:
+        raise ValueError("condition2 needs condition1")
+    if condition1 is not None:
+        if compare_to_future not in COMPARE_TO_FUTURE:
+            raise ValueError(f"compare_to_future must be one of {COMPARE_TO_FUTURE}")
+        for k, cmp in ((1, comparison1), (2, comparison2)):
+            if cmp not in COMPARISONS:
+                raise ValueError(f"comparison{k} must be one of {COMPARISONS}")
+        future1 = condition1_future if compare_to_future in ("1", "both") else None
+        future2 = condition2_future if compare_to_future in ("2", "both") else None
+        if compare_to_future in ("1", "both") and condition1_future is None:
+            raise ValueError("compare_to_future needs condition1_future")
+        if compare_to_future in ("2", "both") and condition2 is not None and condition2_future is None:
+            raise ValueError("compare_to_future needs condition2_future")
+        conditions = {
+            "condition1": _condition(condition1, r.shape, "condition1", dtype),
+            "condition1_future": _condition(future1, r.shape, "condition1_future", dtype),
+            "condition2": _condition(condition2, r.shape, "condition2", dtype),
+            "condition2_future": _condition(future2, r.shape, "condition2_future", dtype),
+            "comparison1": comparison1, "comparison2": comparison2,
+            "condition1_lower": float(condition1_lower), "condition1_upper": float(condition1_upper),
+            "condition2_lower": float(condition2_lower), "condition2_upper": float(condition2_upper),
+        }
+    elif any(c is not None for c in (condition1_future, condition2_future)):
+        raise ValueError("future conditions need condition1")
     compute_fp = bool(flow_potential or normalized_current)
     out = core().circuit_omniscape(r, s, radius, buffer, block_size, bool(four_neighbors),
                                    bool(resistance_is_conductance), bool(correct_artifacts), compute_fp,
-                                   int(n_threads), {}, 0.0 if memory_limit is None else float(memory_limit))
+                                   int(n_threads), conditions, 0.0 if memory_limit is None else float(memory_limit))
     cum = out["current"]
     data = {"cum_currmap": cum}
     nodata = np.isnan(r)
@@ -186,7 +224,7 @@ def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source
         data = {k: v.astype(np.float32) for k, v in data.items()}
     attrs = {"radius": radius, "block_size": block_size, "buffer": buffer, "n_targets": int(out["n_targets"]),
              "four_neighbors": int(bool(four_neighbors)), "correct_artifacts": int(bool(correct_artifacts)),
-             "precision": precision, "conditional": 0, "threads": int(out["threads"])}
+             "precision": precision, "conditional": int(condition1 is not None), "threads": int(out["threads"])}
     order = ["cum_currmap", "flow_potential", "normalized_cum_currmap"]
     data = {k: data[k] for k in order if k in data}
     try:

@@ -419,8 +419,9 @@ std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hi
 // One pass over the target lattice: `kind` 0 the landscape circuits, 1 the
 // flow potential's (conductance 1 on the whole clipped window).
 template <typename T>
-void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, const OmniArgs& a, int kind,
-                  const double* d_corr, double* d_map, unsigned long long* d_ntargets, hipStream_t s, IrStats& st) {
+void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, const OmniArgs& a, const CondArgs& cond,
+                  int kind, const double* d_corr, double* d_map, unsigned long long* d_ntargets, hipStream_t s,
+                  IrStats& st) {
     if (Lt.nlat == 0) return;
     Plan plan = window_plan(Lt.D, kind == 0, a.four != 0, nd_leaf());
     // per-window device memory: maps, vectors, L, update matrices and vectors
@@ -447,8 +448,8 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     for (long long k0 = 0; k0 < Lt.nlat; k0 += B) {
         const int nb = static_cast<int>(std::min<long long>(B, Lt.nlat - k0));
         if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-        hipLaunchKernelGGL(k_fill, dim3(nb), dim3(kThreads), 0, s, d_res, d_src, L2, k0, nb, ws.wb.g.p, ws.wb.src.p,
-                           ws.wb.active.p, d_ntargets);
+        hipLaunchKernelGGL(k_fill, dim3(nb), dim3(kThreads), 0, s, d_res, d_src, L2, k0, nb, cond, ws.wb.g.p,
+                           ws.wb.src.p, ws.wb.active.p, d_ntargets);
         if (kind == 1) {
             // conductance 1 on every on-map cell of the box
             hipLaunchKernelGGL(k_unit_conductance, dim3(nb), dim3(kThreads), 0, s, L2, k0, nb, ws.wb.g.p);
@@ -483,9 +484,103 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
 
 }  // namespace
 
+// The condition value of every lattice point that is a target (a positive
+// block source sum, NaN skipped, column-major): the median ("within", the two
+// middle values halved separately for an even count) or the mode ("equal",
+// values counted by bit pattern, the first to pass the running top count
+// wins) of the block's future values in column-major order, NaN left out
+// (reference omniscape.cpp:168-215); NaN for the other lattice points.
+std::vector<double> condition_values(const double* source, const double* future, const Lattice& L, bool equal) {
+    std::vector<double> out(L.nlat, std::numeric_limits<double>::quiet_NaN());
+    std::vector<double> v;
+    std::vector<std::pair<std::uint64_t, int>> count;
+    for (long long kk = 0; kk < L.nlat; ++kk) {
+        const long long x = L.br + (kk / L.nly) * L.bs, y = L.br + (kk % L.nly) * L.bs;
+        double amps = 0.0;
+        for (long long xx = x - L.br; xx <= x + L.br; ++xx)
+            for (long long yy = y - L.br; yy <= y + L.br; ++yy) {
+                const double q = source[yy * L.nx + xx];
+                if (!std::isnan(q)) amps += q;
+            }
+        if (!(amps > 0)) continue;
+        v.clear();
+        for (long long xx = x - L.br; xx <= std::min(x + L.br, L.nx - 1); ++xx)
+            for (long long yy = y - L.br; yy <= std::min(y + L.br, L.ny - 1); ++yy) {
+                const double q = future[yy * L.nx + xx];
+                if (!std::isnan(q)) v.push_back(q);
+            }
+        if (v.empty()) throw std::runtime_error("a target block has no value of a condition (all NoData)");
+        if (!equal) {
+            std::sort(v.begin(), v.end());
+            const std::size_t n = v.size();
+            out[kk] = n % 2 ? v[n / 2] : v[n / 2 - 1] / 2 + v[n / 2] / 2;
+            continue;
+        }
+        count.clear();
+        double best = v[0];
+        int top = 0;
+        for (double q : v) {
+            std::uint64_t bits;
+            std::memcpy(&bits, &q, sizeof bits);
+            auto it = std::find_if(count.begin(), count.end(), [&](const auto& e) { return e.first == bits; });
+            if (it == count.end()) it = count.insert(count.end(), {bits, 0});
+            if (++it->second > top) top = it->second, best = q;
+        }
+        out[kk] = best;
+    }
+    return out;
+}
+
+// The device form of a run's conditions; `on_device`: the maps (and source)
+// are device pointers, else host pointers.
+struct Conditions {
+    std::vector<DevBuf<double>> bufs;
+    CondArgs args{};
+};
+
+void prepare_conditions(const lsc_omniscape_condition* c, bool on_device, const double* source, const Lattice& L,
+                        hipStream_t s, Conditions& out) {
+    out.args = CondArgs{};
+    if (!c) return;
+    const std::size_t n = static_cast<std::size_t>(L.nx) * L.ny;
+    auto host_of = [&](const double* p) {
+        std::vector<double> h(n);
+        if (n) LSC_HIP(hipMemcpyAsync(h.data(), p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipStreamSynchronize(s));
+        return h;
+    };
+    std::vector<double> src_h;
+    if (on_device) src_h = host_of(source);
+    const double* hs = on_device ? src_h.data() : source;
+    out.bufs.reserve(4);
+    for (int k = 0; k < 2; ++k) {
+        if (!c[k].present) continue;
+        const double* fut = c[k].future ? c[k].future : c[k].present;
+        std::vector<double> fut_h;
+        if (on_device) fut_h = host_of(fut);
+        const std::vector<double> val = condition_values(hs, on_device ? fut_h.data() : fut, L, c[k].equal != 0);
+        out.bufs.push_back(to_device(val, s));
+        const double* vp = out.bufs.back().p;
+        const double* pp = c[k].present;
+        if (!on_device) {
+            DevBuf<double> d(n ? n : 1);
+            if (n) d.upload(c[k].present, n, s);
+            out.bufs.push_back(std::move(d));
+            pp = out.bufs.back().p;
+        }
+        const int i = out.args.n++;
+        out.args.present[i] = pp;
+        out.args.value[i] = vp;
+        out.args.equal[i] = c[k].equal != 0;
+        out.args.lower[i] = c[k].lower;
+        out.args.upper[i] = c[k].upper;
+    }
+}
+
 template <typename T>
 void omniscape_device(const double* d_res, const double* d_src, long long nx, long long ny, const OmniArgs& a,
-                      double* d_cur, double* d_fp, long long* n_targets, IrStats& st, hipStream_t s) {
+                      const CondArgs& cond, double* d_cur, double* d_fp, long long* n_targets, IrStats& st,
+                      hipStream_t s) {
     if (a.radius < 0 || a.buffer < 0 || a.block < 1 || a.block % 2 == 0)
         throw std::invalid_argument("radius and buffer must be >= 0 and block_size odd");
     if (2 * (a.radius + a.buffer) + 1 > kMaxW)
@@ -501,10 +596,10 @@ void omniscape_device(const double* d_res, const double* d_src, long long nx, lo
         IrStats cst;
         corr = to_device(correction<T>(a, nx, ny, s, cst), s);
     }
-    lattice_pass<T>(d_res, d_src, Lt, a, 0, corr.p, d_cur, cnt.p, s, st);
+    lattice_pass<T>(d_res, d_src, Lt, a, cond, 0, corr.p, d_cur, cnt.p, s, st);
     if (a.fp) {
         IrStats fst;
-        lattice_pass<T>(d_res, d_src, Lt, a, 1, corr.p, d_fp, cnt.p + 1, s, fst);
+        lattice_pass<T>(d_res, d_src, Lt, a, cond, 1, corr.p, d_fp, cnt.p + 1, s, fst);
     }
     unsigned long long h = 0;
     LSC_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -548,34 +643,55 @@ void fill_stats(const lsc::IrStats& st, long long n_targets, lsc_solve_stats* ou
     out->ms_accumulate = st.ms_accum;
 }
 
+// The device entry with the condition maps on the host (`cond_on_device`
+// false) or on the device.
+int omniscape_run(const double* resistance, const double* source, const double* host_source, int64_t nx, int64_t ny,
+                  int radius, int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
+                  int correct_artifacts, int flow_potential, const lsc_omniscape_condition* conditions,
+                  bool cond_on_device, const lsc_solve_options* options, double* current, double* flow_potential_out,
+                  int64_t* n_targets, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        const lsc::OmniArgs a = omni_args(radius, buffer, block_size, four_neighbors, resistance_is_conductance,
+                                          correct_artifacts, flow_potential, options);
+        hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
+        lsc::Conditions cond;
+        if (conditions) {
+            if (a.block < 1 || a.block % 2 == 0) throw std::invalid_argument("block_size must be odd");
+            lsc::prepare_conditions(conditions, cond_on_device, cond_on_device ? source : host_source,
+                                    lsc::make_lattice(nx, ny, a), s, cond);
+        }
+        lsc::IrStats st;
+        long long nt = 0;
+        if (a.precision == LSC_PRECISION_FP64)
+            lsc::omniscape_device<double>(resistance, source, nx, ny, a, cond.args, current, flow_potential_out, &nt,
+                                          st, s);
+        else
+            lsc::omniscape_device<float>(resistance, source, nx, ny, a, cond.args, current, flow_potential_out, &nt,
+                                         st, s);
+        if (n_targets) *n_targets = nt;
+        fill_stats(st, nt, stats);
+    });
+}
+
 }  // namespace
 
 extern "C" {
 
 int lsc_circuit_omniscape_device(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
                                  int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
-                                 int correct_artifacts, int flow_potential, const lsc_solve_options* options,
-                                 double* current, double* flow_potential_out, int64_t* n_targets,
-                                 lsc_solve_stats* stats) {
-    return lsc::guarded([&] {
-        const lsc::OmniArgs a = omni_args(radius, buffer, block_size, four_neighbors, resistance_is_conductance,
-                                          correct_artifacts, flow_potential, options);
-        hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
-        lsc::IrStats st;
-        long long nt = 0;
-        if (a.precision == LSC_PRECISION_FP64)
-            lsc::omniscape_device<double>(resistance, source, nx, ny, a, current, flow_potential_out, &nt, st, s);
-        else
-            lsc::omniscape_device<float>(resistance, source, nx, ny, a, current, flow_potential_out, &nt, st, s);
-        if (n_targets) *n_targets = nt;
-        fill_stats(st, nt, stats);
-    });
+                                 int correct_artifacts, int flow_potential, const lsc_omniscape_condition* conditions,
+                                 const lsc_solve_options* options, double* current, double* flow_potential_out,
+                                 int64_t* n_targets, lsc_solve_stats* stats) {
+    return omniscape_run(resistance, source, nullptr, nx, ny, radius, buffer, block_size, four_neighbors,
+                         resistance_is_conductance, correct_artifacts, flow_potential, conditions, true, options,
+                         current, flow_potential_out, n_targets, stats);
 }
 
 int lsc_circuit_omniscape(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
                           int buffer, int block_size, int four_neighbors, int resistance_is_conductance,
-                          int correct_artifacts, int flow_potential, const lsc_solve_options* options,
-                          double* current, double* flow_potential_out, int64_t* n_targets, lsc_solve_stats* stats) {
+                          int correct_artifacts, int flow_potential, const lsc_omniscape_condition* conditions,
+                          const lsc_solve_options* options, double* current, double* flow_potential_out,
+                          int64_t* n_targets, lsc_solve_stats* stats) {
     return lsc::guarded([&] {
         if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
         const std::size_t n = static_cast<std::size_t>(nx) * ny;
@@ -585,9 +701,9 @@ int lsc_circuit_omniscape(const double* resistance, const double* source, int64_
             r.upload(resistance, n, s);
             src.upload(source, n, s);
         }
-        int rc = lsc_circuit_omniscape_device(r.p, src.p, nx, ny, radius, buffer, block_size, four_neighbors,
-                                              resistance_is_conductance, correct_artifacts, flow_potential, options,
-                                              cur.p, fp.p, n_targets, stats);
+        int rc = omniscape_run(r.p, src.p, source, nx, ny, radius, buffer, block_size, four_neighbors,
+                               resistance_is_conductance, correct_artifacts, flow_potential, conditions, false,
+                               options, cur.p, fp.p, n_targets, stats);
         if (rc != 0) throw lsc::HipError(lsc_last_error());
         if (n) {
             LSC_HIP(hipMemcpyAsync(current, cur.p, n * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -67,6 +67,16 @@ struct Lattice {
     int rc;  // resistance_is_conductance
 };
 
+// The conditions of a run on the device: present maps, the value of every
+// target of the lattice, the comparison.
+struct CondArgs {
+    int n;
+    const double* present[2];
+    const double* value[2];
+    int equal[2];
+    double lower[2], upper[2];
+};
+
 struct IrStats {
     long long solved = 0, unconverged = 0, over_bar = 0, iter_sum = 0, solve_launches = 0;
     int batches = 0;

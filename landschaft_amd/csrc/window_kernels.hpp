@@ -78,8 +78,9 @@ __device__ __forceinline__ void target_of(const Lattice& L, long long k, long lo
 // the target block zeroed at D from the clipped window's corner and the
 // block's strength shared out (reference omniscape.cpp:406-421, 495-525).
 static __global__ __launch_bounds__(kThreads) void k_fill(const double* __restrict__ res, const double* __restrict__ source,
-                                                   Lattice Lt, long long k0, int B, double* __restrict__ g,
-                                                   double* __restrict__ src, int* __restrict__ active,
+                                                   Lattice Lt, long long k0, int B, CondArgs cond,
+                                                   double* __restrict__ g, double* __restrict__ src,
+                                                   int* __restrict__ active,
                                                    unsigned long long* __restrict__ n_targets) {
     __shared__ double red[kWaves];
     __shared__ double s_amps;
@@ -137,6 +138,20 @@ static __global__ __launch_bounds__(kThreads) void k_fill(const double* __restri
         for (int c = threadIdx.x; c < W2; c += kThreads) {
             const double v = sw[c];
             if (v > 0) sw[c] = (v * amps) / sum;
+        }
+        // conditional connectivity (reference omniscape.cpp:217-232, 526-530):
+        // sources whose present condition does not match the block's value
+        for (int ci = 0; ci < cond.n; ++ci) {
+            const double value = cond.value[ci][kk];
+            for (int c = threadIdx.x; c < W2; c += kThreads) {
+                const int y = c / W, x = c - (c / W) * W;
+                const long long Y = oy + y, X = ox + x;
+                if (Y < 0 || Y >= Lt.ny || X < 0 || X >= Lt.nx) continue;
+                const double v = cond.present[ci][Y * Lt.nx + X];
+                if (isnan(v)) continue;
+                const bool keep = cond.equal[ci] ? v == value : (v - value <= cond.upper[ci] && v - value >= cond.lower[ci]);
+                if (!keep) sw[c] = 0.0;
+            }
         }
     }
     if (threadIdx.x == 0) {

@@ -718,8 +718,51 @@ struct Target {
 // omniscape.cpp:394-608 (no conditions). `limit` > 0 solves only the first
 // `limit` targets of the sorted order, < 0 only the last -limit (a bounded
 // sample of a run).
+// Conditional connectivity (reference omniscape.cpp:168-232): a condition's
+// present map, its future map (nullptr: the present one), "equal" or
+// "within" [lower, upper].
+struct OCond {
+    const double* present = nullptr;
+    const double* future = nullptr;
+    bool equal = false;
+    double lower = 0, upper = 0;
+};
+
+// The target block's value of a condition: median ("within", the mean of
+// the two middle values halved separately for an even count) or mode
+// ("equal": counted by bit pattern, the first value to pass the running top
+// count wins) of the future values in column-major order, NaN left out.
+double block_value(const OCond& c, i64 nx, i64 ny, i64 x, i64 y, i64 br) {
+    const double* f = c.future ? c.future : c.present;
+    std::vector<double> v;
+    for (i64 xx = x - br; xx <= std::min(x + br, nx - 1); ++xx)
+        for (i64 yy = y - br; yy <= std::min(y + br, ny - 1); ++yy) {
+            const double q = f[yy * nx + xx];
+            if (!std::isnan(q)) v.push_back(q);
+        }
+    if (v.empty()) throw std::runtime_error("a target block has no value of a condition (all NoData)");
+    if (!c.equal) {
+        std::vector<double> s = v;
+        std::sort(s.begin(), s.end());
+        const size_t n = s.size();
+        return n % 2 ? s[n / 2] : s[n / 2 - 1] / 2 + s[n / 2] / 2;
+    }
+    std::vector<std::pair<std::uint64_t, int>> count;
+    double best = v[0];
+    int top = 0;
+    for (double q : v) {
+        std::uint64_t bits;
+        std::memcpy(&bits, &q, sizeof bits);
+        auto it = std::find_if(count.begin(), count.end(), [&](const auto& e) { return e.first == bits; });
+        if (it == count.end()) it = count.insert(count.end(), {bits, 0});
+        if (++it->second > top) top = it->second, best = q;
+    }
+    return best;
+}
+
 i64 omniscape(const double* res, const double* source, i64 nx, i64 ny, int radius, int buffer, int block, bool four,
-              bool rc, bool correct_artifacts, bool fp, int threads, i64 limit, double* cur_out, double* fp_out) {
+              bool rc, bool correct_artifacts, bool fp, int threads, i64 limit, double* cur_out, double* fp_out,
+              const OCond* conds = nullptr) {
     if (radius < 0 || buffer < 0 || block < 1 || block % 2 == 0)
         throw std::invalid_argument("radius and buffer must be >= 0 and block_size odd");
     const i64 R = radius, D = radius + buffer, bs = block, br = (bs - 1) / 2, W = 2 * D + 1;
@@ -799,6 +842,19 @@ i64 omniscape(const double* res, const double* source, i64 nx, i64 ny, int radiu
                         if (src[y * s.w + x] > 0) sum += src[y * s.w + x];
                 for (i64 q = 0; q < s.h * s.w; ++q)
                     if (src[q] > 0) src[q] = (src[q] * t.amps) / sum;
+                // sources whose present condition does not match the block's value
+                for (int ci = 0; conds && ci < 2; ++ci) {
+                    const OCond& c = conds[ci];
+                    if (!c.present) continue;
+                    const double value = block_value(c, nx, ny, t.x, t.y, br);
+                    for (i64 y = 0; y < s.h; ++y)
+                        for (i64 x = 0; x < s.w; ++x) {
+                            const double v = c.present[(t.y0 + y) * nx + (t.x0 + x)];
+                            if (std::isnan(v)) continue;
+                            const bool keep = c.equal ? v == value : (v - value <= c.upper && v - value >= c.lower);
+                            if (!keep) src[y * s.w + x] = 0.0;
+                        }
+                }
                 const size_t sid = shape_id.at(s);
                 solve_window(disk[sid], g.data(), src.data(), s, four, cur[k].data());
                 if (fp) flow_window(rect[sid], s, four, src.data(), fpc[k].data());
@@ -1494,6 +1550,21 @@ int orc_omniscape(const double* r, const double* s, std::int64_t nx, std::int64_
     return orc::guarded([&] {
         *n_targets = orc::omniscape(r, s, nx, ny, radius, buffer, block, four != 0, rc != 0, correct != 0, fp != 0,
                                     threads, limit, cur, fpout);
+    });
+}
+
+// Omniscape with conditional connectivity: condition k (k = 0, 1) is given by
+// present[k] (nullptr: none), future[k], equal[k], lower[k], upper[k].
+int orc_omniscape_cond(const double* r, const double* s, std::int64_t nx, std::int64_t ny, int radius, int buffer,
+                       int block, int four, int rc, int correct, int fp, int threads, const double* const* present,
+                       const double* const* future, const int* equal, const double* lower, const double* upper,
+                       double* cur, double* fpout, std::int64_t* n_targets) {
+    return orc::guarded([&] {
+        orc::OCond c[2];
+        for (int k = 0; k < 2; ++k)
+            c[k] = orc::OCond{present[k], future[k], equal[k] != 0, lower[k], upper[k]};
+        *n_targets = orc::omniscape(r, s, nx, ny, radius, buffer, block, four != 0, rc != 0, correct != 0, fp != 0,
+                                    threads, 0, cur, fpout, c);
     });
 }
 

@@ -89,6 +89,39 @@ def omniscape(resistance, source, radius, buffer, block_size, four_neighbors, re
     return out
 
 
+def omniscape_conditions(resistance, source, radius, buffer, block_size, four_neighbors, resistance_is_conductance,
+                         correct_artifacts, flow_potential, conditions, n_threads=0):
+    """Omniscape with conditional connectivity; ``conditions`` as the binding's dict (condition1,
+    condition1_future, comparison1, condition1_lower, condition1_upper, and the same for 2)."""
+    r, s = _f64(resistance), _f64(source)
+    ny, nx = r.shape
+    keep, present, future, equal, lower, upper = [], [], [], [], [], []
+    for k in (1, 2):
+        p = conditions.get(f"condition{k}")
+        f = conditions.get(f"condition{k}_future")
+        p = None if p is None else _f64(p)
+        f = None if f is None else _f64(f)
+        keep += [p, f]
+        present.append(_p(p))
+        future.append(_p(f))
+        equal.append(1 if conditions.get(f"comparison{k}", "within") == "equal" else 0)
+        lower.append(float(conditions.get(f"condition{k}_lower", 0.0)))
+        upper.append(float(conditions.get(f"condition{k}_upper", 0.0)))
+    dp = ctypes.POINTER(ctypes.c_double)
+    cur = np.empty((ny, nx))
+    fp = np.empty((ny, nx)) if flow_potential else None
+    nt = ctypes.c_int64(0)
+    _check(lib().orc_omniscape_cond(_p(r), _p(s), ctypes.c_int64(nx), ctypes.c_int64(ny), int(radius), int(buffer),
+                                    int(block_size), int(four_neighbors), int(resistance_is_conductance),
+                                    int(correct_artifacts), int(flow_potential), int(n_threads), (dp * 2)(*present),
+                                    (dp * 2)(*future), (ctypes.c_int * 2)(*equal), (ctypes.c_double * 2)(*lower),
+                                    (ctypes.c_double * 2)(*upper), _p(cur), _p(fp), ctypes.byref(nt)))
+    out = {"current": cur, "n_targets": nt.value}
+    if flow_potential:
+        out["flow_potential"] = fp
+    return out
+
+
 def omniscape_correction(radius, buffer, block_size, four_neighbors, nx, ny):
     w = 2 * (radius + buffer) + 1
     out = np.empty((w, w))
