@@ -79,8 +79,8 @@ def circuit_omniscape(resistance, source, radius, buffer, block_size, four_neigh
             conds[i].present = N.ptr(p)
             conds[i].future = N.ptr(f)
             conds[i].equal = 1 if cmp == "equal" else 0
-            conds[i].lower = float(conditions[f"condition{k}_lower"])
-            conds[i].upper = float(conditions[f"condition{k}_upper"])
+            conds[i].lower = float(conditions.get(f"condition{k}_lower", 0.0))
+            conds[i].upper = float(conditions.get(f"condition{k}_upper", 0.0))
     cur = np.empty((ny, nx))
     fp = np.empty((ny, nx)) if flow_potential else None
     nt = ctypes.c_int64(0)
