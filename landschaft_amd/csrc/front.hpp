@@ -25,6 +25,10 @@ __device__ __forceinline__ int tix(int i, int j) {
     return tile_index(i >> 4, j >> 4) * 256 + ((j & 15) << 4) + (i & 15);
 }
 
+// The wavefront's index in its workgroup as a wave-uniform (scalar) value,
+// so that what depends on it stays in scalar registers and scalar branches.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)); }
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -54,6 +58,20 @@ struct Mfma<float> {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int row(int lane, int r) { return ((lane >> 4) << 2) + r; }
+    // the four rows of a lane are consecutive in a column-major tile: one 16-byte access
+    static __device__ __forceinline__ acc load(const float* col, int lane) {
+        return *reinterpret_cast<const acc*>(col + ((lane >> 4) << 2));
+    }
+    static __device__ __forceinline__ void store(float* col, int lane, acc v) {
+        *reinterpret_cast<acc*>(col + ((lane >> 4) << 2)) = v;
+    }
+    // pivot d = sqrt(a) and 1 / d, both correctly rounded: the hardware
+    // reciprocal square root (1 ulp, biased) makes the factor a measurably
+    // worse preconditioner of the FP64 refinement (more refinement steps)
+    static __device__ __forceinline__ void root(float a, float& d, float& inv) {
+        d = sqrtf(a);
+        inv = 1.0f / d;
+    }
 };
 template <>
 struct Mfma<double> {
@@ -62,7 +80,44 @@ struct Mfma<double> {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + (r << 2); }
+    static __device__ __forceinline__ acc load(const double* col, int lane) {
+        acc v;
+        v[0] = col[row(lane, 0)], v[1] = col[row(lane, 1)], v[2] = col[row(lane, 2)], v[3] = col[row(lane, 3)];
+        return v;
+    }
+    static __device__ __forceinline__ void store(double* col, int lane, acc v) {
+        col[row(lane, 0)] = v[0], col[row(lane, 1)] = v[1], col[row(lane, 2)] = v[2], col[row(lane, 3)] = v[3];
+    }
+    static __device__ __forceinline__ void root(double a, double& d, double& inv) {
+        d = sqrt(a);
+        inv = 1.0 / d;
+    }
 };
+
+// The 16 elimination steps of a panel held as P[16] (a row per lane, row i).
+template <typename T, bool kFull>
+__device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int k0, int kend, int cend) {
+    bool ok = true;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        if (kFull || k0 + kk < kend) {
+            const T dkk = bcast(P[kk], kk);
+            ok = ok && (dkk > T(0));
+            T d, inv;
+            Mfma<T>::root(dkk, d, inv);
+            const T l = P[kk] * inv;
+            P[kk] = i == k0 + kk ? d : l;
+#pragma unroll
+            for (int c = kk + 1; c < 16; ++c) {
+                if (kFull || k0 + c < cend) {
+                    const T ljk = bcast(l, c);
+                    P[c] -= l * ljk;
+                }
+            }
+        }
+    }
+    return ok;
+}
 
 // The panel of tile column Kt (columns k0 = 16 Kt .. cend-1 with cend =
 // min(k0 + 16, m)): columns k0..kend-1 factorised (pivots), the others of
@@ -74,35 +129,29 @@ struct Mfma<double> {
 // m - k0 - 16 <= rowbase + 48 nw. Returns false when a pivot is not positive.
 // `sync` orders the team's loads of the diagonal block before wave 0 writes
 // it back. With `diag_out`, the factored diagonal block goes there instead
-// (16 x 16, column-major) and F keeps the unfactored one, for teams spread
-// over several workgroups that cannot be ordered inside one launch.
+// (16 x 16, column-major, zero above the diagonal) and F keeps the
+// unfactored one, for teams spread over several workgroups that cannot be
+// ordered inside one launch.
+// The updates run on every lane without a branch: entries above the
+// diagonal of the diagonal block then hold values that are never read (the
+// broadcasts read rows at or below their column) nor stored.
 template <typename T, typename Sync>
 __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase, T* diag_out, Sync sync) {
     const int cend = min(k0 + 16, m);
     const int i = lane < 16 ? k0 + lane : k0 + 16 + rowbase + 48 * wave + (lane - 16);
     const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0 && diag_out == nullptr));
     T P[16];
+    if (i < m) {
 #pragma unroll
-    for (int c = 0; c < 16; ++c) P[c] = (i < m && k0 + c < cend && i >= k0 + c) ? F[tix(i, k0 + c)] : T(0);
-    sync();
-    bool ok = true;
+        for (int c = 0; c < 16; ++c) P[c] = k0 + c < cend ? F[tix(i, k0 + c)] : T(0);
+    } else {
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-        if (k0 + kk < kend) {
-            const T dkk = bcast(P[kk], kk);
-            ok = ok && (dkk > T(0));
-            const T d = sqrt(dkk), inv = T(1) / d;
-            if (i > k0 + kk) P[kk] *= inv;
-            else if (i == k0 + kk) P[kk] = d;
-#pragma unroll
-            for (int c = kk + 1; c < 16; ++c) {
-                if (k0 + c < cend) {
-                    const T ljk = bcast(P[kk], c);
-                    if (i >= k0 + c) P[c] -= P[kk] * ljk;
-                }
-            }
-        }
+        for (int c = 0; c < 16; ++c) P[c] = T(0);
     }
+    sync();
+    // a full panel (16 pivots, so 16 columns) without any column test
+    const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, k0, kend, cend)
+                                    : panel_steps<T, false>(P, i, k0, kend, cend);
     if (mine) {
 #pragma unroll
         for (int c = 0; c < 16; ++c)
@@ -110,34 +159,52 @@ __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int row
     }
     if (diag_out != nullptr && wave == 0 && lane < 16) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) diag_out[(c << 4) + lane] = P[c];
+        for (int c = 0; c < 16; ++c) diag_out[(c << 4) + lane] = i >= k0 + c ? P[c] : T(0);
     }
     return ok;
 }
 
-// C(I, J) -= P_I P_J^T over the panel columns k0..kend-1 of tile column Kt.
-// MFMA operands as described at Mfma.
+// C(I, J) -= P_I P_J^T over the panel columns k0..kend-1 of tile column Kt,
+// in two steps so that the operands of the next tile can be requested before
+// this one's products: `load` issues every LDS (or global) read of the tile,
+// `apply` runs the MFMAs and writes C back. Columns of the tile column past
+// kend are multiplied by 0. MFMA operands as described at Mfma.
+template <typename T>
+struct TileJob {
+    typedef typename Mfma<T>::acc Acc;
+    T* C;
+    Acc acc;
+    T a[4], b[4];
+
+    __device__ __forceinline__ void load(T* F, int I, int J, int Kt, int lane) {
+        C = F + tile_index(I, J) * 256 + ((lane & 15) << 4);
+        const T* PI = F + tile_index(I, Kt) * 256 + (lane & 15);
+        const T* PJ = F + tile_index(J, Kt) * 256 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = 4 * q + (lane >> 4);
+            a[q] = PI[col << 4];
+            b[q] = PJ[col << 4];
+        }
+        acc = Mfma<T>::load(C, lane);
+    }
+    __device__ __forceinline__ void apply(int nk, int lane) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (4 * q < nk) {
+                const bool valid = 4 * q + (lane >> 4) < nk;
+                acc = Mfma<T>::run(valid ? -a[q] : T(0), valid ? b[q] : T(0), acc);
+            }
+        }
+        Mfma<T>::store(C, lane, acc);
+    }
+};
+
 template <typename T>
 __device__ __forceinline__ void tile_update(T* F, int I, int J, int Kt, int k0, int kend, int lane) {
-    typedef typename Mfma<T>::acc Acc;
-    T* C = F + tile_index(I, J) * 256 + ((lane & 15) << 4);
-    const int r0 = Mfma<T>::row(lane, 0), r1 = Mfma<T>::row(lane, 1), r2 = Mfma<T>::row(lane, 2),
-              r3 = Mfma<T>::row(lane, 3);
-    Acc acc;
-    acc[0] = C[r0], acc[1] = C[r1], acc[2] = C[r2], acc[3] = C[r3];
-    const T* PI = F + tile_index(I, Kt) * 256;
-    const T* PJ = F + tile_index(J, Kt) * 256;
-#pragma unroll
-    for (int kc = 0; kc < 16; kc += 4) {
-        if (k0 + kc < kend) {
-            const int col = kc + (lane >> 4);
-            const bool valid = k0 + col < kend;
-            const T a = valid ? -PI[(col << 4) + (lane & 15)] : T(0);
-            const T b = valid ? PJ[(col << 4) + (lane & 15)] : T(0);
-            acc = Mfma<T>::run(a, b, acc);
-        }
-    }
-    C[r0] = acc[0], C[r1] = acc[1], C[r2] = acc[2], C[r3] = acc[3];
+    TileJob<T> job;
+    job.load(F, I, J, Kt, lane);
+    job.apply(kend - k0, lane);
 }
 
 // Lower-triangle tile (I, J), I >= J, of index t in a T x T triangle.
@@ -149,9 +216,20 @@ __device__ __forceinline__ void tri_tile(int t, int& I, int& J) {
     J = t - (i * (i + 1)) / 2;
 }
 
+// Moves (I, J) to the next lower-triangle tile `step` tiles on in row order.
+__device__ __forceinline__ void next_tile(int& I, int& J, int step, int Tn) {
+    J += step;
+    while (J > I && I < Tn) {
+        J -= I + 1;
+        ++I;
+    }
+}
+
 // Partial Cholesky of the first np columns of a tiled front of m rows by a
 // team of `nw` wavefronts (this one is `wave`); `sync` orders the team. The
 // result flag is returned to every wave through `flag` (team-visible memory).
+// A wave's trailing tiles are pipelined: the next tile's operands are read
+// while the current tile's MFMAs run.
 template <typename T, typename Sync>
 __device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, int* flag, Sync sync) {
     const int MT = (m + 15) >> 4;
@@ -161,11 +239,19 @@ __device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, in
         if (!ok && wave == 0 && lane == 0) *flag = 1;
         sync();
         const int Tn = MT - Kt - 1;
-        const int ntiles = (Tn * (Tn + 1)) >> 1;
-        for (int t = wave; t < ntiles; t += nw) {
-            int I, J;
-            tri_tile(t, I, J);
-            tile_update<T>(F, Kt + 1 + I, Kt + 1 + J, Kt, k0, kend, lane);
+        int I = 0, J = 0;
+        next_tile(I, J, wave, Tn);
+        if (I < Tn) {
+            TileJob<T> cur, nxt;
+            cur.load(F, Kt + 1 + I, Kt + 1 + J, Kt, lane);
+            for (;;) {
+                next_tile(I, J, nw, Tn);
+                const bool more = I < Tn;
+                if (more) nxt.load(F, Kt + 1 + I, Kt + 1 + J, Kt, lane);
+                cur.apply(kend - k0, lane);
+                if (!more) break;
+                cur = nxt;
+            }
         }
         sync();
     }

@@ -86,7 +86,7 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* __restric
         a += __shfl_xor(a, o);
         b += __shfl_xor(b, o);
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     if (lane == 0) red[0][wave] = a, red[1][wave] = b;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_h_panel(DPlan P, int Kt, const int* __r
     if (k0 >= np) return;
     const int below = ma - k0 - 16;
     if (blockIdx.x > 0 && static_cast<int>(192 * blockIdx.x) >= below) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const bool ok = panel<T>(F + hoff[f] * 256, ma, k0, min(k0 + 16, np), wave, lane, 192 * blockIdx.x,
                              blockIdx.x == 0 ? D + static_cast<long long>(f) * 256 : nullptr, [] __device__() {});
     if (!ok && lane == 0) atomicOr(fail, 1);
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256) void k_h_trailing(DPlan P, int Kt, const int* 
     const int np = P.npiv[t], ma = np + P.nupd[t] + 1;
     const int k0 = 16 * Kt;
     if (k0 >= np) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int MT = (ma + 15) >> 4, Tn = MT - Kt - 1;
     const long long ntiles = static_cast<long long>(Tn) * (Tn + 1) / 2;
     const long long tt = static_cast<long long>(blockIdx.x) * 4 + wave;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_h_forward(DPlan P, const int* __restric
     const int t = ht[blockIdx.x];
     const int g0 = blockIdx.y * RB;
     double* tri = reinterpret_cast<double*>(smem_raw);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     double* fv = tri + 256;  // fv[r * m + i]
     for (int r = 0; r < RB; ++r) {
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restri
     double* tri = reinterpret_cast<double*>(smem_raw);
     double* dots = tri + 256;       // dots[r * 16 + cc]
     double* fv = dots + 16 * RB;    // fv[r * m + i]
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     const int* rows = P.rows + P.upd[t];
     for (int r = 0; r < RB; ++r) {

@@ -47,7 +47,7 @@ __device__ __forceinline__ double edge_weight(double a, double b, bool diag) {
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     __syncthreads();
     if (lane == 0) red[wave] = v;
     __syncthreads();
@@ -58,7 +58,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 __device__ __forceinline__ double block_max(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     __syncthreads();
     if (lane == 0) red[wave] = v;
     __syncthreads();
@@ -178,7 +178,7 @@ static __global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const
     const int W = P.W, W2 = W * W, D = P.D, n = P.n, nd = P.nd;
     const double* gw = g + static_cast<long long>(k) * W2;
     const double* sw = src + static_cast<long long>(k) * W2;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int ground = D * W + D;
     if (threadIdx.x == 0) s_ok = gw[ground] > 0;
     __syncthreads();
@@ -354,6 +354,32 @@ __device__ __forceinline__ double chunk_elem<double>(const uint4& v, int r) {
 // One front: assembled (unit diagonal, the scaled entries, RHS row, the
 // children's update matrices), factorised, written out (L packed by columns,
 // y, the update matrix). Team of nw wavefronts (tid / nt over the team).
+// Columns 0..nc-1 of the lower triangle of the tiled front's trailing block
+// F[o.., o..] of `rows` rows, packed column after column (column j holds
+// rows j..rows-1) from `out`: entry e of the packed array is taken by thread
+// e mod nt, four at a time so that the LDS reads overlap.
+template <typename T>
+__device__ __forceinline__ void store_packed(const T* F, int o, int rows, int nc, T* out, int tid, int nt) {
+    const int sz = nc * rows - ((nc * (nc - 1)) >> 1);
+    int j = 0, r = tid;  // entry e = column j, row j + r
+    while (j < nc && r >= rows - j) r -= rows - j, ++j;
+    for (int e0 = tid; e0 < sz; e0 += 4 * nt) {
+        T v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool in = j < nc;
+            v[q] = F[in ? tix(o + j + r, o + j) : 0];
+            r += nt;
+            while (j < nc && r >= rows - j) r -= rows - j, ++j;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * nt;
+            if (e < sz) out[e] = v[q];
+        }
+    }
+}
+
 template <typename T, typename Sync>
 __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid, int nt, int wave, int nw, int lane,
                               int* flag, Sync sync, const T* aval, const double* rsw, T* Lw, double* yw, T* U) {
@@ -426,18 +452,12 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
         sync();
     }
     factor_front<T>(F, ma, np, wave, nw, lane, flag, sync);
-    T* Lt = Lw + P.lpos[t];
-    for (int j = wave; j < np; j += nw) {
-        T* col = Lt + colstart(j, m) - j;
-        for (int i = j + lane; i < m; i += 64) col[i] = F[tix(i, j)];
-    }
+    // L block and update matrix, both packed by columns: the team walks the
+    // packed entries in order (coalesced stores), four LDS reads in flight
+    // per thread
+    store_packed<T>(F, 0, m, np, Lw + P.lpos[t], tid, nt);
     for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
-    const int u1 = nu + 1;
-    T* Ut = U + P.uposA[t];
-    for (int jj = wave; jj <= nu; jj += nw) {
-        T* col = Ut + colstart(jj, u1) - jj;
-        for (int ii = jj + lane; ii <= nu; ii += 64) col[ii] = F[tix(np + ii, np + jj)];
-    }
+    store_packed<T>(F, np, nu + 1, nu + 1, U + P.uposA[t], tid, nt);
     sync();
 }
 
@@ -457,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k_factor_small(DPlan P, int h, int B
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_flag[kWaves];
     __shared__ int s_rel[kWaves][kSmallRows + 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int ns = P.lev_small[h];
     const long long task = static_cast<long long>(blockIdx.x) * kWaves + wave;
     const int k = static_cast<int>(task / ns);
@@ -482,7 +502,7 @@ __global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_flag;
     __shared__ int s_rel[kBigTiles * 16 + 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const int base = P.lev_ptr[h] + P.lev_small[h];
     const int nb = P.lev_ptr[h + 1] - base;
     const int k = blockIdx.x / nb;
@@ -501,6 +521,17 @@ __global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, 
 }
 
 namespace {
+
+// 1 / pivot for the substitutions: for a single-precision pivot, the
+// hardware reciprocal refined by one FP64 Newton step (relative error near
+// 1e-14, two FMAs instead of a double division); the exact quotient for an
+// FP64 pivot.
+__device__ __forceinline__ double inv_pivot(float d) {
+    const double dd = static_cast<double>(d);
+    const double r = static_cast<double>(__builtin_amdgcn_rcpf(d));
+    return fma(r, fma(-dd, r, 1.0), r);
+}
+__device__ __forceinline__ double inv_pivot(double d) { return 1.0 / d; }
 
 __device__ __forceinline__ double pick(const double (&v)[3], int slot) {
     return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
@@ -600,7 +631,7 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double*
             const int j = j0 + q;
             const bool okj = j < np;
             const long long cs = okj ? colstart(j, m) : 0;
-            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[cs]) : 0.0;
+            dinv[q] = okj ? inv_pivot(Lc[cs]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
@@ -701,7 +732,7 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
         for (int q = 0; q < 16; ++q) {
             const int j = j1 - q;
             const bool okj = j >= 0;
-            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[colstart(j, m)]) : 0.0;
+            dinv[q] = okj ? inv_pivot(Lc[colstart(j, m)]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
@@ -797,7 +828,7 @@ __device__ __forceinline__ void forward_compute(const T* Lc, const double* fv, i
             const int j = j0 + q;
             const bool okj = j < np;
             const long long cs = okj ? colstart(j, m) : 0;
-            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[cs]) : 0.0;
+            dinv[q] = okj ? inv_pivot(Lc[cs]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
@@ -850,7 +881,7 @@ __device__ __forceinline__ void backward_compute(const T* Lc, const double* xu, 
         for (int q = 0; q < 16; ++q) {
             const int j = j1 - q;
             const bool okj = j >= 0;
-            dinv[q] = okj ? 1.0 / static_cast<double>(Lc[colstart(j, m)]) : 0.0;
+            dinv[q] = okj ? inv_pivot(Lc[colstart(j, m)]) : 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
