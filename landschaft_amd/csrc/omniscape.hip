@@ -242,13 +242,17 @@ struct WindowSolver {
         };
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {
-                auto kern = plan.lev_mmax[h] <= 64 ? k_forward_level<T, 1> : k_forward_level<T, 3>;
-                hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, P.lsize,
+                const bool small = plan.lev_mmax[h] <= 64;  // rows on lanes, L in registers: 64 doubles of LDS
+                auto kern = small ? k_forward_level<T, 1> : k_forward_level<T, 3>;
+                hipLaunchKernelGGL(kern, grid(h), dim3(64), small ? 64 * sizeof(double) : solve_lds<T>(plan.lev_lmax[h]),
+                                   s, P, h, nb, P.lsize,
                                    wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
             }
         for (int h = P.H - 1; h >= 0; --h) {
-            auto kern = plan.lev_mmax[h] <= 64 ? k_backward_level<T, 1> : k_backward_level<T, 3>;
-            hipLaunchKernelGGL(kern, grid(h), dim3(64), solve_lds<T>(plan.lev_lmax[h]), s, P, h, nb, P.lsize,
+            const bool small = plan.lev_mmax[h] <= 64;  // update rows in 64 doubles of LDS
+            auto kern = small ? k_backward_level<T, 1> : k_backward_level<T, 3>;
+            const std::size_t lds = solve_lds<T>(plan.lev_lmax[h]) - (small ? 128 * sizeof(double) : 0);
+            hipLaunchKernelGGL(kern, grid(h), dim3(64), lds, s, P, h, nb, P.lsize,
                                wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
         }
         hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, static_cast<long long>(P.n),

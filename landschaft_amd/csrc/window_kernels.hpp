@@ -662,6 +662,85 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double*
     }
 }
 
+// Columns j0..j0+15 of front t's L block straight from HBM into registers,
+// a row per lane (lane i holds L[i][j], 0 above the diagonal and past the
+// front): every lane loads from a valid address (clamped), so the values
+// stay in registers.
+template <typename T>
+__device__ __forceinline__ void load_cols(const T* Lt, int np, int m, int j0, int lane, T (&Lr)[16]) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int j = j0 + q;
+        const bool okj = j < np;
+        const int jj = okj ? j : 0;
+        const int i = lane < jj ? jj : (lane < m ? lane : m - 1);
+        const T v = Lt[colstart(jj, m) + (i - jj)];
+        Lr[q] = okj && lane >= j && lane < m ? v : T(0);
+    }
+}
+
+// Forward substitution of a front of at most 64 rows (one wavefront, rows
+// on lanes; the reference's system.cpp:974-989): its vector assembled from
+// the pivots' RHS and the children's update vectors, then y = L11^-1 f and
+// the update vector f_upd - L21 y. L is read from HBM into registers 16
+// columns at a time, the next 16 requested before the current ones are
+// eliminated; only the assembly uses LDS (m doubles).
+template <typename T>
+__device__ void forward_front_small(const DPlan& P, int t, const T* Lw, double* fv, int lane, const double* rsw,
+                                    double* yw, double* V) {
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int cb = P.cb[t], ce = P.ce[t];
+    const T* Lt = Lw + P.lpos[t];
+    T Lr[16];
+    if (np > 0) load_cols<T>(Lt, np, m, 0, lane, Lr);
+    const double pre = lane < np ? rsw[first + lane] : 0.0;
+    double cv[2];
+    int cr[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int ci = cb + k;
+        const int c = ci < ce ? P.kids[ci] : 0;  // (front 0 when absent: no lane reads for it)
+        const int u = ci < ce ? P.nupd[c] : 0;
+        const bool ok = lane < u;
+        cr[k] = ok ? P.relmap[P.upd[c] + lane] : -1;
+        cv[k] = ok ? V[P.vpos[c] + lane] : 0.0;
+    }
+    if (lane < m) fv[lane] = pre;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (cr[k] >= 0) fv[cr[k]] += cv[k];
+        wave_sync();
+    }
+    for (int ci = cb + 2; ci < ce; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c];
+        if (lane < u) fv[P.relmap[P.upd[c] + lane]] += V[P.vpos[c] + lane];
+        wave_sync();
+    }
+    double f = lane < m ? fv[lane] : 0.0;
+    for (int j0 = 0; j0 < np; j0 += 16) {
+        T Ln[16];
+        const bool more = j0 + 16 < np;
+        if (more) load_cols<T>(Lt, np, m, j0 + 16, lane, Ln);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = j0 + q;
+            if (j < np) {
+                const double yj = bcast(f, j) * inv_pivot(bcast(Lr[q], j));
+                f -= static_cast<double>(Lr[q]) * yj;
+                if (lane == j) f = yj;
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) Lr[q] = Ln[q];
+        }
+    }
+    if (lane < np) yw[first + lane] = f;
+    else if (lane < m) V[P.vpos[t] + lane - np] = f;
+}
+
 // Back substitution of front t (one wavefront, L block Lc in LDS): z_piv =
 // L11^-T (y - L21^T z_upd) (reference system.cpp:992-1016). The L block,
 // the solved update rows and the pivots' y are requested together.
@@ -682,11 +761,12 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
     // every lane loads (clamped index): a per-lane conditional load would
     // put lv in private memory. A front without pivots has no L block and
     // reads the start of the (non-empty) L array instead.
-    uint4 lv[8];
+    constexpr int kPre = R == 1 ? 4 : 8;  // 16-byte requests per lane before the first use
+    uint4 lv[kPre];
     const int last = nv > 0 ? nv - 1 : 0;
     const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kPre; ++q) {
         const int e = lane + q * 64;
         lv[q] = srcl[e < nv ? e : last];
     }
@@ -699,12 +779,12 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
     }
     uint4* dst = reinterpret_cast<uint4*>(Lc);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kPre; ++q) {
         const int e = lane + q * 64;
         if (e < nv) dst[e] = lv[q];
     }
 #pragma unroll 8
-    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
+    for (int e = lane + kPre * 64; e < nv; e += 64) dst[e] = src[e];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ii = lane + 64 * r;
@@ -725,11 +805,12 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
     }
     // 16 pivots at a time from the last: L[j][i] of the lane's columns i and
     // the reciprocal pivots in registers, then a register-only chain.
-    for (int j1 = np - 1; j1 >= 0; j1 -= 16) {
-        T Lr[R][16];
-        double dinv[16];
+    constexpr int kB = R == 1 ? 8 : 16;  // pivots per register block (fewer registers for one row slot)
+    for (int j1 = np - 1; j1 >= 0; j1 -= kB) {
+        T Lr[R][kB];
+        double dinv[kB];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = 0; q < kB; ++q) {
             const int j = j1 - q;
             const bool okj = j >= 0;
             dinv[q] = okj ? inv_pivot(Lc[colstart(j, m)]) : 0.0;
@@ -740,7 +821,7 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
             }
         }
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = 0; q < kB; ++q) {
             const int j = j1 - q;
             if (j >= 0) {
                 const int slot = j >> 6;
@@ -783,8 +864,12 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
     if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
     const long long off = static_cast<long long>(k) * P.n;
-    forward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, fv, lane, rs + off, y + off,
-                        Vall + static_cast<long long>(k) * P.vsize);
+    if (R == 1)
+        forward_front_small<T>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off, y + off,
+                               Vall + static_cast<long long>(k) * P.vsize);
+    else
+        forward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, fv, lane, rs + off, y + off,
+                            Vall + static_cast<long long>(k) * P.vsize);
 }
 
 template <typename T, int R>
@@ -794,7 +879,7 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
                                                        const double* __restrict__ y, double* __restrict__ z) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* xu = reinterpret_cast<double*>(smem_raw);
-    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
+    T* Ls = reinterpret_cast<T*>(smem_raw + (R == 1 ? 64 : 192) * sizeof(double));
     const int lane = threadIdx.x;
     const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
     const int k = blockIdx.x / nf;
