@@ -386,7 +386,7 @@ std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hi
     IrStats cst;
     ws.solve(2, std::min(a.tol, 1e-14), std::max(a.maxit, 30), cst, ev, false);
     Lattice Lt{};
-    hipLaunchKernelGGL(k_currents, dim3(2), dim3(kThreads), static_cast<std::size_t>(n) * sizeof(double), s,
+    hipLaunchKernelGGL(k_currents, dim3(2), dim3(kCurThreads), static_cast<std::size_t>(n) * sizeof(double), s,
                        ws.plan.d, Lt, 0ll, 2, ws.wb.g.p, ws.wb.present.p, ws.wb.x.p, ws.wb.active.p,
                        static_cast<const double*>(nullptr), ws.wb.cur.p);
     std::vector<double> cur(2 * n);
@@ -464,7 +464,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
         }
         ws.solve(nb, a.tol, a.maxit, st, ev, timing);
         if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-        hipLaunchKernelGGL(k_currents, dim3(nb), dim3(kThreads), static_cast<std::size_t>(Lt.W) * Lt.W * sizeof(double),
+        hipLaunchKernelGGL(k_currents, dim3(nb), dim3(kCurThreads), static_cast<std::size_t>(Lt.W) * Lt.W * sizeof(double),
                            s, ws.plan.d, Lt, k0, nb, ws.wb.g.p, ws.wb.present.p, ws.wb.x.p, ws.wb.active.p, d_corr,
                            ws.wb.cur.p);
         if (timing) LSC_HIP(hipEventRecord(ev[1], s));

@@ -56,6 +56,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return t;
 }
 
+template <int NW = kWaves>
 __device__ __forceinline__ double block_max(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -63,7 +64,7 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     if (lane == 0) red[wave] = v;
     __syncthreads();
     double t = red[0];
-    for (int w = 1; w < kWaves; ++w) t = fmax(t, red[w]);
+    for (int w = 1; w < NW; ++w) t = fmax(t, red[w]);
     return t;
 }
 
@@ -1142,14 +1143,14 @@ static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, co
 
 // Node currents of every window (reference grid.cpp:19-71), times the
 // correction factors of the clipped window (omniscape.cpp:553-571).
-static __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
+static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
                                                        const double* __restrict__ g,
                                                        const unsigned char* __restrict__ present,
                                                        const double* __restrict__ x, const int* __restrict__ active,
                                                        const double* __restrict__ corr, double* __restrict__ cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* v = reinterpret_cast<double*>(smem_raw);
-    __shared__ double red[kWaves];
+    __shared__ double red[kCurThreads / 64];
     const int k = blockIdx.x;
     if (k >= B || !active[k]) return;
     const int W = P.W, W2 = W * W, nd = P.nd, D = P.D;
@@ -1158,15 +1159,15 @@ static __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice L
     const unsigned char* pw = present + static_cast<long long>(k) * W2;
     const long long off = static_cast<long long>(k) * P.n;
     double* cw = cur + static_cast<long long>(k) * W2;
-    for (int c = threadIdx.x; c < W2; c += kThreads) v[c] = 0.0;
+    for (int c = threadIdx.x; c < W2; c += kCurThreads) v[c] = 0.0;
     __syncthreads();
-    for (int p = threadIdx.x; p < P.n; p += kThreads) {
+    for (int p = threadIdx.x; p < P.n; p += kCurThreads) {
         const int c = P.cell[p];
         if (pw[c]) v[c] = x[off + p];
     }
     __syncthreads();
     double mo = -INFINITY, mi = -INFINITY;
-    for (int q = threadIdx.x; q < W2; q += kThreads) {
+    for (int q = threadIdx.x; q < W2; q += kCurThreads) {
         if (!pw[q]) continue;
         const int y = q / W, xx = q - (q / W) * W;
         for (int d = lower; d < nd; ++d) {
@@ -1179,8 +1180,8 @@ static __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice L
             mi = fmax(mi, -b);
         }
     }
-    const double maxout = block_max(mo, red);
-    const double maxin = block_max(mi, red);
+    const double maxout = block_max<kCurThreads / 64>(mo, red);
+    const double maxin = block_max<kCurThreads / 64>(mi, red);
     // correction window offsets (clipped windows keep Omniscape's slice of it)
     long long tx = 0, ty = 0;
     long long cx0 = 0, cy0 = 0, ox = 0, oy = 0, x0c = 0, y0c = 0;
@@ -1194,7 +1195,7 @@ static __global__ __launch_bounds__(kThreads) void k_currents(DPlan P, Lattice L
         cx0 = (tx < D && tx + D <= Lt.nx - 1) ? W - w : 0;
         cy0 = (ty < D && ty + D <= Lt.ny - 1) ? W - h : 0;
     }
-    for (int q = threadIdx.x; q < W2; q += kThreads) {
+    for (int q = threadIdx.x; q < W2; q += kCurThreads) {
         double out = 0.0;
         if (pw[q]) {
             const int y = q / W, xx = q - (q / W) * W;
