@@ -74,14 +74,6 @@ void allow_lds(const void* fn, std::size_t bytes) {
     LSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
 }
 
-int device_cus() {
-    int dev = 0;
-    LSC_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    LSC_HIP(hipGetDeviceProperties(&prop, dev));
-    return prop.multiProcessorCount;
-}
-
 }  // namespace
 
 void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {

@@ -1083,6 +1083,62 @@ static __global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int 
     }
 }
 
+// The neighbour offsets of ndy / ndx as arithmetic on a compile-time count,
+// so that unrolled loops fold them into immediates.
+template <int ND>
+__device__ __forceinline__ int fixed_dy(int d) {
+    if (ND == 8) return d == 1 || d == 6 ? 0 : (d == 0 || d == 3 || d == 5 ? -1 : 1);
+    return d == 1 ? -1 : (d == 2 ? 1 : 0);
+}
+template <int ND>
+__device__ __forceinline__ int fixed_dx(int d) {
+    if (ND == 8) return d < 3 ? -1 : (d < 5 ? 0 : 1);
+    return d == 0 ? -1 : (d == 3 ? 1 : 0);
+}
+
+// Rows of r = b - A x of one window, a row per thread. The neighbour loop is
+// unrolled and branch-free (clamped indices, masked terms) so that a row's
+// loads are requested in two rounds (its neighbours' indices, then their
+// values) instead of one round trip per neighbour; the terms are added in
+// the neighbour order of the reference.
+template <int ND>
+__device__ __forceinline__ void residual_rows(const DPlan& P, int W, int n, long long off, const double* gw,
+                                              const unsigned char* pw, const double* dg, const double* sc,
+                                              const double* bvec, const double* x, double* rs, double& rr,
+                                              double& bb) {
+    for (int p = threadIdx.x; p < n; p += kThreads) {
+        const int c = P.cell[p];
+        int q[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) q[d] = P.nbr[p * ND + d];
+        const double xp = x[off + p], dgp = dg[off + p], bp = bvec[off + p], scp = sc[off + p];
+        const bool pr = pw[c];
+        const double gc = gw[c];
+        const int y = c / W, xx = c - y * W;
+        double gn[ND], xn[ND];
+        bool on[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const bool in = q[d] >= 0;
+            const int c2 = in ? (y + fixed_dy<ND>(d)) * W + xx + fixed_dx<ND>(d) : c;
+            const bool pc = pw[c2];  // loaded for every neighbour (c2 is clamped)
+            on[d] = in & pc;
+            gn[d] = gw[c2];
+            xn[d] = x[off + (in ? q[d] : p)];
+        }
+        double acc = dgp * xp;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const bool diag = ND == 8 && fixed_dx<ND>(d) != 0 && fixed_dy<ND>(d) != 0;
+            if (on[d]) acc -= edge_weight(gc, gn[d], diag) * xn[d];
+        }
+        const double r = pr ? bp - acc : 0.0;
+        if (pr) bb += bp * bp;
+        rr += r * r;
+        rs[off + p] = scp * r;
+    }
+}
+
 // r = b - A x in FP64 from the conductances (the reference's matrix,
 // shifted, system.cpp:458-487); relative residual ||r|| / ||b||; the scaled
 // residual s r for the next refinement step. A window is done when it
@@ -1103,27 +1159,8 @@ static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, co
     const unsigned char* pw = present + static_cast<long long>(k) * W2;
     const long long off = static_cast<long long>(k) * n;
     double rr = 0.0, bb = 0.0;
-    for (int p = threadIdx.x; p < n; p += kThreads) {
-        const int c = P.cell[p];
-        double r = 0.0;
-        if (pw[c]) {
-            const int y = c / W, xx = c - (c / W) * W;
-            const double gc = gw[c];
-            double acc = dg[off + p] * x[off + p];
-            for (int d = 0; d < nd; ++d) {
-                const int q = P.nbr[p * nd + d];
-                if (q < 0) continue;
-                const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
-                if (!pw[c2]) continue;
-                acc -= edge_weight(gc, gw[c2], ndiag(nd, d)) * x[off + q];
-            }
-            const double b = bvec[off + p];
-            r = b - acc;
-            bb += b * b;
-        }
-        rr += r * r;
-        rs[off + p] = sc[off + p] * r;
-    }
+    if (nd == 8) residual_rows<8>(P, W, n, off, gw, pw, dg, sc, bvec, x, rs, rr, bb);
+    else residual_rows<4>(P, W, n, off, gw, pw, dg, sc, bvec, x, rs, rr, bb);
     rr = block_sum(rr, red);
     bb = block_sum(bb, red);
     if (threadIdx.x == 0) {
@@ -1141,6 +1178,80 @@ static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, co
     }
 }
 
+// The neighbours of cell q of a W x W window for the current kernels: the
+// neighbour's cell (q itself when outside the window) and whether the
+// branch exists (inside, present). Loads are unconditional so that a cell's
+// neighbours are requested together.
+template <int ND>
+__device__ __forceinline__ void cell_neighbours(int W, int q, int y, int xx, const unsigned char* pw, int (&r)[ND],
+                                                bool (&on)[ND]) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const int yy = y + fixed_dy<ND>(d), x2 = xx + fixed_dx<ND>(d);
+        const bool in = yy >= 0 && yy < W && x2 >= 0 && x2 < W;
+        r[d] = in ? yy * W + x2 : q;
+        const bool pr = pw[r[d]];
+        on[d] = in & pr;
+    }
+}
+
+// Largest branch current in each direction over the window (the reference's
+// maxout / maxin of node_currents, each branch once: its lower neighbours).
+template <int ND>
+__device__ __forceinline__ void branch_extremes(int W, const double* gw, const unsigned char* pw, const double* v,
+                                                double& mo, double& mi) {
+    constexpr int lower = ND == 8 ? 4 : 2;
+    for (int q = threadIdx.x; q < W * W; q += kCurThreads) {
+        if (!pw[q]) continue;
+        const int y = q / W, xx = q - y * W;
+        int r[ND];
+        bool on[ND];
+        cell_neighbours<ND>(W, q, y, xx, pw, r, on);
+        const double gq = gw[q], vq = v[q];
+        double gr[ND];
+#pragma unroll
+        for (int d = lower; d < ND; ++d) gr[d] = gw[r[d]];
+#pragma unroll
+        for (int d = lower; d < ND; ++d) {
+            if (!on[d]) continue;
+            const bool diag = ND == 8 && fixed_dx<ND>(d) != 0 && fixed_dy<ND>(d) != 0;
+            const double b = edge_weight(gq, gr[d], diag) * (v[r[d]] - vq);
+            mo = fmax(mo, b);
+            mi = fmax(mi, -b);
+        }
+    }
+}
+
+// A node's current: the larger of its inflow and outflow over branches not
+// negligible against maxin / maxout (reference grid.cpp:45-70).
+template <int ND>
+__device__ __forceinline__ double node_current(int W, int q, int y, int xx, const double* gw,
+                                               const unsigned char* pw, const double* v, double maxin,
+                                               double maxout) {
+    constexpr int lower = ND == 8 ? 4 : 2;
+    int r[ND];
+    bool on[ND];
+    cell_neighbours<ND>(W, q, y, xx, pw, r, on);
+    const double gq = gw[q], vq = v[q];
+    double gr[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) gr[d] = gw[r[d]];
+    double in = 0.0, outf = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        if (!on[d]) continue;
+        const bool dgl = ND == 8 && fixed_dx<ND>(d) != 0 && fixed_dy<ND>(d) != 0;
+        const double vr = v[r[d]];
+        const double b = d < lower ? -(edge_weight(gr[d], gq, dgl) * (vq - vr)) : edge_weight(gq, gr[d], dgl) * (vr - vq);
+        if (b > 0) {
+            if (!(fabs(b / maxin) < 1e-8)) in += b;
+        } else if (b < 0) {
+            if (!(fabs(b / maxout) < 1e-8)) outf -= b;
+        }
+    }
+    return in > outf ? in : outf;
+}
+
 // Node currents of every window (reference grid.cpp:19-71), times the
 // correction factors of the clipped window (omniscape.cpp:553-571).
 static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattice Lt, long long k0, int B,
@@ -1154,7 +1265,6 @@ static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattic
     const int k = blockIdx.x;
     if (k >= B || !active[k]) return;
     const int W = P.W, W2 = W * W, nd = P.nd, D = P.D;
-    const int lower = nd == 8 ? 4 : 2;
     const double* gw = g + static_cast<long long>(k) * W2;
     const unsigned char* pw = present + static_cast<long long>(k) * W2;
     const long long off = static_cast<long long>(k) * P.n;
@@ -1167,19 +1277,8 @@ static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattic
     }
     __syncthreads();
     double mo = -INFINITY, mi = -INFINITY;
-    for (int q = threadIdx.x; q < W2; q += kCurThreads) {
-        if (!pw[q]) continue;
-        const int y = q / W, xx = q - (q / W) * W;
-        for (int d = lower; d < nd; ++d) {
-            const int yy = y + ndy(nd, d), x2 = xx + ndx(nd, d);
-            if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
-            const int r = yy * W + x2;
-            if (!pw[r]) continue;
-            const double b = edge_weight(gw[q], gw[r], ndiag(nd, d)) * (v[r] - v[q]);
-            mo = fmax(mo, b);
-            mi = fmax(mi, -b);
-        }
-    }
+    if (nd == 8) branch_extremes<8>(W, gw, pw, v, mo, mi);
+    else branch_extremes<4>(W, gw, pw, v, mo, mi);
     const double maxout = block_max<kCurThreads / 64>(mo, red);
     const double maxin = block_max<kCurThreads / 64>(mi, red);
     // correction window offsets (clipped windows keep Omniscape's slice of it)
@@ -1197,24 +1296,11 @@ static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattic
     }
     for (int q = threadIdx.x; q < W2; q += kCurThreads) {
         double out = 0.0;
-        if (pw[q]) {
+        const bool pq = pw[q];
+        if (pq) {
             const int y = q / W, xx = q - (q / W) * W;
-            double in = 0.0, outf = 0.0;
-            for (int d = 0; d < nd; ++d) {
-                const int yy = y + ndy(nd, d), x2 = xx + ndx(nd, d);
-                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
-                const int r = yy * W + x2;
-                if (!pw[r]) continue;
-                const bool dgl = ndiag(nd, d);
-                const double b = d < lower ? -(edge_weight(gw[r], gw[q], dgl) * (v[q] - v[r]))
-                                           : edge_weight(gw[q], gw[r], dgl) * (v[r] - v[q]);
-                if (b > 0) {
-                    if (!(fabs(b / maxin) < 1e-8)) in += b;
-                } else if (b < 0) {
-                    if (!(fabs(b / maxout) < 1e-8)) outf -= b;
-                }
-            }
-            out = in > outf ? in : outf;
+            out = nd == 8 ? node_current<8>(W, q, y, xx, gw, pw, v, maxin, maxout)
+                          : node_current<4>(W, q, y, xx, gw, pw, v, maxin, maxout);
             if (corr) {
                 const long long yc = oy + y - y0c, xc = ox + xx - x0c;
                 out *= corr[(cy0 + yc) * W + (cx0 + xc)];
