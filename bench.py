@@ -36,6 +36,11 @@ sys.path.insert(0, str(ROOT))
 MFMA_F32_PEAK = 157.3e12   # MI355X dense FP32 matrix, FLOP/s
 MFMA_F64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12          # bytes/s
+# HBM bytes of one substitution sweep of one window (forward and backward
+# sweeps averaged), from rocprofv3 --pmc FETCH_SIZE (x2, the reading of a
+# 256 MiB copy: python bench.py --copy-probe) and WRITE_SIZE (x1) over the
+# level kernels of this workload at 1024 x 1024; profiles/ holds the runs.
+SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 1027090
 
 
 def fractal_field(n: int, seed: int, exponent: float = -3.0) -> np.ndarray:
@@ -187,7 +192,18 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the pairwise and adjoint graph workloads")
     ap.add_argument("--batch", type=int, default=16384, help="Omniscape windows per batch")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy a 256 MiB buffer three times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+
+        buf = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            buf.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": buf.numel() * 4, "copies": 3}))
+        return 0
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by row bands "
                                    "of target windows, see DESIGN.md)"}))
@@ -242,8 +258,11 @@ def main() -> int:
                           "bytes_per_sweep": l_bytes}
     dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
     kd = kernels[dom]
+    # one launch of k_solve = one substitution sweep over a batch of windows
+    traffic = SOLVE_TRAFFIC_PER_WINDOW_SWEEP * BATCH if dom == "k_solve" else None
     roofline = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
-                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": None}
+                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic,
+                "traffic_note": "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"}
 
     fp64 = None
     if not a.no_fp64:
