@@ -261,8 +261,9 @@ def main() -> int:
     # one launch of k_solve = one substitution sweep over a batch of windows
     traffic = SOLVE_TRAFFIC_PER_WINDOW_SWEEP * BATCH if dom == "k_solve" else None
     roofline = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
-                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic,
-                "traffic_note": "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"}
+                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic}
+    if traffic is not None:
+        roofline["traffic_note"] = "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"
 
     fp64 = None
     if not a.no_fp64:

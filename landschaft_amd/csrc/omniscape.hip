@@ -234,9 +234,9 @@ struct WindowSolver {
         };
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {
-                const bool small = plan.lev_mmax[h] <= 64;  // rows on lanes, L in registers: 64 doubles of LDS
+                const bool small = plan.lev_mmax[h] <= 64;  // rows on lanes, L in registers: the front's vector in LDS
                 auto kern = small ? k_forward_level<T, 1> : k_forward_level<T, 3>;
-                hipLaunchKernelGGL(kern, grid(h), dim3(64), small ? 64 * sizeof(double) : solve_lds<T>(plan.lev_lmax[h]),
+                hipLaunchKernelGGL(kern, grid(h), dim3(64), (small ? 64 : 192) * sizeof(double),
                                    s, P, h, nb, P.lsize,
                                    wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
             }

@@ -538,69 +538,80 @@ __device__ __forceinline__ double pick(const double (&v)[3], int slot) {
     return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
 }
 
-// Forward substitution of front t (one wavefront, L block Lc in LDS): its
-// vector assembled from the pivots' RHS and the children's update vectors,
-// then y = L11^-1 f and the update vector f_upd - L21 y (reference
-// system.cpp:974-989). The L block, the pivots' RHS and the first two
-// children's update vectors are all requested from HBM before the first of
-// them is used, so their latencies overlap.
+// Columns j0..j0+KB-1 of front t's L block straight from HBM into
+// registers, rows on lanes (slot r of lane l holds row l + 64 r; 0 above the
+// diagonal and past the front): every lane loads from a valid address
+// (clamped), so the values stay in registers.
+template <typename T, int R, int KB>
+__device__ __forceinline__ void load_cols(const T* Lt, int np, int m, int j0, int lane, T (&Lr)[R][KB]) {
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const int j = j0 + q;
+        const bool okj = j < np;
+        const int jj = okj ? j : 0;
+        const T* col = Lt + colstart(jj, m) - jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = lane + 64 * r;
+            const int i = row < jj ? jj : (row < m ? row : m - 1);
+            const T v = col[i];
+            Lr[r][q] = okj && row >= j && row < m ? v : T(0);
+        }
+    }
+}
+
+// v[slot] as selects (an indexed read would put v in private memory).
 template <typename T, int R>
-__device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double* fv, int lane, const double* rsw,
-                              double* yw, double* V) {
+__device__ __forceinline__ T pick_slot(const T (&v)[R], int slot) {
+    static_assert(R == 1 || R == 3, "one or three row slots");
+    if constexpr (R == 1) return v[0];
+    else return slot == 0 ? v[0] : (slot == 1 ? v[1] : v[2]);
+}
+
+// Forward substitution of a front of at most 64 R rows (one wavefront, rows
+// on lanes; the reference's system.cpp:974-989): its vector assembled from
+// the pivots' RHS and the children's update vectors, then y = L11^-1 f and
+// the update vector f_upd - L21 y. L is read from HBM into registers KB
+// columns at a time, the next KB requested before the current ones are
+// eliminated; only the assembly uses LDS (m doubles).
+template <typename T, int R, int KB>
+__device__ void forward_front_direct(const DPlan& P, int t, const T* Lw, double* fv, int lane, const double* rsw,
+                                     double* yw, double* V) {
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
     const int cb = P.cb[t], ce = P.ce[t];
-    // requests: L block, RHS, two children (one value per lane and slot)
-    const uint4* src = reinterpret_cast<const uint4*>(Lw + P.lpos[t]);
-    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
-    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
-    // every lane loads (clamped index): a per-lane conditional load would
-    // put lv in private memory. A front without pivots has no L block and
-    // reads the start of the (non-empty) L array instead.
-    uint4 lv[8];
-    const int last = nv > 0 ? nv - 1 : 0;
-    const uint4* srcl = nv > 0 ? src : reinterpret_cast<const uint4*>(Lw);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = lane + q * 64;
-        lv[q] = srcl[e < nv ? e : last];
-    }
+    const T* Lt = Lw + P.lpos[t];
+    T Lr[R][KB];
+    if (np > 0) load_cols<T, R, KB>(Lt, np, m, 0, lane, Lr);
     double pre[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
-        pre[r] = i < np ? rsw[first + i] : 0.0;
+        const double v = rsw[i < np ? first + i : 0];
+        pre[r] = i < np ? v : 0.0;
     }
     double cv[2][R];
     int cr[2][R];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int ci = cb + k;
-        const int c = ci < ce ? P.kids[ci] : -1;
-        const int u = c >= 0 ? P.nupd[c] : 0;
+        const int c = ci < ce ? P.kids[ci] : 0;  // (front 0 when absent: no lane reads for it)
+        const int u = ci < ce ? P.nupd[c] : 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < R; ++r) {  // loads from clamped indices: no per-lane branch
             const int ii = lane + 64 * r;
             const bool ok = ii < u;
-            cr[k][r] = ok ? P.relmap[P.upd[c] + ii] : -1;
-            cv[k][r] = ok ? V[P.vpos[c] + ii] : 0.0;
+            const int rv = P.relmap[ok ? P.upd[c] + ii : 0];
+            const double vv = V[ok ? P.vpos[c] + ii : 0];
+            cr[k][r] = ok ? rv : -1;
+            cv[k][r] = ok ? vv : 0.0;
         }
     }
-    uint4* dst = reinterpret_cast<uint4*>(Lc);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = lane + q * 64;
-        if (e < nv) dst[e] = lv[q];
-    }
-#pragma unroll 8
-    for (int e = lane + 8 * 64; e < nv; e += 64) dst[e] = src[e];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         if (i < m) fv[i] = pre[r];
     }
     wave_sync();
-    // children's contributions: a child's rows map to distinct rows of the
-    // parent, but two children may share one, so they go one after the other
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
 #pragma unroll
@@ -611,48 +622,43 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double*
     for (int ci = cb + 2; ci < ce; ++ci) {
         const int c = P.kids[ci];
         const int u = P.nupd[c];
-        const int* rel = P.relmap + P.upd[c];
-        const double* Vc = V + P.vpos[c];
-        for (int ii = lane; ii < u; ii += 64) fv[rel[ii]] += Vc[ii];
+        for (int ii = lane; ii < u; ii += 64) fv[P.relmap[P.upd[c] + ii]] += V[P.vpos[c] + ii];
         wave_sync();
     }
-    double f[3] = {0.0, 0.0, 0.0};
+    double f[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         f[r] = i < m ? fv[i] : 0.0;
     }
-    // 16 columns at a time: the lane's L entries and the reciprocal pivots
-    // in registers first, so the dependent chain is register-only.
-    for (int j0 = 0; j0 < np; j0 += 16) {
-        T Lr[R][16];
-        double dinv[16];
+    for (int j0 = 0; j0 < np; j0 += KB) {
+        T Ln[R][KB];
+        const bool more = j0 + KB < np;
+        if (more) load_cols<T, R, KB>(Lt, np, m, j0 + KB, lane, Ln);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int j = j0 + q;
-            const bool okj = j < np;
-            const long long cs = okj ? colstart(j, m) : 0;
-            dinv[q] = okj ? inv_pivot(Lc[cs]) : 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = lane + 64 * r;
-                Lr[r][q] = (okj && i > j && i < m) ? Lc[cs + i - j] : T(0);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = 0; q < KB; ++q) {
             const int j = j0 + q;
             if (j < np) {
-                const int slot = j >> 6;
-                const double yj = bcast(pick(f, slot), j & 63) * dinv[q];
+                const int slot = j >> 6, jl = j & 63;
+                T dcol[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) dcol[r] = Lr[r][q];
+                const double yj = bcast(pick_slot<double, R>(f, slot), jl) *
+                                  inv_pivot(bcast(pick_slot<T, R>(dcol, slot), jl));
 #pragma unroll
                 for (int r = 0; r < R; ++r) f[r] -= static_cast<double>(Lr[r][q]) * yj;
-                if (lane == (j & 63)) {
-                    if (slot == 0) f[0] = yj;
-                    else if (slot == 1) f[1] = yj;
-                    else f[2] = yj;
+                if (lane == jl) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (slot == r) f[r] = yj;
                 }
             }
+        }
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int q = 0; q < KB; ++q) Lr[r][q] = Ln[r][q];
         }
     }
 #pragma unroll
@@ -661,85 +667,6 @@ __device__ void forward_front(const DPlan& P, int t, const T* Lw, T* Lc, double*
         if (i < np) yw[first + i] = f[r];
         else if (i < m) V[P.vpos[t] + i - np] = f[r];
     }
-}
-
-// Columns j0..j0+15 of front t's L block straight from HBM into registers,
-// a row per lane (lane i holds L[i][j], 0 above the diagonal and past the
-// front): every lane loads from a valid address (clamped), so the values
-// stay in registers.
-template <typename T>
-__device__ __forceinline__ void load_cols(const T* Lt, int np, int m, int j0, int lane, T (&Lr)[16]) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int j = j0 + q;
-        const bool okj = j < np;
-        const int jj = okj ? j : 0;
-        const int i = lane < jj ? jj : (lane < m ? lane : m - 1);
-        const T v = Lt[colstart(jj, m) + (i - jj)];
-        Lr[q] = okj && lane >= j && lane < m ? v : T(0);
-    }
-}
-
-// Forward substitution of a front of at most 64 rows (one wavefront, rows
-// on lanes; the reference's system.cpp:974-989): its vector assembled from
-// the pivots' RHS and the children's update vectors, then y = L11^-1 f and
-// the update vector f_upd - L21 y. L is read from HBM into registers 16
-// columns at a time, the next 16 requested before the current ones are
-// eliminated; only the assembly uses LDS (m doubles).
-template <typename T>
-__device__ void forward_front_small(const DPlan& P, int t, const T* Lw, double* fv, int lane, const double* rsw,
-                                    double* yw, double* V) {
-    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    const int cb = P.cb[t], ce = P.ce[t];
-    const T* Lt = Lw + P.lpos[t];
-    T Lr[16];
-    if (np > 0) load_cols<T>(Lt, np, m, 0, lane, Lr);
-    const double pre = lane < np ? rsw[first + lane] : 0.0;
-    double cv[2];
-    int cr[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int ci = cb + k;
-        const int c = ci < ce ? P.kids[ci] : 0;  // (front 0 when absent: no lane reads for it)
-        const int u = ci < ce ? P.nupd[c] : 0;
-        const bool ok = lane < u;
-        cr[k] = ok ? P.relmap[P.upd[c] + lane] : -1;
-        cv[k] = ok ? V[P.vpos[c] + lane] : 0.0;
-    }
-    if (lane < m) fv[lane] = pre;
-    wave_sync();
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        if (cr[k] >= 0) fv[cr[k]] += cv[k];
-        wave_sync();
-    }
-    for (int ci = cb + 2; ci < ce; ++ci) {
-        const int c = P.kids[ci];
-        const int u = P.nupd[c];
-        if (lane < u) fv[P.relmap[P.upd[c] + lane]] += V[P.vpos[c] + lane];
-        wave_sync();
-    }
-    double f = lane < m ? fv[lane] : 0.0;
-    for (int j0 = 0; j0 < np; j0 += 16) {
-        T Ln[16];
-        const bool more = j0 + 16 < np;
-        if (more) load_cols<T>(Lt, np, m, j0 + 16, lane, Ln);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int j = j0 + q;
-            if (j < np) {
-                const double yj = bcast(f, j) * inv_pivot(bcast(Lr[q], j));
-                f -= static_cast<double>(Lr[q]) * yj;
-                if (lane == j) f = yj;
-            }
-        }
-        if (more) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) Lr[q] = Ln[q];
-        }
-    }
-    if (lane < np) yw[first + lane] = f;
-    else if (lane < m) V[P.vpos[t] + lane - np] = f;
 }
 
 // Back substitution of front t (one wavefront, L block Lc in LDS): z_piv =
@@ -804,9 +731,9 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
         }
         tv[r] = acc;
     }
-    // 16 pivots at a time from the last: L[j][i] of the lane's columns i and
+    // 8 pivots at a time from the last: L[j][i] of the lane's columns i and
     // the reciprocal pivots in registers, then a register-only chain.
-    constexpr int kB = R == 1 ? 8 : 16;  // pivots per register block (fewer registers for one row slot)
+    constexpr int kB = 8;  // pivots per register block
     for (int j1 = np - 1; j1 >= 0; j1 -= kB) {
         T Lr[R][kB];
         double dinv[kB];
@@ -847,8 +774,10 @@ __device__ void backward_front(const DPlan& P, int t, const T* Lw, T* Lc, double
 }  // namespace
 
 // Batched triangular solves, one launch per level: every (instance, front)
-// pair of the level is the task of a one-wavefront workgroup, which stages
-// the front's L block (contiguous in HBM) into LDS, then substitutes.
+// pair of the level is the task of a one-wavefront workgroup. The forward
+// sweep reads the front's L columns into registers (LDS holds the front's
+// vector, 64 R doubles); the backward sweep stages the L block (contiguous
+// in HBM) into LDS, then substitutes.
 template <typename T, int R>
 __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, long long lstride,
                                                       const int* __restrict__ active,
@@ -857,7 +786,6 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
                                                       double* __restrict__ Vall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* fv = reinterpret_cast<double*>(smem_raw);
-    T* Ls = reinterpret_cast<T*>(smem_raw + 192 * sizeof(double));
     const int lane = threadIdx.x;
     const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
     const int k = blockIdx.x / nf;
@@ -865,12 +793,8 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
     if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
     const long long off = static_cast<long long>(k) * P.n;
-    if (R == 1)
-        forward_front_small<T>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off, y + off,
-                               Vall + static_cast<long long>(k) * P.vsize);
-    else
-        forward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, fv, lane, rs + off, y + off,
-                            Vall + static_cast<long long>(k) * P.vsize);
+    forward_front_direct<T, R, R == 1 ? 16 : 8>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off,
+                                                y + off, Vall + static_cast<long long>(k) * P.vsize);
 }
 
 template <typename T, int R>
