@@ -49,7 +49,7 @@ std::size_t big_lds() {
 }
 template <typename T>
 std::size_t small_lds(int stile) {
-    return static_cast<std::size_t>(kWaves) * stile * 256 * sizeof(T);
+    return static_cast<std::size_t>(kSmallWaves) * stile * 256 * sizeof(T);
 }
 template <typename T>
 std::size_t solve_lds(int lmax) {
@@ -216,8 +216,9 @@ struct WindowSolver {
             const int ns = plan.lev_small_host[h];
             const int nbig = plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h] - ns;
             if (ns > 0)
-                hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kWaves)),
-                                   dim3(kThreads), small_lds<T>(plan.lev_stile[h]), s, P, h, nb, plan.lev_stile[h],
+                hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kSmallWaves)),
+                                   dim3(kSmallWaves * 64), small_lds<T>(plan.lev_stile[h]), s, P, h, nb,
+                                   plan.lev_stile[h],
                                    Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
             if (nbig > 0)
                 hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),

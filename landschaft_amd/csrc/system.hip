@@ -672,15 +672,15 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     int smax = 1;
     for (int v : plan.lev_stile) smax = std::max(smax, v);
     sys_allow(reinterpret_cast<const void*>(&k_factor_small<T>),
-              static_cast<std::size_t>(kWaves) * smax * 256 * sizeof(T));
+              static_cast<std::size_t>(kSmallWaves) * smax * 256 * sizeof(T));
     sys_allow(reinterpret_cast<const void*>(&k_factor_big<T>),
               static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T));
     for (int lv = 0; lv < H; ++lv) {
         const int ns = plan.lev_small_host[lv];
         const int nbig = h.lev_ptr[lv + 1] - h.lev_ptr[lv] - ns;
         if (ns > 0)
-            hipLaunchKernelGGL(k_factor_small<T>, dim3(nblocks(ns, kWaves)), dim3(kThreads),
-                               static_cast<std::size_t>(kWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
+            hipLaunchKernelGGL(k_factor_small<T>, dim3(nblocks(ns, kSmallWaves)), dim3(kSmallWaves * 64),
+                               static_cast<std::size_t>(kSmallWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
                                plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
         if (nbig > 0)
             hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kThreads),

@@ -17,6 +17,7 @@ constexpr int kThreads = 256;      // 4 wavefronts per workgroup
 constexpr int kWaves = 4;
 constexpr int kCurThreads = 1024;  // node currents: 16 wavefronts share a window's voltages in LDS
 constexpr int kSmallRows = 64;     // fronts of <= 64 rows (RHS row included): one wavefront each
+constexpr int kSmallWaves = 1;     // wavefronts (small fronts) per workgroup: LDS is allotted per wavefront
 constexpr int kBigTiles = 10;      // fronts of <= 160 rows: the whole workgroup, tiles in LDS
 constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
 

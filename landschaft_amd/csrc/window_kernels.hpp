@@ -470,17 +470,17 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
 // Small fronts (<= 64 rows with the RHS row): one wavefront each, its own LDS
 // region of `stile` tiles. Large fronts: one workgroup each, up to 55 tiles.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_factor_small(DPlan P, int h, int B, int stile,
+__global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int h, int B, int stile,
                                                            const T* __restrict__ aval, const double* __restrict__ rs,
                                                            const int* __restrict__ active, T* __restrict__ L,
                                                            double* __restrict__ y, T* __restrict__ Uall,
                                                            int* __restrict__ fail) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int s_flag[kWaves];
-    __shared__ int s_rel[kWaves][kSmallRows + 1];
+    __shared__ int s_flag[kSmallWaves];
+    __shared__ int s_rel[kSmallWaves][kSmallRows + 1];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int ns = P.lev_small[h];
-    const long long task = static_cast<long long>(blockIdx.x) * kWaves + wave;
+    const long long task = static_cast<long long>(blockIdx.x) * kSmallWaves + wave;
     const int k = static_cast<int>(task / ns);
     if (k >= B || !active[k]) return;
     const int t = P.order[P.lev_ptr[h] + static_cast<int>(task - static_cast<long long>(k) * ns)];
