@@ -222,7 +222,7 @@ struct WindowSolver {
                                    Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
             if (nbig > 0)
                 hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
-                                   dim3(kThreads), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
+                                   dim3(kBigWaves * 64), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
                                    P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
         }
     }

@@ -683,7 +683,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
                                static_cast<std::size_t>(kSmallWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
                                plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
         if (nbig > 0)
-            hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kThreads),
+            hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kBigWaves * 64),
                                static_cast<std::size_t>(plan.lev_btile[lv]) * 256 * sizeof(T), s, P, lv, 1, Ap, rs0.p,
                                one.p, Lp, y0.p, Up, fail.p);
         HugeLevel& hl = lev_huge[lv];

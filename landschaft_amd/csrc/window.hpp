@@ -19,6 +19,8 @@ constexpr int kCurThreads = 1024;  // node currents: 16 wavefronts share a windo
 constexpr int kSmallRows = 64;     // fronts of <= 64 rows (RHS row included): one wavefront each
 constexpr int kSmallWaves = 1;     // wavefronts (small fronts) per workgroup: LDS is allotted per wavefront
 constexpr int kBigTiles = 10;      // fronts of <= 160 rows: the whole workgroup, tiles in LDS
+constexpr int kBigWaves = 4;       // wavefronts per large-front workgroup (the panel needs 16 + 48 kBigWaves >= 160)
+static_assert(16 + 48 * kBigWaves >= kBigTiles * 16, "the panel's rows must fit the large-front team");
 constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
 
 // Device copy of a Plan plus the per-level chunking of the solve kernels.

@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int 
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, const T* __restrict__ aval,
+__global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int h, int B, const T* __restrict__ aval,
                                                          const double* __restrict__ rs, const int* __restrict__ active,
                                                          T* __restrict__ L, double* __restrict__ y,
                                                          T* __restrict__ Uall, int* __restrict__ fail) {
@@ -513,8 +513,8 @@ __global__ __launch_bounds__(kThreads) void k_factor_big(DPlan P, int h, int B, 
     if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
     auto bsync = [] { __syncthreads(); };
-    process_front<T>(P, t, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kThreads, wave, kWaves, lane, &s_flag,
-                     bsync,
+    process_front<T>(P, t, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kBigWaves * 64, wave, kBigWaves, lane,
+                     &s_flag, bsync,
                      aval + static_cast<long long>(k) * P.nent, rs + static_cast<long long>(k) * P.n,
                      L + static_cast<long long>(k) * P.lsize, y + static_cast<long long>(k) * P.n,
                      Uall + static_cast<long long>(k) * P.usizeA);
