@@ -242,28 +242,49 @@ def main() -> int:
     # nested-dissection leaves): front flops sum np^3/3 + np^2 nu + np nu^2,
     # and L as m x np column blocks in FP32 read once per substitution sweep.
     plan = N.window_plan_stats(a.radius, 0, False, True)
-    kernels = {}
-    f_flops = plan["factor_flops"] * st["n_factored"]
-    f_s = st["ms_factor"] / 1e3
-    kernels["k_factor"] = {"bound": "mfma", "achieved": f_flops / f_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
-                           "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"],
-                           "flops_per_circuit": plan["factor_flops"]}
-    # sweeps: the back substitution after the fused factor-forward, then a
-    # forward + back sweep per refinement step
-    sweeps = st["n_factored"] * (1 + 2 * st["refine_mean"])
-    l_bytes = plan["l_unpacked"] * 4
-    s_s = st["ms_solve"] / 1e3
-    kernels["k_solve"] = {"bound": "hbm", "achieved": sweeps * l_bytes / s_s / 1e9, "peak": HBM_PEAK / 1e9,
-                          "unit": "GB/s", "launches": st["n_solve_launches"], "ms_total": st["ms_solve"],
-                          "bytes_per_sweep": l_bytes}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
-    kd = kernels[dom]
-    # one launch of k_solve = one substitution sweep over a batch of windows
-    traffic = SOLVE_TRAFFIC_PER_WINDOW_SWEEP * BATCH if dom == "k_solve" else None
-    roofline = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
-                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic}
-    if traffic is not None:
-        roofline["traffic_note"] = "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"
+
+    def roofline_of(st):
+        kernels = {}
+        f_flops = plan["factor_flops"] * st["n_factored"]
+        f_s = st["ms_factor"] / 1e3
+        kernels["k_factor"] = {"bound": "mfma", "achieved": f_flops / f_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
+                               "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"],
+                               "flops_per_circuit": plan["factor_flops"]}
+        # sweeps: the back substitution after the fused factor-forward, then a
+        # forward + back sweep per refinement step
+        sweeps = st["n_factored"] * (1 + 2 * st["refine_mean"])
+        l_bytes = plan["l_unpacked"] * 4
+        s_s = st["ms_solve"] / 1e3
+        kernels["k_solve"] = {"bound": "hbm", "achieved": sweeps * l_bytes / s_s / 1e9, "peak": HBM_PEAK / 1e9,
+                              "unit": "GB/s", "launches": st["n_solve_launches"], "ms_total": st["ms_solve"],
+                              "bytes_per_sweep": l_bytes}
+        dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+        kd = kernels[dom]
+        # one launch of k_solve = one substitution sweep over a batch of windows
+        per_batch = st["n_factored"] / max(st["n_batches"], 1)
+        traffic = round(SOLVE_TRAFFIC_PER_WINDOW_SWEEP * per_batch) if dom == "k_solve" else None
+        roof = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
+                "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic,
+                "concurrent_batches": st["n_streams"]}
+        if traffic is not None:
+            roof["traffic_note"] = "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"
+        return roof, kernels
+
+    # the timed region: launch durations are timed on each batch's stream
+    # while the other batches run beside it (per-launch rates, overlapping)
+    roofline, kernels = roofline_of(st)
+    if st["n_streams"] > 1:
+        roofline["achieved_note"] = (f"per launch over the timed region, {st['n_streams']} batches in flight on "
+                                     "their own streams, so launches overlap; single_stream: the same run with one "
+                                     "batch in flight (no overlap), untimed")
+        os.environ["LSC_BATCH_STREAMS"] = "1"
+        try:
+            st1 = run_gpu(*args, "mixed", a.tol)
+        finally:
+            del os.environ["LSC_BATCH_STREAMS"]
+        r1, k1 = roofline_of(st1)
+        roofline["single_stream"] = {k: r1[k] for k in ("kernel", "achieved", "frac", "traffic")}
+        roofline["single_stream"]["phase_ms"] = {k[3:]: round(v, 2) for k, v in st1.items() if k.startswith("ms_")}
 
     fp64 = None
     if not a.no_fp64:

@@ -74,6 +74,9 @@ typedef struct lsc_solve_stats {
     double resid_max;
     double resid_mean;
     double ms_fill, ms_flood, ms_factor, ms_solve, ms_residual, ms_currents, ms_accumulate;
+    int32_t n_streams; /* Omniscape: batches in flight at once, each on its own stream (the
+                          ms_ phases then add up the streams' overlapping times) */
+    int32_t reserved;
 } lsc_solve_stats;
 
 /* Message of the calling thread's last failure ("" if none). */

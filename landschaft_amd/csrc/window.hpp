@@ -85,6 +85,7 @@ struct IrStats {
     long long solved = 0, unconverged = 0, over_bar = 0, iter_sum = 0, solve_launches = 0;
     int batches = 0;
     int iter_max = 0;
+    int streams = 1;  // batches in flight at once
     double resid_max = 0, resid_sum = 0;
     double ms_fill = 0, ms_flood = 0, ms_factor = 0, ms_solve = 0, ms_residual = 0, ms_currents = 0,
            ms_accum = 0;
