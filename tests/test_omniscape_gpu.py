@@ -83,3 +83,26 @@ def test_correction_matches_oracle(backend, oracle):
         got = backend.circuit_omniscape_correction(*args)
         ref = oracle.omniscape_correction(*args)
         np.testing.assert_allclose(got, ref, rtol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_batches_in_flight_give_identical_maps(backend, oracle, precision, monkeypatch):
+    """Many small batches on one stream and on four streams (the accumulation is chained in batch order):
+    the cumulative and flow-potential maps are bit-identical, and match the oracle."""
+    rng = np.random.default_rng(21)
+    r = np.exp(rng.normal(0.0, 1.0, (70, 90)))
+    r[rng.random(r.shape) < 0.08] = np.nan
+    s = np.where(np.isnan(r), 0.0, 1 / np.nan_to_num(r, nan=1.0))
+    backend.settings.precision = precision
+    monkeypatch.setattr(backend.settings, "batch", 512)
+    maps = {}
+    for streams in ("1", "4"):
+        monkeypatch.setenv("LSC_BATCH_STREAMS", streams)
+        print(f"streams {streams} {precision}", flush=True)
+        maps[streams] = backend.circuit_omniscape(r, s, 7, 1, 1, False, False, True, True)
+        assert backend.last_stats["n_batches"] > 8
+    for k in ("current", "flow_potential"):
+        np.testing.assert_array_equal(maps["1"][k], maps["4"][k])
+    ref = oracle.omniscape(r, s, 7, 1, 1, False, False, True, True)
+    top = ref["current"].max()
+    np.testing.assert_allclose(maps["4"]["current"], ref["current"], rtol=1e-9, atol=1e-12 * top)
