@@ -178,6 +178,49 @@ def adjoint_leg(B, n, npairs, seed):
     return out
 
 
+def tune(a) -> int:
+    """A small autotuner: every combination of the window engine's tunables on one map, best first. The
+    library reads LSC_ND_LEAF and LSC_BATCH_STREAMS at each call; the batch size is a solve option."""
+    import torch
+
+    from landschaft_amd import _native as N
+
+    global BATCH
+    n = a.size if a.size != 4096 else 2048
+    r, s = workload(n)
+    dev = torch.device("cuda:0")
+    tr, ts = torch.from_numpy(r).to(dev), torch.from_numpy(s).to(dev)
+    tc = torch.empty((n, n), dtype=torch.float64, device=dev)
+    args = (N, tr.data_ptr(), ts.data_ptr(), tc.data_ptr(), n, n, a.radius, a.block)
+    results = []
+    saved = {k: os.environ.get(k) for k in ("LSC_ND_LEAF", "LSC_BATCH_STREAMS")}
+    try:
+        for leaf in (24, 32, 40):
+            for streams in (1, 2, 4):
+                for batch in (8192, 16384):
+                    os.environ["LSC_ND_LEAF"], os.environ["LSC_BATCH_STREAMS"] = str(leaf), str(streams)
+                    BATCH = batch
+                    run_gpu(*args, "mixed", a.tol, timing=False)  # warm-up (plans, allocations)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    st = run_gpu(*args, "mixed", a.tol, timing=False)
+                    torch.cuda.synchronize()
+                    dt = time.perf_counter() - t0
+                    results.append({"leaf": leaf, "streams": streams, "batch": batch,
+                                    "circuits_per_s": round(st["n_targets"] / dt, 1), "resid_max": st["resid_max"]})
+                    print(json.dumps(results[-1]), file=sys.stderr, flush=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    results.sort(key=lambda d: -d["circuits_per_s"])
+    print(json.dumps({"tune": f"omniscape {n}x{n} radius {a.radius} block {a.block}", "best": results[0],
+                      "all": results}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +237,12 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=16384, help="Omniscape windows per batch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer three times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--tune", action="store_true",
+                    help="time the window engine's tunables (nested-dissection leaf size, batches in flight, batch "
+                         "size) on the --size map (2048 unless given) and print the fastest setting")
     a = ap.parse_args()
+    if a.tune:
+        return tune(a)
     if a.copy_probe:
         import torch
 
@@ -302,7 +350,7 @@ def main() -> int:
     if not a.no_graph:
         from landschaft_amd.circuit import _backend as B
 
-        B.settings.tol = 1e-12
+        B.settings.tol = a.tol
         graph = {"pairwise_256": pairwise_leg(B, 256, False, 4, 101, None, True, cpu=not a.no_cpu),
                  "pairwise_2048": pairwise_leg(B, 2048, True, 16, 102, 2.0, True),
                  "adjoint_1024": adjoint_leg(B, 1024, 32, 104)}

@@ -447,13 +447,14 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     // Several batches in flight on their own streams (four by default,
     // LSC_BATCH_STREAMS), each driven by a host thread: the window kernels
     // are latency-bound (one front per wavefront, the factor limited by LDS),
-    // so other batches fill issue slots one batch leaves idle (measured on
-    // the 4096^2 bench: 1, 2, 3, 4 streams = 104.5k, 114.6k, 116.1k, 118.3k
-    // circuits/s). The cumulative map is added to in batch order (events),
-    // so the result is the same as with one stream, bit for bit.
+    // so other batches fill issue slots one batch leaves idle. The
+    // cumulative map is added to in batch order (events), so the result is
+    // the same as with one stream, bit for bit.
     const int nsol = static_cast<int>(std::max<long long>(1, std::min<long long>({batch_streams(), fit,
                                                                                   (Lt.nlat + want - 1) / want})));
-    const long long B = std::max<long long>(1, std::min<long long>(want, fit / nsol));
+    // the batches in flight share the one-batch budget: allocating (and
+    // freeing) HBM costs about 13 ms per GB, more than extra batches gain
+    const long long B = std::max<long long>(1, std::min<long long>(want, fit) / nsol);
     const long long nbatch = (Lt.nlat + B - 1) / B;
     std::vector<std::unique_ptr<WindowSolver<T>>> ws;
     std::vector<hipStream_t> extra(nsol - 1, nullptr);
