@@ -2,6 +2,7 @@
 // layout self-test.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -17,10 +18,11 @@ namespace lsc {
 
 void progress(const char* fmt, ...) {
     static const bool on = std::getenv("LSC_PROGRESS") != nullptr;
+    static const auto t0 = std::chrono::steady_clock::now();
     if (!on) return;
     std::va_list ap;
     va_start(ap, fmt);
-    std::fputs("[lsc] ", stderr);
+    std::fprintf(stderr, "[lsc %.3f s] ", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     std::vfprintf(stderr, fmt, ap);
     std::fputc('\n', stderr);
     std::fflush(stderr);

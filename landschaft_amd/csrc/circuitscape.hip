@@ -834,6 +834,27 @@ __global__ void k_cell_acc(long long cnt, const int* __restrict__ cells, const i
     if (mx) mx[c] = fmax(mx[c], x);
 }
 
+// Right-hand sides of pairwise jobs, k x n row-major: -1 at the job's source
+// unknown, +1 at its target (-1 index: the grounded node, no entry).
+__global__ void k_pair_rhs(long long n, int k, const int* __restrict__ a, const int* __restrict__ b,
+                           double* __restrict__ B) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n * k) return;
+    const int j = static_cast<int>(i / n);
+    const int u = static_cast<int>(i - static_cast<long long>(j) * n);
+    B[i] = u == a[j] ? -1.0 : (u == b[j] ? 1.0 : 0.0);
+}
+
+// out[j * m + i] = X[j * n + u[i]] (0 where u[i] < 0): the solutions at a
+// few unknowns, so only those cross PCIe.
+__global__ void k_pick(long long n, int k, int m, const int* __restrict__ u, const double* __restrict__ X,
+                       double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k * m) return;
+    const int j = i / m, q = u[i - j * m];
+    out[i] = q >= 0 ? X[static_cast<long long>(j) * n + q] : 0.0;
+}
+
 __global__ void k_zero_at(int cnt, const int* __restrict__ idx, double* __restrict__ v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cnt) v[idx[i]] = 0.0;
@@ -988,36 +1009,72 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         const i32 ref = M.mem[p1 - 1];
         const i64 count = p1 - p0;
         const i32* mb = M.mem.data() + p0;
+        progress("pairwise: component %d matrix (%lld nodes)", k, static_cast<long long>(count));
         HostMatrix A = component_matrix(g, mb, count, nullptr, [&](i32 v) { return v == ref; }, nodes);
         SystemSolver S(cx.precision, cx.s);
         S.factor(A);
+        progress("pairwise: factored; %zu right-hand sides", jobs.size());
         std::vector<i32> uidx(g.n, -1);  // node -> unknown (-1: the grounded node, other components)
         for (int i = 0; i < A.n; ++i) uidx[nodes[i]] = i;
         // every job's voltages (relative to its source) by member
         const std::size_t nj = jobs.size();
-        std::vector<double> B(nj * static_cast<std::size_t>(A.n), 0.0), X(nj * static_cast<std::size_t>(A.n), 0.0);
-        for (std::size_t j = 0; j < nj; ++j) {
-            const i32 a = uidx[jobs[j].first], b = uidx[jobs[j].second];
-            if (a >= 0) B[j * A.n + a] = -1.0;
-            if (b >= 0) B[j * A.n + b] = 1.0;
+        // right-hand sides built and solved on the device; without maps only
+        // the solutions at the component's focal nodes come back
+        std::vector<double> X, Xf;
+        std::unordered_map<i32, int> fpos;  // focal node -> column of Xf
+        if (A.n > 0) {
+            std::vector<int> ja(nj), jb(nj), fu(csub.size());
+            for (std::size_t j = 0; j < nj; ++j) ja[j] = uidx[jobs[j].first], jb[j] = uidx[jobs[j].second];
+            for (std::size_t i = 0; i < csub.size(); ++i) fu[i] = uidx[csub[i]], fpos[csub[i]] = static_cast<int>(i);
+            DevBuf<int> dja = to_device(ja, cx.s), djb = to_device(jb, cx.s), dfu = to_device(fu, cx.s);
+            DevBuf<double> dB, dX;
+            const long long tot = static_cast<long long>(nj) * A.n;
+            dB.alloc(tot), dX.alloc(tot);
+            hipLaunchKernelGGL(k_pair_rhs, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, cx.s,
+                               static_cast<long long>(A.n), static_cast<int>(nj), dja.p, djb.p, dB.p);
+            S.solve_device(static_cast<int>(nj), dB.p, dX.p, cx.tol, cx.maxit, cx.st);
+            if (o.maps()) {
+                X.resize(static_cast<std::size_t>(tot));
+                LSC_HIP(hipMemcpyAsync(X.data(), dX.p, tot * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+            } else {
+                const int nfc = static_cast<int>(csub.size());
+                DevBuf<double> dXf;
+                dXf.alloc(static_cast<std::size_t>(nj) * nfc);
+                hipLaunchKernelGGL(k_pick, dim3(static_cast<unsigned>((nj * nfc + 255) / 256)), dim3(256), 0, cx.s,
+                                   static_cast<long long>(A.n), static_cast<int>(nj), nfc, dfu.p, dX.p, dXf.p);
+                Xf.resize(nj * nfc);
+                LSC_HIP(hipMemcpyAsync(Xf.data(), dXf.p, Xf.size() * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+            }
+            LSC_HIP(hipStreamSynchronize(cx.s));
+        } else if (o.maps()) {
+            X.assign(nj * static_cast<std::size_t>(A.n), 0.0);
         }
-        if (A.n > 0) S.solve(static_cast<int>(nj), B.data(), X.data(), cx.tol, cx.maxit, cx.st);
+        progress("pairwise: solved");
         tally(cx, S);
         auto midx = [&](i32 v) { return static_cast<i32>(std::lower_bound(mb, mb + count, v) - mb); };
-        std::vector<double> vcol(count), vnode, cnode;
+        // voltage of focal node v in job j (relative to the grounded node)
+        auto fvolt = [&](std::size_t j, i32 v) -> double {
+            if (o.maps()) {
+                const i32 u = uidx[v];
+                return u >= 0 ? X[j * A.n + u] : 0.0;
+            }
+            return A.n > 0 ? Xf[j * csub.size() + fpos.at(v)] : 0.0;
+        };
+        std::vector<double> vcol(o.maps() ? count : 0), vnode, cnode;
         if (o.maps()) {
             vnode.assign(g.n, 0.0), cnode.assign(g.n, 0.0);
             dv.alloc(g.n ? g.n : 1), dcur.alloc(g.n ? g.n : 1);
         }
         for (std::size_t j = 0; j < nj; ++j) {
             const auto& job = jobs[j];
-            for (i64 q = 0; q < count; ++q) {
-                const i32 u = uidx[mb[q]];
-                vcol[q] = u >= 0 ? X[j * A.n + u] : 0.0;
+            const double vs = fvolt(j, job.first);
+            if (o.maps()) {
+                for (i64 q = 0; q < count; ++q) {
+                    const i32 u = uidx[mb[q]];
+                    vcol[q] = (u >= 0 ? X[j * A.n + u] : 0.0) - vs;
+                }
             }
-            const double vs = vcol[midx(job.first)];
-            for (i64 q = 0; q < count; ++q) vcol[q] -= vs;
-            const double resistance = vcol[midx(job.second)] - vcol[midx(job.first)];
+            const double resistance = (fvolt(j, job.second) - vs) - (vs - vs);
             if (o.maps()) {
                 for (i64 q = 0; q < count; ++q) vnode[mb[q]] = vcol[q];
                 dv.upload(vnode.data(), g.n, cx.s);
@@ -1033,7 +1090,7 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
                         for (i32 i = 1; i < np; ++i) {
                             const i32 v = points[i];
                             if (v < 0 || g.comp[v] != k) continue;
-                            voltmatrix[static_cast<std::size_t>(i) * np + cj] = 1 - (vcol[midx(v)] / resistance);
+                            voltmatrix[static_cast<std::size_t>(i) * np + cj] = 1 - ((fvolt(j, v) - vs) / resistance);
                         }
                         continue;
                     }
@@ -1250,7 +1307,9 @@ int lsc_circuitscape_pairwise(const double* gmap, const int64_t* polymap, int64_
                               lsc_solve_stats* stats) {
     return lsc::guarded([&] {
         lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::progress("pairwise: raster graph of %lld x %lld cells", static_cast<long long>(nx), static_cast<long long>(ny));
         lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::progress("pairwise: %d nodes, %d components; upload", G.n, G.ncomp);
         lsc::DevGraph DG;
         DG.upload(G, cx.s);
         std::vector<i32> nodes(npts);
