@@ -171,13 +171,96 @@ def circuitscape_advanced(gmap, polymap, source, ground, avg_resistances, four_n
 
 def circuitscape_pairwise(gmap, polymap, point_cells, excluded, avg_resistances, four_neighbors, options, emit=None,
                           n_threads=0):
-    """Pairwise mode (python.cpp:214-277): resistances, cum / max maps; per-pair maps through `emit`."""
+    """Pairwise mode (python.cpp:214-277): resistances, cum / max maps; per-pair maps through `emit`.
+    With ``options["region_ids"]`` (the id of every point cell, sorted) the points are focal regions,
+    solved one graph per pair of ids."""
     g = N.f64(gmap)
     if g.ndim != 2:
         raise ValueError("expected a 2-D habitat raster")
-    ny, nx = g.shape
     if options.get("region_ids") is not None:
-        raise ValueError("focal regions are not supported by the GPU circuit path")
+        return pairwise_regions(_pairwise_points, g, polymap, point_cells, options["region_ids"], excluded,
+                                avg_resistances, four_neighbors, options, emit)
+    return _pairwise_points(g, polymap, point_cells, excluded, avg_resistances, four_neighbors, options, emit)
+
+
+def region_polymap(polymap, ncell, cells, ids, a, b):
+    """The short-circuit map of one pair of focal regions (circuitscape.cpp:749-779), flat.
+
+    Without polygons the cells of ids ``a`` and ``b`` are polygons of those
+    ids. With polygons, a region of several cells becomes one new polygon
+    (numbered past the largest id) that takes in every polygon it overlaps;
+    a single-cell point is left as it is.
+    """
+    if polymap is None:
+        poly = np.zeros(ncell, dtype=np.int64)
+        for pid in (a, b):
+            poly[cells[ids == pid]] = pid
+        return poly
+    poly = np.array(polymap, dtype=np.int64).ravel()
+    top = int(poly.max())
+    for pid in (a, b):
+        mine = cells[ids == pid]
+        if len(mine) == 1:
+            continue
+        top += 1
+        hit = np.unique(poly[mine])
+        hit = hit[hit != 0]
+        if hit.size:
+            poly[np.isin(poly, hit)] = top
+        poly[mine] = top
+    return poly
+
+
+def pairwise_regions(solve, gmap, polymap, point_cells, region_ids, excluded, avg_resistances, four_neighbors,
+                     options, emit=None):
+    """raster_pairwise_regions (circuitscape.cpp:781-855): every pair of distinct ids not excluded
+    gets its own graph (``region_polymap``) and one two-point pairwise solve by ``solve`` (a
+    pairwise function taking this module's arguments), between the first cells of the two ids.
+    Resistances -1 where a pair is not solved, 0 on the diagonal; the cum map is the sum of the
+    pairs' cum maps, the max map their maximum, both floored at -9999."""
+    g = np.asarray(gmap, dtype=np.float64)
+    ny, nx = g.shape
+    cells = np.asarray(point_cells, dtype=np.int64).ravel()
+    ids = np.asarray(region_ids, dtype=np.int64).ravel()
+    if cells.size != ids.size:
+        raise ValueError("region_ids needs one id per point cell")
+    pts = ids[np.r_[True, ids[1:] != ids[:-1]]] if ids.size else ids
+    n = len(pts)
+    ex = np.asarray(excluded, dtype=np.uint8).reshape(n, n)
+    R = np.full((n, n), -1.0)
+    cum = np.zeros((ny, nx)) if options.get("cum") else None
+    mx = np.full((ny, nx), -9999.0) if options.get("max") else None
+    sub = {k: v for k, v in options.items() if k != "region_ids"}
+    for i in range(n):
+        for j in range(i + 1, n):
+            if ex[i, j] or ex[j, i]:
+                continue
+            poly = region_polymap(polymap, nx * ny, cells, ids, pts[i], pts[j]).reshape(ny, nx)
+            ends = np.array([cells[np.flatnonzero(ids == pts[i])[0]], cells[np.flatnonzero(ids == pts[j])[0]]],
+                            dtype=np.int64)
+
+            def remap(ci, cj, v, c, i=i, j=j):
+                if emit is not None:
+                    emit(i if ci == 0 else j, i if cj == 0 else j, v, c)
+
+            r = solve(g, poly, ends, np.zeros((2, 2), dtype=np.uint8), avg_resistances, four_neighbors, sub, remap)
+            R[i, j] = R[j, i] = r["resistances"][0, 1]
+            if cum is not None:
+                cum += r["cum"]
+            if mx is not None:
+                np.maximum(mx, r["max"], out=mx)
+    np.fill_diagonal(R, 0.0)
+    out = {"resistances": R, "nodes": None}
+    if cum is not None:
+        cum[cum < -9999.0] = -9999.0
+        out["cum"] = cum
+    if mx is not None:
+        out["max"] = mx
+    return out
+
+
+def _pairwise_points(g, polymap, point_cells, excluded, avg_resistances, four_neighbors, options, emit=None):
+    ny, nx = g.shape
     poly = _poly(polymap, g.shape)
     pts = np.ascontiguousarray(point_cells, dtype=np.int64).ravel()
     n = pts.size

@@ -1,14 +1,27 @@
 """Circuitscape's raster modes: advanced, pairwise and one-to-all / all-to-one.
 
-The Python layer of the reference's src/landschaft/circuit/circuitscape.py
-(:170-353, :366-516, :519-670, :673-953): the same functions, arguments,
-defaults and results, over arrays (raster files are read by the caller). The
-solves run in the GPU graph engine behind ``core()``.
+The Python layer of the reference's src/landschaft/circuit/circuitscape.py:
+the readers of Circuitscape's file formats (:28-315), the array API
+(:366-953) and the INI driver (:956-1286), with the same functions,
+arguments, defaults and results. The solves run in the GPU graph engine
+behind ``core()``.
+
+Rasters are ESRI ASCII grids (parsed in double, as Circuitscape does) or
+GeoTIFF files read by the small reader below (strips or tiles, uncompressed
+or deflate, integer or float samples, band 1); the reference reads the
+latter through rasterio, which this package does not use.
 """
 
 from __future__ import annotations
 
+import gzip
+import math
+import os
+import struct
+import warnings
+import zlib
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -17,6 +30,262 @@ from .advanced import _grid
 
 NODATA = -9999.0
 POLICIES = ("keepall", "rmvsrc", "rmvgnd", "rmvall")
+SCENARIOS = {"pairwise": "pairwise", "Pairwise": "pairwise", "advanced": "advanced", "Advanced": "advanced",
+             "one-to-all": "one-to-all", "one_to_all": "one-to-all", "all-to-one": "all-to-one",
+             "all_to_one": "all-to-one"}
+TRUELIST = ("True", "true", "1")
+FALSELIST = ("False", "false", "0")
+
+
+# ---- reading Circuitscape's inputs (circuitscape.py:28-315)
+
+@dataclass
+class RasterMeta:
+    """The habitat's grid: size, lower-left corner, cell size, GDAL-order transform
+    ``(x0, dx, rot, y0, rot, -dy)``, projection WKT and, for GeoTIFF inputs, the raw GeoKey tags
+    (``crs``), which written GeoTIFF outputs carry over."""
+
+    nrows: int
+    ncols: int
+    xllcorner: float
+    yllcorner: float
+    cellsize: float
+    transform: tuple = (0.0, 1.0, 0.0, 0.0, 0.0, -1.0)
+    wkt: str = ""
+    crs: object = None
+
+
+def _open(path):
+    p = os.fspath(path)
+    return gzip.open(p, "rt", encoding="utf-8") if p.lower().endswith("gz") else open(p, encoding="utf-8")
+
+
+def _read_bytes(path) -> bytes:
+    p = os.fspath(path)
+    with (gzip.open if p.lower().endswith("gz") else open)(p, "rb") as f:
+        return f.read()
+
+
+def file_type(path) -> str:
+    """Circuitscape's file-type guess (circuitscape.py:48-64): ``geotiff`` (TIFF magic), ``npy``,
+    ``aagrid`` (``ncols`` first), ``pairs_aagrid`` (``min``), ``pairs`` (``mode``) or ``list``."""
+    p = os.fspath(path)
+    with (gzip.open if p.lower().endswith("gz") else open)(p, "rb") as f:
+        head = f.read(4)
+    if head[2:4] == b"\x2a\x00":
+        return "geotiff"
+    with _open(p) as f:
+        first = f.readline()
+    if first.startswith("\x93NUMPY"):
+        return "npy"
+    if first.lower().startswith("ncols"):
+        return "aagrid"
+    if first.startswith("min"):
+        return "pairs_aagrid"
+    if first.startswith("mode"):
+        return "pairs"
+    return "list"
+
+
+_ASC_KEYS = ("ncols", "nrows", "xllcorner", "yllcorner", "xllcenter", "yllcenter", "cellsize", "dx", "dy",
+             "nodata_value")
+
+
+def _read_asc(path):
+    """An ESRI ASCII grid in double (circuitscape.py:70-93): header keys until the first data row,
+    NoData and NaN cells -9999, corners from ``*llcenter`` when given so."""
+    head: dict = {}
+    values: list = []
+    nrow_tokens = 0
+    with _open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if not values and len(parts) == 2 and parts[0].lower() in _ASC_KEYS:
+                head[parts[0].lower()] = float(parts[1])
+                continue
+            values.extend(float(t) for t in parts)
+            nrow_tokens += 1
+    ny, nx = int(head["nrows"]), int(head["ncols"])
+    if nrow_tokens != ny or len(values) != ny * nx:
+        raise ValueError(f"{path}: the ASCII grid does not hold {ny} rows of {nx} values")
+    a = np.array(values, dtype=np.float64).reshape(ny, nx)
+    dx = head.get("dx", head.get("cellsize", math.nan))
+    dy = head.get("dy", dx)
+    x0 = head["xllcorner"] if "xllcorner" in head else head["xllcenter"] - dx / 2
+    y0 = head["yllcorner"] if "yllcorner" in head else head["yllcenter"] - dy / 2
+    a[(a == head.get("nodata_value", NODATA)) | np.isnan(a)] = NODATA
+    p = os.fspath(path)
+    prj = Path((p[:-3] if p.endswith(".gz") else p)[:-4] + ".prj")
+    wkt = prj.read_text().strip() if prj.exists() else ""
+    return a, RasterMeta(ny, nx, x0, y0, dx, (x0, dx, 0.0, y0 + ny * dy, 0.0, -dy), wkt)
+
+
+# TIFF field types: struct code and size
+_TIFF_TYPES = {1: ("B", 1), 2: ("s", 1), 3: ("H", 2), 4: ("I", 4), 5: ("I", 8), 6: ("b", 1), 7: ("B", 1),
+               8: ("h", 2), 9: ("i", 4), 10: ("i", 8), 11: ("f", 4), 12: ("d", 8), 16: ("Q", 8), 17: ("q", 8)}
+_GEOKEY_TAGS = (34735, 34736, 34737)
+
+
+def _tiff_tags(buf: bytes):
+    """The first IFD of a classic or Big TIFF as ``(byte order, {tag: values})``."""
+    order = {b"II": "<", b"MM": ">"}.get(buf[:2])
+    if order is None:
+        raise ValueError("not a TIFF file")
+    magic = struct.unpack_from(order + "H", buf, 2)[0]
+    if magic not in (42, 43):
+        raise ValueError("not a TIFF file")
+    big = magic == 43
+    word = "Q" if big else "I"
+    pos = struct.unpack_from(order + word, buf, 8 if big else 4)[0]
+    n = struct.unpack_from(order + ("Q" if big else "H"), buf, pos)[0]
+    pos += 8 if big else 2
+    tags: dict = {}
+    for _ in range(n):
+        tag, typ = struct.unpack_from(order + "HH", buf, pos)
+        count = struct.unpack_from(order + word, buf, pos + 4)[0]
+        at = pos + (12 if big else 8)
+        pos += 20 if big else 12
+        if typ not in _TIFF_TYPES:
+            continue
+        code, size = _TIFF_TYPES[typ]
+        nbytes = count * size
+        if nbytes > (8 if big else 4):
+            at = struct.unpack_from(order + word, buf, at)[0]
+        raw = buf[at:at + nbytes]
+        if typ == 2:
+            tags[tag] = raw.split(b"\0")[0].decode("latin-1")
+        elif typ in (5, 10):
+            v = struct.unpack(order + f"{2 * count}{code}", raw)
+            tags[tag] = tuple(v[k] / v[k + 1] for k in range(0, len(v), 2))
+        else:
+            tags[tag] = struct.unpack(order + f"{count}{code}", raw)
+    return order, tags
+
+
+def _read_tiff(path):
+    """Band 1 of a GeoTIFF as ``(samples in their own dtype, GDAL nodata text or None, RasterMeta)``."""
+    buf = _read_bytes(path)
+    order, t = _tiff_tags(buf)
+    nx, ny = t[256][0], t[257][0]
+    bits, comp = t.get(258, (1,))[0], t.get(259, (1,))[0]
+    spp, planar = t.get(277, (1,))[0], t.get(284, (1,))[0]
+    fmt, pred = t.get(339, (1,))[0], t.get(317, (1,))[0]
+    if comp not in (1, 8, 32946) or pred != 1 or fmt not in (1, 2, 3) or bits % 8:
+        raise ValueError(f"{path}: only uncompressed or deflate GeoTIFF without a predictor is read here")
+    dt = np.dtype(f"{order}{'uif'[fmt - 1]}{bits // 8}")
+    per = spp if planar == 1 else 1
+
+    def chunk(off, cnt, n):
+        raw = buf[off:off + cnt]
+        raw = zlib.decompress(raw) if comp != 1 else raw
+        return np.frombuffer(raw, dt, count=n * per).reshape(-1, per)[:, 0]
+
+    if 322 in t:
+        tw, th = t[322][0], t[323][0]
+        tx, ty = -(-nx // tw), -(-ny // th)
+        band = np.empty((ty * th, tx * tw), dt)
+        for k in range(tx * ty):  # band 1's tiles come first
+            r0, c0 = (k // tx) * th, (k % tx) * tw
+            band[r0:r0 + th, c0:c0 + tw] = chunk(t[324][k], t[325][k], tw * th).reshape(th, tw)
+        band = band[:ny, :nx]
+    else:
+        rps = min(t.get(278, (ny,))[0], ny)
+        strips = []
+        for k in range(-(-ny // rps)):
+            rows = min(rps, ny - k * rps)
+            strips.append(chunk(t[273][k], t[279][k], rows * nx).reshape(rows, nx))
+        band = np.concatenate(strips)
+    if 34264 in t:
+        m = t[34264]
+        a_, b_, c_, d_, e_, f_ = m[0], m[1], m[3], m[4], m[5], m[7]
+    elif 33550 in t and 33922 in t:
+        sx, sy = t[33550][0], t[33550][1]
+        i, j, _, x, y, _ = t[33922][:6]
+        a_, b_, c_, d_, e_, f_ = sx, 0.0, x - i * sx, 0.0, -sy, y + j * sy
+    else:
+        a_, b_, c_, d_, e_, f_ = 1.0, 0.0, 0.0, 0.0, 1.0, 0.0
+    nodata = t.get(42113)
+    keys = {k: t[k] for k in _GEOKEY_TAGS if k in t}
+    meta = RasterMeta(ny, nx, c_, f_ - ny * a_, a_, (c_, a_, b_, f_, d_, e_), "", keys or None)
+    return band, nodata, meta
+
+
+def read_raster(path):
+    """Circuitscape's raster read (circuitscape.py:96-120): an ESRI ASCII grid parsed in double, or a
+    GeoTIFF, with its nodata value and NaN set to -9999."""
+    p = os.fspath(path)
+    if not os.path.isfile(p):
+        raise FileNotFoundError(f'the file "{p}" does not exist')
+    if p.lower().removesuffix(".gz").endswith(".asc"):
+        return _read_asc(p)
+    if file_type(p) != "geotiff":
+        raise ValueError(f"{p}: only ESRI ASCII grids and GeoTIFF rasters are read")
+    band, nodata, meta = _read_tiff(p)
+    a = band.astype(np.float64)
+    if nodata is not None and nodata.strip():
+        nd = float(nodata.strip())
+        if np.issubdtype(band.dtype, np.integer):
+            nd = float(np.int64(nd)) if math.isfinite(nd) else nd
+        else:
+            nd = float(np.array(nd).astype(band.dtype))
+        a[a == nd] = NODATA
+    a[np.isnan(a)] = NODATA
+    return a, meta
+
+
+def _read_rows(path) -> np.ndarray:
+    rows = []
+    with _open(path) as f:
+        for line in f:
+            parts = line.split()
+            if parts:
+                rows.append([float(t) for t in parts])
+    return rows
+
+
+def _read_list(path) -> np.ndarray:
+    """A point list: ``id x y`` per line."""
+    return np.array(_read_rows(path), dtype=np.float64).reshape(-1, 3)
+
+
+def _list_cells(points: np.ndarray, meta: RasterMeta):
+    # 1-based row and column of a point (circuitscape.py:133-137)
+    i = np.ceil(meta.nrows - (points[:, 2] - meta.yllcorner) / meta.cellsize).astype(np.int64)
+    j = np.ceil((points[:, 1] - meta.xllcorner) / meta.cellsize).astype(np.int64)
+    return i, j
+
+
+def _check_meta(meta: RasterMeta, habitat: RasterMeta) -> None:
+    for name in ("cellsize", "ncols", "nrows", "yllcorner", "xllcorner"):
+        if getattr(meta, name) != getattr(habitat, name):
+            warnings.warn(f"{name} is not the same", stacklevel=4)
+            return
+
+
+def read_polymap(path, habitat: RasterMeta, *, integer: bool = False, nodata_as=0) -> np.ndarray:
+    """A raster on the habitat's grid (circuitscape.py:147-156), NoData set to ``nodata_as``
+    (kept as -9999 with None); ``integer`` rejects non-integer values."""
+    a, meta = read_raster(path)
+    if integer and not np.all(a == np.round(a)):
+        raise ValueError(f"{path} contains non-integer values")
+    if nodata_as is not None:
+        a[a == NODATA] = nodata_as
+    _check_meta(meta, habitat)
+    return a
+
+
+def _read_list_n(path, ncol: int) -> np.ndarray:
+    t = np.array(_read_rows(path), dtype=np.float64)
+    if t.ndim != 2 or t.shape[1] != ncol:
+        raise ValueError(f"{path} must have exactly {ncol} columns")
+    return t
+
+
+def read_reclass_table(path) -> dict:
+    """A reclass table (two columns, old and new value) as a lookup (circuitscape.py:159-163)."""
+    return {float(a): float(b) for a, b in _read_list_n(path, 2)}
 
 
 @dataclass
@@ -60,6 +329,66 @@ class IncludedPairs:
         ids = [int(v) for v in self.ids]
         return {(ids[i], ids[j]) for i in range(len(ids)) for j in range(len(ids))
                 if m[i, j] == skip and m[j, i] == skip}
+
+
+def read_point_map(path, habitat: RasterMeta) -> FocalPoints:
+    """Focal points (circuitscape.py:209-229): a raster of ids, cells of one id a focal region, or a
+    text list of ``id x y``; negative ids dropped, stably sorted by id."""
+    if file_type(path) == "list":
+        pts = _read_list(path)
+        rows, cols = _list_cells(pts, habitat)
+        ids = pts[:, 0]
+    else:
+        a = read_polymap(path, habitat, integer=True)
+        c0, r0 = np.nonzero(a.T)  # column-major cell order
+        rows, cols, ids = r0 + 1, c0 + 1, a[r0, c0]
+    keep = ids >= 0
+    order = np.argsort(ids[keep], kind="stable")
+    rows = rows[keep][order].astype(np.int64)
+    cols = cols[keep][order].astype(np.int64)
+    ids = ids[keep][order].astype(np.int64)
+    if len(ids) and (rows.min() < 0 or cols.min() < 0 or rows.max() > habitat.nrows or cols.max() > habitat.ncols):
+        raise ValueError("At least one focal node location falls outside of habitat map")
+    if len(np.unique(ids)) < 2:
+        raise ValueError("Less than two valid focal nodes found. Please check focal node location file.")
+    return FocalPoints(rows, cols, ids)
+
+
+def read_included_pairs(path, *, starts_from_zero: bool = False) -> IncludedPairs:
+    """Include / exclude pairs (circuitscape.py:283-311): a ``min``/``max`` matrix of ids (a pair is
+    included when min <= value <= max) or a ``mode include|exclude`` list of id pairs.
+    ``starts_from_zero`` shifts the ids of a network numbered from 0 by one."""
+    kind = file_type(path)
+    with _open(path) as f:
+        lines = [ln.split() for ln in f.read().splitlines()]
+    shift = 1 if starts_from_zero else 0
+    if kind == "pairs_aagrid":
+        lo, hi = float(lines[0][1]), float(lines[1][1])
+        a = np.array([[float(t) for t in ln] for ln in lines[2:] if ln])
+        m = a[1:, 1:]
+        m = np.where(m > hi, 0, m)
+        return IncludedPairs("include", a[0, 1:].astype(np.int64) + shift, (m >= lo).astype(np.int64))
+    if kind == "pairs":
+        mode = lines[0][1]
+        pairs = np.array([[float(t) for t in ln] for ln in lines[1:] if ln]).reshape(-1, 2) + shift
+        ids = np.unique(pairs).astype(np.int64)
+        ids = ids[ids != 0]
+        where = {int(v): k for k, v in enumerate(ids)}
+        m = np.zeros((len(ids), len(ids)), dtype=np.int64)
+        for a, b in pairs.astype(np.int64):
+            if int(a) in where and int(b) in where:
+                m[where[int(a)], where[int(b)]] = m[where[int(b)], where[int(a)]] = 1
+        return IncludedPairs(mode, ids, m)
+    raise ValueError("Error reading focal node include/exclude pairs file.")
+
+
+def read_point_strengths(path) -> np.ndarray:
+    """Source strengths (circuitscape.py:704-709): columns id and strength; ids numbered from 0
+    are shifted by one."""
+    t = _read_list_n(path, 2)
+    if t[:, 0].min() == 0:
+        t[:, 0] += 1
+    return t
 
 
 @dataclass
@@ -173,13 +502,20 @@ def circuitscape_advanced(habitat, source, ground, *, habitat_is_resistance: boo
         with np.errstate(divide="ignore"):
             gnd = 1.0 / gnd
     gnd[missing] = 0.0
-    if unit_currents:
+    return _advanced(g, poly, src, gnd, four_neighbors, avg_resistances, remove_src_or_gnd, unit_currents,
+                     direct_grounds, n_threads)
+
+
+def _advanced(g, poly, src, gnd, four, avg, policy, unit, direct, n_threads) -> CircuitscapeResult:
+    # circuitscape.py:499-516: sources in amps and ground conductances per cell, then the solve
+    if unit:
         src = np.where(src != 0, 1.0, src)
-    if direct_grounds:
+    if direct:
         gnd = np.where(gnd != 0, np.inf, gnd)
-    r = core().circuitscape_advanced(np.ascontiguousarray(g), poly, np.ascontiguousarray(src),
-                                     np.ascontiguousarray(gnd), bool(avg_resistances), bool(four_neighbors),
-                                     remove_src_or_gnd, int(n_threads))
+    r = core().circuitscape_advanced(np.ascontiguousarray(g, dtype=np.float64), poly,
+                                     np.ascontiguousarray(src, dtype=np.float64),
+                                     np.ascontiguousarray(gnd, dtype=np.float64), bool(avg), bool(four), policy,
+                                     int(n_threads))
     nm = r["nodemap"]
     on = nm >= 0
     volt = np.zeros(nm.shape)
@@ -233,35 +569,48 @@ def circuitscape_pairwise(habitat, points, *, habitat_is_resistance: bool = True
     if included_pairs is not None and not isinstance(included_pairs, IncludedPairs):
         raise ValueError("included_pairs must be an IncludedPairs (files are read by the caller)")
     points, excluded = _apply_included_pairs(points, included_pairs)
-    if points.has_regions:
-        raise ValueError("focal regions are not supported by the GPU pairwise path")
-    ids = points.ids
-    n = len(ids)
     maps: dict = {}
 
-    def emit(ci, cj, volt, cur):
-        key = (int(ids[ci]), int(ids[cj]))
+    def emit(a, b, volt, cur):
         if volt is not None:
-            maps[("voltage", *key)] = volt
+            maps[("voltage", a, b)] = volt
         if cur is not None:
-            maps[("current", *key)] = cur
+            maps[("current", a, b)] = cur
 
     opts = _pair_options(voltage_maps, current_maps or cum_map, not current_maps and cum_map, max_map,
                          set_focal_node_currents_to_zero, set_null_currents_to_nodata, set_null_voltages_to_nodata,
                          log_transform_maps)
+    res, r = _pairwise(g, poly, points, excluded, four_neighbors, avg_resistances, opts, emit, n_threads)
+    return CircuitscapeResult(resistances=res, cum_current=r.get("cum"), max_current=r.get("max"), maps=maps)
+
+
+def _pairwise(g, poly, points: FocalPoints, excluded: set, four, avg, opts: dict, emit, n_threads):
+    """circuitscape.py:519-540: the resistance table (ids in the first row and column) and the
+    binding's result; ``emit(id_a, id_b, volt, cur)`` receives each pair's maps. Focal regions
+    (ids covering several cells) are solved one graph per pair of ids."""
     rows, cols = points.cells
     cells = np.ascontiguousarray(rows * g.shape[1] + cols, dtype=np.int64)
+    if points.has_regions:
+        opts = {**opts, "region_ids": np.ascontiguousarray(points.ids, dtype=np.int64)}
+        ids = np.unique(points.ids)
+    else:
+        ids = points.ids
+    n = len(ids)
     ex = np.zeros((n, n), dtype=np.uint8)
     for a in range(n):
         for b in range(n):
             ex[a, b] = (int(ids[a]), int(ids[b])) in excluded
-    r = core().circuitscape_pairwise(np.ascontiguousarray(g), poly, cells, ex, bool(avg_resistances),
-                                     bool(four_neighbors), opts, emit, int(n_threads))
+
+    def named(ci, cj, volt, cur):
+        emit(int(ids[ci]), int(ids[cj]), volt, cur)
+
+    r = core().circuitscape_pairwise(np.ascontiguousarray(g, dtype=np.float64), poly, cells, ex, bool(avg),
+                                     bool(four), opts, named, int(n_threads))
     res = np.zeros((n + 1, n + 1))
     res[0, 1:] = ids
     res[1:, 0] = ids
     res[1:, 1:] = r["resistances"]
-    return CircuitscapeResult(resistances=res, cum_current=r.get("cum"), max_current=r.get("max"), maps=maps)
+    return res, r
 
 
 # ---- one-to-all / all-to-one
@@ -448,3 +797,297 @@ def circuitscape_one_to_all(habitat, points, *, all_to_one: bool = False, habita
                     included_pairs, not all_to_one, four_neighbors, avg_resistances, opts, emit, n_threads)
     r.maps = maps
     return r
+
+
+# ---- INI files (circuitscape.py:971-1286)
+
+# Circuitscape's option defaults (its default.ini), which every INI file overrides
+DEFAULTS = {
+    "version": "unknown", "connect_four_neighbors_only": "False", "connect_using_avg_resistances": "False",
+    "use_polygons": "False", "polygon_file": "(Browse for a short-circuit region file)",
+    "source_file": "(Browse for a current source file)", "ground_file": "(Browse for a ground point file)",
+    "ground_file_is_resistances": "True", "use_unit_currents": "False", "use_direct_grounds": "False",
+    "remove_src_or_gnd": "keepall", "use_mask": "False", "mask_file": "None", "data_type": "raster",
+    "scenario": "not entered", "habitat_file": "(Browse for a resistance file)",
+    "habitat_map_is_resistances": "True", "solver": "cg+amg", "parallelize": "False", "precision": "double",
+    "use_variable_source_strengths": "False", "variable_source_file": "None", "use_included_pairs": "False",
+    "included_pairs_file": "(Browse for a file with pairs to include or exclude)",
+    "point_file": "(Browse for file with locations of focal points or regions)", "use_reclass_table": "False",
+    "reclass_file": "(Browse for file with reclassification data)",
+    "output_file": "(Choose a base name for output files)", "write_cur_maps": "False", "write_volt_maps": "False",
+    "write_cum_cur_map_only": "False", "write_max_cur_maps": "False", "set_null_currents_to_nodata": "False",
+    "set_null_voltages_to_nodata": "False", "set_focal_node_currents_to_zero": "False", "compress_grids": "False",
+    "log_transform_maps": "False", "write_as_tif": "False",
+}
+
+
+def _unset(value: str) -> bool:
+    # an empty path, None, or the GUI's "(Browse for ...)" placeholder
+    return value == "" or value == "None" or value.startswith("(")
+
+
+def _flag(cfg: dict, key: str) -> bool:
+    v = cfg[key]
+    if v in TRUELIST:
+        return True
+    if v in FALSELIST:
+        return False
+    raise ValueError(f'{key} = "{v}": expected True or False')
+
+
+def _resolve(name: str, base: Path) -> Path:
+    # paths are read from the working directory, else next to the INI file
+    p = Path(name)
+    if p.is_absolute() or p.exists():
+        return p
+    q = base / p
+    return q if q.exists() else p
+
+
+def parse_config(path) -> dict:
+    """An INI file as Circuitscape reads it (circuitscape.py:1032-1036): ``[section]`` lines skipped,
+    every other line split at its first ``=``."""
+    from .ini import parse_ini
+
+    return parse_ini(path)
+
+
+def _write_tiff(path: str, values: np.ndarray, meta: RasterMeta) -> None:
+    """A float64 GeoTIFF (little-endian, one uncompressed strip) on the habitat's grid, NoData -9999."""
+    ny, nx = values.shape
+    data = np.ascontiguousarray(values, dtype="<f8").tobytes()
+    x0, dx, _, y0, _, ndy = meta.transform
+    entries = [(256, 4, (nx,)), (257, 4, (ny,)), (258, 3, (64,)), (259, 3, (1,)), (262, 3, (1,)),
+               (273, 4, (0,)), (277, 3, (1,)), (278, 4, (ny,)), (279, 4, (len(data),)), (284, 3, (1,)),
+               (339, 3, (3,)), (33550, 12, (dx, -ndy, 0.0)), (33922, 12, (0.0, 0.0, 0.0, x0, y0, 0.0))]
+    for tag, vals in sorted((meta.crs or {}).items()):
+        entries.append((tag, {34735: 3, 34736: 12, 34737: 2}[tag], vals))
+    entries.append((42113, 2, "-9999"))
+    entries.sort(key=lambda e: e[0])
+    code = {2: "s", 3: "H", 4: "I", 12: "d"}
+    size = {2: 1, 3: 2, 4: 4, 12: 8}
+
+    def payload(typ, vals):
+        if typ == 2:
+            raw = vals.encode("latin-1") + b"\0"
+            return raw, len(raw)
+        return struct.pack("<" + f"{len(vals)}{code[typ]}", *vals), len(vals)
+
+    ifd_at = 8
+    ifd_len = 2 + 12 * len(entries) + 4
+    extra = bytearray()
+    blobs = []
+    for tag, typ, vals in entries:
+        raw, count = payload(typ, vals)
+        if len(raw) > 4:
+            blobs.append((tag, typ, count, ifd_at + ifd_len + len(extra), None))
+            extra += raw + (b"\0" if len(raw) % 2 else b"")
+        else:
+            blobs.append((tag, typ, count, None, raw.ljust(4, b"\0")))
+    data_at = ifd_at + ifd_len + len(extra)
+    out = bytearray(b"II" + struct.pack("<HI", 42, ifd_at))
+    out += struct.pack("<H", len(entries))
+    for tag, typ, count, at, inline in blobs:
+        if tag == 273:
+            inline = struct.pack("<I", data_at)
+        out += struct.pack("<HHI", tag, typ, count) + (inline if inline is not None else struct.pack("<I", at))
+    out += struct.pack("<I", 0) + extra + data
+    Path(path).write_bytes(bytes(out))
+
+
+def _write_grid(prefix: str, values: np.ndarray, meta: RasterMeta, tif: bool) -> None:
+    """One output map (circuitscape.py:1011-1029): ``prefix.asc`` with values written in full precision
+    (and ``prefix.prj`` when the habitat had a projection), or ``prefix.tif``; NaN written as -9999."""
+    out = np.where(np.isnan(values), NODATA, values)
+    if tif:
+        _write_tiff(prefix + ".tif", out, meta)
+        return
+    with open(prefix + ".asc", "w", encoding="utf-8") as f:
+        f.write(f"ncols         {meta.ncols}\nnrows         {meta.nrows}\n")
+        f.write(f"xllcorner     {meta.xllcorner!r}\nyllcorner     {meta.yllcorner!r}\n")
+        f.write(f"cellsize      {meta.cellsize!r}\nNODATA_value  -9999\n")
+        for row in out:
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+    if meta.wkt:
+        Path(prefix + ".prj").write_text(meta.wkt)
+
+
+def _write_matrix(path: str, a: np.ndarray) -> None:
+    with open(path, "w", encoding="utf-8") as f:
+        for row in np.atleast_2d(a):
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+
+
+def _prefix(cfg: dict):
+    out = cfg["output_file"]
+    if _unset(out):
+        raise ValueError("output_file is not set")
+    Path(out).parent.mkdir(parents=True, exist_ok=True)
+    return out.split(".out")[0], _flag(cfg, "write_as_tif")
+
+
+def _source_and_ground(source_path, ground_path, meta: RasterMeta, ground_is_resistance: bool):
+    """Advanced mode's sources (amps per cell) and ground conductances (circuitscape.py:1230-1258):
+    rasters on the habitat's grid or ``value x y`` lists; NoData grounds are none, a zero resistance
+    a direct ground."""
+    kind = file_type(ground_path)
+    if kind in ("aagrid", "geotiff"):
+        gnd = read_polymap(ground_path, meta, nodata_as=None)
+    elif kind == "list":
+        pts = _read_list(ground_path)
+        gnd = np.full((meta.nrows, meta.ncols), NODATA)
+        i, j = _list_cells(pts, meta)
+        gnd[i - 1, j - 1] = pts[:, 0]
+    else:
+        raise ValueError("Cannot recognise file type.")
+    kind = file_type(source_path)
+    if kind in ("aagrid", "geotiff"):
+        src = read_polymap(source_path, meta)
+        src[src == NODATA] = 0.0
+    elif kind == "list":
+        pts = _read_list(source_path)
+        src = np.zeros((meta.nrows, meta.ncols))
+        i, j = _list_cells(pts, meta)
+        src[i - 1, j - 1] = pts[:, 0]
+    else:
+        raise ValueError("Cannot recognize file type.")
+    missing = gnd == NODATA
+    if ground_is_resistance:
+        with np.errstate(divide="ignore"):
+            gnd = 1.0 / gnd
+    gnd[missing] = 0.0
+    return src, gnd
+
+
+def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0) -> CircuitscapeResult:
+    """Run a Circuitscape INI file, as ``Circuitscape.compute(path)`` does (circuitscape.py:1039-1213).
+
+    Raster scenarios only (``data_type = raster``): advanced, pairwise,
+    one-to-all and all-to-one, with polygons, masks, reclass tables, include /
+    exclude pairs, variable source strengths and focal regions. Paths are read
+    from the working directory, else next to the INI file. With
+    ``write_outputs`` the maps go next to ``output_file`` under Circuitscape's
+    names (``<base>_voltmap``, ``<base>_curmap_<i>_<j>``, ``<base>_cum_curmap``
+    ...; ASCII grids, or GeoTIFF with ``write_as_tif``) and ``output_file``
+    holds the configuration; otherwise the per-pair / per-point maps are
+    returned in ``result.maps`` under the same names. ``solver``,
+    ``precision`` and ``parallelize`` have no effect: every solve runs on the
+    GPU to the solver settings' residual.
+    """
+    path = Path(path)
+    base = path.resolve().parent
+    cfg = {**DEFAULTS, **parse_config(path)}
+    if cfg["data_type"] in ("network", "Network"):
+        raise ValueError("data_type = network: network (graph) inputs are not read by this package")
+    if cfg["data_type"] not in ("raster", "Raster"):
+        raise ValueError(f'data_type = "{cfg["data_type"]}": expected raster or network')
+    scenario = SCENARIOS.get(cfg["scenario"])
+    if scenario is None:
+        raise ValueError(f'scenario = "{cfg["scenario"]}" is not one of {sorted(set(SCENARIOS.values()))}')
+    for key in ("low_memory_mode", "preemptive_memory_release"):
+        if cfg.get(key, "False") in TRUELIST:
+            raise ValueError(f"{key} is not implemented in Circuitscape.jl; remove it or set it to False")
+
+    def need(key):
+        if _unset(cfg[key]):
+            raise ValueError(f"{key} is not set")
+        p = _resolve(cfg[key], base)
+        if not p.is_file():
+            raise FileNotFoundError(f'{key} = "{cfg[key]}" does not exist')
+        return p
+
+    habitat = need("habitat_file")
+    reclass = read_reclass_table(need("reclass_file")) if _flag(cfg, "use_reclass_table") else None
+    cells, meta = read_raster(habitat)
+    g = habitat_conductance(cells, is_resistance=_flag(cfg, "habitat_map_is_resistances"), reclass=reclass)
+    poly = None
+    if _flag(cfg, "use_polygons"):
+        poly = np.ascontiguousarray(read_polymap(need("polygon_file"), meta, integer=True).astype(np.int64))
+    if _flag(cfg, "use_mask"):
+        m = read_polymap(need("mask_file"), meta)
+        g = g * np.where(m > 0, 1.0, 0.0)
+        if g.sum() == 0:
+            raise ValueError("Mask file deleted everything!")
+    four = _flag(cfg, "connect_four_neighbors_only")
+    avg = _flag(cfg, "connect_using_avg_resistances")
+    policy = cfg["remove_src_or_gnd"]
+    if policy == "not entered":  # Circuitscape 4's default
+        policy = "keepall"
+    if policy not in POLICIES:
+        raise ValueError(f"remove_src_or_gnd must be one of {POLICIES}")
+    prefix, tif = _prefix(cfg) if write_outputs and scenario != "advanced" else (None, False)
+    maps: dict = {}
+
+    def put(name, grid):
+        if grid is None:
+            return
+        if write_outputs:
+            _write_grid(f"{prefix}_{name}", grid, meta, tif)
+        else:
+            maps[name] = grid
+
+    if scenario == "advanced":
+        src, gnd = _source_and_ground(need("source_file"), need("ground_file"), meta,
+                                      _flag(cfg, "ground_file_is_resistances"))
+        result = _advanced(g, poly, src, gnd, four, avg, policy, _flag(cfg, "use_unit_currents"),
+                           _flag(cfg, "use_direct_grounds"), n_threads)
+    elif scenario == "pairwise":
+        points = read_point_map(need("point_file"), meta)
+        pairs = read_included_pairs(need("included_pairs_file")) if _flag(cfg, "use_included_pairs") else None
+        points, excluded = _apply_included_pairs(points, pairs)
+        opts = _pair_options(_flag(cfg, "write_volt_maps"), _flag(cfg, "write_cur_maps"),
+                             _flag(cfg, "write_cum_cur_map_only"), _flag(cfg, "write_max_cur_maps"),
+                             _flag(cfg, "set_focal_node_currents_to_zero"), _flag(cfg, "set_null_currents_to_nodata"),
+                             _flag(cfg, "set_null_voltages_to_nodata"), _flag(cfg, "log_transform_maps"))
+
+        def emit(a, b, volt, cur):
+            put(f"voltmap_{a}_{b}", volt)
+            put(f"curmap_{a}_{b}", cur)
+
+        res, r = _pairwise(g, poly, points, excluded, four, avg, opts, emit, n_threads)
+        result = CircuitscapeResult(resistances=res, cum_current=r.get("cum"), max_current=r.get("max"))
+    else:
+        points = read_point_map(need("point_file"), meta)
+        pairs = read_included_pairs(need("included_pairs_file")) if _flag(cfg, "use_included_pairs") else None
+        strengths = (read_point_strengths(need("variable_source_file"))
+                     if _flag(cfg, "use_variable_source_strengths") else None)
+        cum_only = _flag(cfg, "write_cum_cur_map_only")
+        # Circuitscape writes every point's current map with write_cum_cur_map_only too
+        opts = {"volt_maps": _flag(cfg, "write_volt_maps"), "cur_maps": _flag(cfg, "write_cur_maps") or cum_only,
+                "cum": _flag(cfg, "write_cur_maps") or cum_only, "max": _flag(cfg, "write_max_cur_maps"),
+                "focal_zero": _flag(cfg, "set_focal_node_currents_to_zero")}
+
+        def emit_point(n, volt, cur):
+            put(f"voltmap_{n}", volt)
+            put(f"curmap_{n}", cur)
+
+        result = _one_to_all(g, poly, points, strengths, pairs, scenario == "one-to-all", four, avg, opts,
+                             emit_point, n_threads)
+    result.maps.update(maps)
+    if write_outputs:
+        _write_outputs(cfg, scenario, result, meta)
+    return result
+
+
+def _write_outputs(cfg: dict, scenario: str, result: CircuitscapeResult, meta: RasterMeta) -> None:
+    # circuitscape.py:1261-1286: the configuration, the maps, and pairwise's resistance tables
+    prefix, tif = _prefix(cfg)
+    with open(cfg["output_file"], "w", encoding="utf-8") as f:
+        for k, v in cfg.items():
+            f.write(f"{k} = {v}\n")
+    if scenario == "advanced":
+        if _flag(cfg, "write_volt_maps"):
+            _write_grid(prefix + "_voltmap", result.voltage, meta, tif)
+        if _flag(cfg, "write_cur_maps") or _flag(cfg, "write_cum_cur_map_only"):
+            _write_grid(prefix + "_curmap", result.current, meta, tif)
+        return
+    if result.cum_current is not None:
+        _write_grid(prefix + "_cum_curmap", result.cum_current, meta, tif)
+    if result.max_current is not None:
+        _write_grid(prefix + "_max_curmap", result.max_current, meta, tif)
+    if scenario == "pairwise":
+        r = result.resistances
+        _write_matrix(prefix + "_resistances.out", r)
+        ids = r[1:, 0]
+        three = [(ids[i], ids[j], r[j + 1, i + 1]) for i in range(len(ids)) for j in range(i + 1, len(ids))]
+        _write_matrix(prefix + "_resistances_3columns.out", np.array(three).reshape(-1, 3))
