@@ -148,6 +148,43 @@ int lsc_circuitscape_pairwise(const double* gmap, const int64_t* polymap, int64_
                               const lsc_solve_options* options, double* resistances, double* cum, double* max,
                               lsc_solve_stats* stats);
 
+/* Advanced mode on a network (Circuitscape's data_type = network): edges
+ * a[k] - b[k] (0-based nodes < n_nodes) of conductance w[k], duplicates
+ * summed, zero sums and self loops dropped; source and ground per node
+ * (ground: conductance, +inf = direct); conflict policy as above.
+ * Replaces network_advanced (src/cpp/circuit/python.cpp:310-346 ->
+ * network_graph + raster_advanced + network_currents, circuitscape.cpp:
+ * 265-362, 393-468). voltage and current: n_nodes; the branch currents
+ * (nodes r < c, current) go to arrays of capacity n_edges, n_branches
+ * receiving their count. */
+int lsc_network_advanced(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                         const double* source, const double* ground, int policy, const lsc_solve_options* options,
+                         double* voltage, double* current, int32_t* branch_r, int32_t* branch_c,
+                         double* branch_current, int64_t* n_branches, int32_t* solved, int32_t* n_components,
+                         lsc_solve_stats* stats);
+
+/* Receives a network pair's tables during a network pairwise run: the
+ * component's nodes, their voltages relative to point ci (NULL unless
+ * voltages were asked for) and node currents, and the branch currents. */
+typedef void (*lsc_network_emit_fn)(int ci, int cj, int64_t n_nodes, const int32_t* nodes, const double* voltage,
+                                    const double* current, int64_t n_branches, const int32_t* branch_r,
+                                    const int32_t* branch_c, const double* branch_current, void* ctx);
+
+/* Pairwise mode on a network (graph as in lsc_network_advanced): effective
+ * resistances between the focal nodes `points` (-1: not in the graph),
+ * excluded pairs (n_points x n_points uint8), every pair solved (no
+ * shortcut). flags: 1 voltages, 2 currents, 4 cumulative currents, 8 max;
+ * with any of them every pair's tables go to `emit`, and cum (n_nodes) and
+ * the cumulative branch currents (capacity n_edges, count in
+ * n_cum_branches) receive their sums over the pairs. Replaces
+ * network_pairwise (src/cpp/circuit/python.cpp:348-402 -> raster_pairwise
+ * with network set, circuitscape.cpp:474-743). resistances: n_points^2. */
+int lsc_network_pairwise(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                         const int32_t* points, int64_t n_points, const uint8_t* excluded, int flags,
+                         lsc_network_emit_fn emit, void* ctx, const lsc_solve_options* options, double* resistances,
+                         double* cum, int32_t* cum_branch_r, int32_t* cum_branch_c, double* cum_branch_current,
+                         int64_t* n_cum_branches, int32_t* n_components, lsc_solve_stats* stats);
+
 /* Several advanced problems on one raster graph (the one-to-all driver).
  * Problem p: source cells scells[sptr[p]..sptr[p+1]) with values svals,
  * ground cells likewise, check cell check[p] (-1: every component).

@@ -20,7 +20,7 @@ PRECISION_FP64 = 1
 EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_circuit_omniscape_device",
            "lsc_circuit_omniscape_correction", "lsc_probe_mfma", "lsc_window_plan_stats", "lsc_circuit_advanced",
            "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise", "lsc_circuitscape_multi_advanced",
-           "lsc_circuitscape_multi_advanced_sum",
+           "lsc_circuitscape_multi_advanced_sum", "lsc_network_advanced", "lsc_network_pairwise",
            "lsc_effective_resistance_grad", "lsc_grid_circuit_create", "lsc_grid_circuit_solve",
            "lsc_grid_circuit_adjoint", "lsc_grid_circuit_stats", "lsc_grid_circuit_destroy")
 

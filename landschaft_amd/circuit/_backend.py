@@ -410,3 +410,97 @@ def effective_resistance_grad(resistance, src_cells, gnd_cells, four_neighbors=F
                                                   ctypes.byref(J), N.ptr(R), N.ptr(grad), ctypes.byref(st)))
     _record(st)
     return J.value, R, grad
+
+
+# ---- networks
+
+_NET_EMIT = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+_NET_FLAGS = ("volt_maps", "cur_maps", "cum", "max")
+
+
+def _network_edges(a, b, w, n_nodes):
+    a = np.ascontiguousarray(a, dtype=np.int32).ravel()
+    b = np.ascontiguousarray(b, dtype=np.int32).ravel()
+    w = N.f64(w).ravel()
+    if not a.size == b.size == w.size:
+        raise ValueError("edge arrays of different lengths")
+    if int(n_nodes) <= 0:
+        raise ValueError("a network needs nodes")
+    return a, b, w, int(n_nodes)
+
+
+def _i32(n):
+    return np.empty(max(int(n), 1), dtype=np.int32)
+
+
+def network_advanced(a, b, w, n_nodes, source, ground, policy, n_threads=0):
+    """Advanced mode on a network (python.cpp:310-346): voltage and current by node, the branch
+    currents as ``(r, c, current)`` arrays (nodes r < c), solved, n_components."""
+    a, b, w, m = _network_edges(a, b, w, n_nodes)
+    s, g = N.f64(source).ravel(), N.f64(ground).ravel()
+    if s.size != m or g.size != m:
+        raise ValueError("sources and grounds must have one value per node")
+    volt, cur = np.empty(m), np.empty(m)
+    br, bc, bv = _i32(a.size), _i32(a.size), np.empty(max(a.size, 1))
+    nb, solved, ncomp = ctypes.c_int64(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    st = N.SolveStats()
+    o = _opts()
+    i32p = ctypes.c_int32
+    N.check(N.lib().lsc_network_advanced(N.ptr(a, i32p), N.ptr(b, i32p), N.ptr(w), ctypes.c_int64(a.size),
+                                         ctypes.c_int32(m), N.ptr(s), N.ptr(g), _policy(policy), ctypes.byref(o),
+                                         N.ptr(volt), N.ptr(cur), N.ptr(br, i32p), N.ptr(bc, i32p), N.ptr(bv),
+                                         ctypes.byref(nb), ctypes.byref(solved), ctypes.byref(ncomp),
+                                         ctypes.byref(st)))
+    _record(st)
+    k = nb.value
+    return {"voltage": volt, "current": cur, "branch": (br[:k].copy(), bc[:k].copy(), bv[:k].copy()),
+            "solved": bool(solved.value), "n_components": ncomp.value}
+
+
+def network_pairwise(a, b, w, n_nodes, points, excluded, options, emit=None, n_threads=0):
+    """Pairwise mode on a network (python.cpp:348-402): resistances, n_components; with any map option
+    ``emit(ci, cj, nodes, voltage or None, current, (r, c, branch current))`` per pair, and ``cum``
+    (by node) and ``cum_branch`` ((r, c, current) arrays) summed over the pairs."""
+    a, b, w, m = _network_edges(a, b, w, n_nodes)
+    pts = np.ascontiguousarray(points, dtype=np.int32).ravel()
+    n = pts.size
+    ex = np.ascontiguousarray(excluded, dtype=np.uint8).reshape(n, n)
+    flags = sum(1 << k for k, f in enumerate(_NET_FLAGS) if options.get(f))
+    R = np.empty((n, n))
+    cum = np.empty(m)
+    br, bc, bv = _i32(a.size), _i32(a.size), np.empty(max(a.size, 1))
+    nb, ncomp = ctypes.c_int64(0), ctypes.c_int32(0)
+
+    def _cb(ci, cj, count, nodes, v, c, nbr, rr, cc, vv, _ctx):
+        if emit is None:
+            return
+        as_arr = np.ctypeslib.as_array
+        nd = as_arr(nodes, shape=(count,)).copy() if count else np.zeros(0, np.int32)
+        volt = as_arr(v, shape=(count,)).copy() if v and count else (np.zeros(0) if v else None)
+        cur = as_arr(c, shape=(count,)).copy() if count else np.zeros(0)
+        if nbr:
+            branch = (as_arr(rr, shape=(nbr,)).copy(), as_arr(cc, shape=(nbr,)).copy(),
+                      as_arr(vv, shape=(nbr,)).copy())
+        else:
+            branch = (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        emit(ci, cj, nd, volt, cur, branch)
+
+    cb = _NET_EMIT(_cb)
+    st = N.SolveStats()
+    o = _opts()
+    i32p = ctypes.c_int32
+    N.check(N.lib().lsc_network_pairwise(N.ptr(a, i32p), N.ptr(b, i32p), N.ptr(w), ctypes.c_int64(a.size),
+                                         ctypes.c_int32(m), N.ptr(pts, i32p), ctypes.c_int64(n),
+                                         N.ptr(ex, ctypes.c_uint8), flags, cb, None, ctypes.byref(o), N.ptr(R),
+                                         N.ptr(cum), N.ptr(br, i32p), N.ptr(bc, i32p), N.ptr(bv),
+                                         ctypes.byref(nb), ctypes.byref(ncomp), ctypes.byref(st)))
+    _record(st)
+    out = {"resistances": R, "n_components": ncomp.value}
+    if flags:
+        k = nb.value
+        out["cum"] = cum
+        out["cum_branch"] = (br[:k].copy(), bc[:k].copy(), bv[:k].copy())
+    return out

@@ -991,12 +991,14 @@ def _source_and_ground(source_path, ground_path, meta: RasterMeta, ground_is_res
     return src, gnd
 
 
-def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0) -> CircuitscapeResult:
+def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0):
     """Run a Circuitscape INI file, as ``Circuitscape.compute(path)`` does (circuitscape.py:1039-1213).
 
-    Raster scenarios only (``data_type = raster``): advanced, pairwise,
-    one-to-all and all-to-one, with polygons, masks, reclass tables, include /
-    exclude pairs, variable source strengths and focal regions. Paths are read
+    Rasters (``data_type = raster``): advanced, pairwise, one-to-all and
+    all-to-one, with polygons, masks, reclass tables, include / exclude pairs,
+    variable source strengths and focal regions, giving a
+    :class:`CircuitscapeResult`. Networks (``data_type = network``): pairwise
+    or advanced on an edge list, giving a ``NetworkResult``. Paths are read
     from the working directory, else next to the INI file. With
     ``write_outputs`` the maps go next to ``output_file`` under Circuitscape's
     names (``<base>_voltmap``, ``<base>_curmap_<i>_<j>``, ``<base>_cum_curmap``
@@ -1010,7 +1012,9 @@ def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0) ->
     base = path.resolve().parent
     cfg = {**DEFAULTS, **parse_config(path)}
     if cfg["data_type"] in ("network", "Network"):
-        raise ValueError("data_type = network: network (graph) inputs are not read by this package")
+        from .network import network_ini
+
+        return network_ini(cfg, base, write_outputs=write_outputs, n_threads=n_threads)
     if cfg["data_type"] not in ("raster", "Raster"):
         raise ValueError(f'data_type = "{cfg["data_type"]}": expected raster or network')
     scenario = SCENARIOS.get(cfg["scenario"])

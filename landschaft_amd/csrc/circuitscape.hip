@@ -230,6 +230,70 @@ RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny
     return g;
 }
 
+// A network's graph (reference network_graph, circuitscape.cpp:265-338):
+// sparse(a, b, w) with duplicate entries summed in input order, then A + A'
+// with the zero sums and the self loops left out; node c's neighbours in
+// ascending order. No grid: nx = ny = 0 and no node has a cell.
+RGraph network_graph(const i32* a, const i32* b, const double* w, i64 ne, i32 m) {
+    struct Entry {
+        i32 r, c;
+        double w;
+    };
+    // stable order by (column, row), so entries of one position keep the input order
+    auto by_column = [m](std::vector<Entry>& v) {
+        std::stable_sort(v.begin(), v.end(), [](const Entry& x, const Entry& y) {
+            return x.c != y.c ? x.c < y.c : x.r < y.r;
+        });
+        (void)m;
+    };
+    std::vector<Entry> in(ne);
+    for (i64 k = 0; k < ne; ++k) {
+        if (a[k] < 0 || b[k] < 0 || a[k] >= m || b[k] >= m) throw std::invalid_argument("node index out of range");
+        in[k] = {a[k], b[k], w[k]};
+    }
+    by_column(in);
+    std::vector<Entry> A;
+    for (std::size_t k = 0; k < in.size();) {
+        Entry e = in[k];
+        std::size_t t = k + 1;
+        for (; t < in.size() && in[t].r == e.r && in[t].c == e.c; ++t) e.w += in[t].w;
+        A.push_back(e);
+        k = t;
+    }
+    std::vector<Entry> both;
+    both.reserve(2 * A.size());
+    for (const Entry& e : A) {
+        if (e.r == e.c) continue;
+        both.push_back({e.r, e.c, e.w});
+        both.push_back({e.c, e.r, e.w});
+    }
+    by_column(both);
+    RGraph g;
+    g.n = m;
+    g.node_cell.assign(m, -1);
+    g.ptr.assign(static_cast<std::size_t>(m) + 1, 0);
+    for (std::size_t k = 0; k < both.size();) {
+        double v = both[k].w;
+        std::size_t t = k + 1;
+        if (t < both.size() && both[t].r == both[k].r && both[t].c == both[k].c) v += both[t++].w;
+        if (v != 0) {
+            g.adj.push_back(both[k].r);
+            g.w.push_back(v);
+            ++g.ptr[static_cast<std::size_t>(both[k].c) + 1];
+        }
+        k = t;
+    }
+    for (i32 v = 0; v < m; ++v) g.ptr[v + 1] += g.ptr[v];
+    g.degree.assign(m, 0.0);
+    for (i32 v = 0; v < m; ++v) {
+        double d = 0.0;
+        for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) d += g.w[k];
+        g.degree[v] = d;
+    }
+    label_components(g);
+    return g;
+}
+
 struct Members {
     std::vector<i64> start;
     std::vector<i32> mem;
@@ -954,25 +1018,70 @@ void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
 
 struct PairOpts {
     bool volt_maps, cur_maps, cum, max, focal_zero, null_currents, null_voltages, log_transform;
+    bool network = false;  // a network (no grid): no shortcut; per-pair node and branch currents
     bool maps() const { return volt_maps || cur_maps || cum || max; }
+};
+
+// Branch currents of one component (reference network_currents,
+// circuitscape.cpp:340-362): every edge once, from its larger node c to its
+// smaller neighbour r, |w| (v_r - v_c); values below 1e-8 of the largest
+// set to 0, then all made positive. Per branch: its adjacency entry, its
+// nodes r < c and its current.
+struct Branches {
+    std::vector<i64> e;
+    std::vector<i32> r, c;
+    std::vector<double> cur;
+};
+void branch_currents(const RGraph& g, const i32* mem, i64 count, const double* v, Branches& b) {
+    b.e.clear(), b.r.clear(), b.c.clear(), b.cur.clear();
+    double top = -kInf;
+    for (i64 p = 0; p < count; ++p) {
+        const i32 c = mem[p];
+        for (i64 e = g.ptr[c]; e < g.ptr[c + 1]; ++e) {
+            const i32 r = g.adj[e];
+            if (r >= c) continue;
+            const double x = std::fabs(g.w[e]) * (v[r] - v[c]);
+            top = std::max(top, x);
+            b.e.push_back(e), b.r.push_back(r), b.c.push_back(c), b.cur.push_back(x);
+        }
+    }
+    for (double& x : b.cur) {
+        if (std::fabs(x / top) < 1e-8) x = 0.0;
+        x = std::fabs(x);
+    }
+}
+
+// What a network pairwise run returns besides the resistances: per-pair
+// tables through `emit`, and with any map option the node currents summed
+// by node and the branch currents summed by adjacency entry.
+struct NetPairs {
+    lsc_network_emit_fn emit = nullptr;
+    void* ctx = nullptr;
+    std::vector<double> cum, cum_branch;
 };
 
 void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std::vector<i32>& points,
               const std::uint8_t* excluded, const PairOpts& o, lsc_emit_fn emit, void* ctx, double* R_out,
-              double* cum_out, double* max_out, Ctx& cx) {
+              double* cum_out, double* max_out, Ctx& cx, NetPairs* net = nullptr) {
     const i32 np = static_cast<i32>(points.size());
     const i64 ncell = g.nx * g.ny;
     std::vector<double> Rm(static_cast<std::size_t>(np) * np, -1.0);
     auto R = [&](i32 a, i32 b) -> double& { return Rm[static_cast<std::size_t>(a) * np + b]; };
     bool any_ex = false;
     for (i64 k = 0; k < static_cast<i64>(np) * np; ++k) any_ex = any_ex || excluded[k];
-    const bool shortcut = !o.maps() && !any_ex;
+    const bool shortcut = !o.network && !o.maps() && !any_ex;
     std::vector<double> voltmatrix, sres;
     if (shortcut) voltmatrix.assign(static_cast<std::size_t>(np) * np, 0.0), sres.assign(static_cast<std::size_t>(np) * np, -1.0);
     std::vector<double> cum, mx;
-    if (o.cum) cum.assign(ncell, 0.0);
-    if (o.max) mx.assign(ncell, -9999.0);
+    if (o.network) {
+        if (o.maps()) net->cum.assign(g.n, 0.0), net->cum_branch.assign(g.adj.size(), 0.0);
+    } else {
+        if (o.cum) cum.assign(ncell, 0.0);
+        if (o.max) mx.assign(ncell, -9999.0);
+    }
     const Members M = members_of(g);
+    Branches branch;
+    std::vector<double> cur_mem;
     std::unordered_map<i32, std::vector<i32>> positions;
     for (i32 i = 0; i < np; ++i)
         if (points[i] >= 0) positions[points[i]].push_back(i);
@@ -1081,6 +1190,10 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
                 graph_currents(G, dmem.p + p0, static_cast<int>(count), dv.p, nullptr, dcur.p, scratch, cx.s);
                 LSC_HIP(hipMemcpyAsync(cnode.data(), dcur.p, g.n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
                 LSC_HIP(hipStreamSynchronize(cx.s));
+                if (o.network) {
+                    branch_currents(g, mb, count, vnode.data(), branch);
+                    cur_mem.resize(count);
+                }
             }
             for (i32 ci : positions[job.first])
                 for (i32 cj : positions[job.second]) {
@@ -1095,6 +1208,19 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
                         continue;
                     }
                     if (!o.maps()) continue;
+                    if (o.network) {
+                        // postprocess: the component's node voltages when asked, node and branch currents
+                        for (i64 q = 0; q < count; ++q) {
+                            cur_mem[q] = cnode[mb[q]];
+                            net->cum[mb[q]] += cnode[mb[q]];
+                        }
+                        for (std::size_t b = 0; b < branch.e.size(); ++b) net->cum_branch[branch.e[b]] += branch.cur[b];
+                        if (net->emit)
+                            net->emit(ci, cj, count, mb, o.volt_maps ? vcol.data() : nullptr, cur_mem.data(),
+                                      static_cast<i64>(branch.e.size()), branch.r.data(), branch.c.data(),
+                                      branch.cur.data(), net->ctx);
+                        continue;
+                    }
                     std::vector<double> vmap, cmap(ncell, 0.0);
                     if (o.volt_maps) {
                         vmap.assign(ncell, 0.0);
@@ -1152,10 +1278,40 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
     if (shortcut) Rm = std::move(sres);
     for (i32 i = 0; i < np; ++i) Rm[static_cast<std::size_t>(i) * np + i] = 0.0;
     std::copy(Rm.begin(), Rm.end(), R_out);
+    if (o.network) return;
     if (o.cum && cum_out)
         for (i64 c = 0; c < ncell; ++c) cum_out[c] = cum[c] < -9999 ? -9999 : cum[c];
     if (o.max && max_out)
         for (i64 c = 0; c < ncell; ++c) max_out[c] = mx[c] < -9999 ? -9999 : mx[c];
+}
+
+// ---- networks (the reference's network_advanced / network_pairwise,
+// src/cpp/circuit/python.cpp:310-402)
+
+// Advanced mode on a network: voltages by node (0 where nothing was solved),
+// then the node currents of the whole graph with the finite grounds (taken
+// before the policy) and its branch currents.
+void network_advanced(const RGraph& g, const DevGraph& G, const double* source, const double* ground, int policy,
+                      Ctx& cx, double* voltage, double* current, Branches& br, bool& solved) {
+    const i32 m = g.n;
+    std::vector<double> src(source, source + m), gnd(ground, ground + m), finite(m, 0.0);
+    bool any_finite = false;
+    for (i32 v = 0; v < m; ++v) {
+        finite[v] = gnd[v] < kInf ? gnd[v] : 0.0;
+        any_finite = any_finite || finite[v] != 0;
+    }
+    auto sol = advanced_nodes(g, G, {problem_from_dense(src, gnd)}, {-1}, policy, false, cx);
+    std::vector<i32> all(m);
+    for (i32 v = 0; v < m; ++v) all[v] = v;
+    DevBuf<int> dall = to_device(all, cx.s);
+    DevBuf<double> dv = to_device(sol[0].voltage, cx.s), dfin = to_device(finite, cx.s), dcur, scratch;
+    dcur.alloc(m);
+    graph_currents(G, dall.p, m, dv.p, any_finite ? dfin.p : nullptr, dcur.p, scratch, cx.s);
+    LSC_HIP(hipMemcpyAsync(current, dcur.p, m * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+    LSC_HIP(hipStreamSynchronize(cx.s));
+    std::copy(sol[0].voltage.begin(), sol[0].voltage.end(), voltage);
+    branch_currents(g, all.data(), m, sol[0].voltage.data(), br);
+    solved = sol[0].solved;
 }
 
 // ---- the adjoint of a grid circuit
@@ -1320,6 +1476,68 @@ int lsc_circuitscape_pairwise(const double* gmap, const int64_t* polymap, int64_
         lsc::PairOpts o{(flags & 1) != 0,  (flags & 2) != 0,  (flags & 4) != 0,  (flags & 8) != 0,
                         (flags & 16) != 0, (flags & 32) != 0, (flags & 64) != 0, (flags & 128) != 0};
         lsc::pairwise(G, DG, gmap, nodes, excluded, o, emit, ctx, resistances, cum, max, cx);
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_network_advanced(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                         const double* source, const double* ground, int policy, const lsc_solve_options* options,
+                         double* voltage, double* current, int32_t* branch_r, int32_t* branch_c,
+                         double* branch_current, int64_t* n_branches, int32_t* solved, int32_t* n_components,
+                         lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (policy < 0 || policy > 3) throw std::invalid_argument("remove_src_or_gnd must be keepall, rmvsrc, rmvgnd or rmvall");
+        if (n_nodes <= 0 || n_edges < 0) throw std::invalid_argument("a network needs nodes");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::RGraph G = lsc::network_graph(a, b, w, n_edges, n_nodes);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        lsc::Branches br;
+        bool ok = false;
+        lsc::network_advanced(G, DG, source, ground, policy, cx, voltage, current, br, ok);
+        if (static_cast<int64_t>(br.e.size()) > n_edges) throw std::logic_error("more branches than edges");
+        std::copy(br.r.begin(), br.r.end(), branch_r);
+        std::copy(br.c.begin(), br.c.end(), branch_c);
+        std::copy(br.cur.begin(), br.cur.end(), branch_current);
+        *n_branches = static_cast<int64_t>(br.e.size());
+        *solved = ok;
+        *n_components = G.ncomp;
+        lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_network_pairwise(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                         const int32_t* points, int64_t n_points, const uint8_t* excluded, int flags,
+                         lsc_network_emit_fn emit, void* ctx, const lsc_solve_options* options, double* resistances,
+                         double* cum, int32_t* cum_branch_r, int32_t* cum_branch_c, double* cum_branch_current,
+                         int64_t* n_cum_branches, int32_t* n_components, lsc_solve_stats* stats) {
+    return lsc::guarded([&] {
+        if (n_nodes <= 0 || n_edges < 0) throw std::invalid_argument("a network needs nodes");
+        lsc::Ctx cx = lsc::make_ctx(options);
+        lsc::RGraph G = lsc::network_graph(a, b, w, n_edges, n_nodes);
+        lsc::DevGraph DG;
+        DG.upload(G, cx.s);
+        std::vector<lsc::i32> pts(n_points);
+        for (int64_t i = 0; i < n_points; ++i) pts[i] = points[i] >= 0 && points[i] < n_nodes ? points[i] : -1;
+        lsc::PairOpts o{(flags & 1) != 0, (flags & 2) != 0, (flags & 4) != 0, (flags & 8) != 0,
+                        false, false, false, false};
+        o.network = true;
+        lsc::NetPairs net;
+        net.emit = emit, net.ctx = ctx;
+        lsc::pairwise(G, DG, nullptr, pts, excluded, o, nullptr, nullptr, resistances, nullptr, nullptr, cx, &net);
+        int64_t k = 0;
+        if (o.maps()) {
+            std::copy(net.cum.begin(), net.cum.end(), cum);
+            for (lsc::i32 c = 0; c < G.n; ++c)
+                for (lsc::i64 e = G.ptr[c]; e < G.ptr[c + 1]; ++e)
+                    if (G.adj[e] < c) {
+                        if (k >= n_edges) throw std::logic_error("more branches than edges");
+                        cum_branch_r[k] = G.adj[e], cum_branch_c[k] = c, cum_branch_current[k] = net.cum_branch[e];
+                        ++k;
+                    }
+        }
+        *n_cum_branches = k;
+        *n_components = G.ncomp;
         lsc::fill_stats(cx, stats);
     });
 }

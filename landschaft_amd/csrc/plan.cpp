@@ -65,6 +65,139 @@ struct Dissection {
     }
 };
 
+// Nested dissection of a graph without geometry (George's level-structure
+// scheme; the reference orders networks by the same kind of method,
+// src/cpp/circuit/system.cpp:96-270). A connected set is cut at one level of
+// a breadth-first search from a pseudo-peripheral node, where edges only
+// join consecutive levels: the smallest level that leaves at least a quarter
+// of the nodes on each side (else the median level), keeping in the
+// separator only the nodes that touch the far side. A disconnected set gives
+// one subtree per component. Fronts are laid out in post-order.
+struct GraphDissection {
+    const std::vector<i64>& ptr;
+    const std::vector<i32>& adj;
+    std::size_t leaf;
+    std::vector<i32> stamp, level;
+    i32 stamps = 0;
+    std::vector<i32> order, first, npiv;
+    std::vector<std::vector<i32>> kids;
+
+    GraphDissection(i32 n, const std::vector<i64>& p, const std::vector<i32>& a, int lf)
+        : ptr(p), adj(a), leaf(static_cast<std::size_t>(lf)), stamp(n, -1), level(n, -1) {}
+
+    i32 add(std::vector<i32>& piv, std::vector<i32> ch) {
+        std::sort(piv.begin(), piv.end());
+        first.push_back(static_cast<i32>(order.size()));
+        npiv.push_back(static_cast<i32>(piv.size()));
+        order.insert(order.end(), piv.begin(), piv.end());
+        kids.push_back(std::move(ch));
+        return static_cast<i32>(first.size()) - 1;
+    }
+
+    // Breadth-first search over the nodes stamped s: levels, nodes in visit order.
+    void bfs(i32 root, i32 s, std::vector<i32>& seen) {
+        seen.clear();
+        seen.push_back(root);
+        level[root] = 0;
+        for (std::size_t h = 0; h < seen.size(); ++h) {
+            const i32 v = seen[h];
+            for (i64 e = ptr[v]; e < ptr[v + 1]; ++e) {
+                const i32 u = adj[e];
+                if (u < 0 || stamp[u] != s || level[u] >= 0) continue;
+                level[u] = level[v] + 1;
+                seen.push_back(u);
+            }
+        }
+    }
+    void unlevel(const std::vector<i32>& vs) {
+        for (i32 v : vs) level[v] = -1;
+    }
+
+    // The subtree roots of `set` appended to `roots`.
+    void dissect(std::vector<i32> set, std::vector<i32>& roots) {
+        if (set.empty()) return;
+        if (set.size() <= leaf) {
+            roots.push_back(add(set, {}));
+            return;
+        }
+        const i32 s = stamps++;
+        for (i32 v : set) stamp[v] = s;
+        std::vector<i32> seen;
+        bfs(*std::min_element(set.begin(), set.end()), s, seen);
+        if (seen.size() < set.size()) {
+            std::vector<std::vector<i32>> comps{seen};
+            for (i32 v : set)
+                if (level[v] < 0) {
+                    std::vector<i32> c;
+                    bfs(v, s, c);
+                    comps.push_back(std::move(c));
+                }
+            for (const auto& c : comps) unlevel(c);
+            for (auto& c : comps) dissect(std::move(c), roots);
+            return;
+        }
+        // pseudo-peripheral root: restart from the smallest node of the last
+        // level while the depth grows (at most three searches)
+        i32 depth = level[seen.back()];
+        for (int it = 0; it < 2; ++it) {
+            i32 far = seen.back();
+            for (i32 v : seen)
+                if (level[v] == depth && v < far) far = v;
+            std::vector<i32> again;
+            unlevel(seen);
+            bfs(far, s, again);
+            const i32 d = level[again.back()];
+            seen.swap(again);
+            if (d <= depth) break;
+            depth = d;
+        }
+        depth = level[seen.back()];
+        if (depth < 2) {  // no level has nodes on both sides
+            unlevel(seen);
+            roots.push_back(add(set, {}));
+            return;
+        }
+        std::vector<i64> count(depth + 1, 0);
+        for (i32 v : seen) ++count[level[v]];
+        const i64 n = static_cast<i64>(seen.size());
+        i32 cut = -1, median = -1;
+        i64 before = count[0];
+        for (i32 l = 1; l < depth; ++l) {
+            const i64 after = n - before - count[l];
+            if (median < 0 && before + count[l] >= n / 2) median = l;
+            if (4 * before >= n && 4 * after >= n && (cut < 0 || count[l] < count[cut])) cut = l;
+            before += count[l];
+        }
+        if (cut < 0) cut = median >= 0 ? median : depth / 2;
+        std::vector<i32> a, b, sep;
+        for (i32 v : seen) {
+            const i32 l = level[v];
+            if (l < cut) {
+                a.push_back(v);
+            } else if (l > cut) {
+                b.push_back(v);
+            } else {
+                bool far = false;
+                for (i64 e = ptr[v]; e < ptr[v + 1] && !far; ++e) {
+                    const i32 u = adj[e];
+                    far = u >= 0 && stamp[u] == s && level[u] == cut + 1;
+                }
+                (far ? sep : a).push_back(v);
+            }
+        }
+        unlevel(seen);
+        std::vector<i32>().swap(set);
+        std::vector<i32> ch;
+        dissect(std::move(a), ch);
+        dissect(std::move(b), ch);
+        if (sep.empty() && ch.size() == 1) {
+            roots.push_back(ch[0]);
+            return;
+        }
+        roots.push_back(add(sep, std::move(ch)));
+    }
+};
+
 }  // namespace
 
 namespace {
@@ -199,12 +332,58 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
     return p;
 }
 
+namespace {
+
+// The rest of a graph plan once the elimination order (perm) and the fronts are set.
+void finish_graph(Plan& p, i32 n, i64 bx, i64 by, const std::vector<std::vector<i32>>& kids,
+                  const std::vector<i64>& ptr, const std::vector<i32>& adj) {
+    p.n = n;
+    p.nfront = static_cast<i32>(p.first.size());
+    p.iperm.assign(n, -1);
+    for (i32 k = 0; k < n; ++k) p.iperm[p.perm[k]] = k;
+    p.unk_of.assign(bx * by, -1);
+    for (i32 k = 0; k < n; ++k)
+        if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = k;
+    std::vector<i64> nptr(n + 1, 0), nsrc;
+    std::vector<i32> nidx;
+    for (i32 k = 0; k < n; ++k) {
+        const i32 c = p.perm[k];
+        for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
+            if (adj[e] < 0) continue;
+            nidx.push_back(p.iperm[adj[e]]);
+            nsrc.push_back(e);
+        }
+        nptr[k + 1] = static_cast<i64>(nidx.size());
+    }
+    finish_plan(p, kids, nptr, nidx, nsrc);
+}
+
+}  // namespace
+
 Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
                 const std::vector<i32>& adj, int leaf) {
     if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
     const i32 n = static_cast<i32>(cell.size());
     Plan p;
     p.bx = bx, p.by = by, p.nd = 0;
+    if (std::none_of(cell.begin(), cell.end(), [](i64 c) { return c >= 0; })) {
+        GraphDissection d(n, ptr, adj, leaf);
+        std::vector<i32> all(n), roots;
+        for (i32 c = 0; c < n; ++c) all[c] = c;
+        d.dissect(std::move(all), roots);
+        p.perm = std::move(d.order);
+        p.first = std::move(d.first);
+        p.npiv = std::move(d.npiv);
+        p.cell.assign(n, -1);
+        std::vector<std::vector<i32>> kids = std::move(d.kids);
+        // several components: the last root (the last front) adopts the
+        // others, which share no entry with it, so the tree has one root
+        if (roots.empty() || roots.back() != static_cast<i32>(p.first.size()) - 1)
+            throw std::logic_error("graph dissection: the last root is not the last front");
+        for (std::size_t r = 0; r + 1 < roots.size(); ++r) kids[roots.back()].push_back(roots[r]);
+        finish_graph(p, n, bx, by, kids, ptr, adj);
+        return p;
+    }
     std::vector<std::uint8_t> mark(bx * by, 0);
     std::vector<i32> node_of(bx * by, -1);
     for (i32 c = 0; c < n; ++c)
@@ -231,27 +410,10 @@ Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<
         p.npiv.push_back(static_cast<i32>(p.perm.size()) - first_merged);
         kids.push_back(root >= 0 ? std::vector<i32>{root} : std::vector<i32>{});
     }
-    p.n = n;
-    p.nfront = static_cast<i32>(p.first.size());
-    p.iperm.assign(n, -1);
-    for (i32 k = 0; k < n; ++k) p.iperm[p.perm[k]] = k;
-    p.unk_of.assign(bx * by, -1);
-    for (i32 k = 0; k < n; ++k)
-        if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = k;
-    std::vector<i64> nptr(n + 1, 0), nsrc;
-    std::vector<i32> nidx;
-    for (i32 k = 0; k < n; ++k) {
-        const i32 c = p.perm[k];
-        for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
-            if (adj[e] < 0) continue;
-            nidx.push_back(p.iperm[adj[e]]);
-            nsrc.push_back(e);
-        }
-        nptr[k + 1] = static_cast<i64>(nidx.size());
-    }
-    finish_plan(p, kids, nptr, nidx, nsrc);
+    finish_graph(p, n, bx, by, kids, ptr, adj);
     return p;
 }
+
 
 Plan window_plan(int D, bool disk, bool four, int leaf) {
     const i64 W = 2 * i64(D) + 1;

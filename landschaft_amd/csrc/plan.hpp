@@ -79,7 +79,9 @@ Plan window_plan(int D, bool disk, bool four, int leaf = 16);
 // (a short-circuit region) eliminated last in one front at the root.
 // Adjacency in CSR: node c's neighbours adj[ptr[c] .. ptr[c+1]) (symmetric,
 // -1 ignored). Cell nodes that are grid neighbours may be the only edges
-// between cell nodes.
+// between cell nodes. A graph with no cell node at all (a network, or a
+// component made only of short-circuit regions) is ordered by a nested
+// dissection of the graph itself instead (breadth-first level structures).
 Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
                 const std::vector<i32>& adj, int leaf = 16);
 

@@ -69,3 +69,13 @@ class OracleCore:
                             "active": np.unique(nm[r["active"][p] & on]).astype(np.int32),
                             "solved": bool(r["solved"][p])})
         return {"nodemap": nm, "results": results}
+
+    def network_advanced(self, a, b, w, m, src, gnd, policy, n_threads=0):
+        from oracle import network as ON
+
+        return ON.network_advanced(a, b, w, m, src, gnd, policy)
+
+    def network_pairwise(self, a, b, w, m, points, ex, opts, emit=None, n_threads=0):
+        from oracle import network as ON
+
+        return ON.network_pairwise(a, b, w, m, points, ex, opts, emit)

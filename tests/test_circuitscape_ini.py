@@ -161,8 +161,11 @@ def test_ini_errors(tmp_path):
     f.write_text("scenario = pairwis\nhabitat_file = x\n")
     with pytest.raises(ValueError, match="scenario"):
         cs.circuitscape_ini(f, write_outputs=False)
-    f.write_text("scenario = pairwise\ndata_type = network\n")
-    with pytest.raises(ValueError, match="network"):
+    f.write_text("scenario = pairwise\ndata_type = network\n")  # a network INI goes to the network driver
+    with pytest.raises(ValueError, match="habitat_file is not set"):
+        cs.circuitscape_ini(f, write_outputs=False)
+    f.write_text("scenario = pairwise\ndata_type = graph\n")
+    with pytest.raises(ValueError, match="data_type"):
         cs.circuitscape_ini(f, write_outputs=False)
     f.write_text("[Options]\nscenario = advanced\nuse_polygons = maybe\nhabitat_file = "
                  + (CS / "input/raster/advanced/1/cellmap5x5.asc").as_posix() + "\n")
