@@ -158,6 +158,42 @@ def pairwise_leg(B, n, four, npts, seed, amp, maps_too, cpu=False):
     return out
 
 
+def network_leg(B, n, npts, seed, cpu=False):
+    """Circuitscape's network pairwise on the edge list of an n x n 4-neighbour grid (conductance
+    (g_a + g_b) / 2 per edge: the Laplacian of the raster with four_neighbors), every pair solved; the
+    same pairs on the raster path (geometric ordering, shortcut) as a cross-check, and optionally the
+    CPU oracle's raster pairwise."""
+    r = np.exp(2.0 * fractal_field(n, seed)).astype(np.float32).astype(np.float64)
+    g = 1.0 / r
+    idx = np.arange(n * n, dtype=np.int64).reshape(n, n)
+    a = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    b = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    w = (g.ravel()[a] + g.ravel()[b]) / 2
+    cells = np.random.default_rng(seed + 1000).choice(n * n, npts, replace=False).astype(np.int64)
+    ex = np.zeros((npts, npts), np.uint8)
+    out = {"workload": f"network pairwise: {n}x{n} 4-neighbour grid as an edge list ({n * n} nodes, {a.size} "
+                       f"edges), {npts} focal nodes ({npts * (npts - 1) // 2} pairs, all solved), seed {seed}"}
+    B.settings.precision = "mixed"
+    t0 = time.perf_counter()
+    Rn = B.network_pairwise(a.astype(np.int32), b.astype(np.int32), w, n * n, cells.astype(np.int32), ex, {})
+    out["mixed"] = {"s_wall": round(time.perf_counter() - t0, 3), **_phase(B.last_stats)}
+    t0 = time.perf_counter()
+    Rr = B.circuitscape_pairwise(g, None, cells, ex, False, True, {})["resistances"]
+    out["raster_path_mixed"] = {"s_wall": round(time.perf_counter() - t0, 3), **_phase(B.last_stats)}
+    off = ~np.eye(npts, dtype=bool)
+    out["max_rel_diff_network_vs_raster"] = float(np.max(np.abs(Rn["resistances"] - Rr)[off] / Rr[off]))
+    if cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        Rc = O.cs_pairwise(g, None, cells, ex, False, True, {})["resistances"]
+        out["cpu_oracle_fp64_raster"] = {"s_wall": round(time.perf_counter() - t0, 3),
+                                         "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                         "max_rel_diff_gpu_network": float(
+                                             np.max(np.abs(Rn["resistances"] - Rc)[off] / Rc[off]))}
+    return out
+
+
 def adjoint_leg(B, n, npairs, seed):
     """J = sum of effective resistances and dJ/dr on an n x n 8-neighbour lognormal surface."""
     rng = np.random.default_rng(seed)
@@ -353,7 +389,8 @@ def main() -> int:
         B.settings.tol = a.tol
         graph = {"pairwise_256": pairwise_leg(B, 256, False, 4, 101, None, True, cpu=not a.no_cpu),
                  "pairwise_2048": pairwise_leg(B, 2048, True, 16, 102, 2.0, True),
-                 "adjoint_1024": adjoint_leg(B, 1024, 32, 104)}
+                 "adjoint_1024": adjoint_leg(B, 1024, 32, 104),
+                 "network_1024": network_leg(B, 1024, 8, 105, cpu=not a.no_cpu)}
     line = {
         "metric": "circuit solves/sec at 4096^2 @ rel resid <=1e-10 (Omniscape windows, mixed-precision GPU)",
         "value": round(value, 2), "unit": "circuits/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -234,3 +234,24 @@ def test_top_level_names():
     assert circuit.NetworkResult is not None
     assert callable(circuit.circuitscape_network_advanced)
     assert callable(circuit.read_graph)
+
+
+@pytest.mark.gpu
+def test_grid_network_matches_the_raster_path():
+    # a 4-neighbour grid given as an edge list (conductance (g_a + g_b) / 2) has the raster's
+    # Laplacian: the general-graph ordering must give the raster path's resistances
+    from landschaft_amd.circuit import _backend as B
+
+    n, npts = 384, 6
+    rng = np.random.default_rng(11)
+    g = np.exp(rng.standard_normal((n, n)))
+    idx = np.arange(n * n).reshape(n, n)
+    a = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    b = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    w = (g.ravel()[a] + g.ravel()[b]) / 2
+    cells = rng.choice(n * n, npts, replace=False)
+    ex = np.zeros((npts, npts), np.uint8)
+    Rn = B.network_pairwise(a, b, w, n * n, cells, ex, {})["resistances"]
+    Rr = B.circuitscape_pairwise(g, None, cells.astype(np.int64), ex, False, True, {})["resistances"]
+    np.testing.assert_allclose(Rn, Rr, rtol=1e-9, atol=0)
+    assert np.all(Rn[~np.eye(npts, dtype=bool)] > 0)
