@@ -36,11 +36,14 @@ sys.path.insert(0, str(ROOT))
 MFMA_F32_PEAK = 157.3e12   # MI355X dense FP32 matrix, FLOP/s
 MFMA_F64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12          # bytes/s
-# HBM bytes of one substitution sweep of one window (forward and backward
-# sweeps averaged), from rocprofv3 --pmc FETCH_SIZE (x2, the reading of a
-# 256 MiB copy: python bench.py --copy-probe) and WRITE_SIZE (x1) over the
-# level kernels of this workload at 1024 x 1024; profiles/ holds the runs.
-SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 1027090
+# HBM traffic per window, from rocprofv3 --pmc FETCH_SIZE (x2, the reading
+# of a 256 MiB copy: python bench.py --copy-probe) and WRITE_SIZE (x1) over
+# one single-stream run of this workload at 1024 x 1024 (29,147 windows
+# factored, 5.06 substitution sweeps each); profiles/ holds the runs.
+# One substitution sweep (forward and backward sweeps averaged):
+SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 1927039
+# The factorisation (assembly and every front level, forward fused):
+FACTOR_TRAFFIC_PER_WINDOW = 5289109
 
 
 def fractal_field(n: int, seed: int, exponent: float = -3.0) -> np.ndarray:
@@ -327,13 +330,24 @@ def main() -> int:
     # and L as m x np column blocks in FP32 read once per substitution sweep.
     plan = N.window_plan_stats(a.radius, 0, False, True)
 
+    # The factor's algorithmic bytes: L written once (FP32, the reference's
+    # ordering) and the window's conductances read once (W^2 FP64). Its
+    # intensity, factor_flops / those bytes, is about 11 FLOP/B, below the
+    # FP32 matrix ridge of 157.3 / 8 = 19.7 FLOP/B: the HBM roof bounds it.
+    w_side = 2 * a.radius + 1
+    factor_bytes = plan["l_unpacked"] * 4 + w_side * w_side * 8
+
     def roofline_of(st):
         kernels = {}
         f_flops = plan["factor_flops"] * st["n_factored"]
         f_s = st["ms_factor"] / 1e3
-        kernels["k_factor"] = {"bound": "mfma", "achieved": f_flops / f_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
-                               "unit": "TFLOP/s", "launches": st["n_batches"], "ms_total": st["ms_factor"],
-                               "flops_per_circuit": plan["factor_flops"]}
+        kernels["k_factor"] = {"bound": "hbm", "achieved": st["n_factored"] * factor_bytes / f_s / 1e9,
+                               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "launches": st["n_batches"],
+                               "ms_total": st["ms_factor"], "bytes_per_circuit": factor_bytes,
+                               "flops_per_circuit": plan["factor_flops"],
+                               "flop_per_byte": round(plan["factor_flops"] / factor_bytes, 2),
+                               "mfma_tflops": round(f_flops / f_s / 1e12, 3),
+                               "mfma_frac": round(f_flops / f_s / MFMA_F32_PEAK, 4)}
         # sweeps: the back substitution after the fused factor-forward, then a
         # forward + back sweep per refinement step
         sweeps = st["n_factored"] * (1 + 2 * st["refine_mean"])
@@ -346,12 +360,16 @@ def main() -> int:
         kd = kernels[dom]
         # one launch of k_solve = one substitution sweep over a batch of windows
         per_batch = st["n_factored"] / max(st["n_batches"], 1)
-        traffic = round(SOLVE_TRAFFIC_PER_WINDOW_SWEEP * per_batch) if dom == "k_solve" else None
+        per_launch = {"k_factor": FACTOR_TRAFFIC_PER_WINDOW, "k_solve": SOLVE_TRAFFIC_PER_WINDOW_SWEEP}
+        traffic = round(per_launch[dom] * per_batch)
         roof = {"kernel": dom, "bound": kd["bound"], "achieved": round(kd["achieved"], 3), "peak": kd["peak"],
                 "unit": kd["unit"], "frac": round(kd["achieved"] / kd["peak"], 4), "traffic": traffic,
-                "concurrent_batches": st["n_streams"]}
-        if traffic is not None:
-            roof["traffic_note"] = "HBM bytes per sweep of a batch of windows (rocprofv3 PMC, copy-calibrated)"
+                "concurrent_batches": st["n_streams"],
+                "traffic_note": ("HBM bytes per launch: " + ("the factorisation of a batch of windows"
+                                 if dom == "k_factor" else "one substitution sweep over a batch of windows")
+                                 + " (rocprofv3 PMC, copy-calibrated)")}
+        if dom == "k_factor":
+            roof["mfma_tflops"], roof["mfma_frac"] = kd["mfma_tflops"], kd["mfma_frac"]
         return roof, kernels
 
     # the timed region: launch durations are timed on each batch's stream
@@ -367,7 +385,7 @@ def main() -> int:
         finally:
             del os.environ["LSC_BATCH_STREAMS"]
         r1, k1 = roofline_of(st1)
-        roofline["single_stream"] = {k: r1[k] for k in ("kernel", "achieved", "frac", "traffic")}
+        roofline["single_stream"] = {k: r1[k] for k in ("kernel", "bound", "achieved", "frac", "traffic")}
         roofline["single_stream"]["phase_ms"] = {k[3:]: round(v, 2) for k, v in st1.items() if k.startswith("ms_")}
 
     fp64 = None
