@@ -514,6 +514,235 @@ __global__ __launch_bounds__(256) void k_h_backward(DPlan P, const int* __restri
     }
 }
 
+// ---- very large fronts, row-parallel: the vectors of every right-hand side
+// live in HBM (W, front f at woff[f] * k, right-hand side r at r * m), and
+// each panel of 32 pivots is two launches over every front of the level: one
+// workgroup per front solves the diagonal block for all right-hand sides,
+// then workgroups over row chunks of 256 update the rows below (forward) or
+// take partial dot products (backward, reduced in chunk order by the next
+// diagonal launch). Every L block is read once per sweep for all right-hand
+// sides, and the results do not depend on the launch's scheduling.
+
+constexpr int kWidePanel = 32;
+
+__global__ __launch_bounds__(256) void k_w_fgather(DPlan P, const int* __restrict__ ht,
+                                                   const long long* __restrict__ woff, int n, int k,
+                                                   const double* __restrict__ rs, double* __restrict__ W) {
+    const int f = blockIdx.y, t = ht[f], r = blockIdx.z;
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    W[woff[f] * k + static_cast<long long>(r) * m + i] = i < np ? rs[static_cast<long long>(r) * n + P.first[t] + i] : 0.0;
+}
+
+// One child of each listed front (a child rank): its update vector added into the parent's rows.
+__global__ __launch_bounds__(256) void k_w_extend(DPlan P, const int* __restrict__ chf, const int* __restrict__ chc,
+                                                  const int* __restrict__ ht, const long long* __restrict__ woff,
+                                                  int k, long long vstride, const double* __restrict__ V,
+                                                  double* __restrict__ W) {
+    const int f = chf[blockIdx.y], c = chc[blockIdx.y], t = ht[f], r = blockIdx.z;
+    const int u = P.nupd[c], m = P.npiv[t] + P.nupd[t];
+    const int ii = blockIdx.x * 256 + threadIdx.x;
+    if (ii >= u) return;
+    W[woff[f] * k + static_cast<long long>(r) * m + P.relmap[P.upd[c] + ii]] +=
+        V[static_cast<long long>(r) * vstride + P.vpos[c] + ii];
+}
+
+// tri[c * 32 + r] = L(j0 + r, j0 + c) of the jb x jb diagonal block (0 elsewhere).
+template <typename T>
+__device__ __forceinline__ void stage_tri32(const T* __restrict__ Lt, int m, int j0, int jb, double* __restrict__ tri) {
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+        const int c = e >> 5, r = e & 31;
+        tri[e] = (r < jb && c < jb && r >= c) ? static_cast<double>(Lt[hcol(j0 + c, m) + (r - c)]) : 0.0;
+    }
+}
+
+// Forward, panel s: the diagonal block solved for right-hand sides 16 y .. 16 y + 15.
+template <typename T>
+__global__ __launch_bounds__(256) void k_w_fdiag(DPlan P, const int* __restrict__ ht,
+                                                 const long long* __restrict__ woff, int k, int s,
+                                                 const T* __restrict__ L, double* __restrict__ W) {
+    __shared__ double tri[1024];
+    const int f = blockIdx.x, t = ht[f];
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int j0 = kWidePanel * s;
+    if (j0 >= np) return;
+    const int jb = min(kWidePanel, np - j0);
+    stage_tri32<T>(L + P.lpos[t], m, j0, jb, tri);
+    __syncthreads();
+    double* Wf = W + woff[f] * k;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int rend = min(k, static_cast<int>(blockIdx.y) * 16 + 16);
+    for (int r = blockIdx.y * 16 + wave; r < rend; r += 4) {
+        double* w = Wf + static_cast<long long>(r) * m + j0;
+        double v = lane < jb ? w[lane] : 0.0;
+        for (int q = 0; q < jb; ++q) {
+            const double yq = bcast(v, q) / tri[q * 33];
+            if (lane > q && lane < jb) v -= tri[q * 32 + lane] * yq;
+            if (lane == q) v = yq;
+        }
+        if (lane < jb) w[lane] = v;
+    }
+}
+
+// Forward, panel s: rows below the panel, 256 per workgroup, for right-hand sides 64 z .. 64 z + 63.
+template <typename T>
+__global__ __launch_bounds__(256) void k_w_fupdate(DPlan P, const int* __restrict__ ht,
+                                                   const long long* __restrict__ woff, int k, int s,
+                                                   const T* __restrict__ L, double* __restrict__ W) {
+    __shared__ double ys[64 * kWidePanel];
+    const int f = blockIdx.y, t = ht[f];
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int j0 = kWidePanel * s;
+    if (j0 >= np) return;
+    const int jb = min(kWidePanel, np - j0);
+    const int r0 = 64 * blockIdx.z, rn = min(64, k - r0);
+    double* Wf = W + woff[f] * k;
+    for (int e = threadIdx.x; e < rn * kWidePanel; e += 256) {
+        const int r = e / kWidePanel, q = e - r * kWidePanel;
+        ys[e] = q < jb ? Wf[static_cast<long long>(r0 + r) * m + j0 + q] : 0.0;
+    }
+    __syncthreads();
+    const int i = j0 + jb + blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const T* Lt = L + P.lpos[t];
+    double Lq[kWidePanel];
+#pragma unroll
+    for (int q = 0; q < kWidePanel; ++q) Lq[q] = q < jb ? static_cast<double>(Lt[hcol(j0 + q, m) + i - j0 - q]) : 0.0;
+    for (int r = 0; r < rn; ++r) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWidePanel; ++q) a += Lq[q] * ys[r * kWidePanel + q];
+        Wf[static_cast<long long>(r0 + r) * m + i] -= a;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_w_fscatter(DPlan P, const int* __restrict__ ht,
+                                                    const long long* __restrict__ woff, int n, int k,
+                                                    long long vstride, const double* __restrict__ W,
+                                                    double* __restrict__ y, double* __restrict__ V,
+                                                    const int* __restrict__ done) {
+    const int f = blockIdx.y, t = ht[f], r = blockIdx.z;
+    if (done[r]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const double v = W[woff[f] * k + static_cast<long long>(r) * m + i];
+    if (i < np) y[static_cast<long long>(r) * n + P.first[t] + i] = v;
+    else V[static_cast<long long>(r) * vstride + P.vpos[t] + i - np] = v;
+}
+
+__global__ __launch_bounds__(256) void k_w_bgather(DPlan P, const int* __restrict__ ht,
+                                                   const long long* __restrict__ woff, int n, int k,
+                                                   const double* __restrict__ y, const double* __restrict__ z,
+                                                   double* __restrict__ W) {
+    const int f = blockIdx.y, t = ht[f], r = blockIdx.z;
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const long long off = static_cast<long long>(r) * n;
+    W[woff[f] * k + static_cast<long long>(r) * m + i] =
+        i < np ? y[off + P.first[t] + i] : z[off + P.rows[P.upd[t] + i - np]];
+}
+
+// Backward, step s (the block of pivots s-th from the last): partial dot
+// products of the block's columns with the solved rows below, chunk c of 256
+// rows, right-hand sides 64 z .. 64 z + 63; thread (q, g) takes column q and
+// right-hand sides 8 g .. 8 g + 7.
+template <typename T>
+__global__ __launch_bounds__(256) void k_w_bdots(DPlan P, const int* __restrict__ ht,
+                                                 const long long* __restrict__ woff,
+                                                 const long long* __restrict__ cpre, int k, int s,
+                                                 const T* __restrict__ L, const double* __restrict__ W,
+                                                 double* __restrict__ part) {
+    __shared__ double xs[64 * 64];
+    const int f = blockIdx.y, t = ht[f];
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int b = (np + kWidePanel - 1) / kWidePanel - 1 - s;
+    if (b < 0) return;
+    const int jlo = kWidePanel * b, jb = min(kWidePanel, np - jlo);
+    const int i0 = jlo + jb + 256 * static_cast<int>(blockIdx.x);
+    if (i0 >= m) return;
+    const int i1 = min(m, i0 + 256);
+    const int r0 = 64 * blockIdx.z, rn = min(64, k - r0);
+    const int q = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const T* Lt = L + P.lpos[t];
+    const long long colbase = hcol(jlo + q, m) - (jlo + q);  // L(i, jlo + q) = Lt[colbase + i], i > jlo + q
+    const double* Wf = W + woff[f] * k;
+    double acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+    for (int s0 = i0; s0 < i1; s0 += 64) {
+        const int cnt = min(64, i1 - s0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < rn * 64; e += 256) {
+            const int rr = e >> 6, ii = e & 63;
+            xs[e] = ii < cnt ? Wf[static_cast<long long>(r0 + rr) * m + s0 + ii] : 0.0;
+        }
+        __syncthreads();
+        if (q < jb)
+            for (int ii = 0; ii < cnt; ++ii) {
+                const double l = static_cast<double>(Lt[colbase + s0 + ii]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[j] += l * xs[(g * 8 + j < rn ? g * 8 + j : 0) * 64 + ii];
+            }
+    }
+    double* pf = part + (cpre[f] + blockIdx.x) * static_cast<long long>(k) * kWidePanel;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int rr = g * 8 + j;
+        if (rr < rn) pf[static_cast<long long>(r0 + rr) * kWidePanel + q] = acc[j];
+    }
+}
+
+// Backward, step s: the partial dots summed in chunk order, then the block's
+// transposed triangular solve, right-hand sides 16 y .. 16 y + 15.
+template <typename T>
+__global__ __launch_bounds__(256) void k_w_bdiag(DPlan P, const int* __restrict__ ht,
+                                                 const long long* __restrict__ woff,
+                                                 const long long* __restrict__ cpre, int k, int s,
+                                                 const T* __restrict__ L, const double* __restrict__ part,
+                                                 double* __restrict__ W) {
+    __shared__ double tri[1024];
+    const int f = blockIdx.x, t = ht[f];
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int b = (np + kWidePanel - 1) / kWidePanel - 1 - s;
+    if (b < 0) return;
+    const int jlo = kWidePanel * b, jb = min(kWidePanel, np - jlo);
+    const int nch = (m - jlo - jb + 255) / 256;
+    stage_tri32<T>(L + P.lpos[t], m, jlo, jb, tri);
+    __syncthreads();
+    double* Wf = W + woff[f] * k;
+    const double* pf = part + cpre[f] * static_cast<long long>(k) * kWidePanel;
+    const int lane = threadIdx.x & 63, wave = wave_id();
+    const int rend = min(k, static_cast<int>(blockIdx.y) * 16 + 16);
+    for (int r = blockIdx.y * 16 + wave; r < rend; r += 4) {
+        double d = 0.0;
+        if (lane < jb)
+            for (int c = 0; c < nch; ++c) d += pf[(static_cast<long long>(c) * k + r) * kWidePanel + lane];
+        double* w = Wf + static_cast<long long>(r) * m + jlo;
+        double v = lane < jb ? w[lane] - d : 0.0;
+        for (int qq = jb - 1; qq >= 0; --qq) {
+            const double xq = bcast(v, qq) / tri[qq * 33];
+            if (lane < qq) v -= tri[lane * 32 + qq] * xq;
+            if (lane == qq) v = xq;
+        }
+        if (lane < jb) w[lane] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_w_bscatter(DPlan P, const int* __restrict__ ht,
+                                                    const long long* __restrict__ woff, int n, int k,
+                                                    const double* __restrict__ W, double* __restrict__ z,
+                                                    const int* __restrict__ done) {
+    const int f = blockIdx.y, t = ht[f], r = blockIdx.z;
+    if (done[r]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= np) return;
+    z[static_cast<long long>(r) * n + P.first[t] + j] = W[woff[f] * k + static_cast<long long>(r) * m + j];
+}
+
 // The large fronts of one level, their tile arena and child lists.
 struct HugeLevel {
     std::vector<int> t;                 // fronts
@@ -524,7 +753,13 @@ struct HugeLevel {
     std::vector<int> rb, tb;            // per panel step: largest row-block and tile-block grids
     std::vector<std::vector<int>> chf, chc;  // per child rank: (front index, child)
     std::vector<std::vector<long long>> cbk; // per child rank: workgroup prefix
+    std::vector<int> chmax;                  // per child rank: largest child update size
     long long store_blocks = 1;
+    // row-parallel substitution: row prefix (woff) and 256-row chunk prefix (cpre) per front
+    std::vector<long long> woff, cpre;
+    long long msum = 0, csum = 0;
+    int wkt = 0;                             // 32-pivot panels of the level's largest front
+    DevBuf<long long> dwoff, dcpre;
     DevBuf<int> dt;
     DevBuf<long long> doff;
     std::vector<DevBuf<int>> dchf, dchc;
@@ -562,6 +797,11 @@ struct SystemSolver::Impl {
     void huge_forward(const HugeLevel& H, int k);
     template <typename T>
     void huge_backward(const HugeLevel& H, int k);
+    DevBuf<double> wbuf, pbuf;  // row-parallel substitution: vectors and partial dots
+    template <typename T>
+    void wide_forward(const HugeLevel& H, int k);
+    template <typename T>
+    void wide_backward(const HugeLevel& H, int k);
     template <typename T>
     void solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);
 };
@@ -616,13 +856,22 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
                 const long long u = h.nupd[c];
                 if (u == 0) continue;
                 if (static_cast<int>(hl.chf.size()) <= r) hl.chf.emplace_back(), hl.chc.emplace_back(),
-                                                          hl.cbk.emplace_back(1, 0);
+                                                          hl.cbk.emplace_back(1, 0), hl.chmax.push_back(0);
                 hl.chf[r].push_back(static_cast<int>(f));
                 hl.chc[r].push_back(c);
                 hl.cbk[r].push_back(hl.cbk[r].back() + ((u + 1) * u + 255) / 256);
+                hl.chmax[r] = std::max(hl.chmax[r], static_cast<int>(u));
                 ++r;
             }
+            const long long mf = h.npiv[t] + h.nupd[t];
+            hl.woff.push_back(hl.msum);
+            hl.cpre.push_back(hl.csum);
+            hl.msum += mf;
+            hl.csum += (mf + 255) / 256;
+            hl.wkt = std::max(hl.wkt, (h.npiv[t] + kWidePanel - 1) / kWidePanel);
         }
+        hl.dwoff = to_device(hl.woff, s);
+        hl.dcpre = to_device(hl.cpre, s);
         hl.store_blocks = std::min<long long>(1024, (work + 255) / 256);
         hl.dt = to_device(hl.t, s);
         hl.doff = to_device(hl.off, s);
@@ -751,9 +1000,66 @@ void huge_backward_rb(const DPlan& P, const HugeLevel& H, int k, const T* L, con
                        lds, s, P, H.dt.p, P.n, k, L, y, z, done);
 }
 
+// Rows from which a level's large fronts take the row-parallel substitution
+// (LSC_WIDE_ROWS, default 1024): a workgroup per front then no longer keeps
+// every right-hand side's vector in LDS, nor reads L once per group of them.
+inline int wide_rows() {
+    const char* e = std::getenv("LSC_WIDE_ROWS");
+    return e ? std::atoi(e) : 1024;
+}
+
+inline unsigned cdiv(long long a, long long b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+template <typename T>
+void SystemSolver::Impl::wide_forward(const HugeLevel& H, int k) {
+    const DPlan& P = plan.d;
+    const T* Lp = reinterpret_cast<const T*>(L.p);
+    const unsigned F = static_cast<unsigned>(H.t.size());
+    const std::size_t need = static_cast<std::size_t>(k) * H.msum;
+    if (wbuf.n < need) wbuf.alloc(need);
+    hipLaunchKernelGGL(k_w_fgather, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
+                       rs.p, wbuf.p);
+    for (std::size_t r = 0; r < H.chf.size(); ++r)
+        hipLaunchKernelGGL(k_w_extend, dim3(cdiv(H.chmax[r], 256), static_cast<unsigned>(H.chf[r].size()), k),
+                           dim3(256), 0, s, P, H.dchf[r].p, H.dchc[r].p, H.dt.p, H.dwoff.p, k,
+                           static_cast<long long>(P.vsize), V.p, wbuf.p);
+    for (int st = 0; st < H.wkt; ++st) {
+        hipLaunchKernelGGL(k_w_fdiag<T>, dim3(F, cdiv(k, 16)), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, k, st, Lp,
+                           wbuf.p);
+        const long long below = H.mmax - static_cast<long long>(kWidePanel) * st;
+        if (below > 0)
+            hipLaunchKernelGGL(k_w_fupdate<T>, dim3(cdiv(below, 256), F, cdiv(k, 64)), dim3(256), 0, s, P, H.dt.p,
+                               H.dwoff.p, k, st, Lp, wbuf.p);
+    }
+    hipLaunchKernelGGL(k_w_fscatter, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
+                       static_cast<long long>(P.vsize), wbuf.p, y.p, V.p, done.p);
+}
+
+template <typename T>
+void SystemSolver::Impl::wide_backward(const HugeLevel& H, int k) {
+    const DPlan& P = plan.d;
+    const T* Lp = reinterpret_cast<const T*>(L.p);
+    const unsigned F = static_cast<unsigned>(H.t.size());
+    const std::size_t need = static_cast<std::size_t>(k) * H.msum;
+    if (wbuf.n < need) wbuf.alloc(need);
+    const std::size_t pneed = static_cast<std::size_t>(k) * kWidePanel * H.csum;
+    if (pbuf.n < pneed) pbuf.alloc(pneed);
+    hipLaunchKernelGGL(k_w_bgather, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
+                       y.p, z.p, wbuf.p);
+    for (int st = 0; st < H.wkt; ++st) {
+        hipLaunchKernelGGL(k_w_bdots<T>, dim3(cdiv(H.mmax, 256), F, cdiv(k, 64)), dim3(256), 0, s, P, H.dt.p,
+                           H.dwoff.p, H.dcpre.p, k, st, Lp, wbuf.p, pbuf.p);
+        hipLaunchKernelGGL(k_w_bdiag<T>, dim3(F, cdiv(k, 16)), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, H.dcpre.p, k, st,
+                           Lp, pbuf.p, wbuf.p);
+    }
+    hipLaunchKernelGGL(k_w_bscatter, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
+                       wbuf.p, z.p, done.p);
+}
+
 template <typename T>
 void SystemSolver::Impl::huge_forward(const HugeLevel& H, int k) {
     if (H.t.empty()) return;
+    if (H.mmax >= wide_rows()) return wide_forward<T>(H, k);
     const T* Lp = reinterpret_cast<const T*>(L.p);
     switch (huge_rb(H.mmax, k)) {
         case 8: huge_forward_rb<T, 8>(plan.d, H, k, Lp, rs.p, y.p, V.p, done.p, s); break;
@@ -766,6 +1072,7 @@ void SystemSolver::Impl::huge_forward(const HugeLevel& H, int k) {
 template <typename T>
 void SystemSolver::Impl::huge_backward(const HugeLevel& H, int k) {
     if (H.t.empty()) return;
+    if (H.mmax >= wide_rows()) return wide_backward<T>(H, k);
     const T* Lp = reinterpret_cast<const T*>(L.p);
     switch (huge_rb(H.mmax, k)) {
         case 8: huge_backward_rb<T, 8>(plan.d, H, k, Lp, y.p, z.p, done.p, s); break;

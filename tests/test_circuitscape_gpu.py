@@ -209,3 +209,39 @@ def test_multi_advanced_sum_matches_per_problem_maps(backend, focal_zero):
     np.testing.assert_array_equal(mx, mx0)
     for p, (c, r) in enumerate(zip(pts, full["results"])):
         assert out["check_voltage"][p] == r["voltage"][nm.flat[c]]
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_row_parallel_large_front_substitution(backend, oracle, monkeypatch, precision):
+    # the large fronts' substitutions by rows (every level with a front of >= 64 rows) against the
+    # one-workgroup-per-front kernels (no level) and the oracle: 20 pairs with maps on a 300 x 300
+    # 8-neighbour raster (largest front about 600 rows), so 19 right-hand sides per solve
+    rng = np.random.default_rng(21)
+    g = np.exp(rng.standard_normal((300, 300)))
+    g[rng.random((300, 300)) < 0.05] = 0.0
+    on = np.flatnonzero(g.ravel() > 0)
+    cells = rng.choice(on, 7, replace=False).astype(np.int64)
+    ex = np.zeros((7, 7), np.uint8)
+    opts = {"cum": True, "cur_maps": True}
+    backend.settings.precision = precision
+    out = {}
+    for label, rows in (("per-front", "100000000"), ("by rows", "64")):
+        monkeypatch.setenv("LSC_WIDE_ROWS", rows)
+        print(f"large-front substitution {label} {precision}", flush=True)
+        maps = {}
+        r = backend.circuitscape_pairwise(g, None, cells, ex, False, False, opts,
+                                          lambda ci, cj, v, c: maps.__setitem__((ci, cj), c))
+        out[label] = (r["resistances"], r["cum"], maps)
+    ref = oracle.cs_pairwise(g, None, cells, ex, False, False, opts)
+    for label, (R, cum, maps) in out.items():
+        np.testing.assert_allclose(R, ref["resistances"], rtol=1e-10, atol=0, err_msg=label)
+        np.testing.assert_allclose(cum, ref["cum"], rtol=1e-9, atol=1e-12 * np.abs(ref["cum"]).max(), err_msg=label)
+    a, b = out["per-front"], out["by rows"]
+    np.testing.assert_allclose(b[0], a[0], rtol=1e-12)
+    for key in a[2]:
+        np.testing.assert_allclose(b[2][key], a[2][key], rtol=1e-9, atol=1e-12 * np.abs(a[2][key]).max())
+    # the row-parallel path is deterministic
+    monkeypatch.setenv("LSC_WIDE_ROWS", "64")
+    again = backend.circuitscape_pairwise(g, None, cells, ex, False, False, opts)
+    np.testing.assert_array_equal(again["resistances"], b[0])
+    np.testing.assert_array_equal(again["cum"], b[1])
