@@ -95,56 +95,113 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* __restric
     }
 }
 
-// (A v)_j for the unknown j in elimination order, from the FP64 CSR matrix.
-__device__ __forceinline__ double spmv_row(int j, const int* __restrict__ perm, const int* __restrict__ iperm,
-                                           const long long* __restrict__ ptr, const int* __restrict__ adj,
-                                           const double* __restrict__ diag, const double* __restrict__ off,
-                                           const double* __restrict__ v) {
+// The products run on a copy of the FP64 matrix in elimination order (row j
+// of the copy is the caller's row perm[j], its entries in the same order,
+// their columns in elimination order), kSpmvRhs right-hand sides per thread:
+// a row's entries are read once for all of them.
+constexpr int kSpmvRhs = 8;
+
+// The elimination-order copy: row j takes row perm[j] of the caller's CSR.
+__global__ __launch_bounds__(256) void k_g_permute(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
+                                                   const long long* __restrict__ ptr, const int* __restrict__ adj,
+                                                   const double* __restrict__ diag, const double* __restrict__ off,
+                                                   const long long* __restrict__ eptr, int* __restrict__ eadj,
+                                                   double* __restrict__ ediag, double* __restrict__ eoff) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
     const int c = perm[j];
-    double a = diag[c] * v[j];
-    for (long long e = ptr[c]; e < ptr[c + 1]; ++e) a += off[e] * v[iperm[adj[e]]];
-    return a;
+    ediag[j] = diag[c];
+    long long w = eptr[j];
+    for (long long e = ptr[c]; e < ptr[c + 1]; ++e, ++w) {
+        eadj[w] = iperm[adj[e]];
+        eoff[w] = off[e];
+    }
+}
+
+// Per right-hand side q < kSpmvRhs of the block (inst = r0 + q): the block's
+// sums of a[q] and b[q] into part[(inst * gridDim.x + blockIdx.x) * 2 + {0, 1}],
+// in the order of block_sum2.
+__device__ __forceinline__ void block_sum2_rhs(const double (&a)[kSpmvRhs], const double (&b)[kSpmvRhs], int r0,
+                                               int k, const int* __restrict__ done, double* __restrict__ part) {
+    __shared__ double red[kSpmvRhs][2][4];
+    const int lane = threadIdx.x & 63, wave = wave_id();
+#pragma unroll
+    for (int q = 0; q < kSpmvRhs; ++q) {
+        double x = a[q], y = b[q];
+        for (int o = 32; o > 0; o >>= 1) {
+            x += __shfl_xor(x, o);
+            y += __shfl_xor(y, o);
+        }
+        if (lane == 0) red[q][0][wave] = x, red[q][1][wave] = y;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < kSpmvRhs && r0 + q < k && !done[r0 + q]) {
+        double* out = part + (static_cast<long long>(r0 + q) * gridDim.x + blockIdx.x) * 2;
+        out[0] = (red[q][0][0] + red[q][0][1]) + (red[q][0][2] + red[q][0][3]);
+        out[1] = (red[q][1][0] + red[q][1][1]) + (red[q][1][2] + red[q][1][3]);
+    }
 }
 
 // r = b - A x; rs = s r (the preconditioner's input); partial r.r and b.b.
-__global__ __launch_bounds__(256) void k_g_residual(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
-                                                    const long long* __restrict__ ptr, const int* __restrict__ adj,
-                                                    const double* __restrict__ diag, const double* __restrict__ off,
-                                                    const double* __restrict__ bp, const double* __restrict__ x,
-                                                    const double* __restrict__ sc, double* __restrict__ rs,
-                                                    const int* __restrict__ done, double* __restrict__ part) {
-    const int inst = blockIdx.y;
-    if (done[inst]) return;
+// Right-hand sides kSpmvRhs y .. kSpmvRhs y + kSpmvRhs - 1.
+__global__ __launch_bounds__(256) void k_g_residual(int n, int k, const long long* __restrict__ eptr,
+                                                    const int* __restrict__ eadj, const double* __restrict__ ediag,
+                                                    const double* __restrict__ eoff, const double* __restrict__ bp,
+                                                    const double* __restrict__ x, const double* __restrict__ sc,
+                                                    double* __restrict__ rs, const int* __restrict__ done,
+                                                    double* __restrict__ part) {
+    const int r0 = blockIdx.y * kSpmvRhs;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const long long o = static_cast<long long>(inst) * n;
-    double rr = 0.0, bb = 0.0;
+    double rr[kSpmvRhs], bb[kSpmvRhs];
+#pragma unroll
+    for (int q = 0; q < kSpmvRhs; ++q) rr[q] = 0.0, bb[q] = 0.0;
     if (j < n) {
-        const double b = bp[o + j];
-        const double r = b - spmv_row(j, perm, iperm, ptr, adj, diag, off, x + o);
-        rs[o + j] = sc[j] * r;
-        rr = r * r;
-        bb = b * b;
+        const long long e0 = eptr[j], e1 = eptr[j + 1];
+        const double d = ediag[j], sj = sc[j];
+#pragma unroll
+        for (int q = 0; q < kSpmvRhs; ++q) {
+            const int inst = r0 + q;
+            if (inst >= k || done[inst]) continue;
+            const long long o = static_cast<long long>(inst) * n;
+            double a = d * x[o + j];
+            for (long long e = e0; e < e1; ++e) a += eoff[e] * x[o + eadj[e]];
+            const double b = bp[o + j];
+            const double r = b - a;
+            rs[o + j] = sj * r;
+            rr[q] = r * r;
+            bb[q] = b * b;
+        }
     }
-    block_sum2(rr, bb, part + (static_cast<long long>(inst) * gridDim.x + blockIdx.x) * 2);
+    block_sum2_rhs(rr, bb, r0, k, done, part);
 }
 
-// q = A p; partial p.q.
-__global__ __launch_bounds__(256) void k_g_spmv(int n, const int* __restrict__ perm, const int* __restrict__ iperm,
-                                                const long long* __restrict__ ptr, const int* __restrict__ adj,
-                                                const double* __restrict__ diag, const double* __restrict__ off,
-                                                const double* __restrict__ p, double* __restrict__ q,
-                                                const int* __restrict__ done, double* __restrict__ part) {
-    const int inst = blockIdx.y;
-    if (done[inst]) return;
+// q = A p; partial p.q. Right-hand sides kSpmvRhs y .. kSpmvRhs y + kSpmvRhs - 1.
+__global__ __launch_bounds__(256) void k_g_spmv(int n, int k, const long long* __restrict__ eptr,
+                                                const int* __restrict__ eadj, const double* __restrict__ ediag,
+                                                const double* __restrict__ eoff, const double* __restrict__ p,
+                                                double* __restrict__ q, const int* __restrict__ done,
+                                                double* __restrict__ part) {
+    const int r0 = blockIdx.y * kSpmvRhs;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const long long o = static_cast<long long>(inst) * n;
-    double pq = 0.0;
+    double pq[kSpmvRhs], zero[kSpmvRhs];
+#pragma unroll
+    for (int t = 0; t < kSpmvRhs; ++t) pq[t] = 0.0, zero[t] = 0.0;
     if (j < n) {
-        const double v = spmv_row(j, perm, iperm, ptr, adj, diag, off, p + o);
-        q[o + j] = v;
-        pq = p[o + j] * v;
+        const long long e0 = eptr[j], e1 = eptr[j + 1];
+        const double d = ediag[j];
+#pragma unroll
+        for (int t = 0; t < kSpmvRhs; ++t) {
+            const int inst = r0 + t;
+            if (inst >= k || done[inst]) continue;
+            const long long o = static_cast<long long>(inst) * n;
+            double a = d * p[o + j];
+            for (long long e = e0; e < e1; ++e) a += eoff[e] * p[o + eadj[e]];
+            q[o + j] = a;
+            pq[t] = p[o + j] * a;
+        }
     }
-    block_sum2(pq, 0.0, part + (static_cast<long long>(inst) * gridDim.x + blockIdx.x) * 2);
+    block_sum2_rhs(pq, zero, r0, k, done, part);
 }
 
 // partial r.z = (s r).(L L^T)^-1 (s r), from rs and the sweep's output zz.
@@ -779,6 +836,9 @@ struct SystemSolver::Impl {
     DevBuf<long long> ptr;
     DevBuf<int> adj, perm, iperm;
     DevBuf<double> diag, off, sc;
+    DevBuf<long long> eptr;    // the matrix in elimination order (products)
+    DevBuf<int> eadj;
+    DevBuf<double> ediag, eoff;
     // factor
     DevBuf<char> L, U, A, F, D;
     DevBuf<double> rs0, y0;
@@ -888,6 +948,17 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     off = to_device(m.off.empty() ? std::vector<double>{0.0} : m.off, s);
     perm = to_device(h.perm, s);
     iperm = to_device(h.iperm, s);
+    {
+        std::vector<long long> ep(static_cast<std::size_t>(n) + 1, 0);
+        for (int j = 0; j < n; ++j) ep[j + 1] = ep[j] + (m.ptr[h.perm[j] + 1] - m.ptr[h.perm[j]]);
+        eptr = to_device(ep, s);
+        eadj.alloc(ep[n] ? ep[n] : 1);
+        eoff.alloc(ep[n] ? ep[n] : 1);
+        ediag.alloc(n ? n : 1);
+        if (n)
+            hipLaunchKernelGGL(k_g_permute, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p,
+                               diag.p, off.p, eptr.p, eadj.p, ediag.p, eoff.p);
+    }
     sc.alloc(n ? n : 1);
     if (n) hipLaunchKernelGGL(k_g_scale, dim3(nblocks(n, 256)), dim3(256), 0, s, n, perm.p, diag.p, sc.p);
     L.alloc(static_cast<std::size_t>(P.lsize + 4) * sizeof(T));  // + 4: whole 16-byte loads at the end
@@ -1139,14 +1210,15 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 3>), lds);
     const long long kn = static_cast<long long>(k) * n;
     const dim3 g2(nb, k);
+    const dim3 gk(nb, cdiv(k, kSpmvRhs));  // the products: kSpmvRhs right-hand sides per thread
     sys_reset(done.p, k, s);
     sys_reset(iters.p, k, s);
     hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, x.p);
     hipLaunchKernelGGL(k_g_perm_in, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, d_b, sc.p, bp.p, rs.p, xb.p);
     auto residual_check = [&](int first) {
         sys_reset(open.p, 1, s);
-        hipLaunchKernelGGL(k_g_residual, g2, dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p, diag.p, off.p, bp.p,
-                           x.p, sc.p, rs.p, done.p, part.p);
+        hipLaunchKernelGGL(k_g_residual, gk, dim3(256), 0, s, n, k, eptr.p, eadj.p, ediag.p, eoff.p, bp.p, x.p, sc.p,
+                           rs.p, done.p, part.p);
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, sums.p);
         hipLaunchKernelGGL(k_g_check, dim3(nblocks(k, 256)), dim3(256), 0, s, k, sums.p, rnorm.p, iters.p, stall.p,
                            keep.p, done.p, tol, first, open.p);
@@ -1176,8 +1248,8 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         hipLaunchKernelGGL(k_g_dot, g2, dim3(256), 0, s, n, rs.p, z.p, done.p, part.p);
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, rz_new);
         hipLaunchKernelGGL(k_g_dir, g2, dim3(256), 0, s, n, it == 0 ? 1 : 0, sc.p, z.p, rz_new, rz_old, pv.p, done.p);
-        hipLaunchKernelGGL(k_g_spmv, g2, dim3(256), 0, s, n, perm.p, iperm.p, ptr.p, adj.p, diag.p, off.p, pv.p, qv.p,
-                           done.p, part.p);
+        hipLaunchKernelGGL(k_g_spmv, gk, dim3(256), 0, s, n, k, eptr.p, eadj.p, ediag.p, eoff.p, pv.p, qv.p, done.p,
+                           part.p);
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, pq.p);
         hipLaunchKernelGGL(k_g_step, g2, dim3(256), 0, s, n, pv.p, rz_new, pq.p, x.p, done.p);
         h_open = residual_check(0);
