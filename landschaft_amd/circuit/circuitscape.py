@@ -30,6 +30,9 @@ from .advanced import _grid
 
 NODATA = -9999.0
 POLICIES = ("keepall", "rmvsrc", "rmvgnd", "rmvall")
+# one-to-all / all-to-one problems built on full rasters, as the reference
+# does (circuitscape.py:771-806), instead of from the focal cells alone
+DENSE_PROBLEMS = False
 SCENARIOS = {"pairwise": "pairwise", "Pairwise": "pairwise", "advanced": "advanced", "Advanced": "advanced",
              "one-to-all": "one-to-all", "one_to_all": "one-to-all", "all-to-one": "all-to-one",
              "all_to_one": "all-to-one"}
@@ -661,7 +664,8 @@ def _one_to_all(g, poly, points: FocalPoints, strengths, pairs, one_to_all: bool
     mx = np.full((ny, nx), NODATA) if opts["max"] else None
     policy = "rmvgnd" if one_to_all else "rmvsrc"
 
-    def problem(i):
+    def problem_dense(i):
+        # the reference's construction on full rasters
         pid = int(uids[i])
         pm = point_map.copy()
         newpoly = base_poly
@@ -696,6 +700,60 @@ def _one_to_all(g, poly, points: FocalPoints, strengths, pairs, one_to_all: bool
         return {"n": pid, "strength": amps, "poly": newpoly, "src_cells": sc.astype(np.int64),
                 "src_vals": src.ravel()[sc], "gnd_cells": gc.astype(np.int64), "gnd_vals": gnd.ravel()[gc],
                 "check_cell": int(cells[np.nonzero(ids == pid)[0][0]])}
+
+    # The same problems from the focal cells alone: every raster above is
+    # zero off the focal cells, so its nonzero cells are found among them
+    # (sorted, as flatnonzero gives them). An id 0 would match every empty
+    # cell of the rasters, so such point sets take the full-raster path.
+    pcells = np.unique(cells)
+    pvals = point_map.ravel()[pcells]  # a cell listed twice keeps its last id, as the raster does
+    fcells = np.unique(rows[first] * nx + cols[first])
+    fvals = first_map.ravel()[fcells]
+    fpos = np.searchsorted(pcells, fcells)
+
+    def problem_sparse(i):
+        pid = int(uids[i])
+        pv = pvals.copy()
+        newpoly = base_poly
+        amps = float(strengths[i, 1]) if strengths is not None else 1.0
+        if pairs is not None:
+            row = int(np.nonzero(pairs.ids == pid)[0][0])
+            drop = [pairs.ids[j] for j in range(len(pairs.ids)) if j != row and pairs.matrix[row, j] == mark]
+            pv[np.isin(pv, drop)] = 0
+            if not single:
+                pm = point_map.copy()
+                pm.ravel()[pcells] = pv
+                newpoly = _point_polymap(poly, pm, ids)
+        st = None
+        if strengths is not None:
+            # by position, as the reference maps them (it fails alike when the table and the points differ)
+            st = strengths[:, 1].copy()
+            st[pv[np.searchsorted(pcells, cells)] == 0] = 1.0
+        if pv.sum() == pid:
+            return None
+        fpm = pv[fpos]  # the point map at the first cells
+        if one_to_all:
+            sc = fcells[fvals == pid]
+            sv = np.full(len(sc), amps)
+            gmask = (pv > 0) & (pv != pid)
+            gc, gv = pcells[gmask], np.full(int(gmask.sum()), np.inf)
+        else:
+            if st is not None:
+                sval = dict(zip(cells.tolist(), st[: len(rows)].tolist()))  # a repeated cell: the last value
+                sv_all = np.array([sval[c] for c in pcells.tolist()])
+                sv_all[np.isin(pcells, fcells[fvals == pid])] = 0.0
+                sc, sv = pcells, sv_all
+            else:
+                keep = (fvals != 0) & (fpm != pid)
+                sc, sv = fcells[keep], np.ones(int(keep.sum()))
+            gmask = pv == pid
+            gc, gv = pcells[gmask], np.full(int(gmask.sum()), np.inf)
+        nz = sv != 0
+        return {"n": pid, "strength": amps, "poly": newpoly, "src_cells": sc[nz].astype(np.int64),
+                "src_vals": sv[nz].astype(np.float64), "gnd_cells": gc.astype(np.int64),
+                "gnd_vals": gv.astype(np.float64), "check_cell": int(cells[np.nonzero(ids == pid)[0][0]])}
+
+    problem = problem_dense if (DENSE_PROBLEMS or np.any(ids == 0)) else problem_sparse
 
     probs = [(i, problem(i)) for i in range(len(uids))]
     for i, q in probs:

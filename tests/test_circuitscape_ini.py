@@ -322,3 +322,49 @@ def test_one_to_all_writes_outputs(tmp_path, in_cs, engine):
     r = cs.circuitscape_ini(ini)
     np.testing.assert_array_equal(cs.read_raster(tmp_path / "ota_cum_curmap.asc")[0], r.cum_current)
     assert (tmp_path / "ota_curmap_1.asc").exists() and (tmp_path / "ota_voltmap_1.asc").exists()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_one_to_all_problems_from_focal_cells_match_full_rasters(seed, monkeypatch, oracle):
+    # the problems built from the focal cells alone against the reference's full-raster construction:
+    # repeated cells, focal regions, variable strengths, include and exclude pairs, both modes
+    oc = OracleCore(oracle)
+    monkeypatch.setattr(cs, "core", lambda: oc)
+    rng = np.random.default_rng(seed)
+    ny, nx = 14, 17
+    g = np.exp(rng.standard_normal((ny, nx)))
+    npts = 9
+    rows = rng.integers(1, ny + 1, npts)
+    cols = rng.integers(1, nx + 1, npts)
+    ids = np.sort(rng.integers(1, 6, npts))
+    if seed % 2:
+        rows[3], cols[3] = rows[2], cols[2]  # a cell listed twice
+    points = cs.FocalPoints(rows, cols, ids)
+    uids = np.unique(ids)
+    strengths = np.column_stack([np.arange(1, npts + 1), rng.uniform(0.5, 2.0, npts)]) if seed % 3 == 0 else None
+    pairs = None
+    if seed >= 2:
+        m = (rng.random((len(uids), len(uids))) < 0.6).astype(np.int64)
+        m = np.maximum(m, m.T)
+        pairs = cs.IncludedPairs("include" if seed % 2 == 0 else "exclude", uids, m)
+    for all_to_one in (False, True):
+        out = {}
+        for dense in (True, False):
+            monkeypatch.setattr(cs, "DENSE_PROBLEMS", dense)
+            try:
+                r = cs.circuitscape_one_to_all(g, points, all_to_one=all_to_one, habitat_is_resistance=False,
+                                               strengths=strengths, included_pairs=pairs, cum_map=True,
+                                               current_maps=True, voltage_maps=True)
+                out[dense] = ("ok", r)
+            except Exception as e:  # the two constructions must fail alike too
+                out[dense] = ("error", type(e).__name__)
+        assert out[True][0] == out[False][0], (seed, all_to_one, out[True], out[False])
+        if out[True][0] == "error":
+            assert out[True][1] == out[False][1]
+            continue
+        a, b = out[True][1], out[False][1]
+        np.testing.assert_array_equal(a.resistances, b.resistances)
+        np.testing.assert_array_equal(a.cum_current, b.cum_current)
+        assert set(a.maps) == set(b.maps)
+        for k in a.maps:
+            np.testing.assert_array_equal(a.maps[k], b.maps[k])
