@@ -183,41 +183,62 @@ RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny
     }
     for (i32 v = 0; v < cnt; ++v)
         if (cells_per[v] > 1) g.node_cell[v] = -1;
-    struct E {
-        i32 a, b;
-        double w;
-    };
-    std::vector<E> edges;
-    auto emit = [&](i64 c, i64 d, bool diag) {
-        const i32 a = g.nodemap[c], b = g.nodemap[d];
-        if (a < 0 || b < 0) return;
-        const double w = average(gmap[c], gmap[d], diag, avg_res);
-        if (a == b) return;
-        edges.push_back({a, b, w});
-        edges.push_back({b, a, w});
-    };
-    for (i64 x = 0; x < nx; ++x)
-        for (i64 y = 0; y < ny; ++y) {
-            const i64 c = y * nx + x;
-            if (g.nodemap[c] < 0) continue;
-            if (x + 1 < nx) emit(c, c + 1, false);
-            if (y + 1 < ny) emit(c, c + nx, false);
-            if (!four) {
-                if (y + 1 < ny && x + 1 < nx) emit(c, c + nx + 1, true);
-                if (y > 0 && x + 1 < nx) emit(c, c - nx + 1, true);
+    // Every branch once in each direction, in column-major cell order (right,
+    // down, and the two right diagonals), self loops of merged nodes left out.
+    auto each_branch = [&](auto&& put) {
+        auto emit = [&](i64 c, i64 d, bool diag) {
+            const i32 a = g.nodemap[c], b = g.nodemap[d];
+            if (a < 0 || b < 0 || a == b) return;
+            put(a, b, c, d, diag);
+            put(b, a, c, d, diag);
+        };
+        for (i64 x = 0; x < nx; ++x)
+            for (i64 y = 0; y < ny; ++y) {
+                const i64 c = y * nx + x;
+                if (g.nodemap[c] < 0) continue;
+                if (x + 1 < nx) emit(c, c + 1, false);
+                if (y + 1 < ny) emit(c, c + nx, false);
+                if (!four) {
+                    if (y + 1 < ny && x + 1 < nx) emit(c, c + nx + 1, true);
+                    if (y > 0 && x + 1 < nx) emit(c, c - nx + 1, true);
+                }
             }
-        }
-    std::stable_sort(edges.begin(), edges.end(),
-                     [](const E& p, const E& q) { return std::tie(p.a, p.b) < std::tie(q.a, q.b); });
+    };
+    // bucketed by node in emission order (a stable counting sort), then each
+    // node's entries stably by neighbour: the order of a stable sort of the
+    // whole list by (node, neighbour), so duplicate branches (cells of one
+    // region next to one cell) are summed in emission order
+    std::vector<i64> start(static_cast<std::size_t>(cnt) + 1, 0);
+    each_branch([&](i32 a, i32, i64, i64, bool) { ++start[static_cast<std::size_t>(a) + 1]; });
+    for (i32 v = 0; v < cnt; ++v) start[v + 1] += start[v];
+    std::vector<i32> eb(static_cast<std::size_t>(start[cnt]));
+    std::vector<double> ew(static_cast<std::size_t>(start[cnt]));
+    {
+        std::vector<i64> fill(start.begin(), start.end() - 1);
+        each_branch([&](i32 a, i32 b, i64 c, i64 d, bool diag) {
+            const i64 p = fill[a]++;
+            eb[p] = b;
+            ew[p] = average(gmap[c], gmap[d], diag, avg_res);
+        });
+    }
     g.ptr.assign(cnt + 1, 0);
-    for (std::size_t k = 0; k < edges.size();) {
-        std::size_t e = k;
-        double w = edges[k].w;
-        for (++e; e < edges.size() && edges[e].a == edges[k].a && edges[e].b == edges[k].b; ++e) w += edges[e].w;
-        g.adj.push_back(edges[k].b);
-        g.w.push_back(w);
-        ++g.ptr[edges[k].a + 1];
-        k = e;
+    g.adj.reserve(eb.size());
+    g.w.reserve(eb.size());
+    std::vector<std::pair<i32, double>> tmp;
+    for (i32 v = 0; v < cnt; ++v) {
+        const i64 lo = start[v], hi = start[v + 1];
+        tmp.assign(hi - lo, {});
+        for (i64 p = lo; p < hi; ++p) tmp[p - lo] = {eb[p], ew[p]};
+        std::stable_sort(tmp.begin(), tmp.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
+        for (std::size_t k = 0; k < tmp.size();) {
+            std::size_t e = k + 1;
+            double w = tmp[k].second;
+            for (; e < tmp.size() && tmp[e].first == tmp[k].first; ++e) w += tmp[e].second;
+            g.adj.push_back(tmp[k].first);
+            g.w.push_back(w);
+            ++g.ptr[v + 1];
+            k = e;
+        }
     }
     for (i32 v = 0; v < cnt; ++v) g.ptr[v + 1] += g.ptr[v];
     g.degree.assign(cnt, 0.0);
