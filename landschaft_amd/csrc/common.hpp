@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -66,6 +67,57 @@ DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
     if (!v.empty()) d.upload(v.data(), v.size(), s);
     return d;
 }
+
+// Kernel sequences are replayed as hipGraphs unless LSC_GRAPHS=0 (then
+// launched one by one; the results are the same).
+inline bool graphs_enabled() {
+    const char* e = std::getenv("LSC_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+// A fixed sequence of kernel launches, captured from a stream once and then
+// replayed as one graph launch. `key` stands for everything the launches'
+// arguments depend on (batch size, buffers): a new key captures again. The
+// sequence may hold kernel launches only (no memset, copy or allocation).
+// The legacy null stream cannot be captured: there the launches run eagerly.
+struct LaunchGraph {
+    hipGraphExec_t exec = nullptr;
+    long long key = -1;
+    LaunchGraph() = default;
+    LaunchGraph(const LaunchGraph&) = delete;
+    LaunchGraph& operator=(const LaunchGraph&) = delete;
+    ~LaunchGraph() { clear(); }
+    void clear() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr, key = -1;
+    }
+    template <class F>
+    void run(hipStream_t s, long long k, bool enabled, F&& launches) {
+        if (!enabled || s == nullptr) {
+            launches();
+            return;
+        }
+        if (!exec || key != k) {
+            clear();
+            LSC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                launches();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            LSC_HIP(hipStreamEndCapture(s, &g));
+            hipGraphExec_t x = nullptr;
+            const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            LSC_HIP(e);
+            exec = x, key = k;
+        }
+        LSC_HIP(hipGraphLaunch(exec, s));
+    }
+};
 
 void set_error(const std::string& msg);
 

@@ -17,7 +17,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "system.hpp"
@@ -31,8 +34,18 @@ void sys_reset(int* p, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
 
+// Raises a kernel's dynamic LDS limit once per size (the calls repeat in
+// every sweep; none is made again while a sweep is captured into a graph).
 void sys_allow(const void* fn, std::size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, std::size_t> set;
+    int dev = 0;
+    LSC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    std::size_t& have = set[{dev, fn}];
+    if (have >= bytes) return;
     LSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    have = bytes;
 }
 
 unsigned nblocks(long long tasks, int per) { return static_cast<unsigned>((tasks + per - 1) / per); }
@@ -858,6 +871,22 @@ struct SystemSolver::Impl {
     template <typename T>
     void huge_backward(const HugeLevel& H, int k);
     DevBuf<double> wbuf, pbuf;  // row-parallel substitution: vectors and partial dots
+    // The preconditioner sweep replayed as a graph from the second PCG
+    // iteration on; `sweep_gen` counts the factorisations and scratch
+    // allocations its captured arguments depend on.
+    LaunchGraph g_sweep;
+    long long sweep_gen = 0;
+    hipStream_t owned = nullptr;  // the solver's stream when the caller gave the null stream
+    Impl() = default;
+    Impl(const Impl&) = delete;
+    Impl& operator=(const Impl&) = delete;
+    ~Impl() {
+        g_sweep.clear();
+        if (owned) {
+            (void)hipStreamSynchronize(owned);
+            (void)hipStreamDestroy(owned);
+        }
+    }
     template <typename T>
     void wide_forward(const HugeLevel& H, int k);
     template <typename T>
@@ -869,6 +898,8 @@ struct SystemSolver::Impl {
 template <typename T>
 void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms) {
     const auto h0 = std::chrono::steady_clock::now();
+    g_sweep.clear();
+    ++sweep_gen;
     progress("analysis of %d unknowns", m.n);
     Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
     progress("plan: %d fronts, L %lld entries, largest front %d", p.nfront, static_cast<long long>(p.lsize),
@@ -1087,7 +1118,7 @@ void SystemSolver::Impl::wide_forward(const HugeLevel& H, int k) {
     const T* Lp = reinterpret_cast<const T*>(L.p);
     const unsigned F = static_cast<unsigned>(H.t.size());
     const std::size_t need = static_cast<std::size_t>(k) * H.msum;
-    if (wbuf.n < need) wbuf.alloc(need);
+    if (wbuf.n < need) wbuf.alloc(need), ++sweep_gen;
     hipLaunchKernelGGL(k_w_fgather, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
                        rs.p, wbuf.p);
     for (std::size_t r = 0; r < H.chf.size(); ++r)
@@ -1112,9 +1143,9 @@ void SystemSolver::Impl::wide_backward(const HugeLevel& H, int k) {
     const T* Lp = reinterpret_cast<const T*>(L.p);
     const unsigned F = static_cast<unsigned>(H.t.size());
     const std::size_t need = static_cast<std::size_t>(k) * H.msum;
-    if (wbuf.n < need) wbuf.alloc(need);
+    if (wbuf.n < need) wbuf.alloc(need), ++sweep_gen;
     const std::size_t pneed = static_cast<std::size_t>(k) * kWidePanel * H.csum;
-    if (pbuf.n < pneed) pbuf.alloc(pneed);
+    if (pbuf.n < pneed) pbuf.alloc(pneed), ++sweep_gen;
     hipLaunchKernelGGL(k_w_bgather, dim3(cdiv(H.mmax, 256), F, k), dim3(256), 0, s, P, H.dt.p, H.dwoff.p, P.n, k,
                        y.p, z.p, wbuf.p);
     for (int st = 0; st < H.wkt; ++st) {
@@ -1198,6 +1229,7 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         active.alloc(k), done.alloc(k), iters.alloc(k), stall.alloc(k), keep.alloc(k);
         if (!open.p) open.alloc(1);
         kcap = k;
+        ++sweep_gen;
         std::vector<int> ones(k, 1);
         active.upload(ones.data(), k, s);
     }
@@ -1243,7 +1275,12 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     for (int it = 0; h_open > 0 && it <= maxit; ++it) {
         double* rz_new = rz.p + 2 * k * (it & 1);
         double* rz_old = rz.p + 2 * k * ((it + 1) & 1);
-        sweep_t<T>(k, 1);  // z (scaled space) = (L L^T)^-1 s r
+        // z (scaled space) = (L L^T)^-1 s r; the first sweep runs eagerly and
+        // grows the scratch buffers, later ones replay its graph
+        if (it == 0)
+            sweep_t<T>(k, 1);
+        else
+            g_sweep.run(s, (sweep_gen << 32) | k, graphs_enabled(), [&] { sweep_t<T>(k, 1); });
         st.solve_launches += 1;
         hipLaunchKernelGGL(k_g_dot, g2, dim3(256), 0, s, n, rs.p, z.p, done.p, part.p);
         hipLaunchKernelGGL(k_g_sum, dim3(k), dim3(256), 0, s, static_cast<int>(nb), part.p, done.p, rz_new);
@@ -1276,6 +1313,12 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
 SystemSolver::SystemSolver(int precision, hipStream_t s) : impl_(new Impl) {
     impl_->precision = precision;
     impl_->s = s;
+    // The null stream cannot be captured into graphs: work instead on a
+    // blocking stream of our own, which the null stream orders both ways.
+    if (!s) {
+        LSC_HIP(hipStreamCreateWithFlags(&impl_->owned, hipStreamDefault));
+        impl_->s = impl_->owned;
+    }
 }
 SystemSolver::~SystemSolver() { delete impl_; }
 

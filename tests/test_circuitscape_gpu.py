@@ -245,3 +245,24 @@ def test_row_parallel_large_front_substitution(backend, oracle, monkeypatch, pre
     again = backend.circuitscape_pairwise(g, None, cells, ex, False, False, opts)
     np.testing.assert_array_equal(again["resistances"], b[0])
     np.testing.assert_array_equal(again["cum"], b[1])
+
+
+def test_graph_replay_matches_eager_pcg_sweeps(backend, oracle, monkeypatch):
+    """Mixed precision on a badly conditioned raster (r = exp(2 z)) takes several preconditioned CG
+    iterations; every sweep after the first is replayed as a hipGraph by default. The resistances and
+    the cumulative map are bit-identical to launching the sweeps one by one (LSC_GRAPHS=0)."""
+    rng = np.random.default_rng(23)
+    g = np.exp(2.0 * rng.standard_normal((160, 160)))
+    cells = rng.choice(160 * 160, 6, replace=False).astype(np.int64)
+    ex = np.zeros((6, 6), np.uint8)
+    backend.settings.precision = "mixed"
+    out = {}
+    for graphs in ("0", "1"):
+        monkeypatch.setenv("LSC_GRAPHS", graphs)
+        print(f"pcg graphs {graphs}", flush=True)
+        out[graphs] = backend.circuitscape_pairwise(g, None, cells, ex, False, True, {"cum": True})
+        assert backend.last_stats["refine_max"] >= 3
+    np.testing.assert_array_equal(out["1"]["resistances"], out["0"]["resistances"])
+    np.testing.assert_array_equal(out["1"]["cum"], out["0"]["cum"])
+    ref = oracle.cs_pairwise(g, None, cells, ex, False, True, {"cum": True})
+    np.testing.assert_allclose(out["1"]["resistances"], ref["resistances"], rtol=1e-10, atol=0)
