@@ -180,10 +180,18 @@ struct WindowSolver {
     DevBuf<double> V;
     DevBuf<int> open;
     int Bcap = 0;
+    // the batch's fixed launch sequences, replayed as graphs (keyed by the
+    // batch size; set_plan drops them with the buffers they name)
+    LaunchGraph g_flood, g_factor, g_sweep0, g_residual, g_sweep1;
+    double residual_tol = -1.0;
+    // per level: rows M of the one-window-per-lane substitutions (0: the
+    // one-front-per-wavefront kernels), forward and backward
+    std::vector<int> lane_fwd, lane_bwd;
 
     explicit WindowSolver(hipStream_t st) : s(st) {}
 
     void set_plan(Plan p, int B) {
+        for (LaunchGraph* g : {&g_flood, &g_factor, &g_sweep0, &g_residual, &g_sweep1}) g->clear();
         plan.upload(std::move(p), s);
         int smax = 1, lmax = 1;
         for (int v : plan.lev_stile) smax = std::max(smax, v);
@@ -195,6 +203,7 @@ struct WindowSolver {
         allow_lds(reinterpret_cast<const void*>(&k_forward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_backward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
+        lane_levels();
         const int W = plan.d.W;
         Bcap = B;
         wb.ensure(B, W * W, plan.d.n, plan.d.lsize, sizeof(T));
@@ -202,6 +211,40 @@ struct WindowSolver {
         V.alloc(static_cast<std::size_t>(B) * (plan.d.vsize ? plan.d.vsize : 1));
         A.alloc(static_cast<std::size_t>(B) * (plan.d.nent ? plan.d.nent : 1) * sizeof(T));
         if (!open.p) open.alloc(4);
+    }
+
+    // A level is solved one window per lane when its fronts have at most
+    // kLaneNp pivots and 64 rows (the forward also needs fronts without
+    // children), and every column's reads up to M rows stay inside the
+    // window's L. LSC_LANE_LEVELS=0 keeps every level on the per-front kernels.
+    void lane_levels() {
+        const Plan& h = plan.host;
+        const int H = static_cast<int>(h.lev_ptr.size()) - 1;
+        lane_fwd.assign(H, 0), lane_bwd.assign(H, 0);
+        const char* e = std::getenv("LSC_LANE_LEVELS");
+        if (e && e[0] == '0') return;
+        for (int lv = 0; lv < H; ++lv) {
+            int mmax = 1;
+            bool fits = true, leaves = true;
+            for (int q = h.lev_ptr[lv]; q < h.lev_ptr[lv + 1]; ++q) {
+                const int t = h.order[q];
+                const int np = h.npiv[t], m = np + h.nupd[t];
+                fits = fits && np <= kLaneNp && m <= 64;
+                leaves = leaves && h.cb[t] == h.ce[t];
+                mmax = std::max(mmax, m);
+            }
+            if (!fits) continue;
+            const int M = mmax <= 32 ? 32 : (mmax <= 48 ? 48 : (mmax <= 56 ? 56 : 64));
+            for (int q = h.lev_ptr[lv]; q < h.lev_ptr[lv + 1]; ++q) {
+                const int t = h.order[q];
+                const long long np = h.npiv[t], m = np + h.nupd[t];
+                const long long j = np - 1;  // the last column reads furthest
+                if (np > 0 && h.lpos[t] + j * m - ((j * (j - 1)) >> 1) - j + M > h.lsize) fits = false;
+            }
+            if (!fits) continue;
+            lane_bwd[lv] = M;
+            if (leaves) lane_fwd[lv] = M;
+        }
     }
 
     void time_phase(double& acc, hipEvent_t a, hipEvent_t b, bool timing) {
@@ -243,8 +286,21 @@ struct WindowSolver {
         auto grid = [&](int h) {
             return dim3(static_cast<unsigned>(static_cast<long long>(nb) * (plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h])));
         };
+        auto lanes = [&](int h) {
+            return dim3((static_cast<unsigned>(nb) + 63) / 64,
+                        static_cast<unsigned>(plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h]));
+        };
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {
+                if (const int M = lane_fwd[h]) {
+                    auto kern = M == 32   ? k_forward_lanes<T, 32>
+                                : M == 48 ? k_forward_lanes<T, 48>
+                                : M == 56 ? k_forward_lanes<T, 56>
+                                          : k_forward_lanes<T, 64>;
+                    hipLaunchKernelGGL(kern, lanes(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p,
+                                       L, wb.rs.p, wb.y.p, V.p);
+                    continue;
+                }
                 const bool small = plan.lev_mmax[h] <= 64;  // rows on lanes, L in registers: the front's vector in LDS
                 auto kern = small ? k_forward_level<T, 1> : k_forward_level<T, 3>;
                 hipLaunchKernelGGL(kern, grid(h), dim3(64), (small ? 64 : 192) * sizeof(double),
@@ -252,6 +308,15 @@ struct WindowSolver {
                                    wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
             }
         for (int h = P.H - 1; h >= 0; --h) {
+            if (const int M = lane_bwd[h]) {
+                auto kern = M == 32   ? k_backward_lanes<T, 32>
+                            : M == 48 ? k_backward_lanes<T, 48>
+                            : M == 56 ? k_backward_lanes<T, 56>
+                                      : k_backward_lanes<T, 64>;
+                hipLaunchKernelGGL(kern, lanes(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p, L,
+                                   wb.y.p, wb.z.p);
+                continue;
+            }
             const bool small = plan.lev_mmax[h] <= 64;  // update rows in 64 doubles of LDS
             auto kern = small ? k_backward_level<T, 1> : k_backward_level<T, 3>;
             const std::size_t lds = solve_lds<T>(plan.lev_lmax[h]) - (small ? 128 * sizeof(double) : 0);
@@ -264,17 +329,21 @@ struct WindowSolver {
 
     void solve(int nb, double tol, int maxit, IrStats& st, hipEvent_t* ev, bool timing) {
         const DPlan& P = plan.d;
-        reset(wb.done.p, nb, s);
-        reset(wb.iters.p, nb, s);
-        reset(wb.fail.p, nb, s);
+        const bool gr = graphs_enabled();
+        if (tol != residual_tol) g_residual.clear(), residual_tol = tol;  // the residual graph holds tol
         if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-        hipLaunchKernelGGL(k_flood, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.src.p, wb.active.p,
-                           wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p,
-                           10 * std::numeric_limits<double>::epsilon());
+        g_flood.run(s, nb, gr, [&] {
+            reset(wb.done.p, nb, s);
+            reset(wb.iters.p, nb, s);
+            reset(wb.fail.p, nb, s);
+            hipLaunchKernelGGL(k_flood, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.src.p, wb.active.p,
+                               wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p,
+                               10 * std::numeric_limits<double>::epsilon());
+        });
         if (timing) LSC_HIP(hipEventRecord(ev[1], s));
-        factor(nb);
+        g_factor.run(s, nb, gr, [&] { factor(nb); });
         if (timing) LSC_HIP(hipEventRecord(ev[2], s));
-        triangular(nb, 0);
+        g_sweep0.run(s, nb, gr, [&] { triangular(nb, 0); });
         if (timing) LSC_HIP(hipEventRecord(ev[3], s));
         st.batches += 1;
         st.solve_launches += 1;
@@ -282,11 +351,13 @@ struct WindowSolver {
         time_phase(st.ms_factor, ev[1], ev[2], timing);
         time_phase(st.ms_solve, ev[2], ev[3], timing);
         for (int it = 0;; ++it) {
-            reset(open.p, 1, s);
             if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-            hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.present.p, wb.dg.p,
-                               wb.sc.p, wb.b.p, wb.x.p, wb.rs.p, wb.active.p, wb.done.p, wb.rnorm.p, wb.rprev.p,
-                               wb.iters.p, tol, open.p);
+            g_residual.run(s, nb, gr, [&] {
+                reset(open.p, 1, s);
+                hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.present.p,
+                                   wb.dg.p, wb.sc.p, wb.b.p, wb.x.p, wb.rs.p, wb.active.p, wb.done.p, wb.rnorm.p,
+                                   wb.rprev.p, wb.iters.p, tol, open.p);
+            });
             if (timing) LSC_HIP(hipEventRecord(ev[1], s));
             int h_open = 0;
             LSC_HIP(hipMemcpyAsync(&h_open, open.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -295,7 +366,7 @@ struct WindowSolver {
             if (h_open == 0 || it >= maxit) break;
             st.solve_launches += 1;
             if (timing) LSC_HIP(hipEventRecord(ev[0], s));
-            triangular(nb, 1);
+            g_sweep1.run(s, nb, gr, [&] { triangular(nb, 1); });
             if (timing) LSC_HIP(hipEventRecord(ev[1], s));
             time_phase(st.ms_solve, ev[0], ev[1], timing);
         }
@@ -456,12 +527,22 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     // freeing) HBM costs about 13 ms per GB, more than extra batches gain
     const long long B = std::max<long long>(1, std::min<long long>(want, fit) / nsol);
     const long long nbatch = (Lt.nlat + B - 1) / B;
+    // The first batch runs on the caller's stream, the others on streams of
+    // their own that start after the work already queued there; the host
+    // waits for all of them below. (A stream of our own for the first batch
+    // too made five streams on the box's four hardware queues: 109k instead
+    // of 117k circuits/s on the 4096^2 map.) The caller's stream is often
+    // the null stream, which cannot be captured: its batches launch eagerly.
     std::vector<std::unique_ptr<WindowSolver<T>>> ws;
-    std::vector<hipStream_t> extra(nsol - 1, nullptr);
-    for (auto& x : extra) LSC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    std::vector<hipStream_t> own(nsol - 1, nullptr);
+    for (auto& x : own) LSC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    hipEvent_t queued = nullptr;
+    LSC_HIP(hipEventCreateWithFlags(&queued, hipEventDisableTiming));
+    LSC_HIP(hipEventRecord(queued, s));
+    for (auto& x : own) LSC_HIP(hipStreamWaitEvent(x, queued, 0));
     std::vector<hipEvent_t> acc_ev(nsol, nullptr);
     for (int j = 0; j < nsol; ++j) {
-        ws.emplace_back(new WindowSolver<T>(j == 0 ? s : extra[j - 1]));
+        ws.emplace_back(new WindowSolver<T>(j == 0 ? s : own[j - 1]));
         ws.back()->set_plan(j == 0 ? plan : Plan(plan), static_cast<int>(B));
         LSC_HIP(hipEventCreateWithFlags(&acc_ev[j], hipEventDisableTiming));
     }
@@ -545,7 +626,8 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     for (auto& h : helpers) h.join();
     for (auto& e : acc_ev) (void)hipEventDestroy(e);
     ws.clear();
-    for (auto& x : extra) (void)hipStreamDestroy(x);
+    (void)hipEventDestroy(queued);
+    for (auto& x : own) (void)hipStreamDestroy(x);
     for (auto& e : err)
         if (e) std::rethrow_exception(e);
     st.streams = std::max(st.streams, nsol);

@@ -815,6 +815,96 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
 
+// Levels of small fronts solved one window per lane: lane l of workgroup
+// (g, f) substitutes front f of the level for window 64 g + l, the front's
+// vector in registers (rows statically indexed, loops unrolled to kLaneNp
+// pivots and M rows). A lane reads its own window's L column by column,
+// every row up to M at constant offsets from the column's start, so the
+// loads of a column are in flight together and no value crosses lanes. Rows
+// past the front's m read the entries that follow the column in the same
+// window's L (the host checks that they lie inside it); the forward
+// discards them, the backward masks their terms.
+constexpr int kLaneNp = 32;
+
+// Forward substitution of a front without children (reference
+// system.cpp:974-989): y = L11^-1 rs, update vector -L21 y.
+template <typename T, int M>
+__global__ __launch_bounds__(64) void k_forward_lanes(DPlan P, int h, int B, long long lstride,
+                                                      const int* __restrict__ active, const int* __restrict__ done,
+                                                      const T* __restrict__ L, const double* __restrict__ rs,
+                                                      double* __restrict__ y, double* __restrict__ Vall) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
+    if (k >= B || !active[k] || done[k]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
+    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t];
+    const double* rw = rs + static_cast<long long>(k) * P.n + first;
+    double f[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) f[i] = i < np ? rw[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kLaneNp; ++j) {
+        if (j < np) {
+            const T* col = Lt + colstart(j, m) - j;  // col[i] = L[i][j] for i >= j
+            T lv[M];
+#pragma unroll
+            for (int i = j; i < M; ++i) lv[i] = col[i];
+            const double yj = f[j] * inv_pivot(lv[j]);
+            f[j] = yj;
+#pragma unroll
+            for (int i = j + 1; i < M; ++i) f[i] -= static_cast<double>(lv[i]) * yj;
+        }
+    }
+    double* yw = y + static_cast<long long>(k) * P.n + first;
+    double* Vt = Vall + static_cast<long long>(k) * P.vsize + P.vpos[t] - np;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        if (i < np) yw[i] = f[i];
+        else if (i < m) Vt[i] = f[i];
+    }
+}
+
+// Back substitution of a front (reference system.cpp:992-1016): z_piv =
+// L11^-T (y - L21^T z_upd), column j of L dotted with the solved rows below
+// it, from the last pivot up.
+template <typename T, int M>
+__global__ __launch_bounds__(64) void k_backward_lanes(DPlan P, int h, int B, long long lstride,
+                                                       const int* __restrict__ active, const int* __restrict__ done,
+                                                       const T* __restrict__ L, const double* __restrict__ y,
+                                                       double* __restrict__ z) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
+    if (k >= B || !active[k] || done[k]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
+    const int* rows = P.rows + P.upd[t];
+    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t];
+    const long long off = static_cast<long long>(k) * P.n;
+    double x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        if (i < np) x[i] = y[off + first + i];
+        else if (i < m) x[i] = z[off + rows[i - np]];
+        else x[i] = 0.0;
+    }
+#pragma unroll
+    for (int j = kLaneNp - 1; j >= 0; --j) {
+        if (j < np) {
+            const T* col = Lt + colstart(j, m) - j;
+            T lv[M];
+#pragma unroll
+            for (int i = j; i < M; ++i) lv[i] = col[i];
+            double acc = 0.0;
+#pragma unroll
+            for (int i = j + 1; i < M; ++i) acc += i < m ? static_cast<double>(lv[i]) * x[i] : 0.0;
+            x[j] = (x[j] - acc) * inv_pivot(lv[j]);
+        }
+    }
+    double* zw = z + off + first;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < np) zw[i] = x[i];
+}
+
 // Solves of one system for many right-hand sides (graphs: one L shared by
 // every instance): workgroup (f, g) is one wavefront that stages front f's L
 // block in LDS once and solves right-hand sides g kRhs.. against it, so L is
