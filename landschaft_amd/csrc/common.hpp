@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -75,6 +76,15 @@ inline bool graphs_enabled() {
     return !(e && e[0] == '0');
 }
 
+// HIP rejects a wait on an event while the stream that recorded it is being
+// captured (hipErrorStreamCaptureIsolation), even when the event was recorded
+// before the capture began: captures and waits on other streams' events hold
+// this lock.
+inline std::mutex& capture_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 // A fixed sequence of kernel launches, captured from a stream once and then
 // replayed as one graph launch. `key` stands for everything the launches'
 // arguments depend on (batch size, buffers): a new key captures again. The
@@ -99,6 +109,7 @@ struct LaunchGraph {
         }
         if (!exec || key != k) {
             clear();
+            std::lock_guard<std::mutex> lk(capture_mutex());
             LSC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipGraph_t g = nullptr;
             try {

@@ -590,6 +590,8 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
                     std::unique_lock<std::mutex> lk(mu);
                     cv.wait(lk, [&] { return acc_done >= bi - 1 || failed; });
                     if (failed) break;
+                    lk.unlock();
+                    std::lock_guard<std::mutex> cl(capture_mutex());
                     LSC_HIP(hipStreamWaitEvent(sj, acc_ev[(j + nsol - 1) % nsol], 0));
                 }
                 hipLaunchKernelGGL(k_accumulate, dim3(static_cast<unsigned>((cells + kThreads - 1) / kThreads)),
