@@ -893,10 +893,12 @@ __global__ __launch_bounds__(64) void k_backward_lanes(DPlan P, int h, int B, lo
             T lv[M];
 #pragma unroll
             for (int i = j; i < M; ++i) lv[i] = col[i];
-            double acc = 0.0;
+            // four partial sums: the dot is not one chain of dependent adds
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int i = j + 1; i < M; ++i) acc += i < m ? static_cast<double>(lv[i]) * x[i] : 0.0;
-            x[j] = (x[j] - acc) * inv_pivot(lv[j]);
+            for (int i = j + 1; i < M; ++i)
+                acc[i & 3] = fma(static_cast<double>(i < m ? lv[i] : T(0)), x[i], acc[i & 3]);
+            x[j] = (x[j] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * inv_pivot(lv[j]);
         }
     }
     double* zw = z + off + first;
