@@ -46,6 +46,7 @@ struct Dissection {
             return add(f, {});
         }
         std::vector<i32> ch;
+        ch.reserve(2);
         const bool by_row = h >= w;
         const i64 mid = by_row ? r0 + h / 2 : c0 + w / 2;
         const i32 a = by_row ? cut(r0, mid, c0, c1) : cut(r0, r1, c0, mid);
@@ -211,6 +212,12 @@ void finish_plan(Plan& p, const std::vector<std::vector<i32>>& kids, const std::
     p.nupd.assign(nf, 0), p.upd.assign(nf, 0), p.cb.assign(nf, 0), p.ce.assign(nf, 0), p.height.assign(nf, 0);
     std::vector<i32> stamp(n, -1), loc(n, -1), upd;
     p.fe_ptr.assign(nf + 1, 0);
+    // every entry (q, r) with r > q is assembled into q's front: at most half
+    // the neighbour lists
+    const std::size_t nent = static_cast<std::size_t>(nptr[n] / 2 + 1);
+    p.fe_row.reserve(nent), p.fe_col.reserve(nent), p.fe_p.reserve(nent), p.fe_q.reserve(nent);
+    p.fe_src.reserve(nent);
+    p.kids.reserve(nf);
     for (i32 t = 0; t < nf; ++t) {
         p.cb[t] = static_cast<i32>(p.kids.size());
         i32 h = 0;
@@ -237,7 +244,9 @@ void finish_plan(Plan& p, const std::vector<std::vector<i32>>& kids, const std::
         std::sort(upd.begin(), upd.end());
         p.upd[t] = static_cast<i32>(p.rows.size());
         p.nupd[t] = static_cast<i32>(upd.size());
+        if (p.rows.capacity() < p.rows.size() + upd.size()) p.rows.reserve(2 * (p.rows.size() + upd.size()));
         p.rows.insert(p.rows.end(), upd.begin(), upd.end());
+        if (p.relmap.capacity() < p.rows.size()) p.relmap.reserve(2 * p.rows.size());
         p.relmap.resize(p.rows.size(), -1);
         for (i32 q = p.first[t]; q < end; ++q) loc[q] = q - p.first[t];
         for (i32 i = 0; i < p.nupd[t]; ++i) loc[upd[i]] = p.npiv[t] + i;
@@ -346,6 +355,7 @@ void finish_graph(Plan& p, i32 n, i64 bx, i64 by, const std::vector<std::vector<
         if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = k;
     std::vector<i64> nptr(n + 1, 0), nsrc;
     std::vector<i32> nidx;
+    nidx.reserve(static_cast<std::size_t>(ptr[n])), nsrc.reserve(static_cast<std::size_t>(ptr[n]));
     for (i32 k = 0; k < n; ++k) {
         const i32 c = p.perm[k];
         for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
@@ -397,6 +407,7 @@ Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<
     p.cell = std::move(d.cells);
     p.first = std::move(d.first);
     p.npiv = std::move(d.npiv);
+    p.perm.reserve(n);
     for (i32 b : p.cell) p.perm.push_back(node_of[b]);
     const i32 first_merged = static_cast<i32>(p.perm.size());
     for (i32 c = 0; c < n; ++c)
