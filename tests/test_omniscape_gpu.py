@@ -137,3 +137,26 @@ def test_graph_replay_matches_eager_launches(backend, oracle, precision, monkeyp
     ref = oracle.omniscape(r, s, 7, 1, 1, False, False, True, True)
     top = ref["current"].max()
     np.testing.assert_allclose(maps["1"]["current"], ref["current"], rtol=1e-9, atol=1e-12 * top)
+
+
+@pytest.mark.parametrize("four", [False, True])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_lane_substitutions_match_per_front_kernels(backend, oracle, precision, four, monkeypatch):
+    """The small levels' substitutions one window per lane (the default) against the
+    one-front-per-wavefront kernels (LSC_LANE_LEVELS=0), radius 12 windows with 8 and 4 neighbours:
+    the maps agree to rounding (the two sum in different orders) and both match the oracle."""
+    rng = np.random.default_rng(31)
+    r = np.exp(rng.normal(0.0, 1.0, (60, 80)))
+    r[rng.random(r.shape) < 0.1] = np.nan
+    s = np.where(np.isnan(r), 0.0, 1 / np.nan_to_num(r, nan=1.0))
+    backend.settings.precision = precision
+    maps = {}
+    for lanes in ("0", "1"):
+        monkeypatch.setenv("LSC_LANE_LEVELS", lanes)
+        print(f"lane levels {lanes} {precision} four {four}", flush=True)
+        maps[lanes] = backend.circuit_omniscape(r, s, 12, 0, 3, four, False, True, True)
+    ref = oracle.omniscape(r, s, 12, 0, 3, four, False, True, True)
+    for k in ("current", "flow_potential"):
+        top = np.nanmax(np.abs(ref[k]))
+        np.testing.assert_allclose(maps["1"][k], maps["0"][k], rtol=1e-11, atol=1e-13 * top)
+        np.testing.assert_allclose(maps["1"][k], ref[k], rtol=1e-9, atol=1e-12 * top)
