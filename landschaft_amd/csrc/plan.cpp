@@ -1,7 +1,10 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <stdexcept>
+#include <thread>
 
 namespace lsc {
 
@@ -75,25 +78,53 @@ struct Dissection {
 // separator only the nodes that touch the far side. A disconnected set gives
 // one subtree per component. Fronts are laid out in post-order.
 struct GraphDissection {
+    // fronts of one subtree in post-order: pivots in elimination order,
+    // each front's first pivot, pivot count and children (front indices)
+    struct Out {
+        std::vector<i32> order, first, npiv;
+        std::vector<std::vector<i32>> kids;
+    };
     const std::vector<i64>& ptr;
     const std::vector<i32>& adj;
     std::size_t leaf;
-    std::vector<i32> stamp, level;
-    i32 stamps = 0;
-    std::vector<i32> order, first, npiv;
-    std::vector<std::vector<i32>> kids;
+    // stamp[v]: the search that owns v (unique per call, so a search only
+    // ever matches its own nodes, also while other threads re-stamp theirs);
+    // level[v] is touched only by the owner of v
+    std::unique_ptr<std::atomic<i32>[]> stamp;
+    std::vector<i32> level;
+    std::atomic<i32> stamps{0};
+    std::atomic<int> helpers{0};  // threads running the far halves of large sets
+    int max_helpers;
+    Out out;
 
     GraphDissection(i32 n, const std::vector<i64>& p, const std::vector<i32>& a, int lf)
-        : ptr(p), adj(a), leaf(static_cast<std::size_t>(lf)), stamp(n, -1), level(n, -1) {}
-
-    i32 add(std::vector<i32>& piv, std::vector<i32> ch) {
-        std::sort(piv.begin(), piv.end());
-        first.push_back(static_cast<i32>(order.size()));
-        npiv.push_back(static_cast<i32>(piv.size()));
-        order.insert(order.end(), piv.begin(), piv.end());
-        kids.push_back(std::move(ch));
-        return static_cast<i32>(first.size()) - 1;
+        : ptr(p), adj(a), leaf(static_cast<std::size_t>(lf)), stamp(new std::atomic<i32>[n > 0 ? n : 1]),
+          level(n, -1),
+          max_helpers(static_cast<int>(std::min<unsigned>(15, std::max(1u, std::thread::hardware_concurrency()) - 1))) {
+        for (i32 v = 0; v < n; ++v) stamp[v].store(-1, std::memory_order_relaxed);
     }
+
+    static i32 add(Out& o, std::vector<i32>& piv, std::vector<i32> ch) {
+        std::sort(piv.begin(), piv.end());
+        o.first.push_back(static_cast<i32>(o.order.size()));
+        o.npiv.push_back(static_cast<i32>(piv.size()));
+        o.order.insert(o.order.end(), piv.begin(), piv.end());
+        o.kids.push_back(std::move(ch));
+        return static_cast<i32>(o.first.size()) - 1;
+    }
+    // appends subtree b after o's fronts; returns the index shift of b's fronts
+    static i32 append(Out& o, Out& b) {
+        const i32 shift = static_cast<i32>(o.first.size()), base = static_cast<i32>(o.order.size());
+        for (i32 f : b.first) o.first.push_back(f + base);
+        o.npiv.insert(o.npiv.end(), b.npiv.begin(), b.npiv.end());
+        o.order.insert(o.order.end(), b.order.begin(), b.order.end());
+        for (auto& k : b.kids) {
+            for (i32& c : k) c += shift;
+            o.kids.push_back(std::move(k));
+        }
+        return shift;
+    }
+    i32 owner(i32 u) const { return stamp[u].load(std::memory_order_relaxed); }
 
     // Breadth-first search over the nodes stamped s: levels, nodes in visit order.
     void bfs(i32 root, i32 s, std::vector<i32>& seen) {
@@ -104,7 +135,7 @@ struct GraphDissection {
             const i32 v = seen[h];
             for (i64 e = ptr[v]; e < ptr[v + 1]; ++e) {
                 const i32 u = adj[e];
-                if (u < 0 || stamp[u] != s || level[u] >= 0) continue;
+                if (u < 0 || owner(u) != s || level[u] >= 0) continue;
                 level[u] = level[v] + 1;
                 seen.push_back(u);
             }
@@ -114,15 +145,15 @@ struct GraphDissection {
         for (i32 v : vs) level[v] = -1;
     }
 
-    // The subtree roots of `set` appended to `roots`.
-    void dissect(std::vector<i32> set, std::vector<i32>& roots) {
+    // The subtree roots of `set` appended to `roots`, its fronts to `o`.
+    void dissect(std::vector<i32> set, Out& o, std::vector<i32>& roots) {
         if (set.empty()) return;
         if (set.size() <= leaf) {
-            roots.push_back(add(set, {}));
+            roots.push_back(add(o, set, {}));
             return;
         }
-        const i32 s = stamps++;
-        for (i32 v : set) stamp[v] = s;
+        const i32 s = stamps.fetch_add(1, std::memory_order_relaxed);
+        for (i32 v : set) stamp[v].store(s, std::memory_order_relaxed);
         std::vector<i32> seen;
         bfs(*std::min_element(set.begin(), set.end()), s, seen);
         if (seen.size() < set.size()) {
@@ -134,7 +165,7 @@ struct GraphDissection {
                     comps.push_back(std::move(c));
                 }
             for (const auto& c : comps) unlevel(c);
-            for (auto& c : comps) dissect(std::move(c), roots);
+            for (auto& c : comps) dissect(std::move(c), o, roots);
             return;
         }
         // pseudo-peripheral root: restart from the smallest node of the last
@@ -155,7 +186,7 @@ struct GraphDissection {
         depth = level[seen.back()];
         if (depth < 2) {  // no level has nodes on both sides
             unlevel(seen);
-            roots.push_back(add(set, {}));
+            roots.push_back(add(o, set, {}));
             return;
         }
         std::vector<i64> count(depth + 1, 0);
@@ -181,7 +212,7 @@ struct GraphDissection {
                 bool far = false;
                 for (i64 e = ptr[v]; e < ptr[v + 1] && !far; ++e) {
                     const i32 u = adj[e];
-                    far = u >= 0 && stamp[u] == s && level[u] == cut + 1;
+                    far = u >= 0 && owner(u) == s && level[u] == cut + 1;
                 }
                 (far ? sep : a).push_back(v);
             }
@@ -189,13 +220,27 @@ struct GraphDissection {
         unlevel(seen);
         std::vector<i32>().swap(set);
         std::vector<i32> ch;
-        dissect(std::move(a), ch);
-        dissect(std::move(b), ch);
+        // the far half of a large set on a thread of its own; the fronts are
+        // laid out as the serial recursion lays them out (a, then b)
+        if (b.size() >= (std::size_t(1) << 16) && helpers.fetch_add(1) < max_helpers) {
+            Out ob;
+            std::vector<i32> rb;
+            std::thread t([&] { dissect(std::move(b), ob, rb); });
+            dissect(std::move(a), o, ch);
+            t.join();
+            helpers.fetch_sub(1);
+            const i32 shift = append(o, ob);
+            for (i32 r : rb) ch.push_back(r + shift);
+        } else {
+            if (b.size() >= (std::size_t(1) << 16)) helpers.fetch_sub(1);
+            dissect(std::move(a), o, ch);
+            dissect(std::move(b), o, ch);
+        }
         if (sep.empty() && ch.size() == 1) {
             roots.push_back(ch[0]);
             return;
         }
-        roots.push_back(add(sep, std::move(ch)));
+        roots.push_back(add(o, sep, std::move(ch)));
     }
 };
 
@@ -380,12 +425,12 @@ Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<
         GraphDissection d(n, ptr, adj, leaf);
         std::vector<i32> all(n), roots;
         for (i32 c = 0; c < n; ++c) all[c] = c;
-        d.dissect(std::move(all), roots);
-        p.perm = std::move(d.order);
-        p.first = std::move(d.first);
-        p.npiv = std::move(d.npiv);
+        d.dissect(std::move(all), d.out, roots);
+        p.perm = std::move(d.out.order);
+        p.first = std::move(d.out.first);
+        p.npiv = std::move(d.out.npiv);
         p.cell.assign(n, -1);
-        std::vector<std::vector<i32>> kids = std::move(d.kids);
+        std::vector<std::vector<i32>> kids = std::move(d.out.kids);
         // several components: the last root (the last front) adopts the
         // others, which share no entry with it, so the tree has one root
         if (roots.empty() || roots.back() != static_cast<i32>(p.first.size()) - 1)
