@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <memory>
 #include <stdexcept>
 #include <thread>
@@ -225,10 +226,24 @@ struct GraphDissection {
         if (b.size() >= (std::size_t(1) << 16) && helpers.fetch_add(1) < max_helpers) {
             Out ob;
             std::vector<i32> rb;
-            std::thread t([&] { dissect(std::move(b), ob, rb); });
-            dissect(std::move(a), o, ch);
+            std::exception_ptr err;
+            std::thread t([&] {
+                try {
+                    dissect(std::move(b), ob, rb);
+                } catch (...) {
+                    err = std::current_exception();
+                }
+            });
+            try {
+                dissect(std::move(a), o, ch);
+            } catch (...) {
+                t.join();
+                helpers.fetch_sub(1);
+                throw;
+            }
             t.join();
             helpers.fetch_sub(1);
+            if (err) std::rethrow_exception(err);
             const i32 shift = append(o, ob);
             for (i32 r : rb) ch.push_back(r + shift);
         } else {
