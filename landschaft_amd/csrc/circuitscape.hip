@@ -17,6 +17,8 @@
 #include <tuple>
 #include <unordered_map>
 #include <unordered_set>
+#include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "../../include/landschaft_circuit.h"
@@ -80,59 +82,110 @@ void label_components(RGraph& g) {
 // written directly, with the same entries as the sorted edge list (no pair
 // of cells is joined twice, and the edge average is symmetric in its two
 // arguments, bit for bit).
+// f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads (each
+// index writes only its own slots, so the result does not depend on the split).
+template <class F>
+void host_chunks(i64 n, F&& f) {
+    const i64 hw = std::max<i64>(1, std::min<i64>(16, std::thread::hardware_concurrency()));
+    const i64 nt = std::min<i64>(hw, std::max<i64>(1, n / (i64(1) << 16)));
+    std::vector<std::thread> ts;
+    for (i64 t = 1; t < nt; ++t) ts.emplace_back([&, t] { f(t, n * t / nt, n * (t + 1) / nt); });
+    f(i64(0), i64(0), n / nt);
+    for (auto& th : ts) th.join();
+}
+
 RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool four) {
     RGraph g;
     g.nx = nx, g.ny = ny;
     const i64 n = nx * ny;
     g.nodemap.assign(n, -1);
-    i32 cnt = 0;
-    for (i64 x = 0; x < nx; ++x)
-        for (i64 y = 0; y < ny; ++y)
-            if (gmap[y * nx + x] > 0) g.nodemap[y * nx + x] = cnt++;
-    g.n = cnt;
-    g.node_cell.resize(cnt);
-    for (i64 x = 0; x < nx; ++x)
-        for (i64 y = 0; y < ny; ++y) {
-            const i32 v = g.nodemap[y * nx + x];
-            if (v >= 0) g.node_cell[v] = y * nx + x;
-        }
-    // neighbour offsets (dx, dy) in column-major order
+    // Column-major labels (x outer, y inner) from two row-major passes over
+    // blocks of rows: each block counts its cells per column, then labels
+    // them from the column's start plus the blocks above.
+    const i64 hw = std::max<i64>(1, std::min<i64>(16, std::thread::hardware_concurrency()));
+    const i64 nb = std::min<i64>(hw, std::max<i64>(1, n / (i64(1) << 16)));
+    std::vector<std::vector<i32>> cnt(nb, std::vector<i32>(nx, 0));
+    std::vector<std::thread> ts;
+    auto rows = [&](i64 b) { return std::make_pair(ny * b / nb, ny * (b + 1) / nb); };
+    for (i64 b = 0; b < nb; ++b)
+        ts.emplace_back([&, b] {
+            const auto [r0, r1] = rows(b);
+            i32* c = cnt[b].data();
+            for (i64 y = r0; y < r1; ++y)
+                for (i64 x = 0; x < nx; ++x) c[x] += gmap[y * nx + x] > 0;
+        });
+    for (auto& t : ts) t.join();
+    ts.clear();
+    std::vector<i64> start(nx + 1, 0);
+    for (i64 x = 0; x < nx; ++x) {
+        i64 tot = 0;
+        for (i64 b = 0; b < nb; ++b) tot += cnt[b][x];
+        start[x + 1] = start[x] + tot;
+    }
+    const i64 total = start[nx];
+    if (total >= (i64(1) << 31)) throw std::invalid_argument("too many nodes for 32-bit labels");
+    g.n = static_cast<i32>(total);
+    g.node_cell.resize(g.n);
+    for (i64 b = 0; b < nb; ++b)
+        ts.emplace_back([&, b] {
+            const auto [r0, r1] = rows(b);
+            std::vector<i64> next(nx);
+            for (i64 x = 0; x < nx; ++x) {
+                i64 o = start[x];
+                for (i64 bb = 0; bb < b; ++bb) o += cnt[bb][x];
+                next[x] = o;
+            }
+            for (i64 y = r0; y < r1; ++y)
+                for (i64 x = 0; x < nx; ++x)
+                    if (gmap[y * nx + x] > 0) {
+                        const i32 v = static_cast<i32>(next[x]++);
+                        g.nodemap[y * nx + x] = v;
+                        g.node_cell[v] = y * nx + x;
+                    }
+        });
+    for (auto& t : ts) t.join();
+    const i32 cntn = g.n;
     static const int k8[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1}, {0, 1}, {1, -1}, {1, 0}, {1, 1}};
     static const int k4[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
     const int nd = four ? 4 : 8;
-    g.ptr.assign(static_cast<std::size_t>(cnt) + 1, 0);
-    for (i32 v = 0; v < cnt; ++v) {
-        const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
-        i64 deg = 0;
-        for (int d = 0; d < nd; ++d) {
-            const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
-            const i64 xx = x + dx, yy = y + dy;
-            if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && g.nodemap[yy * nx + xx] >= 0) ++deg;
+    g.ptr.assign(static_cast<std::size_t>(cntn) + 1, 0);
+    host_chunks(cntn, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v) {
+            const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
+            i64 deg = 0;
+            for (int d = 0; d < nd; ++d) {
+                const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
+                const i64 xx = x + dx, yy = y + dy;
+                if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && g.nodemap[yy * nx + xx] >= 0) ++deg;
+            }
+            g.ptr[v + 1] = deg;
         }
-        g.ptr[v + 1] = g.ptr[v] + deg;
-    }
-    g.adj.resize(g.ptr[cnt]);
-    g.w.resize(g.ptr[cnt]);
-    g.degree.assign(cnt, 0.0);
-    for (i32 v = 0; v < cnt; ++v) {
-        const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
-        i64 k = g.ptr[v];
-        double dsum = 0.0;
-        for (int d = 0; d < nd; ++d) {
-            const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
-            const i64 xx = x + dx, yy = y + dy;
-            if (xx < 0 || xx >= nx || yy < 0 || yy >= ny) continue;
-            const i64 e = yy * nx + xx;
-            const i32 u = g.nodemap[e];
-            if (u < 0) continue;
-            const double w = average(gmap[c], gmap[e], dx != 0 && dy != 0, avg_res);
-            g.adj[k] = u;
-            g.w[k] = w;
-            dsum += w;
-            ++k;
+    });
+    for (i32 v = 0; v < cntn; ++v) g.ptr[v + 1] += g.ptr[v];
+    g.adj.resize(g.ptr[cntn]);
+    g.w.resize(g.ptr[cntn]);
+    g.degree.assign(cntn, 0.0);
+    host_chunks(cntn, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v) {
+            const i64 c = g.node_cell[v], x = c % nx, y = c / nx;
+            i64 k = g.ptr[v];
+            double dsum = 0.0;
+            for (int d = 0; d < nd; ++d) {
+                const int dx = four ? k4[d][0] : k8[d][0], dy = four ? k4[d][1] : k8[d][1];
+                const i64 xx = x + dx, yy = y + dy;
+                if (xx < 0 || xx >= nx || yy < 0 || yy >= ny) continue;
+                const i64 e = yy * nx + xx;
+                const i32 u = g.nodemap[e];
+                if (u < 0) continue;
+                const double w = average(gmap[c], gmap[e], dx != 0 && dy != 0, avg_res);
+                g.adj[k] = u;
+                g.w[k] = w;
+                dsum += w;
+                ++k;
+            }
+            g.degree[v] = dsum;
         }
-        g.degree[v] = dsum;
-    }
+    });
     label_components(g);
     return g;
 }
@@ -1609,15 +1662,20 @@ int lsc_circuitscape_multi_advanced_sum(const double* gmap, const int64_t* polym
         if (!cum) throw std::invalid_argument("the cumulative map is required");
         lsc::Ctx cx = lsc::make_ctx(options);
         const auto h0 = std::chrono::steady_clock::now();
+        lsc::progress("multi-advanced: raster graph of %lld x %lld cells", static_cast<long long>(nx),
+                      static_cast<long long>(ny));
         lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::progress("multi-advanced: %d nodes, %d components; upload", G.n, G.ncomp);
         lsc::DevGraph DG;
         DG.upload(G, cx.s);
         std::vector<lsc::Problem> probs;
         std::vector<i32> only;
+        lsc::progress("multi-advanced: %lld problems from their cells", static_cast<long long>(nprob));
         lsc::problems_from_cells(G, nprob, sptr, scells, svals, gptr, gcells, gvals, check, probs, only);
         std::vector<i32> cnode(nprob, -1);
         for (int64_t p = 0; p < nprob; ++p) cnode[p] = only[p] >= 0 ? only[p] : -1;
         lsc::SumSink sink(G, cx.s, focal_zero != 0, cum, max, cnode);
+        lsc::progress("multi-advanced: maps on the device; problems resolved");
         cx.st.ms_fill += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
         const std::vector<char> ok = lsc::advanced_many(G, DG, probs, only, policy, true, cx, sink);
         const i64 n = nx * ny;
@@ -1628,6 +1686,7 @@ int lsc_circuitscape_multi_advanced_sum(const double* gmap, const int64_t* polym
             if (max) LSC_HIP(hipMemcpyAsync(max, sink.mx.p, n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
         }
         LSC_HIP(hipStreamSynchronize(cx.s));
+        lsc::progress("multi-advanced: maps read back");
         for (i64 c = 0; c < n; ++c) nodemap[c] = G.nodemap[c];
         for (int64_t p = 0; p < nprob; ++p) {
             check_voltage[p] = sink.vcheck[p];
