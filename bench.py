@@ -80,6 +80,40 @@ def workload(n: int = 4096, seed: int = 103, nodata: float = 0.30, sigma: float 
     return np.ascontiguousarray(r), np.ascontiguousarray(s)
 
 
+def pairwise_inputs(n: int, seed: int, npts: int, amp):
+    """The pairwise workload: conductance 1 / r on an n x n raster, r = exp(amp z) of the fractal field
+    rounded to float32 (amp None: r = exp(N(0, 1)) iid in float64), and npts distinct seeded focal cells."""
+    if amp is None:
+        r = np.exp(np.random.default_rng(seed).standard_normal((n, n)))
+    else:
+        r = np.exp(amp * fractal_field(n, seed)).astype(np.float32).astype(np.float64)
+    cells = np.random.default_rng(seed + 1000).choice(n * n, npts, replace=False).astype(np.int64)
+    return 1.0 / r, cells
+
+
+def adjoint_inputs(n: int, npairs: int, seed: int):
+    """The gradient workload: an n x n lognormal (sigma 1) resistance rounded to float32 and npairs
+    seeded (source, ground) cells."""
+    rng = np.random.default_rng(seed)
+    r = np.exp(rng.standard_normal((n, n))).astype(np.float32).astype(np.float64)
+    src = rng.choice(n * n, npairs, replace=False)
+    dst = rng.choice(n * n, npairs, replace=False)
+    return r, src, dst
+
+
+def large_graph_inputs(n: int = 8192, seed: int = 105, npts: int = 64, nodata: float = 0.10,
+                       sigma: float | None = None):
+    """The large single-graph workload: r = exp(2 z) of the fractal field in float32, NaN on clustered
+    NoData blobs (a `nodata` fraction, sigma n / 128 cells), and npts seeded focal cells among the data
+    cells."""
+    z = fractal_field(n, seed)
+    r = np.exp(2.0 * z).astype(np.float32).astype(np.float64)
+    r[nodata_blobs(n, seed + 1, sigma if sigma is not None else n / 128.0, nodata)] = np.nan
+    data = np.flatnonzero(~np.isnan(r))
+    cells = np.random.default_rng(seed + 1000).choice(data, npts, replace=False).astype(np.int64)
+    return np.ascontiguousarray(r), cells
+
+
 BATCH = 16384  # windows per batch
 
 
@@ -123,12 +157,7 @@ def _phase(st):
 
 def pairwise_leg(B, n, four, npts, seed, amp, maps_too, cpu=False):
     """Circuitscape pairwise on one n x n graph: mixed and FP64 GPU (and optionally the CPU oracle)."""
-    if amp is None:  # lognormal iid
-        r = np.exp(np.random.default_rng(seed).standard_normal((n, n)))
-    else:
-        r = np.exp(amp * fractal_field(n, seed)).astype(np.float32).astype(np.float64)
-    g = 1.0 / r
-    cells = np.random.default_rng(seed + 1000).choice(n * n, npts, replace=False).astype(np.int64)
+    g, cells = pairwise_inputs(n, seed, npts, amp)
     ex = np.zeros((npts, npts), np.uint8)
     out = {"workload": f"pairwise {n}x{n} {'4' if four else '8'}-neighbour, {npts} focal cells "
                        f"({npts * (npts - 1) // 2} pairs), seed {seed}"}
@@ -199,10 +228,7 @@ def network_leg(B, n, npts, seed, cpu=False):
 
 def adjoint_leg(B, n, npairs, seed):
     """J = sum of effective resistances and dJ/dr on an n x n 8-neighbour lognormal surface."""
-    rng = np.random.default_rng(seed)
-    r = np.exp(rng.standard_normal((n, n))).astype(np.float32).astype(np.float64)
-    src = rng.choice(n * n, npairs, replace=False)
-    dst = rng.choice(n * n, npairs, replace=False)
+    r, src, dst = adjoint_inputs(n, npairs, seed)
     out = {"workload": f"adjoint {n}x{n} 8-neighbour lognormal sigma 1, {npairs} pairs, seed {seed}"}
     grads = {}
     for prec in ("mixed", "fp64"):

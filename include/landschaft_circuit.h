@@ -237,11 +237,26 @@ int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t 
 typedef struct lsc_grid_circuit lsc_grid_circuit;
 int lsc_grid_circuit_create(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
                             const lsc_solve_options* options, lsc_grid_circuit** out);
+/* update factorises the circuit again for a new host resistance on the same
+ * cells (finite r > 0 where they were, status 1 otherwise), keeping the
+ * symbolic plans: a calibration loop pays the analysis once. */
+int lsc_grid_circuit_update(lsc_grid_circuit* circuit, const double* resistance);
 int lsc_grid_circuit_solve(lsc_grid_circuit* circuit, int k, const double* rhs_cells, double* out_cells);
 int lsc_grid_circuit_adjoint(lsc_grid_circuit* circuit, int k, const double* dJdV_cells, const double* V_cells,
                              double* grad_r);
 int lsc_grid_circuit_stats(lsc_grid_circuit* circuit, lsc_solve_stats* stats);
 void lsc_grid_circuit_destroy(lsc_grid_circuit* circuit);
+
+/* The body of an ESRI ASCII grid of ny x nx float64 values (row-major):
+ * every value in the shortest text that reads back to the same double (as
+ * C++17 std::to_chars writes it), NaN written as -9999, one space between
+ * values and a newline after each row. Replaces ascii_grid_rows
+ * (src/cpp/circuit/python.cpp:484-521, registered at :554; the writer of
+ * src/landschaft/circuit/circuitscape.py:1027 calls it). With out NULL,
+ * *size receives the number of bytes of the text; otherwise the text goes to
+ * out (capacity bytes, no terminating zero) and *size receives its length,
+ * status 1 when capacity is too small. Host code: no device is needed. */
+int lsc_ascii_grid_rows(const double* values, int64_t nx, int64_t ny, char* out, int64_t capacity, int64_t* size);
 
 /* Symbolic statistics of a window plan with the reference's ordering
  * (16-cell nested-dissection leaves; disk: 1 for the circuit windows, 0 for

@@ -21,9 +21,9 @@ EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_ci
            "lsc_circuit_omniscape_correction", "lsc_probe_mfma", "lsc_window_plan_stats", "lsc_circuit_advanced",
            "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise", "lsc_circuitscape_multi_advanced",
            "lsc_circuitscape_multi_advanced_sum", "lsc_network_advanced", "lsc_network_pairwise",
-           "lsc_effective_resistance_grad", "lsc_grid_circuit_create", "lsc_grid_circuit_solve",
-           "lsc_grid_circuit_adjoint", "lsc_grid_circuit_stats", "lsc_grid_circuit_destroy")
-
+           "lsc_effective_resistance_grad", "lsc_grid_circuit_create", "lsc_grid_circuit_update",
+           "lsc_grid_circuit_solve", "lsc_grid_circuit_adjoint", "lsc_grid_circuit_stats",
+           "lsc_grid_circuit_destroy", "lsc_ascii_grid_rows")
 
 class SolveOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_refine", ctypes.c_int32), ("tol", ctypes.c_double),
@@ -50,6 +50,49 @@ class SolveStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+def _signatures() -> dict:
+    """restype and argtypes of every export, as include/landschaft_circuit.h declares them: ctypes then
+    converts (and range-checks) each argument instead of passing Python ints as C ints."""
+    c = ctypes
+    i32, i64, f64, vp = c.c_int32, c.c_int64, c.c_double, c.c_void_p
+    pi32, pi64, pf64, pu8 = c.POINTER(i32), c.POINTER(i64), c.POINTER(f64), c.POINTER(c.c_uint8)
+    ci = c.c_int
+    opt, st = c.POINTER(SolveOptions), c.POINTER(SolveStats)
+    cond = c.POINTER(OmniscapeCondition)
+    omni = [vp, vp, i64, i64, ci, ci, ci, ci, ci, ci, ci, cond, opt, vp, vp, pi64, st]
+    return {
+        "lsc_last_error": (c.c_char_p, []),
+        "lsc_device_info": (ci, [c.c_char_p, i64, pi32, pi64]),
+        "lsc_circuit_omniscape": (ci, omni),
+        "lsc_circuit_omniscape_device": (ci, omni),
+        "lsc_circuit_omniscape_correction": (ci, [ci, ci, ci, ci, i64, i64, opt, pf64]),
+        "lsc_probe_mfma": (ci, [ci, pf64]),
+        "lsc_window_plan_stats": (ci, [ci, ci, ci, ci, pf64]),
+        "lsc_circuit_advanced": (ci, [pf64, pf64, pf64, i64, i64, ci, opt, pf64, pf64, st]),
+        "lsc_circuitscape_advanced": (ci, [pf64, pi64, pf64, pf64, i64, i64, ci, ci, ci, opt, pi32, pf64, pf64,
+                                           pi64, pi32, pi32, st]),
+        "lsc_circuitscape_pairwise": (ci, [pf64, pi64, i64, i64, pi64, i64, pu8, ci, ci, ci, vp, vp, opt, pf64,
+                                           pf64, pf64, st]),
+        "lsc_circuitscape_multi_advanced": (ci, [pf64, pi64, i64, i64, ci, ci, ci, i64, pi64, pi64, pf64, pi64,
+                                                 pi64, pf64, pi64, opt, pi32, pf64, pf64, pu8, pi32, st]),
+        "lsc_circuitscape_multi_advanced_sum": (ci, [pf64, pi64, i64, i64, ci, ci, ci, i64, pi64, pi64, pf64,
+                                                     pi64, pi64, pf64, pi64, ci, opt, pi32, pf64, pf64, pf64, pi32,
+                                                     st]),
+        "lsc_network_advanced": (ci, [pi32, pi32, pf64, i64, i32, pf64, pf64, ci, opt, pf64, pf64, pi32, pi32,
+                                      pf64, pi64, pi32, pi32, st]),
+        "lsc_network_pairwise": (ci, [pi32, pi32, pf64, i64, i32, pi32, i64, pu8, ci, vp, vp, opt, pf64, pf64,
+                                      pi32, pi32, pf64, pi64, pi32, st]),
+        "lsc_effective_resistance_grad": (ci, [pf64, i64, i64, ci, pi64, pi64, i64, opt, pf64, pf64, pf64, st]),
+        "lsc_grid_circuit_create": (ci, [pf64, i64, i64, ci, opt, c.POINTER(vp)]),
+        "lsc_grid_circuit_update": (ci, [vp, pf64]),
+        "lsc_grid_circuit_solve": (ci, [vp, ci, vp, vp]),
+        "lsc_grid_circuit_adjoint": (ci, [vp, ci, vp, vp, vp]),
+        "lsc_grid_circuit_stats": (ci, [vp, st]),
+        "lsc_grid_circuit_destroy": (None, [vp]),
+        "lsc_ascii_grid_rows": (ci, [pf64, i64, i64, c.c_char_p, i64, pi64]),
+    }
+
+
 _lib = None
 
 
@@ -66,8 +109,11 @@ def lib() -> ctypes.CDLL:
             import torch  # noqa: F401
         except ImportError:
             pass
-        _lib = ctypes.CDLL(str(LIB_PATH))
-        _lib.lsc_last_error.restype = ctypes.c_char_p
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _signatures().items():
+            fn = getattr(handle, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = handle
     return _lib
 
 

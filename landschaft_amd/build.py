@@ -22,7 +22,7 @@ LIB = OUT / "liblandschaft_circuit.so"
 ROOT = PKG.parent
 ARCH = os.environ.get("LSC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.hip", "omniscape.hip", "system.hip", "circuitscape.hip", "plan.cpp"]
+SOURCES = ["capi.hip", "omniscape.hip", "system.hip", "circuitscape.hip", "plan.cpp", "grid_text.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-I", str(ROOT / "include")]
 

@@ -393,6 +393,20 @@ def circuitscape_multi_advanced_sum(gmap, polymap, avg_resistances, four_neighbo
     return {"nodemap": nm, "check_voltage": vcheck, "solved": solved.astype(bool)}
 
 
+def ascii_grid_rows(values) -> bytes:
+    """The body of an ESRI ASCII grid (python.cpp:484-521): every value in its shortest round-trip
+    text, NaN as -9999, rows ended by newlines."""
+    a = N.f64(values)
+    if a.ndim != 2:
+        raise ValueError("expected a 2-D array")
+    ny, nx = a.shape
+    size = ctypes.c_int64(0)
+    N.check(N.lib().lsc_ascii_grid_rows(N.ptr(a), nx, ny, None, 0, ctypes.byref(size)))
+    buf = ctypes.create_string_buffer(max(size.value, 1))
+    N.check(N.lib().lsc_ascii_grid_rows(N.ptr(a), nx, ny, buf, size.value, ctypes.byref(size)))
+    return buf.raw[:size.value]
+
+
 def effective_resistance_grad(resistance, src_cells, gnd_cells, four_neighbors=False):
     """J = sum of effective resistances of the pairs and dJ/dr per cell (the adjoint on the GPU)."""
     r = N.f64(resistance)

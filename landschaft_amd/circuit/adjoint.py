@@ -24,10 +24,26 @@ class _Circuit:
     def __init__(self, r: np.ndarray, four: bool):
         r = N.f64(r)
         self.shape = r.shape
+        self.r = r.copy()
+        self.version = 0
         self.h = ctypes.c_void_p()
         o = _opts()
         N.check(N.lib().lsc_grid_circuit_create(N.ptr(r), ctypes.c_int64(r.shape[1]), ctypes.c_int64(r.shape[0]),
                                                 int(bool(four)), ctypes.byref(o), ctypes.byref(self.h)))
+
+    def update(self, r: np.ndarray) -> None:
+        """Factorise again for new resistances on the same cells (the symbolic plans are kept)."""
+        r = N.f64(r)
+        if r.shape != self.shape:
+            raise ValueError("the new resistance must have the circuit's shape")
+        N.check(N.lib().lsc_grid_circuit_update(self.h, N.ptr(r)))
+        self.r = r.copy()
+        self.version += 1
+
+    def ensure(self, r: np.ndarray, version: int) -> None:
+        """The factor of r again if the circuit was refactorised since `version` was taken."""
+        if self.version != version:
+            self.update(r)
 
     def solve(self, rhs: torch.Tensor) -> torch.Tensor:
         rhs = rhs.to(torch.float64).contiguous()
@@ -49,6 +65,28 @@ class _Circuit:
             N.lib().lsc_grid_circuit_destroy(self.h)
 
 
+_cached: dict = {}
+
+
+def _circuit_for(r: np.ndarray, four: bool) -> _Circuit:
+    """The factorised circuit of r: the last circuit of the same shape, neighbours and cells in the
+    circuit (finite r > 0) and solver settings, refactorised on its plans when the values changed, else
+    a new one. A
+    calibration loop that changes the values of r, not its NoData cells, pays the symbolic analysis
+    once. A backward pass whose forward values were replaced since refactorises them again."""
+    valid = np.isfinite(r) & (r > 0)
+    o = _opts()
+    key = (r.shape, bool(four), valid.tobytes(), o.precision, o.tol, o.max_refine)
+    c = _cached.get("circuit")
+    if c is not None and _cached.get("key") == key:
+        if not np.array_equal(c.r, r, equal_nan=True):
+            c.update(r)
+        return c
+    c = _Circuit(r, four)
+    _cached.update(circuit=c, key=key)
+    return c
+
+
 class PairPotentials(torch.autograd.Function):
     """V[k] = potentials of the grid circuit of conductance 1 / r with 1 A entering
     at src[k] and leaving at dst[k] (each component grounded at its last node)."""
@@ -58,20 +96,21 @@ class PairPotentials(torch.autograd.Function):
         if r.device.type != "cuda":
             raise ValueError("the resistance must be a CUDA (HIP) tensor")
         ny, nx = r.shape
-        circuit = _Circuit(r.detach().to(torch.float64).cpu().numpy(), four)
+        circuit = _circuit_for(r.detach().to(torch.float64).cpu().numpy(), four)
         k = src.numel()
         rhs = torch.zeros((k, ny, nx), dtype=torch.float64, device=r.device)
         idx = torch.arange(k, device=r.device)
         rhs.view(k, -1)[idx, dst.to(r.device)] += 1.0
         rhs.view(k, -1)[idx, src.to(r.device)] -= 1.0
         V = circuit.solve(rhs)
-        ctx.circuit = circuit
+        ctx.circuit, ctx.version, ctx.r = circuit, circuit.version, circuit.r
         ctx.save_for_backward(V)
         return V
 
     @staticmethod
     def backward(ctx, gV: torch.Tensor):
         (V,) = ctx.saved_tensors
+        ctx.circuit.ensure(ctx.r, ctx.version)
         grad = ctx.circuit.adjoint(gV, V)
         return grad, None, None, None
 

@@ -953,38 +953,6 @@ def _write_tiff(path: str, values: np.ndarray, meta: RasterMeta) -> None:
     Path(path).write_bytes(bytes(out))
 
 
-def shortest(v: float) -> str:
-    """A double in the shortest text that reads back to it, fixed or scientific notation, whichever
-    is shorter (fixed on a tie), as C++'s ``std::to_chars(double)`` writes it: ``1``, ``0.5``,
-    ``1e-05``, ``123456789012345680``. Circuitscape's ASCII grids are written this way
-    (python.cpp:484-521); NaN is written as -9999."""
-    v = float(v)
-    if v != v:
-        v = NODATA
-    if v == 0.0:
-        return "-0" if math.copysign(1.0, v) < 0 else "0"
-    if math.isinf(v):
-        return "-inf" if v < 0 else "inf"
-    r = repr(v)  # Python's repr holds the shortest round-trip digits
-    sign = "-" if r[0] == "-" else ""
-    mant, _, e = r.lstrip("-").partition("e")
-    whole, _, frac = mant.partition(".")
-    exp = int(e) if e else 0
-    digits = (whole + frac).lstrip("0").rstrip("0")
-    # the value is 0.<digits> x 10^point
-    point = len(whole) + exp if whole != "0" else exp - (len(frac) - len(frac.lstrip("0")))
-    sexp = point - 1
-    sci = digits[0] + ("." + digits[1:] if len(digits) > 1 else "") + ("e-" if sexp < 0 else "e+") + f"{abs(sexp):02d}"
-    if point <= 0:
-        fix = "0." + "0" * -point + digits
-    elif point >= len(digits):
-        # an integer: as many characters whatever the digits, so the exact value, the closest
-        fix = str(int(abs(v)))
-    else:
-        fix = digits[:point] + "." + digits[point:]
-    return sign + (fix if len(fix) <= len(sci) else sci)
-
-
 def _write_grid(prefix: str, values: np.ndarray, meta: RasterMeta, tif: bool) -> None:
     """One output map (circuitscape.py:1011-1029): ``prefix.asc`` with every value in its shortest
     round-trip form (and ``prefix.prj`` when the habitat had a projection), or ``prefix.tif``; NaN written as -9999."""
@@ -992,12 +960,12 @@ def _write_grid(prefix: str, values: np.ndarray, meta: RasterMeta, tif: bool) ->
     if tif:
         _write_tiff(prefix + ".tif", out, meta)
         return
-    with open(prefix + ".asc", "w", encoding="utf-8") as f:
-        f.write(f"ncols         {meta.ncols}\nnrows         {meta.nrows}\n")
-        f.write(f"xllcorner     {meta.xllcorner!r}\nyllcorner     {meta.yllcorner!r}\n")
-        f.write(f"cellsize      {meta.cellsize!r}\nNODATA_value  -9999\n")
-        for row in out:
-            f.write(" ".join(shortest(v) for v in row) + "\n")
+    head = (f"ncols         {meta.ncols}\nnrows         {meta.nrows}\n"
+            f"xllcorner     {meta.xllcorner!r}\nyllcorner     {meta.yllcorner!r}\n"
+            f"cellsize      {meta.cellsize!r}\nNODATA_value  -9999\n")
+    with open(prefix + ".asc", "wb") as f:
+        f.write(head.encode())
+        f.write(core().ascii_grid_rows(out))
     if meta.wkt:
         Path(prefix + ".prj").write_text(meta.wkt)
 

@@ -1428,19 +1428,82 @@ __global__ void k_adjoint_grad(long long nx, long long ny, int four, int npairs,
 
 }  // namespace
 
+// Right-hand sides by cell to a component's unknowns (k x nu), and back.
+__global__ void k_cells_to_nodes(int nu, int k, long long ncell, const long long* __restrict__ cell,
+                                 const double* __restrict__ by_cell, double* __restrict__ by_node) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= static_cast<long long>(nu) * k) return;
+    const long long j = q / nu, i = q - j * nu;
+    by_node[q] = by_cell[j * ncell + cell[i]];
+}
+
+__global__ void k_nodes_to_cells(int nu, int k, long long ncell, const long long* __restrict__ cell,
+                                 const double* __restrict__ by_node, double* __restrict__ by_cell) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= static_cast<long long>(nu) * k) return;
+    const long long j = q / nu, i = q - j * nu;
+    by_cell[j * ncell + cell[i]] = by_node[q];
+}
+
+__global__ void k_fill_zero(long long n, double* __restrict__ a) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = 0.0;
+}
+
 // A grid circuit kept on the GPU for gradients: conductance 1 / r on the
 // cells with a finite r > 0 (8 or 4 neighbours, average conductance), every
-// component grounded at its last node, factorised once.
+// component grounded at its last node, factorised once; new resistances on
+// the same cells are factorised again on the same symbolic plans.
 struct GridCircuit {
     Ctx cx;
     RGraph g;
     Members M;
     std::vector<std::unique_ptr<SystemSolver>> sys;  // per component
     std::vector<std::vector<i32>> nodes;             // unknown -> graph node, per component
+    std::vector<DevBuf<long long>> dcell;            // unknown -> cell, per component (device)
+    std::vector<unsigned char> valid;                // cells in the circuit (finite r > 0)
     DevBuf<double> r;                                // resistance (cells)
+    DevBuf<double> B, X;                             // right-hand sides and solutions by unknown
     long long nx = 0, ny = 0;
     int four = 0;
 };
+
+bool valid_resistance(double v) { return v > 0 && std::isfinite(v); }
+
+// The component matrices of the grid of conductance 1 / r: each component
+// grounded at its last member, factorised (`refactor`: on the plans of an
+// earlier factorisation of the same cells).
+void grid_factor(GridCircuit& C, const double* resistance, bool refactor) {
+    const i64 n = C.nx * C.ny;
+    std::vector<double> g(n);
+    for (i64 c = 0; c < n; ++c) g[c] = valid_resistance(resistance[c]) ? 1 / resistance[c] : 0.0;
+    C.g = raster_graph(g.data(), nullptr, C.nx, C.ny, false, C.four != 0);
+    C.M = members_of(C.g);
+    if (refactor && static_cast<std::size_t>(C.g.ncomp) != C.sys.size())
+        throw std::invalid_argument("the circuit's components changed");
+    for (i32 k = 0; k < C.g.ncomp; ++k) {
+        const i64 p0 = C.M.start[k], p1 = C.M.start[k + 1];
+        const i32 ref = C.M.mem[p1 - 1];
+        std::vector<i32> nodes;
+        HostMatrix A = component_matrix(C.g, C.M.mem.data() + p0, p1 - p0, nullptr, [&](i32 v) { return v == ref; },
+                                        nodes);
+        if (refactor) {
+            if (nodes != C.nodes[k]) throw std::invalid_argument("the circuit's components changed");
+            C.sys[k]->refactor(A);
+            continue;
+        }
+        auto S = std::make_unique<SystemSolver>(C.cx.precision, C.cx.s);
+        S->factor(A);
+        std::vector<long long> cells(nodes.size());
+        for (std::size_t i = 0; i < nodes.size(); ++i) cells[i] = C.g.node_cell[nodes[i]];
+        C.dcell.push_back(to_device(cells.empty() ? std::vector<long long>{0} : cells, C.cx.s));
+        C.sys.push_back(std::move(S));
+        C.nodes.push_back(std::move(nodes));
+    }
+    C.r.alloc(n ? n : 1);
+    if (n) C.r.upload(resistance, n, C.cx.s);
+    LSC_HIP(hipStreamSynchronize(C.cx.s));
+}
 
 }  // namespace lsc
 
@@ -1449,26 +1512,27 @@ namespace {
 using lsc::i32;
 using lsc::i64;
 
-// Solves potentials (by cell) of per-pair right-hand sides e_dst - e_src
-// (or arbitrary per-node vectors) on a GridCircuit.
+// Potentials by cell (device, k x ny x nx) of right-hand sides given by cell
+// (device, k x ny x nx): every component solved on the device, cells outside
+// the circuit 0.
 void grid_solve(lsc::GridCircuit& C, int k, const double* d_rhs_cells, double* d_out_cells) {
     const i64 ncell = C.nx * C.ny;
-    std::vector<double> rhs(static_cast<std::size_t>(k) * ncell), out(static_cast<std::size_t>(k) * ncell, 0.0);
-    LSC_HIP(hipMemcpyAsync(rhs.data(), d_rhs_cells, rhs.size() * sizeof(double), hipMemcpyDeviceToHost, C.cx.s));
-    LSC_HIP(hipStreamSynchronize(C.cx.s));
+    const long long tot = static_cast<long long>(k) * ncell;
+    if (tot > 0)
+        hipLaunchKernelGGL(lsc::k_fill_zero, dim3(lsc::nblk(tot, 256)), dim3(256), 0, C.cx.s, tot, d_out_cells);
     for (std::size_t ci = 0; ci < C.sys.size(); ++ci) {
         auto& S = *C.sys[ci];
-        const auto& nodes = C.nodes[ci];
         const int nu = S.n();
-        if (nu == 0) continue;
-        std::vector<double> B(static_cast<std::size_t>(k) * nu), X(B.size());
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < nu; ++i) B[static_cast<std::size_t>(j) * nu + i] = rhs[j * ncell + C.g.node_cell[nodes[i]]];
-        S.solve(k, B.data(), X.data(), C.cx.tol, C.cx.maxit, C.cx.st);
-        for (int j = 0; j < k; ++j)
-            for (int i = 0; i < nu; ++i) out[j * ncell + C.g.node_cell[nodes[i]]] = X[static_cast<std::size_t>(j) * nu + i];
+        if (nu == 0 || k <= 0) continue;
+        const std::size_t kn = static_cast<std::size_t>(k) * nu;
+        if (C.B.n < kn) C.B.alloc(kn), C.X.alloc(kn);
+        const long long* cell = C.dcell[ci].p;
+        hipLaunchKernelGGL(lsc::k_cells_to_nodes, dim3(lsc::nblk(static_cast<long long>(kn), 256)), dim3(256), 0,
+                           C.cx.s, nu, k, static_cast<long long>(ncell), cell, d_rhs_cells, C.B.p);
+        S.solve_device(k, C.B.p, C.X.p, C.cx.tol, C.cx.maxit, C.cx.st);
+        hipLaunchKernelGGL(lsc::k_nodes_to_cells, dim3(lsc::nblk(static_cast<long long>(kn), 256)), dim3(256), 0,
+                           C.cx.s, nu, k, static_cast<long long>(ncell), cell, C.X.p, d_out_cells);
     }
-    LSC_HIP(hipMemcpyAsync(d_out_cells, out.data(), out.size() * sizeof(double), hipMemcpyHostToDevice, C.cx.s));
     LSC_HIP(hipStreamSynchronize(C.cx.s));
 }
 
@@ -1699,28 +1763,26 @@ int lsc_circuitscape_multi_advanced_sum(const double* gmap, const int64_t* polym
 int lsc_grid_circuit_create(const double* resistance, int64_t nx, int64_t ny, int four_neighbors,
                             const lsc_solve_options* options, lsc_grid_circuit** out) {
     return lsc::guarded([&] {
+        if (!out || (nx * ny > 0 && !resistance)) throw std::invalid_argument("null argument");
+        if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
         auto C = std::make_unique<lsc::GridCircuit>();
         C->cx = lsc::make_ctx(options);
         C->nx = nx, C->ny = ny, C->four = four_neighbors;
-        const i64 n = nx * ny;
-        std::vector<double> g(n);
-        for (i64 c = 0; c < n; ++c) g[c] = (resistance[c] > 0 && std::isfinite(resistance[c])) ? 1 / resistance[c] : 0.0;
-        C->g = lsc::raster_graph(g.data(), nullptr, nx, ny, false, four_neighbors != 0);
-        C->M = lsc::members_of(C->g);
-        for (i32 k = 0; k < C->g.ncomp; ++k) {
-            const i64 p0 = C->M.start[k], p1 = C->M.start[k + 1];
-            const i32 ref = C->M.mem[p1 - 1];
-            std::vector<i32> nodes;
-            lsc::HostMatrix A = lsc::component_matrix(C->g, C->M.mem.data() + p0, p1 - p0, nullptr,
-                                                      [&](i32 v) { return v == ref; }, nodes);
-            auto S = std::make_unique<lsc::SystemSolver>(C->cx.precision, C->cx.s);
-            S->factor(A);
-            C->sys.push_back(std::move(S));
-            C->nodes.push_back(std::move(nodes));
-        }
-        C->r.alloc(n ? n : 1);
-        if (n) C->r.upload(resistance, n, C->cx.s);
+        C->valid.resize(static_cast<std::size_t>(nx * ny));
+        for (i64 c = 0; c < nx * ny; ++c) C->valid[c] = lsc::valid_resistance(resistance[c]);
+        lsc::grid_factor(*C, resistance, false);
         *out = reinterpret_cast<lsc_grid_circuit*>(C.release());
+    });
+}
+
+int lsc_grid_circuit_update(lsc_grid_circuit* h, const double* resistance) {
+    return lsc::guarded([&] {
+        if (!h || !resistance) throw std::invalid_argument("null argument");
+        auto& C = *reinterpret_cast<lsc::GridCircuit*>(h);
+        for (i64 c = 0; c < C.nx * C.ny; ++c)
+            if (static_cast<bool>(C.valid[c]) != lsc::valid_resistance(resistance[c]))
+                throw std::invalid_argument("the new resistance must keep the circuit's cells (finite r > 0)");
+        lsc::grid_factor(C, resistance, true);
     });
 }
 
@@ -1759,8 +1821,9 @@ int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t 
     return lsc::guarded([&] {
         const i64 n = nx * ny;
         lsc_grid_circuit* h = nullptr;
-        if (lsc_grid_circuit_create(resistance, nx, ny, four_neighbors, options, &h) != 0)
-            throw lsc::HipError(lsc_last_error());
+        const int created = lsc_grid_circuit_create(resistance, nx, ny, four_neighbors, options, &h);
+        if (created == 1) throw std::invalid_argument(lsc_last_error());
+        if (created != 0) throw lsc::HipError(lsc_last_error());
         std::unique_ptr<lsc::GridCircuit> C(reinterpret_cast<lsc::GridCircuit*>(h));
         for (int64_t k = 0; k < npairs; ++k) {
             if (src_cells[k] < 0 || src_cells[k] >= n || gnd_cells[k] < 0 || gnd_cells[k] >= n)

@@ -856,7 +856,8 @@ int lsc_circuit_omniscape(const double* resistance, const double* source, int64_
                           int correct_artifacts, int flow_potential, const lsc_omniscape_condition* conditions,
                           const lsc_solve_options* options, double* current, double* flow_potential_out,
                           int64_t* n_targets, lsc_solve_stats* stats) {
-    return lsc::guarded([&] {
+    int inner = 0;  // the device run's status: an argument error stays status 1
+    const int rc = lsc::guarded([&] {
         if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
         const std::size_t n = static_cast<std::size_t>(nx) * ny;
         hipStream_t s = options ? reinterpret_cast<hipStream_t>(options->stream) : nullptr;
@@ -865,10 +866,10 @@ int lsc_circuit_omniscape(const double* resistance, const double* source, int64_
             r.upload(resistance, n, s);
             src.upload(source, n, s);
         }
-        int rc = omniscape_run(r.p, src.p, source, nx, ny, radius, buffer, block_size, four_neighbors,
-                               resistance_is_conductance, correct_artifacts, flow_potential, conditions, false,
-                               options, cur.p, fp.p, n_targets, stats);
-        if (rc != 0) throw lsc::HipError(lsc_last_error());
+        inner = omniscape_run(r.p, src.p, source, nx, ny, radius, buffer, block_size, four_neighbors,
+                              resistance_is_conductance, correct_artifacts, flow_potential, conditions, false,
+                              options, cur.p, fp.p, n_targets, stats);
+        if (inner != 0) return;  // its message is set
         if (n) {
             LSC_HIP(hipMemcpyAsync(current, cur.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
             if (flow_potential)
@@ -876,6 +877,7 @@ int lsc_circuit_omniscape(const double* resistance, const double* source, int64_
         }
         LSC_HIP(hipStreamSynchronize(s));
     });
+    return inner != 0 ? inner : rc;
 }
 
 int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, double* out) {

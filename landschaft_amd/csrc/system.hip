@@ -865,6 +865,11 @@ struct SystemSolver::Impl {
     template <typename T>
     void factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
     template <typename T>
+    void numeric_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
+    // the structure the plan was built for (a refactorisation must match it)
+    std::vector<i64> ptr_host;
+    std::vector<i32> adj_host;
+    template <typename T>
     void sweep_t(int k, int mode);
     template <typename T>
     void huge_forward(const HugeLevel& H, int k);
@@ -975,6 +980,25 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     std::vector<long long> ptrl(m.ptr.begin(), m.ptr.end());
     ptr = to_device(ptrl, s);
     adj = to_device(m.adj, s);
+    ptr_host = m.ptr;
+    adj_host = m.adj;
+    // the solve scratch is sized by n and the plan's update vectors: a new
+    // plan sizes it again at the next solve
+    kcap = 0;
+    analyze_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    numeric_t<T>(m, analyze_ms, factor_ms);
+}
+
+// The values of `m` (the structure of the last analysis) uploaded, permuted
+// and scaled, then the numeric factorisation.
+template <typename T>
+void SystemSolver::Impl::numeric_t(const HostMatrix& m, double& analyze_ms, double& factor_ms) {
+    const auto h0 = std::chrono::steady_clock::now();
+    g_sweep.clear();
+    ++sweep_gen;
+    const Plan& h = plan.host;
+    const DPlan& P = plan.d;
+    const int H = P.H, n = P.n;
     diag = to_device(m.diag, s);
     off = to_device(m.off.empty() ? std::vector<double>{0.0} : m.off, s);
     perm = to_device(h.perm, s);
@@ -1329,6 +1353,14 @@ void SystemSolver::factor(const HostMatrix& m) {
     }
     if (impl_->precision == 1) impl_->factor_t<double>(m, analyze_ms, factor_ms);
     else impl_->factor_t<float>(m, analyze_ms, factor_ms);
+}
+
+void SystemSolver::refactor(const HostMatrix& m) {
+    if (m.n != impl_->plan.host.n || m.ptr != impl_->ptr_host || m.adj != impl_->adj_host)
+        throw std::invalid_argument("a refactorisation needs the structure of the analysed matrix");
+    if (m.n <= 0) return;
+    if (impl_->precision == 1) impl_->numeric_t<double>(m, analyze_ms, factor_ms);
+    else impl_->numeric_t<float>(m, analyze_ms, factor_ms);
 }
 
 void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {

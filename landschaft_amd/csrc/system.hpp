@@ -34,6 +34,9 @@ public:
 
     // Symbolic analysis, upload and numeric factorisation of `m`.
     void factor(const HostMatrix& m);
+    // Numeric factorisation of new values on the structure (n, ptr, adj) of
+    // the last factor(): the symbolic plan is kept.
+    void refactor(const HostMatrix& m);
     // X = A^-1 B for k right-hand sides in the caller's node order (device
     // pointers, k x n row-major), refined to ||b - Ax|| / ||b|| <= tol.
     void solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);

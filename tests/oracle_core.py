@@ -70,6 +70,11 @@ class OracleCore:
                             "solved": bool(r["solved"][p])})
         return {"nodemap": nm, "results": results}
 
+    @staticmethod
+    def ascii_grid_rows(values):
+        # grid text is host code of the library (no device work): the writer is not under test here
+        return _backend.ascii_grid_rows(values)
+
     def network_advanced(self, a, b, w, m, src, gnd, policy, n_threads=0):
         from oracle import network as ON
 

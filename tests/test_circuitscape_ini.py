@@ -131,18 +131,6 @@ def test_region_polymap():
     np.testing.assert_array_equal(region_polymap(poly, 9, cells, ids, 1, 2), [8, 8, 8, 0, 0, 0, 0, 7, 0])
 
 
-def test_grid_values_are_written_as_to_chars():
-    # golden vectors from C++17's std::to_chars (tests/golden/make_to_chars_vectors.py)
-    lines = (Path(__file__).resolve().parent / "golden" / "to_chars_vectors.txt").read_text().splitlines()
-    assert len(lines) == 3000
-    for line in lines:
-        bits, text = line.split()
-        v = np.array([int(bits, 16)], dtype=np.uint64).view(np.float64)[0]
-        assert cs.shortest(v) == text, (bits, text)
-        assert float(text) == v
-    assert cs.shortest(np.nan) == "-9999"
-
-
 def test_written_ascii_grid_reads_back(tmp_path):
     a, meta = cs.read_raster(CS / "input" / "raster" / "advanced" / "1" / "cellmap5x5.asc")
     vals = np.random.default_rng(3).standard_normal(a.shape) * 10.0 ** np.arange(a.size).reshape(a.shape)

@@ -1,0 +1,297 @@
+"""The benchmark workloads on the GPU against the FP64 oracle: live where the oracle solves them in
+seconds (a 448 x 448 crop of the 4096 x 4096 Omniscape map at radius 50), from the oracle's outputs in
+tests/golden/workloads.npz at full size (2048 x 2048 pairwise, 1024 x 1024 gradient, a 1024 x 1024
+crop of the 8192 x 8192 map all-to-one), and through size-independent properties at full size
+(effective resistance is a metric, central differences of J, Kirchhoff's current law, mixed against
+FP64). Each test prints what it runs before it runs it."""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import pytest
+
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def backend():
+    from landschaft_amd.circuit import _backend
+
+    saved = (_backend.settings.precision, _backend.settings.tol)
+    yield _backend
+    _backend.settings.precision, _backend.settings.tol = saved
+
+
+@pytest.fixture(scope="module")
+def golden():
+    z = np.load(W.GOLDEN)
+    return z
+
+
+def _check_inputs(z, key, *arrays):
+    assert str(z[f"{key}/input_sha"]) == W.digest(*arrays), \
+        f"the {key} inputs differ from those the golden file was made from (numpy's FFT or generator changed)"
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# ---- Omniscape, radius 50, block 5, correction on: the windows of the headline map
+
+@pytest.fixture(scope="module")
+def omni(oracle):
+    r, s = W.omniscape_crop()
+    t0 = time.perf_counter()
+    ref = oracle.omniscape(r, s, 50, 0, 5, False, False, True, False)
+    _say(f"oracle omniscape 448^2 radius 50: {ref['n_targets']} windows, {time.perf_counter() - t0:.1f} s")
+    return r, s, ref
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("tol", [1e-10, 1e-12])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_omniscape_radius50_matches_oracle(backend, omni, precision, tol):
+    """Every window path of the benchmark (fronts up to 150 rows on the workgroup kernels, 11 levels,
+    the lane substitutions with M = 48 / 64) against the oracle, at the benchmark's residual bar and the
+    library default, with the reference's Omniscape tolerance (tests/test_omniscape.py:157)."""
+    from landschaft_amd import _native as N
+
+    plan = N.window_plan_stats(50)
+    assert plan["max_front"] > 64 and plan["levels"] >= 10  # the large-front kernels are reached
+    r, s, ref = omni
+    backend.settings.precision, backend.settings.tol = precision, tol
+    _say(f"omniscape 448^2 radius 50 block 5 {precision} tol {tol}")
+    out = backend.circuit_omniscape(r, s, 50, 0, 5, False, False, True, False)
+    st = dict(backend.last_stats)
+    assert out["n_targets"] == ref["n_targets"]
+    # Every window meets the benchmark's bar. Below it lies the FP64 floor of these windows' residual
+    # (r = exp(2 z) spans 1e5 within a window): the FP64 direct solve's own residual is 2e-12 on average
+    # and 7e-12 at worst on this crop, so at tol 1e-12 refinement stops when it stalls there.
+    assert st["n_over_1e10"] == 0 and st["resid_max"] <= max(tol, 1e-10), st
+    if tol >= 1e-10:
+        assert st["n_unconverged"] == 0, st
+    got, exp = out["current"], ref["current"]
+    top = np.nanmax(exp)
+    np.testing.assert_array_equal(got == 0, exp == 0)
+    np.testing.assert_allclose(got, exp, rtol=1e-9, atol=1e-13 * top)
+    _say(f"  max rel diff {np.max(np.abs(got - exp)) / top:.2e}, resid max {st['resid_max']:.2e} "
+         f"mean {st['resid_mean']:.2e}, refinement mean {st['refine_mean']:.2f}")
+
+
+def test_correction_radius50_matches_oracle(backend, oracle):
+    backend.settings.precision, backend.settings.tol = "mixed", 1e-12
+    for four in (False, True):
+        got = backend.circuit_omniscape_correction(50, 0, 5, four, 448, 448)
+        ref = oracle.omniscape_correction(50, 0, 5, four, 448, 448)
+        np.testing.assert_allclose(got, ref, rtol=1e-9)
+
+
+# ---- pairwise, 2048 x 2048, 4 neighbours, 16 focal cells
+
+@pytest.fixture(scope="module")
+def pairwise(golden):
+    g, cells = W.pairwise_2048()
+    _check_inputs(golden, "pairwise2048", g, cells)
+    return g, cells
+
+
+def _metric_checks(R):
+    n = R.shape[0]
+    np.testing.assert_array_equal(np.diag(R), 0.0)
+    np.testing.assert_allclose(R, R.T, rtol=1e-10)
+    off = ~np.eye(n, dtype=bool)
+    assert np.all(R[off] > 0)
+    # effective resistance is a metric: R_ij <= R_ik + R_kj
+    slack = R[:, :, None] + R[None, :, :] - R[:, None, :]  # [i, k, j]: R_ik + R_kj - R_ij
+    assert slack.min() >= -1e-10 * R.max()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-10), ("mixed", 1e-12), ("fp64", 1e-12)])
+def test_pairwise_2048_matches_oracle(backend, pairwise, golden, precision, tol):
+    """The shortcut (15 solves) and all 120 pairs with the cumulative map against the oracle's 120
+    per-pair FP64 solves (Circuitscape's pairwise tolerance, rtol 1e-8, reference
+    tests/test_circuitscape.py:131-229; here 1e-9), plus the metric properties at full size."""
+    g, cells = pairwise
+    n = cells.size
+    ex = np.zeros((n, n), np.uint8)
+    backend.settings.precision, backend.settings.tol = precision, tol
+    _say(f"pairwise 2048^2 shortcut {precision} tol {tol}")
+    Rs = backend.circuitscape_pairwise(g, None, cells, ex, False, True, {})["resistances"]
+    assert backend.last_stats["resid_max"] <= tol
+    _say(f"pairwise 2048^2 all pairs + cum map {precision} tol {tol}")
+    out = backend.circuitscape_pairwise(g, None, cells, ex, False, True, {"cum": True})
+    assert backend.last_stats["resid_max"] <= tol and backend.last_stats["n_solved"] == 120
+    R0 = golden["pairwise2048/R"]
+    for R in (Rs, out["resistances"]):
+        _metric_checks(R)
+        np.testing.assert_allclose(R, R0, rtol=1e-9)
+    d = W.assert_summary_close(out["cum"], golden, "pairwise2048/cum", rtol=1e-9, atol_frac=1e-12)
+    np.testing.assert_allclose(out["cum"].ravel()[cells], golden["pairwise2048/cum_focal"], rtol=1e-9)
+    _say(f"  R max rel diff {np.max(np.abs(out['resistances'] - R0)[R0 > 0] / R0[R0 > 0]):.2e}, "
+         f"cum sample max diff {d:.2e} of max")
+
+
+# ---- gradient, 1024 x 1024, 8 neighbours, 32 pairs
+
+@pytest.fixture(scope="module")
+def adjoint_inputs(golden):
+    r, src, dst = W.adjoint_1024()
+    _check_inputs(golden, "adjoint1024", r, src, dst)
+    return r, src, dst
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_adjoint_1024_matches_oracle(backend, adjoint_inputs, golden, precision):
+    r, src, dst = adjoint_inputs
+    backend.settings.precision, backend.settings.tol = precision, 1e-12
+    _say(f"adjoint 1024^2 32 pairs {precision}")
+    J, R, grad = backend.effective_resistance_grad(r, src, dst)
+    np.testing.assert_allclose(R, golden["adjoint1024/R"], rtol=1e-9)
+    np.testing.assert_allclose(J, float(golden["adjoint1024/J"]), rtol=1e-10)
+    d = W.assert_summary_close(grad, golden, "adjoint1024/grad", rtol=1e-8, atol_frac=1e-10)
+    assert np.all(grad >= 0)  # Rayleigh monotonicity
+    _say(f"  J {J:.15g} (oracle {float(golden['adjoint1024/J']):.15g}), grad sample diff {d:.2e} of max")
+
+
+def _fd_cells(r, src, dst, count, seed):
+    """count seeded cells within 3 cells of the pairs' ends (where dJ/dr is large enough for a central
+    difference in FP64 to resolve), in the circuit."""
+    ny, nx = r.shape
+    rng = np.random.default_rng(seed)
+    ends = np.concatenate([src, dst])
+    out = []
+    while len(out) < count:
+        e = int(ends[len(out) % ends.size])
+        y = min(max(e // nx + int(rng.integers(-3, 4)), 0), ny - 1)
+        x = min(max(e % nx + int(rng.integers(-3, 4)), 0), nx - 1)
+        if np.isfinite(r[y, x]) and y * nx + x not in out:
+            out.append(y * nx + x)
+    return np.array(out)
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_1024_central_differences(backend, adjoint_inputs):
+    """dJ/dr from the adjoint kernel against central differences (h = 1e-4 r) at 64 seeded cells: J at
+    r +- h from FP64 solves on the same circuit refactorised with the perturbed values (the symbolic
+    plans kept), refined to the limit of FP64."""
+    import torch
+
+    from landschaft_amd import _native as N
+
+    r, src, dst = adjoint_inputs
+    ny, nx = r.shape
+    backend.settings.precision, backend.settings.tol = "fp64", 1e-12
+    J0, _, grad = backend.effective_resistance_grad(r, src, dst)
+    backend.settings.tol = 1e-15
+    cells = _fd_cells(r, src, dst, 64, 1104)
+    o = backend._opts()
+    h = ctypes.c_void_p()
+    N.check(N.lib().lsc_grid_circuit_create(N.ptr(r), nx, ny, 0, ctypes.byref(o), ctypes.byref(h)))
+    k = src.size
+    rhs = torch.zeros((k, ny, nx), dtype=torch.float64, device="cuda")
+    idx = torch.arange(k, device="cuda")
+    rhs.view(k, -1)[idx, torch.as_tensor(src, device="cuda")] += 1.0
+    rhs.view(k, -1)[idx, torch.as_tensor(dst, device="cuda")] -= 1.0
+    V = torch.empty_like(rhs)
+    s_t, d_t = torch.as_tensor(src, device="cuda"), torch.as_tensor(dst, device="cuda")
+
+    def J_of(rr):
+        N.check(N.lib().lsc_grid_circuit_update(h, N.ptr(rr)))
+        N.check(N.lib().lsc_grid_circuit_solve(h, k, ctypes.c_void_p(rhs.data_ptr()), ctypes.c_void_p(V.data_ptr())))
+        flat = V.view(k, -1)
+        return float((flat[idx, s_t] - flat[idx, d_t]).sum())
+
+    _say(f"central differences at {cells.size} cells (two refactorisations each)")
+    try:
+        assert abs(J_of(r) - J0) <= 1e-10 * J0
+        fd = np.empty(cells.size)
+        for i, c in enumerate(cells):
+            hc = 1e-4 * r.flat[c]
+            rp, rm = r.copy(), r.copy()
+            rp.flat[c] += hc
+            rm.flat[c] -= hc
+            fd[i] = (J_of(rp) - J_of(rm)) / (2 * hc)
+    finally:
+        N.lib().lsc_grid_circuit_destroy(h)
+    g = grad.ravel()[cells]
+    rel = np.abs(fd - g) / np.abs(g)
+    _say(f"  FD vs adjoint: max rel diff {rel.max():.2e}, median {np.median(rel):.2e}, |grad| {np.abs(g).min():.2e}"
+         f"..{np.abs(g).max():.2e}")
+    # truncation O(h^2) ~ 1e-8 relative; FP64 rounding of J over 2h ~ 1e-15 J / 1e-4 r
+    np.testing.assert_allclose(fd, g, rtol=1e-5, atol=1e-10 * J0 / r.flat[cells].min())
+
+
+# ---- all-to-one: a 1024 x 1024 crop against the oracle; 8192 x 8192 through properties
+
+@pytest.fixture(scope="module")
+def large_map():
+    t0 = time.perf_counter()
+    r, cells = W.bench.large_graph_inputs()
+    _say(f"8192^2 map generated in {time.perf_counter() - t0:.1f} s")
+    return r, cells
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_all_to_one_1024_crop_matches_oracle(backend, large_map, golden, precision):
+    r, pm = W.large_graph_crop(large_map[0])
+    _check_inputs(golden, "alltoone1024", r, pm)
+    backend.settings.precision, backend.settings.tol = precision, 1e-12
+    _say(f"all-to-one 1024^2 crop, 64 points, {precision}")
+    res = W.all_to_one(r, pm)
+    d = W.assert_summary_close(res.cum_current, golden, "alltoone1024/cum", rtol=1e-9, atol_frac=1e-12)
+    _say(f"  cum sample max diff {d:.2e} of max, resid max {backend.last_stats['resid_max']:.2e}")
+
+
+@pytest.mark.timeout(900)
+def test_all_to_one_8192_properties(backend, large_map):
+    """The full 8192 x 8192 map (60 M unknowns), 64 points, all-to-one: every circuit at the residual
+    bar, mixed against FP64 on the cumulative map, and Kirchhoff's current law on one problem solved
+    alone: the current into the grounded point equals the amps injected in its component."""
+    from scipy import ndimage
+
+    r, cells = large_map
+    ny, nx = r.shape
+    pm = W.points_map(r.shape, cells)
+    cums = {}
+    for precision in ("mixed", "fp64"):
+        backend.settings.precision, backend.settings.tol = precision, 1e-10
+        _say(f"all-to-one 8192^2, 64 points, {precision}")
+        t0 = time.perf_counter()
+        res = W.all_to_one(r, pm)
+        st = dict(backend.last_stats)
+        _say(f"  {time.perf_counter() - t0:.1f} s, resid max {st['resid_max']:.2e}, solved {st['n_solved']}")
+        assert st["n_unconverged"] == 0 and st["resid_max"] <= 1e-10, st
+        cums[precision] = res.cum_current
+    top = np.nanmax(cums["fp64"])
+    diff = float(np.nanmax(np.abs(cums["mixed"] - cums["fp64"])) / top)
+    _say(f"  cum mixed vs fp64: {diff:.2e} of max")
+    assert diff <= 1e-8
+    # Kirchhoff on one problem: point 1 grounded, 1 A at every other point of its component
+    lab, _ = ndimage.label(np.isfinite(r))
+    comp = lab.flat[cells]
+    gcell = int(cells[0])
+    srcs = cells[1:][comp[1:] == comp[0]]
+    assert srcs.size > 10
+    backend.settings.precision, backend.settings.tol = "mixed", 1e-10
+    g = np.where(np.isfinite(r), 1.0 / np.where(np.isfinite(r), r, 1.0), 0.0)
+    prob = {"src_cells": srcs.astype(np.int64), "src_vals": np.ones(srcs.size), "gnd_cells": np.array([gcell]),
+            "gnd_vals": np.array([np.inf]), "check_cell": gcell}
+    _say("Kirchhoff: one problem on the 8192^2 graph")
+    out = backend.circuitscape_multi_advanced(g, None, False, True, [prob], "rmvsrc")
+    res0 = out["results"][0]
+    nm = out["nodemap"]
+    assert res0["solved"]
+    inflow = res0["current"][nm.flat[gcell]]
+    _say(f"  current into the ground {inflow:.15g}, injected {srcs.size}")
+    np.testing.assert_allclose(inflow, srcs.size, rtol=1e-9)
+    assert res0["voltage"][nm.flat[gcell]] == 0.0
+    assert np.all(res0["voltage"][nm.flat[srcs]] > 0)
