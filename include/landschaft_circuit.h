@@ -76,7 +76,9 @@ typedef struct lsc_solve_stats {
     double ms_fill, ms_flood, ms_factor, ms_solve, ms_residual, ms_currents, ms_accumulate;
     int32_t n_streams; /* Omniscape: batches in flight at once, each on its own stream (the
                           ms_ phases then add up the streams' overlapping times) */
-    int32_t reserved;
+    int32_t n_fallback; /* mixed precision: circuits (Omniscape windows) or factorisations (graphs)
+                           solved again in FP64 because the FP32 factor met a non-positive pivot or
+                           its refinement stalled more than 1000 times above tol */
 } lsc_solve_stats;
 
 /* Message of the calling thread's last failure ("" if none). */

@@ -609,6 +609,7 @@ void tally(Ctx& cx, const SystemSolver& S) {
     cx.st.ms_fill += S.analyze_ms;
     cx.st.ms_factor += S.factor_ms;
     cx.st.ms_solve += S.solve_ms;
+    cx.st.fallback += S.fallbacks();
 }
 
 Ctx make_ctx(const lsc_solve_options* o) {
@@ -1086,6 +1087,7 @@ void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
     out->refine_mean = cx.st.solved ? static_cast<double>(cx.st.iter_sum) / cx.st.solved : 0.0;
     out->resid_max = cx.st.resid_max;
     out->resid_mean = cx.st.solved ? cx.st.resid_sum / cx.st.solved : 0.0;
+    out->n_fallback = cx.st.fallback;
 }
 
 // ---- pairwise (reference raster_pairwise, circuitscape.cpp:474-743)

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <thread>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/landschaft_circuit.h"
@@ -187,6 +188,10 @@ struct WindowSolver {
     // per level: rows M of the one-window-per-lane substitutions (0: the
     // one-front-per-wavefront kernels), forward and backward
     std::vector<int> lane_fwd, lane_bwd;
+    // mixed precision: the FP64 solver of the windows whose FP32 factor
+    // failed or whose refinement stalled (made on first use)
+    std::unique_ptr<WindowSolver<double>> fb;
+    int fb_cap = 0;
 
     explicit WindowSolver(hipStream_t st) : s(st) {}
 
@@ -327,6 +332,9 @@ struct WindowSolver {
                            wb.active.p, wb.done.p, wb.sc.p, wb.z.p, wb.x.p);
     }
 
+    void fallback_fp64(const std::vector<int>& bad, double tol, int maxit, hipEvent_t* ev, std::vector<double>& rn,
+                       std::vector<int>& its, std::vector<int>& fl, IrStats& st);
+
     void solve(int nb, double tol, int maxit, IrStats& st, hipEvent_t* ev, bool timing) {
         const DPlan& P = plan.d;
         const bool gr = graphs_enabled();
@@ -378,6 +386,15 @@ struct WindowSolver {
         LSC_HIP(hipMemcpyAsync(its.data(), wb.iters.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipMemcpyAsync(rn.data(), wb.rnorm.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
         LSC_HIP(hipStreamSynchronize(s));
+        if constexpr (std::is_same<T, float>::value) {
+            // A window whose FP32 factor met a non-positive pivot, or whose refinement ended more than
+            // kFallbackGap times above tol (failed, rather than stopped at the FP64 floor of its
+            // residual), is solved again in FP64; its solution replaces the mixed one.
+            std::vector<int> bad;
+            for (int k = 0; k < nb; ++k)
+                if (act[k] && (fl[k] || !(rn[k] <= kFallbackGap * tol))) bad.push_back(k);
+            if (!bad.empty()) fallback_fp64(bad, tol, maxit, ev, rn, its, fl, st);
+        }
         for (int k = 0; k < nb; ++k) {
             if (!act[k]) continue;
             if (fl[k]) throw std::runtime_error("an Omniscape window is not positive definite");
@@ -391,6 +408,53 @@ struct WindowSolver {
         }
     }
 };
+
+template <typename T>
+void WindowSolver<T>::fallback_fp64(const std::vector<int>& bad, double tol, int maxit, hipEvent_t* ev,
+                                    std::vector<double>& rn, std::vector<int>& its, std::vector<int>& fl,
+                                    IrStats& st) {
+    const int W2 = plan.d.W * plan.d.W, n = plan.d.n;
+    const int nbad = static_cast<int>(bad.size());
+    for (int c0 = 0; c0 < nbad; c0 += Bcap) {  // in chunks of at most the batch size
+        const int m = std::min(Bcap, nbad - c0);
+        if (!fb || fb_cap < m) {
+            fb_cap = std::max(m, std::min(Bcap, 256));
+            fb.reset(new WindowSolver<double>(s));
+            fb->set_plan(Plan(plan.host), fb_cap);
+        }
+        auto& w = fb->wb;
+        for (int j = 0; j < m; ++j) {
+            const std::size_t k = static_cast<std::size_t>(bad[c0 + j]);
+            LSC_HIP(hipMemcpyAsync(w.g.p + static_cast<std::size_t>(j) * W2, wb.g.p + k * W2, W2 * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+            LSC_HIP(hipMemcpyAsync(w.src.p + static_cast<std::size_t>(j) * W2, wb.src.p + k * W2,
+                                   W2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        const std::vector<int> ones(m, 1);
+        LSC_HIP(hipMemcpyAsync(w.active.p, ones.data(), m * sizeof(int), hipMemcpyHostToDevice, s));
+        IrStats fst;  // the windows are counted once, by the caller
+        fb->solve(m, tol, maxit, fst, ev, false);
+        std::vector<int> fact(m), fits(m);
+        std::vector<double> frn(m);
+        LSC_HIP(hipMemcpyAsync(fact.data(), w.active.p, m * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(fits.data(), w.iters.p, m * sizeof(int), hipMemcpyDeviceToHost, s));
+        LSC_HIP(hipMemcpyAsync(frn.data(), w.rnorm.p, m * sizeof(double), hipMemcpyDeviceToHost, s));
+        for (int j = 0; j < m; ++j) {
+            const std::size_t k = static_cast<std::size_t>(bad[c0 + j]);
+            LSC_HIP(hipMemcpyAsync(wb.x.p + k * n, w.x.p + static_cast<std::size_t>(j) * n, n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        LSC_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < m; ++j) {
+            const int k = bad[c0 + j];
+            // the same window floods to the same component: it stays active
+            if (!fact[j]) throw std::logic_error("a window's FP64 solve dropped it");
+            rn[k] = frn[j], its[k] += fits[j], fl[k] = 0;
+        }
+        st.fallback += m;
+        st.solve_launches += fst.solve_launches;
+    }
+}
 
 struct OmniArgs {
     int radius, buffer, block, four, rc, correct, fp, precision;
@@ -642,6 +706,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
         st.ms_fill += q.ms_fill, st.ms_flood += q.ms_flood, st.ms_factor += q.ms_factor;
         st.ms_solve += q.ms_solve, st.ms_residual += q.ms_residual, st.ms_currents += q.ms_currents;
         st.ms_accum += q.ms_accum;
+        st.fallback += q.fallback;
     }
 }
 
@@ -805,6 +870,7 @@ void fill_stats(const lsc::IrStats& st, long long n_targets, lsc_solve_stats* ou
     out->ms_solve = st.ms_solve, out->ms_residual = st.ms_residual, out->ms_currents = st.ms_currents;
     out->ms_accumulate = st.ms_accum;
     out->n_streams = st.streams;
+    out->n_fallback = st.fallback;
 }
 
 // The device entry with the condition maps on the host (`cond_on_device`

@@ -865,7 +865,10 @@ struct SystemSolver::Impl {
     template <typename T>
     void factor_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
     template <typename T>
-    void numeric_t(const HostMatrix& m, double& analyze_ms, double& factor_ms);
+    void numeric_t(const HostMatrix* m, double& analyze_ms, double& factor_ms);
+    // FP64 in place of a failed FP32 factor: the same plan and values, L in FP64 from now on
+    void to_fp64(double& analyze_ms, double& factor_ms);
+    int fallbacks = 0;  // FP64 refactorisations after a failed or stalled FP32 factor
     // the structure the plan was built for (a refactorisation must match it)
     std::vector<i64> ptr_host;
     std::vector<i32> adj_host;
@@ -897,7 +900,8 @@ struct SystemSolver::Impl {
     template <typename T>
     void wide_backward(const HugeLevel& H, int k);
     template <typename T>
-    void solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st);
+    void solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st, std::vector<double>& rn,
+                 std::vector<int>& its);
 };
 
 template <typename T>
@@ -986,26 +990,28 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     // plan sizes it again at the next solve
     kcap = 0;
     analyze_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-    numeric_t<T>(m, analyze_ms, factor_ms);
+    numeric_t<T>(&m, analyze_ms, factor_ms);
 }
 
 // The values of `m` (the structure of the last analysis) uploaded, permuted
 // and scaled, then the numeric factorisation.
 template <typename T>
-void SystemSolver::Impl::numeric_t(const HostMatrix& m, double& analyze_ms, double& factor_ms) {
+void SystemSolver::Impl::numeric_t(const HostMatrix* m, double& analyze_ms, double& factor_ms) {
     const auto h0 = std::chrono::steady_clock::now();
     g_sweep.clear();
     ++sweep_gen;
     const Plan& h = plan.host;
     const DPlan& P = plan.d;
     const int H = P.H, n = P.n;
-    diag = to_device(m.diag, s);
-    off = to_device(m.off.empty() ? std::vector<double>{0.0} : m.off, s);
+    if (m) {  // new values (m == nullptr: the values already on the device)
+        diag = to_device(m->diag, s);
+        off = to_device(m->off.empty() ? std::vector<double>{0.0} : m->off, s);
+    }
     perm = to_device(h.perm, s);
     iperm = to_device(h.iperm, s);
     {
         std::vector<long long> ep(static_cast<std::size_t>(n) + 1, 0);
-        for (int j = 0; j < n; ++j) ep[j + 1] = ep[j] + (m.ptr[h.perm[j] + 1] - m.ptr[h.perm[j]]);
+        for (int j = 0; j < n; ++j) ep[j + 1] = ep[j] + (ptr_host[h.perm[j] + 1] - ptr_host[h.perm[j]]);
         eptr = to_device(ep, s);
         eadj.alloc(ep[n] ? ep[n] : 1);
         eoff.alloc(ep[n] ? ep[n] : 1);
@@ -1091,7 +1097,7 @@ void SystemSolver::Impl::numeric_t(const HostMatrix& m, double& analyze_ms, doub
     factor_ms += fms;
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
-    if (h_fail) throw std::runtime_error("a circuit component is not positive definite");
+    if (h_fail) throw NotPositiveDefinite("a circuit component is not positive definite");
     // the update matrices are no longer needed
     U.release();
     A.release();
@@ -1238,7 +1244,8 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
 // converges when the FP32 factor of a badly conditioned Laplacian is only a
 // rough inverse.
 template <typename T>
-void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
+void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st,
+                                 std::vector<double>& rn, std::vector<int>& its) {
     const DPlan& P = plan.d;
     const int n = P.n;
     if (k <= 0 || n == 0) return;
@@ -1318,20 +1325,19 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
         progress("solve of %d right-hand sides: iteration %d, %d open", k, it + 1, h_open);
     }
     hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(kn, 256)), dim3(256), 0, s, n, k, perm.p, xb.p, d_x);
-    std::vector<int> its(k);
-    std::vector<double> rn(k);
+    its.assign(k, 0);
+    rn.assign(k, 0.0);
     LSC_HIP(hipMemcpyAsync(its.data(), iters.p, k * sizeof(int), hipMemcpyDeviceToHost, s));
     LSC_HIP(hipMemcpyAsync(rn.data(), rnorm.p, k * sizeof(double), hipMemcpyDeviceToHost, s));
     LSC_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < k; ++i) {
-        ++st.solved;
-        st.iter_sum += its[i];
-        st.iter_max = std::max(st.iter_max, its[i]);
-        st.resid_max = std::max(st.resid_max, rn[i]);
-        st.resid_sum += rn[i];
-        if (!(rn[i] <= tol)) ++st.unconverged;
-        if (!(rn[i] <= 1e-10)) ++st.over_bar;
-    }
+}
+
+void SystemSolver::Impl::to_fp64(double& analyze_ms, double& factor_ms) {
+    progress("FP64 refactorisation of %d unknowns after the FP32 factor failed or stalled", plan.d.n);
+    precision = 1;
+    ++fallbacks;
+    L.release();
+    numeric_t<double>(nullptr, analyze_ms, factor_ms);
 }
 
 SystemSolver::SystemSolver(int precision, hipStream_t s) : impl_(new Impl) {
@@ -1351,17 +1357,33 @@ void SystemSolver::factor(const HostMatrix& m) {
         impl_->plan = DevPlan{};
         return;
     }
-    if (impl_->precision == 1) impl_->factor_t<double>(m, analyze_ms, factor_ms);
-    else impl_->factor_t<float>(m, analyze_ms, factor_ms);
+    if (impl_->precision == 1) {
+        impl_->factor_t<double>(m, analyze_ms, factor_ms);
+        return;
+    }
+    try {
+        impl_->factor_t<float>(m, analyze_ms, factor_ms);
+    } catch (const NotPositiveDefinite&) {
+        impl_->to_fp64(analyze_ms, factor_ms);  // throws again if the matrix is not positive definite
+    }
 }
 
 void SystemSolver::refactor(const HostMatrix& m) {
     if (m.n != impl_->plan.host.n || m.ptr != impl_->ptr_host || m.adj != impl_->adj_host)
         throw std::invalid_argument("a refactorisation needs the structure of the analysed matrix");
     if (m.n <= 0) return;
-    if (impl_->precision == 1) impl_->numeric_t<double>(m, analyze_ms, factor_ms);
-    else impl_->numeric_t<float>(m, analyze_ms, factor_ms);
+    if (impl_->precision == 1) {
+        impl_->numeric_t<double>(&m, analyze_ms, factor_ms);
+        return;
+    }
+    try {
+        impl_->numeric_t<float>(&m, analyze_ms, factor_ms);
+    } catch (const NotPositiveDefinite&) {
+        impl_->to_fp64(analyze_ms, factor_ms);
+    }
 }
+
+int SystemSolver::fallbacks() const { return impl_->fallbacks; }
 
 void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double tol, int maxit, IrStats& st) {
     if (impl_->plan.host.n == 0) return;
@@ -1369,8 +1391,51 @@ void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double to
     LSC_HIP(hipEventCreate(&a));
     LSC_HIP(hipEventCreate(&b));
     LSC_HIP(hipEventRecord(a, impl_->s));
-    if (impl_->precision == 1) impl_->solve_t<double>(k, d_b, d_x, tol, maxit, st);
-    else impl_->solve_t<float>(k, d_b, d_x, tol, maxit, st);
+    std::vector<double> rn;
+    std::vector<int> its;
+    if (impl_->precision == 1) {
+        impl_->solve_t<double>(k, d_b, d_x, tol, maxit, st, rn, its);
+    } else {
+        impl_->solve_t<float>(k, d_b, d_x, tol, maxit, st, rn, its);
+        // A right-hand side whose FP32-preconditioned iteration ended far above tol (more than
+        // kFallbackGap times) has failed rather than met the FP64 floor of the residual: the solver
+        // refactorises in FP64 (for this and every later solve) and solves those again, keeping the
+        // better of the two solutions.
+        std::vector<int> bad;
+        for (int i = 0; i < k; ++i)
+            if (!(rn[i] <= kFallbackGap * tol)) bad.push_back(i);
+        if (!bad.empty()) {
+            impl_->to_fp64(analyze_ms, factor_ms);
+            const int kb = static_cast<int>(bad.size());
+            const std::size_t n = static_cast<std::size_t>(impl_->plan.d.n);
+            DevBuf<double> b2(kb * n), x2(kb * n);
+            for (int j = 0; j < kb; ++j)
+                LSC_HIP(hipMemcpyAsync(b2.p + j * n, d_b + bad[j] * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                       impl_->s));
+            std::vector<double> rn2;
+            std::vector<int> its2;
+            impl_->solve_t<double>(kb, b2.p, x2.p, tol, maxit, st, rn2, its2);
+            for (int j = 0; j < kb; ++j) {
+                const int i = bad[j];
+                its[i] += its2[j];
+                if (rn2[j] < rn[i] || !(rn[i] == rn[i])) {
+                    LSC_HIP(hipMemcpyAsync(d_x + i * n, x2.p + j * n, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                           impl_->s));
+                    rn[i] = rn2[j];
+                }
+            }
+            LSC_HIP(hipStreamSynchronize(impl_->s));
+        }
+    }
+    for (int i = 0; i < k; ++i) {
+        ++st.solved;
+        st.iter_sum += its[i];
+        st.iter_max = std::max(st.iter_max, its[i]);
+        st.resid_max = std::max(st.resid_max, rn[i]);
+        st.resid_sum += rn[i];
+        if (!(rn[i] <= tol)) ++st.unconverged;
+        if (!(rn[i] <= 1e-10)) ++st.over_bar;
+    }
     LSC_HIP(hipEventRecord(b, impl_->s));
     LSC_HIP(hipEventSynchronize(b));
     float ms = 0;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 
 #include "common.hpp"
@@ -13,6 +14,12 @@
 #include "window.hpp"
 
 namespace lsc {
+
+// A factorisation met a non-positive pivot (status 2, the reference's
+// message, src/cpp/circuit/grid.cpp:208).
+struct NotPositiveDefinite : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 // The matrix in the caller's node order: diagonal, and CSR off-diagonal
 // values (the negated edge conductances) over a symmetric adjacency.
@@ -44,6 +51,9 @@ public:
     void solve(int k, const double* b, double* x, double tol, int maxit, IrStats& st);
 
     int n() const;
+    // FP64 refactorisations after the FP32 factor failed (a non-positive
+    // pivot) or a mixed solve stalled: 0 or 1.
+    int fallbacks() const;
     const Plan& plan() const;
     double analyze_ms = 0;  // host symbolic analysis and upload (wall clock)
     double factor_ms = 0;   // numeric factorisation (device)

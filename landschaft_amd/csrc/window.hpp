@@ -21,6 +21,10 @@ constexpr int kSmallWaves = 1;     // wavefronts (small fronts) per workgroup: L
 constexpr int kBigTiles = 10;      // fronts of <= 160 rows: the whole workgroup, tiles in LDS
 constexpr int kBigWaves = 4;       // wavefronts per large-front workgroup (the panel needs 16 + 48 kBigWaves >= 160)
 static_assert(16 + 48 * kBigWaves >= kBigTiles * 16, "the panel's rows must fit the large-front team");
+// A mixed-precision solve that ends more than this many times above its
+// tolerance has failed (FP32-preconditioned refinement that stalls at the
+// FP64 floor of the residual ends within it): it is solved again in FP64.
+constexpr double kFallbackGap = 1000.0;
 constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
 
 // Device copy of a Plan plus the per-level chunking of the solve kernels.
@@ -86,6 +90,7 @@ struct IrStats {
     int batches = 0;
     int iter_max = 0;
     int streams = 1;  // batches in flight at once
+    int fallback = 0; // circuits (graphs: factorisations) solved again in FP64 after the FP32 factor failed or stalled
     double resid_max = 0, resid_sum = 0;
     double ms_fill = 0, ms_flood = 0, ms_factor = 0, ms_solve = 0, ms_residual = 0, ms_currents = 0,
            ms_accum = 0;

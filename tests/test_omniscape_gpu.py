@@ -160,3 +160,31 @@ def test_lane_substitutions_match_per_front_kernels(backend, oracle, precision, 
         top = np.nanmax(np.abs(ref[k]))
         np.testing.assert_allclose(maps["1"][k], maps["0"][k], rtol=1e-11, atol=1e-13 * top)
         np.testing.assert_allclose(maps["1"][k], ref[k], rtol=1e-9, atol=1e-12 * top)
+
+
+@pytest.mark.parametrize("contrast,all_windows", [(1e6, False), (1e10, True)])
+def test_fp64_fallback_of_high_contrast_windows(backend, oracle, contrast, all_windows):
+    """Corridors of conductance `contrast` times the background's: where the FP32 factor's refinement
+    stalls (part of the windows at 1e6, every window at 1e10) the window is solved again in FP64
+    (n_fallback); the map matches the FP64 path and the oracle."""
+    rng = np.random.default_rng(3)
+    r = np.full((60, 80), np.sqrt(contrast))
+    r[:, 38:42] = 1 / np.sqrt(contrast)
+    r[28:32, :] = 1 / np.sqrt(contrast)
+    r *= np.exp(rng.normal(0, 0.3, r.shape))
+    s = 1 / r
+    ref = oracle.omniscape(r, s, 10, 0, 3, False, False, True, False)
+    out = {}
+    for precision in ("mixed", "fp64"):
+        backend.settings.precision, backend.settings.tol = precision, 1e-10
+        print(f"corridor contrast {contrast:g} {precision}", flush=True)
+        out[precision] = backend.circuit_omniscape(r, s, 10, 0, 3, False, False, True, False)["current"]
+        st = backend.last_stats
+        if precision == "mixed":
+            assert st["n_fallback"] > 0
+            assert (st["n_fallback"] == st["n_solved"]) == all_windows
+        else:
+            assert st["n_fallback"] == 0
+    top = ref["current"].max()
+    np.testing.assert_allclose(out["mixed"], out["fp64"], rtol=1e-9, atol=1e-12 * top)
+    np.testing.assert_allclose(out["mixed"], ref["current"], rtol=1e-9, atol=1e-10 * top)

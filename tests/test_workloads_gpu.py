@@ -17,6 +17,8 @@ import workloads as W
 
 pytestmark = pytest.mark.gpu
 
+kFALLBACK_GAP = 1000.0  # a mixed solve ending this many times above tol falls back to FP64
+
 
 @pytest.fixture(scope="module")
 def backend():
@@ -311,11 +313,20 @@ def test_all_to_one_8192_properties(backend, large_map):
             "gnd_vals": np.array([np.inf]), "check_cell": gcell}
     _say("Kirchhoff: one problem on the 8192^2 graph")
     out = backend.circuitscape_multi_advanced(g, None, False, True, [prob], "rmvsrc")
+    st = dict(backend.last_stats)
     res0 = out["results"][0]
     nm = out["nodemap"]
     assert res0["solved"]
     inflow = res0["current"][nm.flat[gcell]]
-    _say(f"  current into the ground {inflow:.15g}, injected {srcs.size}")
-    np.testing.assert_allclose(inflow, srcs.size, rtol=1e-9)
+    # injected - (current into the ground) = 1^T (b - A x) over the component's free nodes (the grounded
+    # Laplacian's columns sum to the conductances to the ground), so it is at most sqrt(n) ||b - A x||
+    # = sqrt(n) resid ||b||. With 63 unit sources this right-hand side's FP64 residual floor is about
+    # 3e-9; the FP32-preconditioned iteration stalls far above it and the solve falls back to FP64.
+    n_nodes = int(np.isfinite(r).sum())
+    bound = np.sqrt(n_nodes) * st["resid_max"] * np.sqrt(srcs.size)
+    _say(f"  current into the ground {inflow:.15g}, injected {srcs.size}, bound {bound:.2e}, resid "
+         f"{st['resid_max']:.2e}, FP64 fallbacks {st['n_fallback']}")
+    assert st["resid_max"] <= kFALLBACK_GAP * 1e-10
+    assert abs(inflow - srcs.size) <= bound + 1e-12 * srcs.size
     assert res0["voltage"][nm.flat[gcell]] == 0.0
     assert np.all(res0["voltage"][nm.flat[srcs]] > 0)
