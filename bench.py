@@ -117,6 +117,11 @@ def large_graph_inputs(n: int = 8192, seed: int = 105, npts: int = 64, nodata: f
 BATCH = 16384  # windows per batch
 
 
+def log(msg: str) -> None:
+    """A progress line on stderr (the JSON result line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=True):
     o = N.options(precision, tol, 30, BATCH, timing)
     st = N.SolveStats()
@@ -127,8 +132,26 @@ def run_gpu(N, d_r, d_s, d_cur, nx, ny, radius, block, precision, tol, timing=Tr
     return st.as_dict()
 
 
+def host_info() -> dict:
+    """The host's CPU model, logical CPUs and RAM (from /proc)."""
+    model, ram = platform.processor() or platform.machine(), 0
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+        for line in Path("/proc/meminfo").read_text().splitlines():
+            if line.startswith("MemTotal"):
+                ram = int(line.split()[1]) * 1024
+                break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count() or 1, "ram_gb": round(ram / 2**30, 1)}
+
+
 def cpu_baseline(r, s, radius, block, seconds: float):
-    """The FP64 CPU oracle (a port of the reference's C++) on full-size windows of the same map."""
+    """The FP64 CPU oracle (a port of the reference's C++) on full-size windows of the same map: the
+    sample sized to about `seconds` of work on all threads, best of three; one thread on a tenth of it."""
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -139,20 +162,54 @@ def cpu_baseline(r, s, radius, block, seconds: float):
     O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-64 - 32 * threads)
     t2 = time.perf_counter()
     rate = 32 * threads / max((t2 - t1) - (t1 - t0), 1e-3)
-    sample = int(min(400000, max(256, rate * seconds)))
+    sample = int(min(400000, max(256, rate * seconds / 3)))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-sample)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    one = max(64, sample // (10 * max(threads // 4, 1)))
     t0 = time.perf_counter()
-    out = O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=threads, limit=-sample)
-    dt = time.perf_counter() - t0
+    out1 = O.omniscape(r, s, radius, 0, block, False, False, True, False, n_threads=1, limit=-one)
+    dt1 = time.perf_counter() - t0
     return {"value": out["n_targets"] / dt, "unit": "circuits/s", "cores": threads, "kind": "port",
             "sample": f"{out['n_targets']} unclipped radius-{radius} windows of the same map (FP64 multifrontal, "
-                      f"{threads} OpenMP threads, {dt:.1f} s, incl. the correction solves)",
-            "cpu": platform.processor() or platform.machine()}
+                      f"{threads} OpenMP threads, best of 3: {', '.join(f'{t:.1f}' for t in times)} s, incl. the "
+                      "correction solves)",
+            "one_thread": {"value": out1["n_targets"] / dt1, "windows": out1["n_targets"], "s": round(dt1, 1)},
+            **host_info()}
+
+
+def accuracy_crop(N, radius, block, tol, size=448):
+    """The cumulative map of a size x size crop of the benchmark map (seeded offset), mixed and FP64
+    GPU, against the FP64 CPU oracle: max |difference| over the oracle's max, and the mixed path's
+    residuals."""
+    from landschaft_amd.circuit import _backend as B
+    from oracle import oracle as O
+
+    r, s = workload(4096)
+    rng = np.random.default_rng(1103)
+    y0, x0 = (int(v) for v in rng.integers(0, 4096 - size, 2))
+    r, s = np.ascontiguousarray(r[y0:y0 + size, x0:x0 + size]), np.ascontiguousarray(s[y0:y0 + size, x0:x0 + size])
+    ref = O.omniscape(r, s, radius, 0, block, False, False, True, False)["current"]
+    top = float(np.max(ref))
+    out = {"crop": f"{size}x{size} at ({y0}, {x0})"}
+    for prec in ("mixed", "fp64"):
+        B.settings.precision, B.settings.tol = prec, tol
+        cur = B.circuit_omniscape(r, s, radius, 0, block, False, False, True, False)["current"]
+        st = B.last_stats
+        out[prec] = {"cum_max_rel_err_vs_cpu_fp64": float(np.max(np.abs(cur - ref)) / top),
+                     "resid_max": st["resid_max"], "resid_median": st["resid_median"], "windows": st["n_solved"]}
+    B.settings.precision = "mixed"
+    return out
 
 
 def _phase(st):
     return {"ms_analysis": round(st["ms_fill"], 2), "ms_factor": round(st["ms_factor"], 2),
-            "ms_solve": round(st["ms_solve"], 2), "resid_max": st["resid_max"], "refine_max": st["refine_max"],
-            "refine_mean": round(st["refine_mean"], 2), "n_solves": st["n_solved"]}
+            "ms_solve": round(st["ms_solve"], 2), "resid_max": st["resid_max"], "resid_median": st["resid_median"],
+            "refine_max": st["refine_max"], "refine_mean": round(st["refine_mean"], 2), "n_solves": st["n_solved"],
+            "n_fallback": st["n_fallback"]}
 
 
 def pairwise_leg(B, n, four, npts, seed, amp, maps_too, cpu=False):
@@ -243,6 +300,44 @@ def adjoint_leg(B, n, npairs, seed):
     return out
 
 
+def large_graph_leg(B, npairs=16):
+    """The 8192 x 8192 single graph (60 M unknowns, 10 % NoData, 4 neighbours): all-to-one over 64
+    focal cells with the cumulative map, mixed and FP64, and dJ/dr of J = sum of the effective
+    resistances of npairs pairs of focal cells (one calibration step)."""
+    from landschaft_amd.circuit import circuitscape as cs
+
+    t0 = time.perf_counter()
+    r, cells = large_graph_inputs()
+    gen = time.perf_counter() - t0
+    pm = np.zeros(r.shape)
+    pm.flat[cells] = np.arange(1, cells.size + 1)
+    hab = np.where(np.isnan(r), cs.NODATA, r)
+    out = {"workload": "all-to-one 8192x8192 fractal exp(2z) f32, 10% nodata blobs, 4-neighbour, 64 focal cells, "
+                       f"cum map; dJ/dr of {npairs} focal pairs", "gen_s": round(gen, 1)}
+    cums = {}
+    for prec in ("mixed", "fp64"):
+        B.settings.precision = prec
+        t0 = time.perf_counter()
+        res = cs.circuitscape_one_to_all(hab, pm, all_to_one=True, four_neighbors=True, cum_map=True)
+        out[prec] = {"s_wall": round(time.perf_counter() - t0, 2), **_phase(B.last_stats)}
+        cums[prec] = res.cum_current
+    top = float(np.max(cums["fp64"]))
+    out["cum_max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(cums["mixed"] - cums["fp64"])) / top)
+    del cums
+    B.settings.precision = "mixed"
+    t0 = time.perf_counter()
+    from scipy import ndimage
+
+    lab, _ = ndimage.label(~np.isnan(r))  # 4-connected components of the data cells
+    same = cells[lab.flat[cells] == lab.flat[cells[0]]]
+    src, dst = same[:npairs], same[npairs:2 * npairs]
+    J, _, grad = B.effective_resistance_grad(r, src, dst, True)
+    out["grad_mixed"] = {"s_wall": round(time.perf_counter() - t0, 2), "J": J, "pairs": int(src.size),
+                         **_phase(B.last_stats),
+                         "grad_min": float(np.min(grad)), "cells": int(grad.size)}
+    return out
+
+
 def tune(a) -> int:
     """A small autotuner: every combination of the window engine's tunables on one map, best first. The
     library reads LSC_ND_LEAF and LSC_BATCH_STREAMS at each call; the batch size is a solve option."""
@@ -299,6 +394,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the pairwise and adjoint graph workloads")
+    ap.add_argument("--no-large", action="store_true", help="skip the 8192^2 single-graph workload")
     ap.add_argument("--batch", type=int, default=16384, help="Omniscape windows per batch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy a 256 MiB buffer three times (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -338,6 +434,7 @@ def main() -> int:
     tc = torch.empty((n, n), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     args = (N, tr.data_ptr(), ts.data_ptr(), tc.data_ptr(), n, n, a.radius, a.block)
+    log(f"map {n}x{n} generated in {gen_s:.1f} s; {a.warmup} warm-up and {a.steps} timed steps")
     for _ in range(a.warmup):
         run_gpu(*args, "mixed", a.tol)
     torch.cuda.synchronize()
@@ -414,6 +511,7 @@ def main() -> int:
         roofline["single_stream"] = {k: r1[k] for k in ("kernel", "bound", "achieved", "frac", "traffic")}
         roofline["single_stream"]["phase_ms"] = {k[3:]: round(v, 2) for k, v in st1.items() if k.startswith("ms_")}
 
+    log(f"mixed: {value:.0f} circuits/s")
     fp64 = None
     if not a.no_fp64:
         torch.cuda.synchronize()
@@ -425,16 +523,27 @@ def main() -> int:
         top = np.nanmax(np.abs(cum64))
         fp64 = {"value": st64["n_targets"] / d64, "ms": d64 * 1e3, "resid_max": st64["resid_max"],
                 "cum_max_rel_diff_vs_mixed": float(np.nanmax(np.abs(cum_mixed - cum64)) / top)}
+    log("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline(r, s, a.radius, a.block, a.cpu_seconds)
     graph = None
     if not a.no_graph:
         from landschaft_amd.circuit import _backend as B
 
         B.settings.tol = a.tol
-        graph = {"pairwise_256": pairwise_leg(B, 256, False, 4, 101, None, True, cpu=not a.no_cpu),
-                 "pairwise_2048": pairwise_leg(B, 2048, True, 16, 102, 2.0, True),
-                 "adjoint_1024": adjoint_leg(B, 1024, 32, 104),
-                 "network_1024": network_leg(B, 1024, 8, 105, cpu=not a.no_cpu)}
+        graph = {}
+        log("graph workloads: pairwise 256^2")
+        graph["pairwise_256"] = pairwise_leg(B, 256, False, 4, 101, None, True, cpu=not a.no_cpu)
+        log("pairwise 2048^2")
+        graph["pairwise_2048"] = pairwise_leg(B, 2048, True, 16, 102, 2.0, True)
+        log("adjoint 1024^2")
+        graph["adjoint_1024"] = adjoint_leg(B, 1024, 32, 104)
+        log("network 1024^2")
+        graph["network_1024"] = network_leg(B, 1024, 8, 105, cpu=not a.no_cpu)
+        if not a.no_large:
+            log("large graph 8192^2")
+            graph["large_8192"] = large_graph_leg(B)
+    log("accuracy crop against the CPU oracle")
+    accuracy = None if a.no_cpu else accuracy_crop(N, a.radius, a.block, a.tol)
     line = {
         "metric": "circuit solves/sec at 4096^2 @ rel resid <=1e-10 (Omniscape windows, mixed-precision GPU)",
         "value": round(value, 2), "unit": "circuits/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -443,7 +552,8 @@ def main() -> int:
         "config": {"workload": f"omniscape {n}x{n} fractal exp(2z) f32, 30% nodata blobs, radius {a.radius}, "
                                f"block {a.block}, 8-neighbour, correction on",
                    "radius": a.radius, "block_size": a.block, "tol": a.tol, "seed": 103},
-        "n_targets": st["n_targets"], "resid_max": st["resid_max"], "n_over_1e10": st["n_over_1e10"],
+        "n_targets": st["n_targets"], "resid_max": st["resid_max"], "resid_median": st["resid_median"],
+        "n_over_1e10": st["n_over_1e10"], "n_fallback": st["n_fallback"], "accuracy_vs_cpu_fp64": accuracy,
         "refine_mean": round(st["refine_mean"], 3), "refine_max": st["refine_max"],
         "phase_ms": {k[3:]: round(v, 2) for k, v in st.items() if k.startswith("ms_")},
         "kernels": kernels, "roofline": roofline, "fp64_gpu": fp64, "cpu_baseline": cpu, "graph": graph,

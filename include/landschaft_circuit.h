@@ -79,6 +79,7 @@ typedef struct lsc_solve_stats {
     int32_t n_fallback; /* mixed precision: circuits (Omniscape windows) or factorisations (graphs)
                            solved again in FP64 because the FP32 factor met a non-positive pivot or
                            its refinement stalled more than 1000 times above tol */
+    double resid_median; /* median relative residual of the solved circuits */
 } lsc_solve_stats;
 
 /* Message of the calling thread's last failure ("" if none). */

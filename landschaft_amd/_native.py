@@ -44,7 +44,7 @@ class SolveStats(ctypes.Structure):
                 ("ms_fill", ctypes.c_double), ("ms_flood", ctypes.c_double), ("ms_factor", ctypes.c_double),
                 ("ms_solve", ctypes.c_double), ("ms_residual", ctypes.c_double),
                 ("ms_currents", ctypes.c_double), ("ms_accumulate", ctypes.c_double),
-                ("n_streams", ctypes.c_int32), ("n_fallback", ctypes.c_int32)]
+                ("n_streams", ctypes.c_int32), ("n_fallback", ctypes.c_int32), ("resid_median", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}

@@ -1087,6 +1087,7 @@ void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
     out->refine_mean = cx.st.solved ? static_cast<double>(cx.st.iter_sum) / cx.st.solved : 0.0;
     out->resid_max = cx.st.resid_max;
     out->resid_mean = cx.st.solved ? cx.st.resid_sum / cx.st.solved : 0.0;
+    out->resid_median = cx.st.resid_median();
     out->n_fallback = cx.st.fallback;
 }
 

@@ -403,6 +403,7 @@ struct WindowSolver {
             st.iter_max = std::max(st.iter_max, its[k]);
             st.resid_max = std::max(st.resid_max, rn[k]);
             st.resid_sum += rn[k];
+            st.resids.push_back(static_cast<float>(rn[k]));
             if (!(rn[k] <= tol)) ++st.unconverged;
             if (!(rn[k] <= 1e-10)) ++st.over_bar;
         }
@@ -703,6 +704,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
         st.iter_max = std::max(st.iter_max, q.iter_max);
         st.resid_max = std::max(st.resid_max, q.resid_max);
         st.resid_sum += q.resid_sum;
+        st.resids.insert(st.resids.end(), q.resids.begin(), q.resids.end());
         st.ms_fill += q.ms_fill, st.ms_flood += q.ms_flood, st.ms_factor += q.ms_factor;
         st.ms_solve += q.ms_solve, st.ms_residual += q.ms_residual, st.ms_currents += q.ms_currents;
         st.ms_accum += q.ms_accum;
@@ -866,6 +868,7 @@ void fill_stats(const lsc::IrStats& st, long long n_targets, lsc_solve_stats* ou
     out->refine_mean = st.solved ? static_cast<double>(st.iter_sum) / st.solved : 0.0;
     out->resid_max = st.resid_max;
     out->resid_mean = st.solved ? st.resid_sum / st.solved : 0.0;
+    out->resid_median = st.resid_median();
     out->ms_fill = st.ms_fill, out->ms_flood = st.ms_flood, out->ms_factor = st.ms_factor;
     out->ms_solve = st.ms_solve, out->ms_residual = st.ms_residual, out->ms_currents = st.ms_currents;
     out->ms_accumulate = st.ms_accum;

@@ -1405,13 +1405,14 @@ void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double to
         for (int i = 0; i < k; ++i)
             if (!(rn[i] <= kFallbackGap * tol)) bad.push_back(i);
         if (!bad.empty()) {
-            impl_->to_fp64(analyze_ms, factor_ms);
             const int kb = static_cast<int>(bad.size());
             const std::size_t n = static_cast<std::size_t>(impl_->plan.d.n);
             DevBuf<double> b2(kb * n), x2(kb * n);
+            // the right-hand sides from the solve's own copy in elimination order (d_b may be d_x)
             for (int j = 0; j < kb; ++j)
-                LSC_HIP(hipMemcpyAsync(b2.p + j * n, d_b + bad[j] * n, n * sizeof(double), hipMemcpyDeviceToDevice,
-                                       impl_->s));
+                hipLaunchKernelGGL(k_g_perm_out, dim3(nblocks(static_cast<long long>(n), 256)), dim3(256), 0, impl_->s,
+                                   static_cast<int>(n), 1, impl_->perm.p, impl_->bp.p + bad[j] * n, b2.p + j * n);
+            impl_->to_fp64(analyze_ms, factor_ms);
             std::vector<double> rn2;
             std::vector<int> its2;
             impl_->solve_t<double>(kb, b2.p, x2.p, tol, maxit, st, rn2, its2);
@@ -1433,6 +1434,7 @@ void SystemSolver::solve_device(int k, const double* d_b, double* d_x, double to
         st.iter_max = std::max(st.iter_max, its[i]);
         st.resid_max = std::max(st.resid_max, rn[i]);
         st.resid_sum += rn[i];
+        st.resids.push_back(static_cast<float>(rn[i]));
         if (!(rn[i] <= tol)) ++st.unconverged;
         if (!(rn[i] <= 1e-10)) ++st.over_bar;
     }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -92,6 +93,16 @@ struct IrStats {
     int streams = 1;  // batches in flight at once
     int fallback = 0; // circuits (graphs: factorisations) solved again in FP64 after the FP32 factor failed or stalled
     double resid_max = 0, resid_sum = 0;
+    std::vector<float> resids;  // every solved circuit's relative residual (for the median)
+    double resid_median() const {
+        if (resids.empty()) return 0.0;
+        std::vector<float> v(resids);
+        const std::size_t h = v.size() / 2;
+        std::nth_element(v.begin(), v.begin() + h, v.end());
+        if (v.size() % 2) return v[h];
+        const float hi = v[h];
+        return 0.5 * (static_cast<double>(*std::max_element(v.begin(), v.begin() + h)) + hi);
+    }
     double ms_fill = 0, ms_flood = 0, ms_factor = 0, ms_solve = 0, ms_residual = 0, ms_currents = 0,
            ms_accum = 0;
 };
