@@ -110,6 +110,27 @@ int lsc_circuit_omniscape_device(const double* resistance, const double* source,
                                  double* current, double* flow_potential_out, int64_t* n_targets,
                                  lsc_solve_stats* stats);
 
+/* The same as lsc_circuit_omniscape_device, allocating nothing: no
+ * reference counterpart; it is for callers that cannot allocate device
+ * memory (tight serving loops, code that must not call hipMalloc). Every
+ * buffer of the run (plans, batch buffers, the correction's circuits) comes
+ * from a static scratch of 128 MiB that each GPU the library runs on holds
+ * until the process ends; batches are sized to fit it (one batch in flight,
+ * on the caller's stream, kernels launched one by one). A run that needs
+ * more than the scratch holds (one window at most, its plan, and room for
+ * the FP64 fallback) fails with status 1 before any work. Calls on one GPU
+ * share its scratch: the library runs them one at a time, and work that
+ * overlaps a call on the same GPU must not touch its buffers. Every window is
+ * solved as lsc_circuit_omniscape_device solves it; the cumulative maps
+ * differ by the rounding of their sums, which add the windows of smaller
+ * batches at a time. */
+int lsc_circuit_omniscape_device_static(const double* resistance, const double* source, int64_t nx, int64_t ny,
+                                        int radius, int buffer, int block_size, int four_neighbors,
+                                        int resistance_is_conductance, int correct_artifacts, int flow_potential,
+                                        const lsc_omniscape_condition* conditions, const lsc_solve_options* options,
+                                        double* current, double* flow_potential_out, int64_t* n_targets,
+                                        lsc_solve_stats* stats);
+
 /* Omniscape's block-artifact correction factors, (2 (radius + buffer) + 1)^2
  * float64 into out. Replaces circuit_omniscape_correction
  * (src/cpp/circuit/python.cpp:139-153 -> omniscape.cpp:326-392). */

@@ -18,6 +18,7 @@ PRECISION_FP64 = 1
 
 # Every symbol declared in include/landschaft_circuit.h.
 EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_circuit_omniscape_device",
+           "lsc_circuit_omniscape_device_static",
            "lsc_circuit_omniscape_correction", "lsc_probe_mfma", "lsc_window_plan_stats", "lsc_circuit_advanced",
            "lsc_circuitscape_advanced", "lsc_circuitscape_pairwise", "lsc_circuitscape_multi_advanced",
            "lsc_circuitscape_multi_advanced_sum", "lsc_network_advanced", "lsc_network_pairwise",
@@ -65,6 +66,7 @@ def _signatures() -> dict:
         "lsc_device_info": (ci, [c.c_char_p, i64, pi32, pi64]),
         "lsc_circuit_omniscape": (ci, omni),
         "lsc_circuit_omniscape_device": (ci, omni),
+        "lsc_circuit_omniscape_device_static": (ci, omni),
         "lsc_circuit_omniscape_correction": (ci, [ci, ci, ci, ci, i64, i64, opt, pf64]),
         "lsc_probe_mfma": (ci, [ci, pf64]),
         "lsc_window_plan_stats": (ci, [ci, ci, ci, ci, pf64]),

@@ -24,21 +24,47 @@ inline void hip_check(hipError_t e, const char* what) {
 }
 #define LSC_HIP(x) ::lsc::hip_check((x), #x)
 
+// Device memory given by the caller (the allocation-free entry points'
+// static scratch), handed out in order, 256-byte aligned. While one is
+// active on the calling thread, every DevBuf of that thread takes its memory
+// from it instead of hipMalloc, and gives nothing back one by one: an
+// ArenaMark returns what was taken in its scope.
+struct Arena {
+    char* base = nullptr;
+    std::size_t size = 0, used = 0;
+};
+inline Arena*& active_arena() {
+    thread_local Arena* a = nullptr;
+    return a;
+}
+struct ArenaMark {
+    Arena* a = active_arena();
+    std::size_t used = a ? a->used : 0;
+    ArenaMark() = default;
+    ArenaMark(const ArenaMark&) = delete;
+    ArenaMark& operator=(const ArenaMark&) = delete;
+    ~ArenaMark() {
+        if (a) a->used = used;
+    }
+};
+inline std::size_t arena_round(std::size_t bytes) { return (bytes + 255) & ~static_cast<std::size_t>(255); }
+
 // A device allocation owned by the host object.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     std::size_t n = 0;
+    bool borrowed = false;  // from the active arena: not freed
     DevBuf() = default;
     explicit DevBuf(std::size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), borrowed(o.borrowed) { o.p = nullptr, o.n = 0, o.borrowed = false; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
-            p = o.p, n = o.n;
-            o.p = nullptr, o.n = 0;
+            p = o.p, n = o.n, borrowed = o.borrowed;
+            o.p = nullptr, o.n = 0, o.borrowed = false;
         }
         return *this;
     }
@@ -46,12 +72,23 @@ struct DevBuf {
     void alloc(std::size_t count) {
         release();
         if (count == 0) return;
-        LSC_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        if (Arena* a = active_arena()) {
+            const std::size_t at = arena_round(a->used), bytes = count * sizeof(T);
+            if (at + bytes > a->size)
+                throw std::invalid_argument("the run needs more device memory than the static scratch of " +
+                                            std::to_string(a->size >> 20) + " MiB holds (" +
+                                            std::to_string((at + bytes) >> 20) + " MiB): use the allocating entry point");
+            p = reinterpret_cast<T*>(a->base + at);
+            a->used = at + bytes;
+            borrowed = true;
+        } else {
+            LSC_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        }
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr, n = 0;
+        if (p && !borrowed) (void)hipFree(p);
+        p = nullptr, n = 0, borrowed = false;
     }
     void upload(const T* h, std::size_t count, hipStream_t s) {
         LSC_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
@@ -72,6 +109,7 @@ DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
 // Kernel sequences are replayed as hipGraphs unless LSC_GRAPHS=0 (then
 // launched one by one; the results are the same).
 inline bool graphs_enabled() {
+    if (active_arena()) return false;  // instantiating a graph may allocate
     const char* e = std::getenv("LSC_GRAPHS");
     return !(e && e[0] == '0');
 }

@@ -30,6 +30,17 @@
 
 namespace lsc {
 
+// Zeroes n doubles (a kernel rather than a memset, so a caller's capture of
+// the stream records kernels only).
+__global__ void k_zero_f64(long long n, double* a) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = 0.0;
+}
+
+__global__ void k_zero_u64(int n, unsigned long long* a) {
+    if (static_cast<int>(threadIdx.x) < n) a[threadIdx.x] = 0ull;
+}
+
 // Flow potential circuits: conductance 1 on every on-map cell of the window
 // (reference omniscape.cpp:127-166 uses a window of ones).
 __global__ void k_unit_conductance(Lattice Lt, long long k0, int B, double* g) {
@@ -64,6 +75,7 @@ std::size_t solve_lds(int lmax) {
 // default 32: half the fronts of 16-cell leaves for 25 % more flops).
 // Batches in flight at once (LSC_BATCH_STREAMS, 1-4; default 4).
 long long batch_streams() {
+    if (active_arena()) return 1;  // the static scratch serves one batch on the caller's stream
     const char* e = std::getenv("LSC_BATCH_STREAMS");
     const int v = e ? std::atoi(e) : 4;
     return v >= 1 && v <= 4 ? v : 4;
@@ -197,7 +209,9 @@ struct WindowSolver {
 
     void set_plan(Plan p, int B) {
         for (LaunchGraph* g : {&g_flood, &g_factor, &g_sweep0, &g_residual, &g_sweep1}) g->clear();
+        const std::size_t used0 = active_arena() ? active_arena()->used : 0;
         plan.upload(std::move(p), s);
+        if (active_arena()) plan_bytes = static_cast<long long>(active_arena()->used - used0);
         int smax = 1, lmax = 1;
         for (int v : plan.lev_stile) smax = std::max(smax, v);
         for (int v : plan.lev_lmax) lmax = std::max(lmax, v);
@@ -210,6 +224,7 @@ struct WindowSolver {
         allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
         lane_levels();
         const int W = plan.d.W;
+        if (active_arena()) B = arena_batch(B);
         Bcap = B;
         wb.ensure(B, W * W, plan.d.n, plan.d.lsize, sizeof(T));
         U.alloc(static_cast<std::size_t>(B) * plan.d.usizeA * sizeof(T));
@@ -332,6 +347,33 @@ struct WindowSolver {
                            wb.active.p, wb.done.p, wb.sc.p, wb.z.p, wb.x.p);
     }
 
+    // Device bytes of one window's batch buffers with L entries of `elem` bytes.
+    long long window_bytes(int elem) const {
+        const DPlan& P = plan.d;
+        const long long W2 = static_cast<long long>(P.W) * P.W;
+        return W2 * 25 + static_cast<long long>(P.n) * 56 + 48 +
+               (P.lsize + P.usizeA + std::max(P.nent, 1)) * elem + std::max<long long>(P.vsize, 1) * 8;
+    }
+    static constexpr long long kBufferSlack = 24 * 256 + 64;  // alignment of the batch buffers
+    static constexpr int kArenaFallback = 8;                  // FP64 fallback windows at once in the scratch
+
+    // The largest batch (at most `want`) whose buffers fit what is left of
+    // the active arena once the plan is in it; with FP32 fronts, room is left
+    // for the FP64 fallback (its plan and kArenaFallback windows).
+    int arena_batch(int want) const {
+        const Arena* a = active_arena();
+        long long left = static_cast<long long>(a->size) - static_cast<long long>(arena_round(a->used));
+        if (std::is_same<T, float>::value)
+            left -= plan_bytes + kBufferSlack + kArenaFallback * window_bytes(8) + 4 * 8;
+        const long long fit = (left - kBufferSlack - 4 * static_cast<long long>(sizeof(T))) / window_bytes(sizeof(T));
+        if (fit < 1)
+            throw std::invalid_argument("one window of radius " + std::to_string(plan.d.D) +
+                                        " needs more device memory than the static scratch of " +
+                                        std::to_string(a->size >> 20) + " MiB holds");
+        return static_cast<int>(std::min<long long>(want, fit));
+    }
+    long long plan_bytes = 0;  // arena bytes the plan took (the arena's)
+
     void fallback_fp64(const std::vector<int>& bad, double tol, int maxit, hipEvent_t* ev, std::vector<double>& rn,
                        std::vector<int>& its, std::vector<int>& fl, IrStats& st);
 
@@ -416,10 +458,12 @@ void WindowSolver<T>::fallback_fp64(const std::vector<int>& bad, double tol, int
                                     IrStats& st) {
     const int W2 = plan.d.W * plan.d.W, n = plan.d.n;
     const int nbad = static_cast<int>(bad.size());
-    for (int c0 = 0; c0 < nbad; c0 += Bcap) {  // in chunks of at most the batch size
-        const int m = std::min(Bcap, nbad - c0);
+    // in chunks of at most the batch size (kArenaFallback windows in the static scratch)
+    const int chunk = active_arena() ? kArenaFallback : Bcap;
+    for (int c0 = 0; c0 < nbad; c0 += chunk) {
+        const int m = std::min(chunk, nbad - c0);
         if (!fb || fb_cap < m) {
-            fb_cap = std::max(m, std::min(Bcap, 256));
+            fb_cap = active_arena() ? chunk : std::max(m, std::min(Bcap, 256));
             fb.reset(new WindowSolver<double>(s));
             fb->set_plan(Plan(plan.host), fb_cap);
         }
@@ -577,8 +621,10 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
                               static_cast<double>(plan.lsize + ua + plan.fe_row.size()) * sizeof(T) +
                               plan.vsize * 8.0;
     std::size_t free_b = 0, total_b = 0;
-    LSC_HIP(hipMemGetInfo(&free_b, &total_b));
-    const long long fit = std::max<long long>(1, static_cast<long long>(0.5 * static_cast<double>(free_b) / per_window));
+    if (!active_arena()) LSC_HIP(hipMemGetInfo(&free_b, &total_b));
+    // in the static scratch the solver sizes the batch to what is left of it
+    const long long fit = active_arena() ? std::numeric_limits<int>::max()
+                                         : std::max<long long>(1, static_cast<long long>(0.5 * static_cast<double>(free_b) / per_window));
     const long long want = std::min<long long>(a.batch > 0 ? a.batch : 16384, Lt.nlat);
     // Several batches in flight on their own streams (four by default,
     // LSC_BATCH_STREAMS), each driven by a host thread: the window kernels
@@ -590,8 +636,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
                                                                                   (Lt.nlat + want - 1) / want})));
     // the batches in flight share the one-batch budget: allocating (and
     // freeing) HBM costs about 13 ms per GB, more than extra batches gain
-    const long long B = std::max<long long>(1, std::min<long long>(want, fit) / nsol);
-    const long long nbatch = (Lt.nlat + B - 1) / B;
+    long long B = std::max<long long>(1, std::min<long long>(want, fit) / nsol);
     // The first batch runs on the caller's stream, the others on streams of
     // their own that start after the work already queued there; the host
     // waits for all of them below. (A stream of our own for the first batch
@@ -602,15 +647,19 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     std::vector<hipStream_t> own(nsol - 1, nullptr);
     for (auto& x : own) LSC_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
     hipEvent_t queued = nullptr;
-    LSC_HIP(hipEventCreateWithFlags(&queued, hipEventDisableTiming));
-    LSC_HIP(hipEventRecord(queued, s));
-    for (auto& x : own) LSC_HIP(hipStreamWaitEvent(x, queued, 0));
     std::vector<hipEvent_t> acc_ev(nsol, nullptr);
+    if (nsol > 1) {  // the other streams start after the caller's queued work; accumulation in batch order
+        LSC_HIP(hipEventCreateWithFlags(&queued, hipEventDisableTiming));
+        LSC_HIP(hipEventRecord(queued, s));
+        for (auto& x : own) LSC_HIP(hipStreamWaitEvent(x, queued, 0));
+    }
     for (int j = 0; j < nsol; ++j) {
         ws.emplace_back(new WindowSolver<T>(j == 0 ? s : own[j - 1]));
         ws.back()->set_plan(j == 0 ? plan : Plan(plan), static_cast<int>(B));
-        LSC_HIP(hipEventCreateWithFlags(&acc_ev[j], hipEventDisableTiming));
+        if (nsol > 1) LSC_HIP(hipEventCreateWithFlags(&acc_ev[j], hipEventDisableTiming));
     }
+    B = ws[0]->Bcap;  // the static scratch may hold fewer windows
+    const long long nbatch = (Lt.nlat + B - 1) / B;
     std::vector<IrStats> part(nsol);
     std::vector<std::exception_ptr> err(nsol);
     std::mutex mu;
@@ -661,7 +710,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
                 }
                 hipLaunchKernelGGL(k_accumulate, dim3(static_cast<unsigned>((cells + kThreads - 1) / kThreads)),
                                    dim3(kThreads), 0, sj, Lt, k0, nb, X0, ncol, w.wb.active.p, w.wb.cur.p, d_map);
-                LSC_HIP(hipEventRecord(acc_ev[j], sj));
+                if (nsol > 1) LSC_HIP(hipEventRecord(acc_ev[j], sj));
                 {
                     std::lock_guard<std::mutex> lk(mu);
                     acc_done = bi;
@@ -691,9 +740,10 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     for (int j = 1; j < nsol; ++j) helpers.emplace_back(run, j);
     run(0);
     for (auto& h : helpers) h.join();
-    for (auto& e : acc_ev) (void)hipEventDestroy(e);
+    for (auto& e : acc_ev)
+        if (e) (void)hipEventDestroy(e);
     ws.clear();
-    (void)hipEventDestroy(queued);
+    if (queued) (void)hipEventDestroy(queued);
     for (auto& x : own) (void)hipStreamDestroy(x);
     for (auto& e : err)
         if (e) std::rethrow_exception(e);
@@ -816,19 +866,30 @@ void omniscape_device(const double* d_res, const double* d_src, long long nx, lo
     if (2 * (a.radius + a.buffer) + 1 > kMaxW)
         throw std::invalid_argument("window side 2 (radius + buffer) + 1 exceeds " + std::to_string(kMaxW));
     const Lattice Lt = make_lattice(nx, ny, a);
-    LSC_HIP(hipMemsetAsync(d_cur, 0, nx * ny * sizeof(double), s));
-    if (a.fp) LSC_HIP(hipMemsetAsync(d_fp, 0, nx * ny * sizeof(double), s));
+    const long long ncell = nx * ny;
+    const unsigned zb = static_cast<unsigned>((ncell + 255) / 256);
+    if (ncell > 0) hipLaunchKernelGGL(k_zero_f64, dim3(zb), dim3(256), 0, s, ncell, d_cur);
+    if (a.fp && ncell > 0) hipLaunchKernelGGL(k_zero_f64, dim3(zb), dim3(256), 0, s, ncell, d_fp);
     DevBuf<unsigned long long> cnt(2);
-    LSC_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, 2, cnt.p);
     DevBuf<double> corr;
     const bool correct = a.correct && a.block > 1;
     if (correct && Lt.nlat > 0) {
         IrStats cst;
-        corr = to_device(correction<T>(a, nx, ny, s, cst), s);
+        std::vector<double> c;
+        {
+            ArenaMark m;  // the correction's circuits give their scratch back
+            c = correction<T>(a, nx, ny, s, cst);
+        }
+        corr = to_device(c, s);
     }
-    lattice_pass<T>(d_res, d_src, Lt, a, cond, 0, corr.p, d_cur, cnt.p, s, st);
+    {
+        ArenaMark m;
+        lattice_pass<T>(d_res, d_src, Lt, a, cond, 0, corr.p, d_cur, cnt.p, s, st);
+    }
     if (a.fp) {
         IrStats fst;
+        ArenaMark m;
         lattice_pass<T>(d_res, d_src, Lt, a, cond, 1, corr.p, d_fp, cnt.p + 1, s, fst);
     }
     unsigned long long h = 0;
@@ -836,6 +897,47 @@ void omniscape_device(const double* d_res, const double* d_src, long long nx, lo
     LSC_HIP(hipStreamSynchronize(s));
     *n_targets = static_cast<long long>(h);
 }
+
+// ---- the static scratch of the allocation-free entry point
+
+// Device memory of lsc_circuit_omniscape_device_static: a file-scope array
+// each GPU holds from the library's load until the process ends. Host code
+// never uses its name as an address (it is a host-side stand-in): the
+// address on each device comes from hipGetSymbolAddress.
+constexpr std::size_t kScratchBytes = std::size_t{128} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+std::mutex g_device_mu[kMaxDevices];  // one run at a time per device: they share its scratch
+}  // namespace
+
+// The scratch's address on the current device, looked up once per device
+// (with that device current).
+char* scratch_on_current_device(int& dev) {
+    LSC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        LSC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("the static scratch has no device address");
+        g_scratch[dev] = static_cast<char*>(p);
+    }
+    return g_scratch[dev];
+}
+
+std::mutex& device_run_mutex(int dev) { return g_device_mu[dev]; }
+
+// Makes `a` the calling thread's arena for the scope.
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { active_arena() = a; }
+    ~ArenaScope() { active_arena() = nullptr; }
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+};
 
 }  // namespace lsc
 
@@ -918,6 +1020,26 @@ int lsc_circuit_omniscape_device(const double* resistance, const double* source,
     return omniscape_run(resistance, source, nullptr, nx, ny, radius, buffer, block_size, four_neighbors,
                          resistance_is_conductance, correct_artifacts, flow_potential, conditions, true, options,
                          current, flow_potential_out, n_targets, stats);
+}
+
+int lsc_circuit_omniscape_device_static(const double* resistance, const double* source, int64_t nx, int64_t ny,
+                                        int radius, int buffer, int block_size, int four_neighbors,
+                                        int resistance_is_conductance, int correct_artifacts, int flow_potential,
+                                        const lsc_omniscape_condition* conditions, const lsc_solve_options* options,
+                                        double* current, double* flow_potential_out, int64_t* n_targets,
+                                        lsc_solve_stats* stats) {
+    int inner = 0;
+    const int rc = lsc::guarded([&] {
+        int dev = 0;
+        char* base = lsc::scratch_on_current_device(dev);
+        std::lock_guard<std::mutex> lk(lsc::device_run_mutex(dev));
+        lsc::Arena arena{base, lsc::kScratchBytes, 0};
+        lsc::ArenaScope scope(&arena);
+        inner = omniscape_run(resistance, source, nullptr, nx, ny, radius, buffer, block_size, four_neighbors,
+                              resistance_is_conductance, correct_artifacts, flow_potential, conditions, true, options,
+                              current, flow_potential_out, n_targets, stats);
+    });
+    return inner != 0 ? inner : rc;
 }
 
 int lsc_circuit_omniscape(const double* resistance, const double* source, int64_t nx, int64_t ny, int radius,
