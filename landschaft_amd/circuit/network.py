@@ -1,9 +1,12 @@
-"""Circuitscape's network (graph) mode: pairwise and advanced on an edge list.
+"""Circuitscape's network (graph) mode, advanced and pairwise, on the GPU graph engine.
 
-Mirrors the reference's src/landschaft/circuit/network.py (readers :24-113,
-result :116-157, advanced :177-262, pairwise :266-374, INI :377-446); the
-solves run in the GPU graph engine behind ``core()`` (the general-graph
-nested dissection orders the unknowns, there being no grid).
+The public names, arguments, results and error messages are those of the reference's
+src/landschaft/circuit/network.py (readers :24-113, NetworkResult :116-157, advanced :177-262,
+pairwise :266-374, the INI driver :377-446), so callers can switch modules. The engine behind
+``core()`` orders a graph without a grid by a general-graph nested dissection.
+
+Inputs are normalised once into a :class:`_Network` (0-based int32 endpoints, conductances, the node
+count, whether the caller counted from 0); everything downstream works on it.
 """
 
 from __future__ import annotations
@@ -20,77 +23,112 @@ from .circuitscape import (POLICIES, SCENARIOS, IncludedPairs, _flag, _open, _pr
 
 __all__ = ["NetworkResult", "circuitscape_network_advanced", "circuitscape_network_pairwise", "read_graph"]
 
+# branch currents within this of 0 are not written (Circuitscape's tables)
+_BRANCH_FLOOR = 1e-6
 
-# ---- reading (network.py:24-113)
 
-def _read_table(path) -> list:
-    rows = []
+# ---- text tables
+
+def _rows_of(path) -> list:
+    """Every non-empty line of a (possibly gzipped) text file as a list of floats; commas count as
+    blanks."""
     with _open(path) as f:
-        for line in f:
-            parts = line.replace(",", " ").split()
-            if parts:
-                rows.append([float(t) for t in parts])
-    return rows
+        tokens = (line.replace(",", " ").split() for line in f)
+        return [[float(t) for t in toks] for toks in tokens if toks]
 
 
-def _as_table(value, ncol: int, name: str) -> np.ndarray:
+def _matrix(value, ncol: int, name: str) -> np.ndarray:
+    """A path or array as a float64 table of `ncol` columns (an empty one as 0 x ncol)."""
     if isinstance(value, (str, os.PathLike)):
-        rows = _read_table(value)
-        if not rows or any(len(r) != len(rows[0]) for r in rows):
+        rows = _rows_of(value)
+        widths = {len(r) for r in rows}
+        if len(widths) != 1:
             raise ValueError(f"{value}: expected a table with the same number of columns on every line")
-        t = np.array(rows, dtype=np.float64)
+        table = np.asarray(rows, dtype=np.float64)
     else:
-        t = np.array(value, dtype=np.float64)
-    if t.size == 0:
-        t = t.reshape(-1, ncol)
-    if t.ndim != 2 or t.shape[1] != ncol:
+        table = np.asarray(value, dtype=np.float64).copy()
+    if table.size == 0:
+        table = np.zeros((0, ncol))
+    if table.ndim != 2 or table.shape[1] != ncol:
         raise ValueError(f"the {name} must have {ncol} columns")
-    return t
+    return table
 
 
 def read_graph(edges):
-    """A Circuitscape edge list (path or ``(k, 3)`` array: node, node, value) as
-    ``(from, to, value, shifted)``: nodes numbered from 1 (a list numbered from 0 is shifted by one,
-    and ``shifted`` says so)."""
-    t = _as_table(edges, 3, "graph")
-    if len(t) == 0:
+    """A Circuitscape edge list (path or ``(k, 3)`` array of node, node, value) as
+    ``(from, to, value, shifted)``: node numbers from 1 (a list counting from 0 is moved up by one,
+    and ``shifted`` says so) and the edge values."""
+    table = _matrix(edges, 3, "graph")
+    if table.shape[0] == 0:
         raise ValueError("the graph has no edges")
-    i, j = t[:, 0].astype(np.int64), t[:, 1].astype(np.int64)
-    low = min(i.min(), j.min())
-    if low > 1:
-        raise ValueError(f"the graph numbers its nodes from {low}; Circuitscape counts from 1 (or 0)")
-    if low < 0:
+    ends = table[:, :2].astype(np.int64)
+    base = int(ends.min())
+    if base < 0:
         raise ValueError("negative node numbers")
-    if low == 0:
-        i, j = i + 1, j + 1
-    return i, j, t[:, 2].copy(), bool(low == 0)
+    if base > 1:
+        raise ValueError(f"the graph numbers its nodes from {base}; Circuitscape counts from 1 (or 0)")
+    ends += 1 - base
+    return ends[:, 0].copy(), ends[:, 1].copy(), table[:, 2].copy(), base == 0
 
 
-def _read_focal_nodes(points) -> np.ndarray:
-    # every number of the file, column after column; a list from 0 shifted to 1
+def _focal_nodes(points) -> np.ndarray:
+    """Focal node numbers from a file (all its numbers, read down the columns when every line has
+    as many, else line by line) or an array; a list holding 0 counts from 0 and moves up by one."""
     if isinstance(points, (str, os.PathLike)):
-        rows = _read_table(points)
-        if rows and all(len(r) == len(rows[0]) for r in rows):
-            fp = np.array(rows).ravel(order="F")
+        rows = _rows_of(points)
+        if rows and len({len(r) for r in rows}) == 1:
+            nums = np.asarray(rows).T.ravel()
         else:
-            fp = np.array([v for r in rows for v in r])
+            nums = np.asarray([v for r in rows for v in r])
     else:
-        fp = np.asarray(points, dtype=np.float64).ravel()
-    fp = fp.astype(np.int64)
-    if len(fp) and fp.min() == 0:
-        fp = fp + 1
-    return fp
+        nums = np.asarray(points, dtype=np.float64).ravel()
+    nodes = nums.astype(np.int64)
+    return nodes + 1 if nodes.size and nodes.min() == 0 else nodes
 
 
-def _read_strengths(value, starts_from_zero: bool) -> np.ndarray:
-    # (node, value) rows, shifted when they or the graph count from 0
-    t = _as_table(value, 2, "node values")
-    if len(t) and (t[:, 0].min() == 0 or starts_from_zero):
-        t[:, 0] += 1
-    return t
+def _node_table(value, graph_from_zero: bool) -> np.ndarray:
+    """(node, value) rows; moved up by one when they count from 0 or the graph does."""
+    table = _matrix(value, 2, "node values")
+    if table.shape[0] and (graph_from_zero or table[:, 0].min() == 0):
+        table[:, 0] += 1
+    return table
 
 
-# ---- the result (network.py:116-157)
+# ---- the graph handed to the library
+
+@dataclass
+class _Network:
+    frm: np.ndarray  # edge endpoints as given, from 1
+    to: np.ndarray
+    a: np.ndarray  # the same from 0, int32
+    b: np.ndarray
+    w: np.ndarray  # conductances
+    m: int  # nodes 1..m
+    from_zero: bool
+
+    @classmethod
+    def of(cls, graph, is_resistance: bool) -> "_Network":
+        frm, to, value, from_zero = read_graph(graph)
+        w = np.ascontiguousarray(1.0 / value if is_resistance else value, dtype=np.float64)
+        return cls(frm, to, np.ascontiguousarray(frm - 1, dtype=np.int32), np.ascontiguousarray(to - 1, dtype=np.int32),
+                   w, int(max(frm.max(), to.max())), from_zero)
+
+    def per_node(self, table: np.ndarray, what: str) -> np.ndarray:
+        """Node values as a dense array over the nodes (a node listed twice: its last value)."""
+        out = np.zeros(self.m)
+        nodes = table[:, 0].astype(np.int64)
+        bad = (nodes < 1) | (nodes > self.m)
+        if bad.any():
+            raise ValueError(f"a {what} node is not in the graph (nodes 1..{self.m})")
+        out[nodes - 1] = table[:, 1]
+        return out
+
+    def numbered(self, values: np.ndarray) -> np.ndarray:
+        """(node, value) rows over every node."""
+        return np.column_stack([np.arange(1.0, self.m + 1.0), values])
+
+
+# ---- results
 
 @dataclass
 class NetworkResult:
@@ -113,21 +151,14 @@ class NetworkResult:
     n_components: int = 0
 
 
-def _branch_table(r, c, v) -> np.ndarray:
-    # the branches Circuitscape writes: current not within 1e-6 of 0, nodes from 1
-    r, c, v = np.asarray(r), np.asarray(c), np.asarray(v)
-    keep = ~(np.abs(v) <= 1e-6)
-    return np.column_stack([r[keep] + 1.0, c[keep] + 1.0, v[keep]])
+def _written_branches(r, c, v) -> np.ndarray:
+    """Branches whose current is more than 1e-6 from 0 (NaN kept), nodes from 1."""
+    r, c, v = (np.asarray(x) for x in (r, c, v))
+    shown = ~(np.abs(v) <= _BRANCH_FLOOR)
+    return np.stack([r[shown] + 1.0, c[shown] + 1.0, v[shown]], axis=1) if shown.any() else np.zeros((0, 3))
 
 
-def _edges(i, j, w, is_resistance: bool):
-    w = 1 / w if is_resistance else w
-    m = int(max(i.max(), j.max()))
-    return (np.ascontiguousarray(i - 1, dtype=np.int32), np.ascontiguousarray(j - 1, dtype=np.int32),
-            np.ascontiguousarray(w, dtype=np.float64), m)
-
-
-# ---- advanced (network.py:177-262)
+# ---- advanced
 
 def circuitscape_network_advanced(graph, sources, grounds, *, graph_is_resistance: bool = True,
                                   ground_is_resistance: bool = True, remove_src_or_gnd: str = "keepall",
@@ -143,27 +174,49 @@ def circuitscape_network_advanced(graph, sources, grounds, *, graph_is_resistanc
     """
     if remove_src_or_gnd not in POLICIES:
         raise ValueError(f"remove_src_or_gnd must be one of {POLICIES}")
-    i, j, w, zero = read_graph(graph)
-    a, b, w, m = _edges(i, j, w, graph_is_resistance)
-    src_t = _read_strengths(sources, zero)
-    gnd_t = _read_strengths(grounds, zero)
+    net = _Network.of(graph, graph_is_resistance)
+    src = net.per_node(_node_table(sources, net.from_zero), "source")
+    gnd_t = _node_table(grounds, net.from_zero)
     if ground_is_resistance:
         with np.errstate(divide="ignore"):
-            gnd_t[:, 1] = 1 / gnd_t[:, 1]
-    src, gnd = np.zeros(m), np.zeros(m)
-    for table, target, name in ((src_t, src, "source"), (gnd_t, gnd, "ground")):
-        ids = table[:, 0].astype(np.int64)
-        if len(ids) and (ids.min() < 1 or ids.max() > m):
-            raise ValueError(f"a {name} node is not in the graph (nodes 1..{m})")
-        target[ids - 1] = table[:, 1]  # a node listed twice keeps its last value
-    r = core().network_advanced(a, b, w, m, src, gnd, remove_src_or_gnd, int(n_threads))
-    nodes = np.arange(1, m + 1, dtype=np.float64)
-    return NetworkResult(voltages=np.column_stack([nodes, r["voltage"]]),
-                         node_currents=np.column_stack([nodes, r["current"]]),
-                         branch_currents=_branch_table(*r["branch"]), n_components=int(r["n_components"]))
+            gnd_t[:, 1] = np.reciprocal(gnd_t[:, 1])  # a 0 resistance: an infinite conductance (direct)
+    gnd = net.per_node(gnd_t, "ground")
+    out = core().network_advanced(net.a, net.b, net.w, net.m, src, gnd, remove_src_or_gnd, int(n_threads))
+    return NetworkResult(voltages=net.numbered(out["voltage"]), node_currents=net.numbered(out["current"]),
+                         branch_currents=_written_branches(*out["branch"]), n_components=int(out["n_components"]))
 
 
-# ---- pairwise (network.py:266-374)
+# ---- pairwise
+
+def _excluded_matrix(nodes: np.ndarray, excluded: set) -> np.ndarray:
+    """uint8 n x n: 1 where the ordered pair of focal nodes is excluded."""
+    n = nodes.size
+    ex = np.zeros((n, n), dtype=np.uint8)
+    if excluded:
+        pos = {}
+        for k, v in enumerate(nodes.tolist()):
+            pos.setdefault(v, []).append(k)
+        for p, q in excluded:
+            for i in pos.get(p, ()):
+                ex[i, pos.get(q, [])] = 1
+    return ex
+
+
+def _cumulative_branches(net: _Network, cum_branch) -> np.ndarray:
+    """The pairs' summed branch currents laid out on the input lines, as Circuitscape writes its
+    cumulative vector: every unordered node pair's current on its first input line, 0 on repeats;
+    lines more than 1e-6 from 0 kept."""
+    cr, cc, cv = (np.asarray(x) for x in cum_branch)
+    lo = np.minimum(net.frm, net.to)
+    hi = np.maximum(net.frm, net.to)
+    key = lo * (net.m + 1) + hi
+    sums = dict(zip(((np.minimum(cr, cc) + 1) * (net.m + 1) + np.maximum(cr, cc) + 1).tolist(), cv.tolist()))
+    _, first = np.unique(key, return_index=True)
+    line_value = np.zeros(key.size)
+    line_value[first] = [sums.get(k, 0.0) for k in key[first].tolist()]
+    shown = ~(np.abs(line_value) <= _BRANCH_FLOOR)
+    return np.stack([net.frm[shown].astype(np.float64), net.to[shown].astype(np.float64), line_value[shown]], axis=1)
+
 
 def circuitscape_network_pairwise(graph, points, *, graph_is_resistance: bool = True, included_pairs=None,
                                   voltage_maps: bool = False, current_maps: bool = False,
@@ -178,127 +231,108 @@ def circuitscape_network_pairwise(graph, points, *, graph_is_resistance: bool = 
     node and branch currents, go to ``maps``; the currents are summed over
     the pairs into ``cum_node_currents`` / ``cum_branch_currents``.
     """
-    i, j, w, zero = read_graph(graph)
-    a, b, w, m = _edges(i, j, w, graph_is_resistance)
-    fp = _read_focal_nodes(points)
-    if included_pairs is not None and not isinstance(included_pairs, IncludedPairs):
-        included_pairs = read_included_pairs(included_pairs, starts_from_zero=zero)
-    excluded: set = set()
-    if included_pairs is not None:
-        if included_pairs.mode == "include":
-            fp = fp[np.isin(fp, included_pairs.ids)]
-        excluded = included_pairs.excluded()
-    n = len(fp)
-    ex = np.zeros((n, n), dtype=np.uint8)
-    if excluded:
-        for p in range(n):
-            for q in range(n):
-                ex[p, q] = (int(fp[p]), int(fp[q])) in excluded
+    net = _Network.of(graph, graph_is_resistance)
+    focal = _focal_nodes(points)
+    pairs = included_pairs
+    if pairs is not None and not isinstance(pairs, IncludedPairs):
+        pairs = read_included_pairs(pairs, starts_from_zero=net.from_zero)
+    excluded = set()
+    if pairs is not None:
+        if pairs.mode == "include":
+            focal = focal[np.isin(focal, pairs.ids)]
+        excluded = pairs.excluded()
+    ex = _excluded_matrix(focal, excluded)
     maps: dict = {}
 
-    def emit(ci, cj, nodes, volt, cur, branch):
-        key = (int(fp[ci]), int(fp[cj]))
-        ids = nodes + 1.0
+    def record(ci, cj, nodes, volt, cur, branch):
+        pair = (int(focal[ci]), int(focal[cj]))
+        numbered = nodes + 1.0
         if volt is not None:
-            maps[("voltages", *key)] = np.column_stack([ids, volt])
-        maps[("node_currents", *key)] = np.column_stack([ids, cur])
-        maps[("branch_currents", *key)] = _branch_table(*branch)
+            maps[("voltages", *pair)] = np.stack([numbered, volt], axis=1)
+        maps[("node_currents", *pair)] = np.stack([numbered, cur], axis=1)
+        maps[("branch_currents", *pair)] = _written_branches(*branch)
 
-    opts = {"volt_maps": bool(voltage_maps), "cur_maps": bool(current_maps), "cum": bool(current_maps)}
-    pts = np.ascontiguousarray(np.where((fp >= 1) & (fp <= m), fp - 1, -1), dtype=np.int32)
-    r = core().network_pairwise(a, b, w, m, pts, ex, opts, emit, int(n_threads))
-    res = np.zeros((n + 1, n + 1))
-    res[0, 1:] = fp
-    res[1:, 0] = fp
-    res[1:, 1:] = r["resistances"]
-    out = NetworkResult(resistances=res, maps=maps, n_components=int(r["n_components"]))
-    if "cum" in r:
-        out.cum_node_currents = np.column_stack([np.arange(1, m + 1, dtype=np.float64), r["cum"]])
-        # one slot per input line: the first line of a node pair (either way round) takes the pair's
-        # summed current, as Circuitscape's cumulative vectors are laid out
-        cr, cc, cv = (np.asarray(x) for x in r["cum_branch"])
-        total = {(int(p), int(q)): float(v) for p, q, v in zip(cr + 1, cc + 1, cv)}
-        seen: set = set()
-        vals = np.zeros(len(i))
-        for k, (p, q) in enumerate(zip(i.tolist(), j.tolist())):
-            key = (min(p, q), max(p, q))
-            if key in seen:
-                continue
-            seen.add(key)
-            vals[k] = total.get(key, 0.0)
-        keep = ~(np.abs(vals) <= 1e-6)
-        out.cum_branch_currents = np.column_stack([i[keep].astype(np.float64), j[keep].astype(np.float64),
-                                                   vals[keep]])
-    return out
+    options = {"volt_maps": bool(voltage_maps), "cur_maps": bool(current_maps), "cum": bool(current_maps)}
+    on_graph = (focal >= 1) & (focal <= net.m)
+    library_points = np.ascontiguousarray(np.where(on_graph, focal - 1, -1), dtype=np.int32)
+    out = core().network_pairwise(net.a, net.b, net.w, net.m, library_points, ex, options, record, int(n_threads))
+    table = np.zeros((focal.size + 1, focal.size + 1))
+    table[0, 1:] = table[1:, 0] = focal
+    table[1:, 1:] = out["resistances"]
+    result = NetworkResult(resistances=table, maps=maps, n_components=int(out["n_components"]))
+    if "cum" in out:
+        result.cum_node_currents = net.numbered(out["cum"])
+        result.cum_branch_currents = _cumulative_branches(net, out["cum_branch"])
+    return result
 
 
-# ---- INI files (network.py:377-446)
+# ---- network INI files
 
 def _write_table(path: str, a: np.ndarray, sep: str = "\t") -> None:
-    with open(path, "w", encoding="utf-8") as f:
-        for row in np.atleast_2d(a):
-            f.write(sep.join(repr(float(v)) for v in row) + "\n")
+    lines = (sep.join(repr(float(v)) for v in row) for row in np.atleast_2d(a))
+    Path(path).write_text("".join(line + "\n" for line in lines), encoding="utf-8")
+
+
+def _ini_file(cfg: dict, base: Path, key: str) -> Path:
+    """The file an INI key names (relative paths: the working directory, then the INI's)."""
+    if _unset(cfg[key]):
+        raise ValueError(f"{key} is not set")
+    path = _resolve(cfg[key], base)
+    if not path.is_file():
+        raise FileNotFoundError(f'{key} = "{cfg[key]}" does not exist')
+    return path
+
+
+def _write_network_outputs(cfg: dict, scenario: str, result: NetworkResult) -> None:
+    prefix, _ = _prefix(cfg)
+    Path(cfg["output_file"]).write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()), encoding="utf-8")
+    if scenario != "pairwise":
+        if _flag(cfg, "write_volt_maps"):
+            _write_table(f"{prefix}_voltages.txt", result.voltages)
+        if _flag(cfg, "write_cur_maps") or _flag(cfg, "write_cum_cur_map_only"):
+            _write_table(f"{prefix}_node_currents.txt", result.node_currents)
+            _write_table(f"{prefix}_branch_currents.txt", result.branch_currents)
+        return
+    res = result.resistances
+    _write_matrix(prefix + "_resistances.out", res)
+    ids = res[1:, 0]
+    iu, ju = np.triu_indices(ids.size, 1)
+    _write_matrix(prefix + "_resistances_3columns.out",
+                  np.stack([ids[iu], ids[ju], res[ju + 1, iu + 1]], axis=1).reshape(-1, 3))
+    for (kind, p, q), table in result.maps.items():
+        _write_table(f"{prefix}_{kind}_{p}_{q}.txt", table)
+    if result.cum_node_currents is not None:
+        _write_table(f"{prefix}_node_currents_cum.txt", result.cum_node_currents)
+        _write_table(f"{prefix}_branch_currents_cum.txt", result.cum_branch_currents)
 
 
 def network_ini(cfg: dict, base: Path, *, write_outputs: bool = True, n_threads: int = 0) -> NetworkResult:
     """``circuitscape_ini`` for ``data_type = network`` (``cfg``: the INI over Circuitscape's defaults)."""
-
-    def need(key):
-        if _unset(cfg[key]):
-            raise ValueError(f"{key} is not set")
-        p = _resolve(cfg[key], base)
-        if not p.is_file():
-            raise FileNotFoundError(f'{key} = "{cfg[key]}" does not exist')
-        return p
-
     scenario = SCENARIOS.get(cfg["scenario"])
     if scenario is None:
         raise ValueError(f'scenario = "{cfg["scenario"]}" is not one of {sorted(set(SCENARIOS.values()))}')
-    graph = need("habitat_file")
+    graph = _ini_file(cfg, base, "habitat_file")
     is_res = _flag(cfg, "habitat_map_is_resistances")
-    volt, cur = _flag(cfg, "write_volt_maps"), _flag(cfg, "write_cur_maps")
-    cum_only = _flag(cfg, "write_cum_cur_map_only")
     if scenario == "pairwise":
         pairs = None
         if _flag(cfg, "use_included_pairs"):
-            pairs = read_included_pairs(need("included_pairs_file"), starts_from_zero=read_graph(graph)[3])
-        # a pair's currents are written with any map option; the cumulative tables need write_cur_maps
-        any_maps = volt or cur or cum_only or _flag(cfg, "write_max_cur_maps")
-        result = circuitscape_network_pairwise(graph, need("point_file"), graph_is_resistance=is_res,
-                                               included_pairs=pairs, voltage_maps=volt, current_maps=any_maps,
-                                               n_threads=n_threads)
-        if not cur:
+            pairs = read_included_pairs(_ini_file(cfg, base, "included_pairs_file"),
+                                        starts_from_zero=read_graph(graph)[3])
+        # any map option writes each pair's currents; the cumulative tables come with write_cur_maps
+        any_map = any(_flag(cfg, k) for k in ("write_volt_maps", "write_cur_maps", "write_cum_cur_map_only",
+                                               "write_max_cur_maps"))
+        result = circuitscape_network_pairwise(graph, _ini_file(cfg, base, "point_file"), graph_is_resistance=is_res,
+                                               included_pairs=pairs, voltage_maps=_flag(cfg, "write_volt_maps"),
+                                               current_maps=any_map, n_threads=n_threads)
+        if not _flag(cfg, "write_cur_maps"):
             result.cum_node_currents = result.cum_branch_currents = None
     else:
-        # networks have no one-to-all / all-to-one: every other scenario runs as advanced
-        policy = cfg["remove_src_or_gnd"]
-        if policy == "not entered":
-            policy = "keepall"
-        result = circuitscape_network_advanced(graph, need("source_file"), need("ground_file"),
-                                               graph_is_resistance=is_res,
+        # a network has no one-to-all or all-to-one: every other scenario is solved as advanced
+        policy = "keepall" if cfg["remove_src_or_gnd"] == "not entered" else cfg["remove_src_or_gnd"]
+        result = circuitscape_network_advanced(graph, _ini_file(cfg, base, "source_file"),
+                                               _ini_file(cfg, base, "ground_file"), graph_is_resistance=is_res,
                                                ground_is_resistance=_flag(cfg, "ground_file_is_resistances"),
                                                remove_src_or_gnd=policy, n_threads=n_threads)
-    if not write_outputs:
-        return result
-    prefix, _ = _prefix(cfg)
-    with open(cfg["output_file"], "w", encoding="utf-8") as f:
-        for k, v in cfg.items():
-            f.write(f"{k} = {v}\n")
-    if scenario == "pairwise":
-        r = result.resistances
-        _write_matrix(prefix + "_resistances.out", r)
-        ids = r[1:, 0]
-        three = [(ids[p], ids[q], r[q + 1, p + 1]) for p in range(len(ids)) for q in range(p + 1, len(ids))]
-        _write_matrix(prefix + "_resistances_3columns.out", np.array(three).reshape(-1, 3))
-        for (kind, p, q), table in result.maps.items():
-            _write_table(f"{prefix}_{kind}_{p}_{q}.txt", table)
-        if result.cum_node_currents is not None:
-            _write_table(f"{prefix}_node_currents_cum.txt", result.cum_node_currents)
-            _write_table(f"{prefix}_branch_currents_cum.txt", result.cum_branch_currents)
-    else:
-        if volt:
-            _write_table(f"{prefix}_voltages.txt", result.voltages)
-        if cur or cum_only:
-            _write_table(f"{prefix}_node_currents.txt", result.node_currents)
-            _write_table(f"{prefix}_branch_currents.txt", result.branch_currents)
+    if write_outputs:
+        _write_network_outputs(cfg, scenario, result)
     return result
