@@ -1,9 +1,11 @@
-"""Omniscape: omnidirectional circuit connectivity over moving windows.
+"""Omniscape: omnidirectional circuit connectivity over moving windows, on the GPU window engine.
 
-A numpy restatement of the reference's Python front end
-(src/landschaft/circuit/omniscape.py:102-404) over the GPU backend. Results
-are an ``xarray.Dataset`` when xarray is importable, else a light
-:class:`Dataset` with the same variable names and ``attrs``.
+The public function, its arguments, outputs (``cum_currmap``, ``flow_potential``,
+``normalized_cum_currmap`` and ``attrs``) and error messages are those of the reference's
+src/landschaft/circuit/omniscape.py:102-404, so callers can switch modules. The call is prepared in
+three steps here: the rasters (precision, reclassification, sources), the conditions, then the maps
+from ``core().circuit_omniscape``. Results are an ``xarray.Dataset`` when xarray is importable, else a
+light :class:`Dataset` with the same variable names and ``attrs``.
 """
 
 from __future__ import annotations
@@ -18,6 +20,7 @@ from .advanced import _grid
 COMPARISONS = ("within", "equal")
 COMPARE_TO_FUTURE = ("none", "1", "2", "both")
 PRECISIONS = ("double", "single")
+_OUTPUT_ORDER = ("cum_currmap", "flow_potential", "normalized_cum_currmap")
 
 
 class Variable(np.ndarray):
@@ -47,44 +50,50 @@ class Dataset(dict):
 
 
 def _to_int(value, name: str) -> int:
-    # omniscape.py:43-48: Int64(round(parse(Float64, value))), halves to even.
+    """An integer option as Omniscape reads one (omniscape.py:43-48): the nearest integer, halves to
+    even."""
     v = float(value)
-    if not np.isfinite(v):
+    if v != v or v in (np.inf, -np.inf):
         raise ValueError(f"{name} must be finite")
-    return int(round(v))
+    return round(v)
 
 
 def read_reclass_table(path) -> np.ndarray:
     """Omniscape's reclass table: (value, new value) rows, ``missing`` = NaN (omniscape.py:51-82)."""
-    rows = []
+
+    def number(tok: str) -> float:
+        return np.nan if tok.strip('"') == "missing" else float(tok)
+
+    pairs = []
     with open(path, encoding="utf-8") as f:
-        for line in f:
-            parts = line.split()
-            if not parts:
-                continue
-            if len(parts) != 2:
-                raise ValueError(f"reclass table lines need two columns: {line.strip()!r}")
-            rows.append([np.nan if p.strip('"') == "missing" else float(p) for p in parts])
-    return np.array(rows, dtype=np.float64).reshape(-1, 2)
+        for raw in f:
+            cols = raw.split()
+            if cols and len(cols) != 2:
+                raise ValueError(f"reclass table lines need two columns: {raw.strip()!r}")
+            if cols:
+                pairs.append((number(cols[0]), number(cols[1])))
+    return np.asarray(pairs, dtype=np.float64).reshape(-1, 2)
 
 
-def _reclass_table(table) -> np.ndarray:
+def _reclass_pairs(table) -> np.ndarray:
+    """A reclass table given as a path, a {value: new} dict (None = NoData) or (value, new) rows."""
     if isinstance(table, (str, bytes)) or hasattr(table, "__fspath__"):
         return read_reclass_table(table)
     if isinstance(table, dict):
-        table = [[k, np.nan if v is None else v] for k, v in table.items()]
-    t = np.array(table, dtype=np.float64)
-    if t.ndim != 2 or t.shape[1] != 2:
+        table = [(k, np.nan if v is None else v) for k, v in table.items()]
+    pairs = np.asarray(table, dtype=np.float64)
+    if pairs.ndim != 2 or pairs.shape[1] != 2:
         raise ValueError("reclass_table must have two columns (value, new value)")
-    return t
+    return pairs
 
 
 def reclassify(resistance: np.ndarray, table) -> np.ndarray:
-    """reclassify_resistance!: rows applied in turn, matched on the original values (omniscape.py:102-133)."""
-    t = _reclass_table(table)
-    out = resistance.copy()
-    for old, new in t:
-        out[resistance == old] = new
+    """reclassify_resistance!: each (value, new value) row applied in turn to the cells whose
+    original value it names (omniscape.py:102-133), so chains do not cascade."""
+    original = np.asarray(resistance)
+    out = original.copy()
+    for old, new in _reclass_pairs(table):
+        np.copyto(out, new, where=original == old)
     return out
 
 
@@ -100,20 +109,99 @@ def source_from_resistance(resistance, *, r_cutoff: float = np.inf, resistance_i
     r, _ = _grid(resistance, "resistance")
     if reclass_table is not None:
         r = reclassify(r, reclass_table)
-    with np.errstate(divide="ignore"):
-        s = r if resistance_is_conductance else 1 / r
-    return np.where(np.isnan(s) | (s < 1 / float(r_cutoff)), 0.0, s)
+    if resistance_is_conductance:
+        strength = r
+    else:
+        with np.errstate(divide="ignore"):
+            strength = np.reciprocal(r)
+    low = 1.0 / float(r_cutoff)
+    return np.where(np.isnan(strength) | (strength < low), 0.0, strength)
 
 
-def _condition(value, shape, name, dtype):
-    if value is None:
-        return None
-    v, _ = _grid(value, name)
-    if v.shape != shape:
-        raise ValueError(f"{name} must have the resistance's shape")
-    if dtype is np.float32:
-        v = v.astype(np.float32).astype(np.float64)
-    return np.ascontiguousarray(v)
+def _rounded(a: np.ndarray, single: bool) -> np.ndarray:
+    """float64, through float32 first for the single-precision option (which rounds the inputs only)."""
+    return a.astype(np.float32).astype(np.float64) if single else a
+
+
+def _require_positive(r: np.ndarray, message: str) -> None:
+    data = r[r == r]
+    if data.size and data.min() <= 0:
+        raise ValueError(message)
+
+
+def _window_options(radius, buffer, block_size):
+    radius, buffer, block_size = (_to_int(v, k) for v, k in ((radius, "radius"), (buffer, "buffer"),
+                                                              (block_size, "block_size")))
+    if min(radius, buffer) < 0 or block_size < 1:
+        raise ValueError("radius and buffer must be >= 0 and block_size >= 1")
+    if not block_size & 1:
+        warnings.warn("block_size is even, but must be odd. Using block_size + 1.", stacklevel=3)
+        block_size += 1
+    return radius, buffer, block_size
+
+
+def _source_map(r, source, single, source_threshold, r_cutoff, resistance_is_conductance, reclass_table):
+    if source is None:
+        s = source_from_resistance(r, r_cutoff=r_cutoff, resistance_is_conductance=resistance_is_conductance,
+                                   reclass_table=reclass_table)
+    else:
+        s, _ = _grid(source, "source")
+        if s.shape != r.shape:
+            raise ValueError("the source raster must have the resistance's shape")
+        s = _rounded(s, single)
+        s = np.where((s != s) | (s < source_threshold), 0.0, s)
+    return np.ascontiguousarray(s, dtype=np.float64)
+
+
+def _condition_layers(shape, single, c1, c2, f1, f2, compare_to_future, comparison1, comparison2, bounds):
+    """The binding's conditions dict (empty without condition1), validated."""
+    if c1 is None:
+        if c2 is not None:
+            raise ValueError("condition2 needs condition1")
+        if f1 is not None or f2 is not None:
+            raise ValueError("future conditions need condition1")
+        return {}
+    if compare_to_future not in COMPARE_TO_FUTURE:
+        raise ValueError(f"compare_to_future must be one of {COMPARE_TO_FUTURE}")
+    for k, cmp in enumerate((comparison1, comparison2), start=1):
+        if cmp not in COMPARISONS:
+            raise ValueError(f"comparison{k} must be one of {COMPARISONS}")
+    use1 = compare_to_future in ("1", "both")
+    use2 = compare_to_future in ("2", "both")
+    if use1 and f1 is None:
+        raise ValueError("compare_to_future needs condition1_future")
+    if use2 and c2 is not None and f2 is None:
+        raise ValueError("compare_to_future needs condition2_future")
+
+    def layer(value, name):
+        if value is None:
+            return None
+        v, _ = _grid(value, name)
+        if v.shape != shape:
+            raise ValueError(f"{name} must have the resistance's shape")
+        return np.ascontiguousarray(_rounded(v, single))
+
+    out = {"condition1": layer(c1, "condition1"), "condition1_future": layer(f1 if use1 else None, "condition1_future"),
+           "condition2": layer(c2, "condition2"), "condition2_future": layer(f2 if use2 else None, "condition2_future"),
+           "comparison1": comparison1, "comparison2": comparison2}
+    out.update({k: float(v) for k, v in bounds.items()})
+    return out
+
+
+def _maps(out, nodata, flow_potential, normalized_current, mask_nodata, single) -> dict:
+    maps = {"cum_currmap": out["current"]}
+    if flow_potential or normalized_current:
+        fp = out["flow_potential"]
+        if flow_potential:
+            maps["flow_potential"] = fp
+        if normalized_current:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ratio = out["current"] / fp
+            maps["normalized_cum_currmap"] = np.nan_to_num(ratio, nan=0.0, posinf=np.inf, neginf=-np.inf)
+    for m in maps.values():
+        if mask_nodata:
+            m[nodata] = np.nan
+    return {k: (maps[k].astype(np.float32) if single else maps[k]) for k in _OUTPUT_ORDER if k in maps}
 
 
 def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source_threshold: float = 0.0,
@@ -139,94 +227,27 @@ def omniscape(resistance, radius, *, source=None, block_size=1, buffer=0, source
     """
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {PRECISIONS}")
-    dtype = np.float32 if precision == "single" else np.float64
-    r, _ = _grid(resistance, "resistance")
-    if dtype is np.float32:
-        r = r.astype(np.float32).astype(np.float64)
-    finite = r[~np.isnan(r)]
-    if finite.size and finite.min() <= 0:
-        raise ValueError("resistance (or conductance) values must be > 0")
-    radius = _to_int(radius, "radius")
-    buffer = _to_int(buffer, "buffer")
-    block_size = _to_int(block_size, "block_size")
-    if radius < 0 or buffer < 0 or block_size < 1:
-        raise ValueError("radius and buffer must be >= 0 and block_size >= 1")
-    if block_size % 2 == 0:
-        warnings.warn("block_size is even, but must be odd. Using block_size + 1.", stacklevel=2)
-        block_size += 1
-    if source is None:
-        s = source_from_resistance(r, r_cutoff=r_cutoff, resistance_is_conductance=resistance_is_conductance,
-                                   reclass_table=reclass_table)
-    else:
-        s, _ = _grid(source, "source")
-        if s.shape != r.shape:
-            raise ValueError("the source raster must have the resistance's shape")
-        if dtype is np.float32:
-            s = s.astype(np.float32).astype(np.float64)
-        s = np.where(np.isnan(s), 0.0, s)
-        s = np.where(s < source_threshold, 0.0, s)
-    s = np.ascontiguousarray(s, dtype=np.float64)
+    single = precision == "single"
+    r = _rounded(_grid(resistance, "resistance")[0], single)
+    _require_positive(r, "resistance (or conductance) values must be > 0")
+    radius, buffer, block_size = _window_options(radius, buffer, block_size)
+    s = _source_map(r, source, single, source_threshold, r_cutoff, resistance_is_conductance, reclass_table)
     if reclass_table is not None:
-        r = reclassify(r, reclass_table)
-        if dtype is np.float32:
-            r = r.astype(np.float32).astype(np.float64)
-        finite = r[~np.isnan(r)]
-        if finite.size and finite.min() <= 0:
-            raise ValueError("the reclassified resistance must be > 0")
+        r = _rounded(reclassify(r, reclass_table), single)
+        _require_positive(r, "the reclassified resistance must be > 0")
     r = np.ascontiguousarray(r)
-    # conditional connectivity (omniscape.py:344-368 of the reference)
-    conditions = {}
-    if condition2 is not None and condition1 is None:
-        raise ValueError("condition2 needs condition1")
-    if condition1 is not None:
-        if compare_to_future not in COMPARE_TO_FUTURE:
-            raise ValueError(f"compare_to_future must be one of {COMPARE_TO_FUTURE}")
-        for k, cmp in ((1, comparison1), (2, comparison2)):
-            if cmp not in COMPARISONS:
-                raise ValueError(f"comparison{k} must be one of {COMPARISONS}")
-        future1 = condition1_future if compare_to_future in ("1", "both") else None
-        future2 = condition2_future if compare_to_future in ("2", "both") else None
-        if compare_to_future in ("1", "both") and condition1_future is None:
-            raise ValueError("compare_to_future needs condition1_future")
-        if compare_to_future in ("2", "both") and condition2 is not None and condition2_future is None:
-            raise ValueError("compare_to_future needs condition2_future")
-        conditions = {
-            "condition1": _condition(condition1, r.shape, "condition1", dtype),
-            "condition1_future": _condition(future1, r.shape, "condition1_future", dtype),
-            "condition2": _condition(condition2, r.shape, "condition2", dtype),
-            "condition2_future": _condition(future2, r.shape, "condition2_future", dtype),
-            "comparison1": comparison1, "comparison2": comparison2,
-            "condition1_lower": float(condition1_lower), "condition1_upper": float(condition1_upper),
-            "condition2_lower": float(condition2_lower), "condition2_upper": float(condition2_upper),
-        }
-    elif any(c is not None for c in (condition1_future, condition2_future)):
-        raise ValueError("future conditions need condition1")
-    compute_fp = bool(flow_potential or normalized_current)
+    conditions = _condition_layers(r.shape, single, condition1, condition2, condition1_future, condition2_future,
+                                   compare_to_future, comparison1, comparison2,
+                                   {"condition1_lower": condition1_lower, "condition1_upper": condition1_upper,
+                                    "condition2_lower": condition2_lower, "condition2_upper": condition2_upper})
     out = core().circuit_omniscape(r, s, radius, buffer, block_size, bool(four_neighbors),
-                                   bool(resistance_is_conductance), bool(correct_artifacts), compute_fp,
-                                   int(n_threads), conditions, 0.0 if memory_limit is None else float(memory_limit))
-    cum = out["current"]
-    data = {"cum_currmap": cum}
-    nodata = np.isnan(r)
-    if compute_fp:
-        fp = out["flow_potential"]
-        if normalized_current:
-            with np.errstate(divide="ignore", invalid="ignore"):
-                norm = cum / fp
-            norm[np.isnan(norm)] = 0.0
-            data["normalized_cum_currmap"] = norm
-        if flow_potential:
-            data["flow_potential"] = fp
-    if mask_nodata:
-        for v in data.values():
-            v[nodata] = np.nan
-    if dtype is np.float32:
-        data = {k: v.astype(np.float32) for k, v in data.items()}
+                                   bool(resistance_is_conductance), bool(correct_artifacts),
+                                   bool(flow_potential or normalized_current), int(n_threads), conditions,
+                                   float(memory_limit or 0.0))
+    data = _maps(out, np.isnan(r), flow_potential, normalized_current, mask_nodata, single)
     attrs = {"radius": radius, "block_size": block_size, "buffer": buffer, "n_targets": int(out["n_targets"]),
              "four_neighbors": int(bool(four_neighbors)), "correct_artifacts": int(bool(correct_artifacts)),
              "precision": precision, "conditional": int(condition1 is not None), "threads": int(out["threads"])}
-    order = ["cum_currmap", "flow_potential", "normalized_cum_currmap"]
-    data = {k: data[k] for k in order if k in data}
     try:
         import xarray as xr
     except ImportError:
