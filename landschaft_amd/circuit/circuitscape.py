@@ -509,25 +509,26 @@ def circuitscape_advanced(habitat, source, ground, *, habitat_is_resistance: boo
                      direct_grounds, n_threads)
 
 
+def _on_cells(nodemap: np.ndarray, per_node: np.ndarray) -> np.ndarray:
+    """Values per node spread over each node's cells; 0 on cells without a node."""
+    if per_node.size == 0:
+        return np.zeros(nodemap.shape)
+    return np.where(nodemap >= 0, per_node[np.maximum(nodemap, 0)], 0.0)
+
+
 def _advanced(g, poly, src, gnd, four, avg, policy, unit, direct, n_threads) -> CircuitscapeResult:
-    # circuitscape.py:499-516: sources in amps and ground conductances per cell, then the solve
-    if unit:
-        src = np.where(src != 0, 1.0, src)
-    if direct:
-        gnd = np.where(gnd != 0, np.inf, gnd)
-    r = core().circuitscape_advanced(np.ascontiguousarray(g, dtype=np.float64), poly,
-                                     np.ascontiguousarray(src, dtype=np.float64),
-                                     np.ascontiguousarray(gnd, dtype=np.float64), bool(avg), bool(four), policy,
-                                     int(n_threads))
-    nm = r["nodemap"]
-    on = nm >= 0
-    volt = np.zeros(nm.shape)
-    cur = np.zeros(nm.shape)
-    volt[on] = r["voltage"][nm[on]]
-    cur[on] = r["current"][nm[on]]
-    res = CircuitscapeResult(voltage=volt, current=cur, nodemap=nm)
-    res.maps["solved"] = bool(r["solved"])
-    return res
+    """One advanced solve (circuitscape.py:499-516): amps per cell (1 A with ``unit``), ground
+    conductances per cell (every ground direct with ``direct``), the maps back on the cells."""
+    amps = np.where(src != 0, 1.0, src) if unit else src
+    to_ground = np.where(gnd != 0, np.inf, gnd) if direct else gnd
+    as_f64 = [np.ascontiguousarray(x, dtype=np.float64) for x in (g, amps, to_ground)]
+    out = core().circuitscape_advanced(as_f64[0], poly, as_f64[1], as_f64[2], bool(avg), bool(four), policy,
+                                       int(n_threads))
+    nm = out["nodemap"]
+    result = CircuitscapeResult(voltage=_on_cells(nm, out["voltage"]), current=_on_cells(nm, out["current"]),
+                                nodemap=nm)
+    result.maps["solved"] = bool(out["solved"])
+    return result
 
 
 # ---- pairwise
@@ -1017,6 +1018,58 @@ def _source_and_ground(source_path, ground_path, meta: RasterMeta, ground_is_res
     return src, gnd
 
 
+class _IniRun:
+    """A Circuitscape INI file over its defaults: flags, and the input files it names (found in the
+    working directory, else next to the INI file)."""
+
+    def __init__(self, cfg: dict, base: Path):
+        self.cfg, self.base = cfg, base
+
+    def flag(self, key: str) -> bool:
+        return _flag(self.cfg, key)
+
+    def file(self, key: str) -> Path:
+        value = self.cfg[key]
+        if _unset(value):
+            raise ValueError(f"{key} is not set")
+        path = _resolve(value, self.base)
+        if not path.is_file():
+            raise FileNotFoundError(f'{key} = "{value}" does not exist')
+        return path
+
+    def optional_file(self, switch: str, key: str):
+        return self.file(key) if self.flag(switch) else None
+
+    def scenario(self) -> str:
+        name = SCENARIOS.get(self.cfg["scenario"])
+        if name is None:
+            raise ValueError(f'scenario = "{self.cfg["scenario"]}" is not one of {sorted(set(SCENARIOS.values()))}')
+        return name
+
+    def policy(self) -> str:
+        chosen = self.cfg["remove_src_or_gnd"]
+        chosen = "keepall" if chosen == "not entered" else chosen  # Circuitscape 4's default
+        if chosen not in POLICIES:
+            raise ValueError(f"remove_src_or_gnd must be one of {POLICIES}")
+        return chosen
+
+    def habitat(self):
+        """The conductance on the habitat's grid (reclassified, masked), the polygons and the grid."""
+        habitat = self.file("habitat_file")
+        reclass_file = self.optional_file("use_reclass_table", "reclass_file")
+        cells, meta = read_raster(habitat)
+        g = habitat_conductance(cells, is_resistance=self.flag("habitat_map_is_resistances"),
+                                reclass=None if reclass_file is None else read_reclass_table(reclass_file))
+        poly = None
+        if self.flag("use_polygons"):
+            poly = np.ascontiguousarray(read_polymap(self.file("polygon_file"), meta, integer=True), dtype=np.int64)
+        if self.flag("use_mask"):
+            g = g * (read_polymap(self.file("mask_file"), meta) > 0)
+            if g.sum() == 0:
+                raise ValueError("Mask file deleted everything!")
+        return g, poly, meta
+
+
 def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0):
     """Run a Circuitscape INI file, as ``Circuitscape.compute(path)`` does (circuitscape.py:1039-1213).
 
@@ -1034,122 +1087,87 @@ def circuitscape_ini(path, *, write_outputs: bool = True, n_threads: int = 0):
     ``precision`` and ``parallelize`` have no effect: every solve runs on the
     GPU to the solver settings' residual.
     """
-    path = Path(path)
-    base = path.resolve().parent
+    base = Path(path).resolve().parent
     cfg = {**DEFAULTS, **parse_config(path)}
-    if cfg["data_type"] in ("network", "Network"):
+    kind = cfg["data_type"].lower() if cfg["data_type"] in ("raster", "Raster", "network", "Network") else None
+    if kind == "network":
         from .network import network_ini
 
         return network_ini(cfg, base, write_outputs=write_outputs, n_threads=n_threads)
-    if cfg["data_type"] not in ("raster", "Raster"):
+    if kind != "raster":
         raise ValueError(f'data_type = "{cfg["data_type"]}": expected raster or network')
-    scenario = SCENARIOS.get(cfg["scenario"])
-    if scenario is None:
-        raise ValueError(f'scenario = "{cfg["scenario"]}" is not one of {sorted(set(SCENARIOS.values()))}')
+    run = _IniRun(cfg, base)
+    scenario = run.scenario()
     for key in ("low_memory_mode", "preemptive_memory_release"):
         if cfg.get(key, "False") in TRUELIST:
             raise ValueError(f"{key} is not implemented in Circuitscape.jl; remove it or set it to False")
-
-    def need(key):
-        if _unset(cfg[key]):
-            raise ValueError(f"{key} is not set")
-        p = _resolve(cfg[key], base)
-        if not p.is_file():
-            raise FileNotFoundError(f'{key} = "{cfg[key]}" does not exist')
-        return p
-
-    habitat = need("habitat_file")
-    reclass = read_reclass_table(need("reclass_file")) if _flag(cfg, "use_reclass_table") else None
-    cells, meta = read_raster(habitat)
-    g = habitat_conductance(cells, is_resistance=_flag(cfg, "habitat_map_is_resistances"), reclass=reclass)
-    poly = None
-    if _flag(cfg, "use_polygons"):
-        poly = np.ascontiguousarray(read_polymap(need("polygon_file"), meta, integer=True).astype(np.int64))
-    if _flag(cfg, "use_mask"):
-        m = read_polymap(need("mask_file"), meta)
-        g = g * np.where(m > 0, 1.0, 0.0)
-        if g.sum() == 0:
-            raise ValueError("Mask file deleted everything!")
-    four = _flag(cfg, "connect_four_neighbors_only")
-    avg = _flag(cfg, "connect_using_avg_resistances")
-    policy = cfg["remove_src_or_gnd"]
-    if policy == "not entered":  # Circuitscape 4's default
-        policy = "keepall"
-    if policy not in POLICIES:
-        raise ValueError(f"remove_src_or_gnd must be one of {POLICIES}")
+    g, poly, meta = run.habitat()
+    four, avg = run.flag("connect_four_neighbors_only"), run.flag("connect_using_avg_resistances")
+    policy = run.policy()
+    # the per-pair and per-point maps: written as they come, or kept in result.maps
     prefix, tif = _prefix(cfg) if write_outputs and scenario != "advanced" else (None, False)
-    maps: dict = {}
+    kept: dict = {}
 
-    def put(name, grid):
-        if grid is None:
-            return
-        if write_outputs:
-            _write_grid(f"{prefix}_{name}", grid, meta, tif)
-        else:
-            maps[name] = grid
+    def deliver(name, grid):
+        if grid is not None:
+            if write_outputs:
+                _write_grid(f"{prefix}_{name}", grid, meta, tif)
+            else:
+                kept[name] = grid
 
     if scenario == "advanced":
-        src, gnd = _source_and_ground(need("source_file"), need("ground_file"), meta,
-                                      _flag(cfg, "ground_file_is_resistances"))
-        result = _advanced(g, poly, src, gnd, four, avg, policy, _flag(cfg, "use_unit_currents"),
-                           _flag(cfg, "use_direct_grounds"), n_threads)
-    elif scenario == "pairwise":
-        points = read_point_map(need("point_file"), meta)
-        pairs = read_included_pairs(need("included_pairs_file")) if _flag(cfg, "use_included_pairs") else None
-        points, excluded = _apply_included_pairs(points, pairs)
-        opts = _pair_options(_flag(cfg, "write_volt_maps"), _flag(cfg, "write_cur_maps"),
-                             _flag(cfg, "write_cum_cur_map_only"), _flag(cfg, "write_max_cur_maps"),
-                             _flag(cfg, "set_focal_node_currents_to_zero"), _flag(cfg, "set_null_currents_to_nodata"),
-                             _flag(cfg, "set_null_voltages_to_nodata"), _flag(cfg, "log_transform_maps"))
-
-        def emit(a, b, volt, cur):
-            put(f"voltmap_{a}_{b}", volt)
-            put(f"curmap_{a}_{b}", cur)
-
-        res, r = _pairwise(g, poly, points, excluded, four, avg, opts, emit, n_threads)
-        result = CircuitscapeResult(resistances=res, cum_current=r.get("cum"), max_current=r.get("max"))
+        src, gnd = _source_and_ground(run.file("source_file"), run.file("ground_file"), meta,
+                                      run.flag("ground_file_is_resistances"))
+        result = _advanced(g, poly, src, gnd, four, avg, policy, run.flag("use_unit_currents"),
+                           run.flag("use_direct_grounds"), n_threads)
     else:
-        points = read_point_map(need("point_file"), meta)
-        pairs = read_included_pairs(need("included_pairs_file")) if _flag(cfg, "use_included_pairs") else None
-        strengths = (read_point_strengths(need("variable_source_file"))
-                     if _flag(cfg, "use_variable_source_strengths") else None)
-        cum_only = _flag(cfg, "write_cum_cur_map_only")
-        # Circuitscape writes every point's current map with write_cum_cur_map_only too
-        opts = {"volt_maps": _flag(cfg, "write_volt_maps"), "cur_maps": _flag(cfg, "write_cur_maps") or cum_only,
-                "cum": _flag(cfg, "write_cur_maps") or cum_only, "max": _flag(cfg, "write_max_cur_maps"),
-                "focal_zero": _flag(cfg, "set_focal_node_currents_to_zero")}
-
-        def emit_point(n, volt, cur):
-            put(f"voltmap_{n}", volt)
-            put(f"curmap_{n}", cur)
-
-        result = _one_to_all(g, poly, points, strengths, pairs, scenario == "one-to-all", four, avg, opts,
-                             emit_point, n_threads)
-    result.maps.update(maps)
+        points = read_point_map(run.file("point_file"), meta)
+        pairs_file = run.optional_file("use_included_pairs", "included_pairs_file")
+        pairs = None if pairs_file is None else read_included_pairs(pairs_file)
+        if scenario == "pairwise":
+            points, excluded = _apply_included_pairs(points, pairs)
+            opts = _pair_options(*(run.flag(k) for k in (
+                "write_volt_maps", "write_cur_maps", "write_cum_cur_map_only", "write_max_cur_maps",
+                "set_focal_node_currents_to_zero", "set_null_currents_to_nodata", "set_null_voltages_to_nodata",
+                "log_transform_maps")))
+            res, out = _pairwise(g, poly, points, excluded, four, avg, opts,
+                                 lambda a, b, volt, cur: (deliver(f"voltmap_{a}_{b}", volt),
+                                                          deliver(f"curmap_{a}_{b}", cur)), n_threads)
+            result = CircuitscapeResult(resistances=res, cum_current=out.get("cum"), max_current=out.get("max"))
+        else:
+            strengths_file = run.optional_file("use_variable_source_strengths", "variable_source_file")
+            strengths = None if strengths_file is None else read_point_strengths(strengths_file)
+            # Circuitscape writes every point's current map with write_cum_cur_map_only too
+            point_maps = run.flag("write_cur_maps") or run.flag("write_cum_cur_map_only")
+            opts = {"volt_maps": run.flag("write_volt_maps"), "cur_maps": point_maps, "cum": point_maps,
+                    "max": run.flag("write_max_cur_maps"), "focal_zero": run.flag("set_focal_node_currents_to_zero")}
+            result = _one_to_all(g, poly, points, strengths, pairs, scenario == "one-to-all", four, avg, opts,
+                                 lambda n, volt, cur: (deliver(f"voltmap_{n}", volt), deliver(f"curmap_{n}", cur)),
+                                 n_threads)
+    result.maps.update(kept)
     if write_outputs:
         _write_outputs(cfg, scenario, result, meta)
     return result
 
 
 def _write_outputs(cfg: dict, scenario: str, result: CircuitscapeResult, meta: RasterMeta) -> None:
-    # circuitscape.py:1261-1286: the configuration, the maps, and pairwise's resistance tables
+    """The configuration echo, the run's maps and, for pairwise, the resistance tables
+    (circuitscape.py:1261-1286)."""
     prefix, tif = _prefix(cfg)
-    with open(cfg["output_file"], "w", encoding="utf-8") as f:
-        for k, v in cfg.items():
-            f.write(f"{k} = {v}\n")
+    Path(cfg["output_file"]).write_text("".join(f"{k} = {v}\n" for k, v in cfg.items()), encoding="utf-8")
     if scenario == "advanced":
-        if _flag(cfg, "write_volt_maps"):
-            _write_grid(prefix + "_voltmap", result.voltage, meta, tif)
-        if _flag(cfg, "write_cur_maps") or _flag(cfg, "write_cum_cur_map_only"):
-            _write_grid(prefix + "_curmap", result.current, meta, tif)
-        return
-    if result.cum_current is not None:
-        _write_grid(prefix + "_cum_curmap", result.cum_current, meta, tif)
-    if result.max_current is not None:
-        _write_grid(prefix + "_max_curmap", result.max_current, meta, tif)
+        grids = [("_voltmap", result.voltage, _flag(cfg, "write_volt_maps")),
+                 ("_curmap", result.current, _flag(cfg, "write_cur_maps") or _flag(cfg, "write_cum_cur_map_only"))]
+    else:
+        grids = [("_cum_curmap", result.cum_current, result.cum_current is not None),
+                 ("_max_curmap", result.max_current, result.max_current is not None)]
+    for suffix, grid, wanted in grids:
+        if wanted:
+            _write_grid(prefix + suffix, grid, meta, tif)
     if scenario == "pairwise":
-        r = result.resistances
-        _write_matrix(prefix + "_resistances.out", r)
-        ids = r[1:, 0]
-        three = [(ids[i], ids[j], r[j + 1, i + 1]) for i in range(len(ids)) for j in range(i + 1, len(ids))]
-        _write_matrix(prefix + "_resistances_3columns.out", np.array(three).reshape(-1, 3))
+        table = result.resistances
+        ids = table[1:, 0]
+        lo, hi = np.triu_indices(ids.size, 1)
+        _write_matrix(prefix + "_resistances.out", table)
+        _write_matrix(prefix + "_resistances_3columns.out",
+                      np.stack([ids[lo], ids[hi], table[hi + 1, lo + 1]], axis=1).reshape(-1, 3))
