@@ -12,27 +12,34 @@ import numpy as np
 from ._backend import core
 
 
-def _grid(values, name: str, *, band=None):
-    """A raster (array or DataArray) as contiguous float64 with NaN, and the DataArray it came from.
+def _is_path(value) -> bool:
+    return isinstance(value, (str, bytes)) or hasattr(value, "__fspath__")
 
-    The reference also reads paths through rasterio (advanced.py:34-54); this
-    path takes arrays (and DataArrays when xarray is importable).
-    """
-    like = None
-    if hasattr(values, "dims") and hasattr(values, "values"):
-        da = values.squeeze(drop=True) if "band" in values.dims else values
-        v = np.array(da.values, dtype=np.float64)
-        nodata = da.rio.nodata if hasattr(da, "rio") else None
-        if nodata is not None and not np.isnan(nodata):
-            v[v == nodata] = np.nan
-        like = da
-    elif isinstance(values, (str, bytes)) or hasattr(values, "__fspath__"):
+
+def _dataarray_values(da) -> np.ndarray:
+    """A DataArray's cells as float64 (a band dimension squeezed away), its NoData value as NaN."""
+    if "band" in da.dims:
+        da = da.squeeze(drop=True)
+    cells = np.asarray(da.values, dtype=np.float64).copy()
+    marker = getattr(getattr(da, "rio", None), "nodata", None)
+    if marker is not None and marker == marker:
+        cells[cells == marker] = np.nan
+    return cells, da
+
+
+def _grid(values, name: str, *, band=None):
+    """A raster as contiguous float64 with NaN for NoData, and the DataArray it came from (None for an
+    array). Arrays (masked cells as NaN) and DataArrays (with their NoData value) are taken; the
+    reference also opens paths with rasterio (advanced.py:34-54), which this path does not use."""
+    if _is_path(values):
         raise ValueError(f"the {name} must be an array (raster files are read by the caller)")
+    if hasattr(values, "dims") and hasattr(values, "values"):
+        cells, like = _dataarray_values(values)
     else:
-        v = np.array(np.ma.filled(np.ma.asarray(values, dtype=np.float64), np.nan), dtype=np.float64)
-    if v.ndim != 2:
+        cells, like = np.ma.asarray(values, dtype=np.float64).filled(np.nan).astype(np.float64), None
+    if cells.ndim != 2:
         raise ValueError(f"the {name} must be a 2-D raster")
-    return np.ascontiguousarray(v), like
+    return np.ascontiguousarray(cells), like
 
 
 @dataclass
