@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -304,65 +305,64 @@ RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny
     return g;
 }
 
-// A network's graph (reference network_graph, circuitscape.cpp:265-338):
-// sparse(a, b, w) with duplicate entries summed in input order, then A + A'
-// with the zero sums and the self loops left out; node c's neighbours in
-// ascending order. No grid: nx = ny = 0 and no node has a cell.
+// A network's graph (reference network_graph, circuitscape.cpp:265-338): the
+// edge list as a sparse matrix whose repeated positions add up in input order,
+// made symmetric (a position holds its value plus its mirror's), self loops
+// and zero sums left out; each node's neighbours ascending. No grid: nx = ny =
+// 0 and no node has a cell.
 RGraph network_graph(const i32* a, const i32* b, const double* w, i64 ne, i32 m) {
-    struct Entry {
-        i32 r, c;
-        double w;
-    };
-    // stable order by (column, row), so entries of one position keep the input order
-    auto by_column = [m](std::vector<Entry>& v) {
-        std::stable_sort(v.begin(), v.end(), [](const Entry& x, const Entry& y) {
-            return x.c != y.c ? x.c < y.c : x.r < y.r;
-        });
-        (void)m;
-    };
-    std::vector<Entry> in(ne);
-    for (i64 k = 0; k < ne; ++k) {
+    for (i64 k = 0; k < ne; ++k)
         if (a[k] < 0 || b[k] < 0 || a[k] >= m || b[k] >= m) throw std::invalid_argument("node index out of range");
-        in[k] = {a[k], b[k], w[k]};
+    // 1. positions (column-major keys); a stable order keeps repeats in input order
+    const auto key = [&](i64 k) { return static_cast<std::uint64_t>(b[k]) * static_cast<std::uint64_t>(m) + a[k]; };
+    std::vector<i64> order(static_cast<std::size_t>(ne));
+    std::iota(order.begin(), order.end(), i64{0});
+    std::stable_sort(order.begin(), order.end(), [&](i64 x, i64 y) { return key(x) < key(y); });
+    std::vector<i32> pr, pc;  // the distinct off-diagonal positions and their summed values
+    std::vector<double> pv;
+    for (std::size_t q = 0; q < order.size();) {
+        const i64 k0 = order[q];
+        double sum = w[k0];
+        std::size_t t = q + 1;
+        for (; t < order.size() && key(order[t]) == key(k0); ++t) sum += w[order[t]];
+        if (a[k0] != b[k0]) pr.push_back(a[k0]), pc.push_back(b[k0]), pv.push_back(sum);
+        q = t;
     }
-    by_column(in);
-    std::vector<Entry> A;
-    for (std::size_t k = 0; k < in.size();) {
-        Entry e = in[k];
-        std::size_t t = k + 1;
-        for (; t < in.size() && in[t].r == e.r && in[t].c == e.c; ++t) e.w += in[t].w;
-        A.push_back(e);
-        k = t;
-    }
-    std::vector<Entry> both;
-    both.reserve(2 * A.size());
-    for (const Entry& e : A) {
-        if (e.r == e.c) continue;
-        both.push_back({e.r, e.c, e.w});
-        both.push_back({e.c, e.r, e.w});
-    }
-    by_column(both);
+    // 2. every position and its mirror under the column of each end: at most two
+    // values per (node, neighbour), added (the sum of two does not depend on order)
     RGraph g;
     g.n = m;
     g.node_cell.assign(m, -1);
-    g.ptr.assign(static_cast<std::size_t>(m) + 1, 0);
-    for (std::size_t k = 0; k < both.size();) {
-        double v = both[k].w;
-        std::size_t t = k + 1;
-        if (t < both.size() && both[t].r == both[k].r && both[t].c == both[k].c) v += both[t++].w;
-        if (v != 0) {
-            g.adj.push_back(both[k].r);
-            g.w.push_back(v);
-            ++g.ptr[static_cast<std::size_t>(both[k].c) + 1];
+    std::vector<i64> slot(static_cast<std::size_t>(m) + 1, 0);
+    for (std::size_t e = 0; e < pv.size(); ++e) ++slot[pc[e] + 1], ++slot[pr[e] + 1];
+    for (i32 v = 0; v < m; ++v) slot[v + 1] += slot[v];
+    std::vector<std::pair<i32, double>> nb(static_cast<std::size_t>(slot[m]));
+    {
+        std::vector<i64> fill(slot.begin(), slot.end() - 1);
+        for (std::size_t e = 0; e < pv.size(); ++e) {
+            nb[fill[pc[e]]++] = {pr[e], pv[e]};
+            nb[fill[pr[e]]++] = {pc[e], pv[e]};
         }
-        k = t;
     }
-    for (i32 v = 0; v < m; ++v) g.ptr[v + 1] += g.ptr[v];
+    g.ptr.assign(static_cast<std::size_t>(m) + 1, 0);
     g.degree.assign(m, 0.0);
     for (i32 v = 0; v < m; ++v) {
-        double d = 0.0;
-        for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) d += g.w[k];
-        g.degree[v] = d;
+        auto first = nb.begin() + slot[v], last = nb.begin() + slot[v + 1];
+        std::sort(first, last, [](const auto& x, const auto& y) { return x.first < y.first; });
+        double deg = 0.0;
+        for (auto it = first; it != last;) {
+            double sum = it->second;
+            auto next = it + 1;
+            if (next != last && next->first == it->first) sum += (next++)->second;
+            if (sum != 0) {
+                g.adj.push_back(it->first);
+                g.w.push_back(sum);
+                deg += sum;
+            }
+            it = next;
+        }
+        g.degree[v] = deg;
+        g.ptr[v + 1] = static_cast<i64>(g.adj.size());
     }
     label_components(g);
     return g;

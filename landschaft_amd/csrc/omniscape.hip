@@ -531,31 +531,30 @@ template <typename T>
 std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hipStream_t s, IrStats& st) {
     const long long R = a.radius, D = a.radius + a.buffer, W = 2 * D + 1, bs = a.block, br = (bs - 1) / 2;
     const long long n = W * W;
-    std::vector<double> g(n, -9999.0), null_src(n, 0.0), block_src(n, 0.0);
-    double n_sources = 0;
-    for (long long y = 0; y < W; ++y)
-        for (long long x = 0; x < W; ++x) {
-            const long long d2 = (y - D) * (y - D) + (x - D) * (x - D);
-            if (d2 <= D * D) g[y * W + x] = 1.0;
-            if (d2 <= R * R) {
-                null_src[y * W + x] = 1.0;
-                block_src[y * W + x] = 1.0;
-                n_sources += 1;
-            }
-        }
-    null_src[D * W + D] = 0.0;
-    for (auto& v : null_src)
-        if (v != 0.0) v = 1 / (n_sources - 1);
-    for (long long y = D - br; y <= std::min(D + br, ny - 1); ++y)
-        for (long long x = D - br; x <= std::min(D + br, nx - 1); ++x)
-            if (y >= 0 && y < W && x >= 0 && x < W) block_src[y * W + x] = 0.0;
-    double sum = 0.0;
-    for (long long x = 0; x < W; ++x)
-        for (long long y = 0; y < W; ++y)
-            if (block_src[y * W + x] > 0) sum += block_src[y * W + x];
-    const double amps = static_cast<double>(bs * bs);
-    for (auto& v : block_src)
-        if (v > 0) v = (v * amps) / sum;
+    auto within = [&](long long c, long long rad) {
+        const long long dy = c / W - D, dx = c % W - D;
+        return dy * dy + dx * dx <= rad * rad;
+    };
+    // the target block around the centre, as the reference clips it: at the
+    // window and at the map's last row and column
+    auto in_block = [&](long long c) {
+        const long long y = c / W, x = c % W;
+        return y >= D - br && y <= std::min(D + br, ny - 1) && x >= D - br && x <= std::min(D + br, nx - 1);
+    };
+    long long n_src = 0, n_block = 0;  // sources within R; those outside the block
+    for (long long c = 0; c < n; ++c)
+        if (within(c, R)) ++n_src, n_block += !in_block(c);
+    // every source 1 / (n_src - 1) except at the ground (null circuit); the
+    // block's amps bs^2 shared by the sources outside it (block circuit)
+    const double null_amps = 1 / static_cast<double>(n_src - 1);
+    const double block_amps = static_cast<double>(bs * bs) / static_cast<double>(n_block ? n_block : 1);
+    std::vector<double> g(n), block_src(n, 0.0), null_src(n, 0.0);
+    for (long long c = 0; c < n; ++c) {
+        g[c] = within(c, D) ? 1.0 : -9999.0;
+        if (!within(c, R)) continue;
+        if (c != D * W + D) null_src[c] = null_amps;
+        if (!in_block(c)) block_src[c] = block_amps;
+    }
 
     WindowSolver<T> ws(s);
     ws.set_plan(window_plan(static_cast<int>(D), true, a.four != 0), 2);
@@ -581,25 +580,30 @@ std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hi
     if (!act[0]) std::fill(bc.begin(), bc.end(), 0.0);
     if (!act[1]) std::fill(nc.begin(), nc.end(), 0.0);
     (void)st;
-    const long long T2 = 2 * D + bs;
-    std::vector<double> total(T2 * T2, 0.0);
-    for (long long i = 0; i < bs; ++i)
-        for (long long j = 0; j < bs; ++j)
-            for (long long y = 0; y < W; ++y)
-                for (long long x = 0; x < W; ++x) total[(i + y) * T2 + (j + x)] += nc[y * W + x];
-    double top = -std::numeric_limits<double>::infinity();
-    for (double v : bc) top = std::max(top, v);
-    const double floor = 1e-6 * top;
+    // factor = (null currents summed over the bs x bs block of targets
+    // centred on the cell) / (block circuit's current), 1 where the block
+    // current is within 1e-6 of its maximum from 0; the block sum adds its
+    // terms by block row, then column, as the reference's shifted sums do
+    const double floor = 1e-6 * *std::max_element(bc.begin(), bc.end());
     std::vector<double> corr(n);
-    for (long long y = 0; y < W; ++y)
-        for (long long x = 0; x < W; ++x) {
-            double b = bc[y * W + x], t = total[(y + br) * T2 + (x + br)];
-            if (b <= floor) b = 0.0;
-            if (b == 0.0) t = 0.0;
-            if (t == 0.0) t = 1.0;
-            if (b == 0.0) b = 1.0;
-            corr[y * W + x] = t / b;
+    for (long long c = 0; c < n; ++c) {
+        const double b = bc[c];
+        if (b <= floor) {
+            corr[c] = 1.0;
+            continue;
         }
+        const long long y = c / W, x = c % W;
+        double t = 0.0;
+        for (long long i = 0; i < bs; ++i) {
+            const long long yy = y + br - i;
+            if (yy < 0 || yy >= W) continue;
+            for (long long j = 0; j < bs; ++j) {
+                const long long xx = x + br - j;
+                if (xx >= 0 && xx < W) t += nc[yy * W + xx];
+            }
+        }
+        corr[c] = (t == 0.0 ? 1.0 : t) / b;
+    }
     return corr;
 }
 
