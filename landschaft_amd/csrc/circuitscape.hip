@@ -1131,6 +1131,63 @@ void branch_currents(const RGraph& g, const i32* mem, i64 count, const double* v
 // What a network pairwise run returns besides the resistances: per-pair
 // tables through `emit`, and with any map option the node currents summed
 // by node and the branch currents summed by adjacency entry.
+// Job j's voltages relative to its source, by graph node, over one
+// component's members (unknown u of node c: uidx[c]; the grounded node 0).
+__global__ void k_job_voltages(int cnt, const int* __restrict__ mem, const int* __restrict__ uidx,
+                               const double* __restrict__ Xj, double vs, double* __restrict__ vnode) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= cnt) return;
+    const int c = mem[p], u = uidx[c];
+    vnode[c] = (u >= 0 ? Xj[u] : 0.0) - vs;
+}
+
+// One pair's maps on the cells (the raster pairwise's postprocessing): the
+// voltage map (vmap, optional), the current map (cmap, optional) with the
+// focal-zero, log10 and null options, added `times` times into cum and
+// maximised into mx (each when given), one thread per cell, so the sums run
+// in job order as on a single thread. flags: 1 focal currents zero, 2 log10,
+// 4 null currents, 8 null voltages (nodata where the conductance is 0).
+__global__ void k_pair_cells(long long ncell, const int* __restrict__ nodemap, const int* __restrict__ comp, int k,
+                             const double* __restrict__ vnode, const double* __restrict__ cnode,
+                             const unsigned char* __restrict__ zero_g, int flags, int na, int nb, int times,
+                             double* __restrict__ cum, double* __restrict__ mx, double* __restrict__ vmap,
+                             double* __restrict__ cmap) {
+    const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int v = nodemap[c];
+    const bool in = v >= 0 && comp[v] == k;
+    const bool null_cell = zero_g && zero_g[c];
+    if (vmap) vmap[c] = (flags & 8) && null_cell ? -9999.0 : (in ? vnode[v] : 0.0);
+    double x = in ? cnode[v] : 0.0;
+    if (in && (flags & 1) && (v == na || v == nb)) x = 0.0;
+    if (flags & 2) x = x > 0 ? log10(x) : -9999.0;
+    if ((flags & 4) && null_cell) x = -9999.0;
+    if (cmap) cmap[c] = x;
+    if (cum)
+        for (int t = 0; t < times; ++t) cum[c] += x;
+    if (mx && times > 0) mx[c] = mx[c] < x ? x : mx[c];
+}
+
+__global__ void k_fill_value(long long n, double value, double* __restrict__ a) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = value;
+}
+
+// Right-hand sides solved at once in pairwise mode: as many as a dozen
+// vectors of the component's size each fit in half the free device memory,
+// at most 1024 (LSC_PAIR_CHUNK lowers the cap; the results do not depend on
+// it: every right-hand side is solved on its own and the maps add in job order).
+long long pair_chunk(long long jobs, long long n) {
+    std::size_t free_b = 0, total_b = 0;
+    LSC_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double per_rhs = 16.0 * 8.0 * static_cast<double>(std::max<long long>(n, 1));
+    long long cap = 1024;
+    if (const char* e = std::getenv("LSC_PAIR_CHUNK"))
+        if (std::atoi(e) > 0) cap = std::min<long long>(cap, std::atoi(e));
+    const long long fit = static_cast<long long>(0.5 * static_cast<double>(free_b) / per_rhs);
+    return std::max<long long>(1, std::min<long long>({jobs, fit, cap}));
+}
+
 struct NetPairs {
     lsc_network_emit_fn emit = nullptr;
     void* ctx = nullptr;
@@ -1149,13 +1206,7 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
     const bool shortcut = !o.network && !o.maps() && !any_ex;
     std::vector<double> voltmatrix, sres;
     if (shortcut) voltmatrix.assign(static_cast<std::size_t>(np) * np, 0.0), sres.assign(static_cast<std::size_t>(np) * np, -1.0);
-    std::vector<double> cum, mx;
-    if (o.network) {
-        if (o.maps()) net->cum.assign(g.n, 0.0), net->cum_branch.assign(g.adj.size(), 0.0);
-    } else {
-        if (o.cum) cum.assign(ncell, 0.0);
-        if (o.max) mx.assign(ncell, -9999.0);
-    }
+    if (o.network && o.maps()) net->cum.assign(g.n, 0.0), net->cum_branch.assign(g.adj.size(), 0.0);
     const Members M = members_of(g);
     Branches branch;
     std::vector<double> cur_mem;
@@ -1164,6 +1215,32 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         if (points[i] >= 0) positions[points[i]].push_back(i);
     DevBuf<int> dmem = to_device(M.mem.empty() ? std::vector<i32>{0} : M.mem, cx.s);
     DevBuf<double> dv, dcur, scratch;
+    // raster maps on the device: node of every cell, component of every node,
+    // cells of conductance 0 (for the null options), the running cum / max
+    // maps, one pair's maps when they are emitted
+    DevBuf<int> dnodemap, dcomp;
+    DevBuf<unsigned char> dzero;
+    DevBuf<double> dcum, dmx, dvmap, dcmap;
+    const int cell_flags = (o.focal_zero ? 1 : 0) | (o.log_transform ? 2 : 0) | (o.null_currents ? 4 : 0) |
+                           (o.null_voltages ? 8 : 0);
+    if (o.maps() && !o.network) {
+        dnodemap = to_device(g.nodemap.empty() ? std::vector<i32>{-1} : g.nodemap, cx.s);
+        dcomp = to_device(g.comp.empty() ? std::vector<i32>{0} : g.comp, cx.s);
+        if (o.null_currents || o.null_voltages) {
+            std::vector<unsigned char> z(static_cast<std::size_t>(ncell));
+            for (i64 c = 0; c < ncell; ++c) z[c] = gmap[c] == 0;
+            dzero = to_device(z.empty() ? std::vector<unsigned char>{0} : z, cx.s);
+        }
+        const unsigned cb = nblk(ncell, 256);
+        if (o.cum) {
+            dcum.alloc(ncell ? ncell : 1);
+            if (ncell) hipLaunchKernelGGL(k_fill_value, dim3(cb), dim3(256), 0, cx.s, ncell, 0.0, dcum.p);
+        }
+        if (o.max) {
+            dmx.alloc(ncell ? ncell : 1);
+            if (ncell) hipLaunchKernelGGL(k_fill_value, dim3(cb), dim3(256), 0, cx.s, ncell, -9999.0, dmx.p);
+        }
+    }
     std::vector<i32> nodes;
     for (i32 k = 0; k < g.ncomp; ++k) {
         const i64 p0 = M.start[k], p1 = M.start[k + 1];
@@ -1202,91 +1279,115 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         progress("pairwise: factored; %zu right-hand sides", jobs.size());
         std::vector<i32> uidx(g.n, -1);  // node -> unknown (-1: the grounded node, other components)
         for (int i = 0; i < A.n; ++i) uidx[nodes[i]] = i;
-        // every job's voltages (relative to its source) by member
         const std::size_t nj = jobs.size();
-        // right-hand sides built and solved on the device; without maps only
-        // the solutions at the component's focal nodes come back
-        std::vector<double> X, Xf;
-        std::unordered_map<i32, int> fpos;  // focal node -> column of Xf
-        if (A.n > 0) {
-            std::vector<int> ja(nj), jb(nj), fu(csub.size());
-            for (std::size_t j = 0; j < nj; ++j) ja[j] = uidx[jobs[j].first], jb[j] = uidx[jobs[j].second];
-            for (std::size_t i = 0; i < csub.size(); ++i) fu[i] = uidx[csub[i]], fpos[csub[i]] = static_cast<int>(i);
-            DevBuf<int> dja = to_device(ja, cx.s), djb = to_device(jb, cx.s), dfu = to_device(fu, cx.s);
-            DevBuf<double> dB, dX;
-            const long long tot = static_cast<long long>(nj) * A.n;
-            dB.alloc(tot), dX.alloc(tot);
-            hipLaunchKernelGGL(k_pair_rhs, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, cx.s,
-                               static_cast<long long>(A.n), static_cast<int>(nj), dja.p, djb.p, dB.p);
-            S.solve_device(static_cast<int>(nj), dB.p, dX.p, cx.tol, cx.maxit, cx.st);
-            if (o.maps()) {
-                X.resize(static_cast<std::size_t>(tot));
-                LSC_HIP(hipMemcpyAsync(X.data(), dX.p, tot * sizeof(double), hipMemcpyDeviceToHost, cx.s));
-            } else {
-                const int nfc = static_cast<int>(csub.size());
-                DevBuf<double> dXf;
-                dXf.alloc(static_cast<std::size_t>(nj) * nfc);
-                hipLaunchKernelGGL(k_pick, dim3(static_cast<unsigned>((nj * nfc + 255) / 256)), dim3(256), 0, cx.s,
-                                   static_cast<long long>(A.n), static_cast<int>(nj), nfc, dfu.p, dX.p, dXf.p);
-                Xf.resize(nj * nfc);
-                LSC_HIP(hipMemcpyAsync(Xf.data(), dXf.p, Xf.size() * sizeof(double), hipMemcpyDeviceToHost, cx.s));
-            }
-            LSC_HIP(hipStreamSynchronize(cx.s));
-        } else if (o.maps()) {
-            X.assign(nj * static_cast<std::size_t>(A.n), 0.0);
-        }
-        progress("pairwise: solved");
-        tally(cx, S);
-        auto midx = [&](i32 v) { return static_cast<i32>(std::lower_bound(mb, mb + count, v) - mb); };
-        // voltage of focal node v in job j (relative to the grounded node)
-        auto fvolt = [&](std::size_t j, i32 v) -> double {
-            if (o.maps()) {
-                const i32 u = uidx[v];
-                return u >= 0 ? X[j * A.n + u] : 0.0;
-            }
-            return A.n > 0 ? Xf[j * csub.size() + fpos.at(v)] : 0.0;
-        };
-        std::vector<double> vcol(o.maps() ? count : 0), vnode, cnode;
-        if (o.maps()) {
-            vnode.assign(g.n, 0.0), cnode.assign(g.n, 0.0);
+        const int nfc = static_cast<int>(csub.size());
+        std::unordered_map<i32, int> fpos;  // focal node -> column of the picked solutions
+        std::vector<int> fu(nfc);
+        for (int i = 0; i < nfc; ++i) fu[i] = uidx[csub[i]], fpos[csub[i]] = i;
+        const bool raster_maps = o.maps() && !o.network;
+        // the raster maps stay on the device: the component's unknowns by node,
+        // the job's node voltages and currents
+        DevBuf<int> duidx, dfu = to_device(fu, cx.s);
+        if (raster_maps) {
+            duidx = to_device(uidx, cx.s);
             dv.alloc(g.n ? g.n : 1), dcur.alloc(g.n ? g.n : 1);
         }
-        for (std::size_t j = 0; j < nj; ++j) {
-            const auto& job = jobs[j];
-            const double vs = fvolt(j, job.first);
-            if (o.maps()) {
-                for (i64 q = 0; q < count; ++q) {
-                    const i32 u = uidx[mb[q]];
-                    vcol[q] = (u >= 0 ? X[j * A.n + u] : 0.0) - vs;
+        std::vector<double> vnode, cnode, vcol;
+        if (o.network && o.maps()) vnode.assign(g.n, 0.0), cnode.assign(g.n, 0.0), vcol.resize(count);
+        if (o.network && o.maps()) dv.alloc(g.n ? g.n : 1), dcur.alloc(g.n ? g.n : 1);
+        // jobs in chunks whose right-hand sides fit the device (all at once when they do)
+        const long long chunk = A.n > 0 ? pair_chunk(static_cast<long long>(nj), A.n) : static_cast<long long>(nj);
+        DevBuf<double> dB, dX, dXf;
+        std::vector<double> Xf, X;
+        for (std::size_t j0 = 0; j0 < nj; j0 += static_cast<std::size_t>(chunk)) {
+            const int kc = static_cast<int>(std::min<std::size_t>(static_cast<std::size_t>(chunk), nj - j0));
+            Xf.assign(static_cast<std::size_t>(kc) * nfc, 0.0);
+            if (A.n > 0) {
+                std::vector<int> ja(kc), jb(kc);
+                for (int j = 0; j < kc; ++j) ja[j] = uidx[jobs[j0 + j].first], jb[j] = uidx[jobs[j0 + j].second];
+                DevBuf<int> dja = to_device(ja, cx.s), djb = to_device(jb, cx.s);
+                const long long tot = static_cast<long long>(kc) * A.n;
+                if (dB.n < static_cast<std::size_t>(tot)) dB.alloc(tot), dX.alloc(tot);
+                if (dXf.n < Xf.size()) dXf.alloc(Xf.size());
+                hipLaunchKernelGGL(k_pair_rhs, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, cx.s,
+                                   static_cast<long long>(A.n), kc, dja.p, djb.p, dB.p);
+                S.solve_device(kc, dB.p, dX.p, cx.tol, cx.maxit, cx.st);
+                hipLaunchKernelGGL(k_pick, dim3(static_cast<unsigned>((Xf.size() + 255) / 256)), dim3(256), 0, cx.s,
+                                   static_cast<long long>(A.n), kc, nfc, dfu.p, dX.p, dXf.p);
+                LSC_HIP(hipMemcpyAsync(Xf.data(), dXf.p, Xf.size() * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                if (o.network && o.maps()) {
+                    X.resize(static_cast<std::size_t>(tot));
+                    LSC_HIP(hipMemcpyAsync(X.data(), dX.p, tot * sizeof(double), hipMemcpyDeviceToHost, cx.s));
                 }
-            }
-            const double resistance = (fvolt(j, job.second) - vs) - (vs - vs);
-            if (o.maps()) {
-                for (i64 q = 0; q < count; ++q) vnode[mb[q]] = vcol[q];
-                dv.upload(vnode.data(), g.n, cx.s);
-                graph_currents(G, dmem.p + p0, static_cast<int>(count), dv.p, nullptr, dcur.p, scratch, cx.s);
-                LSC_HIP(hipMemcpyAsync(cnode.data(), dcur.p, g.n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
                 LSC_HIP(hipStreamSynchronize(cx.s));
-                if (o.network) {
-                    branch_currents(g, mb, count, vnode.data(), branch);
-                    cur_mem.resize(count);
-                }
+            } else if (o.network && o.maps()) {
+                X.assign(static_cast<std::size_t>(kc) * A.n, 0.0);
             }
-            for (i32 ci : positions[job.first])
-                for (i32 cj : positions[job.second]) {
-                    if (excluded[static_cast<std::size_t>(ci) * np + cj]) continue;
-                    R(ci, cj) = R(cj, ci) = resistance;
-                    if (shortcut) {
+            // voltage of focal node v in job j of the chunk (relative to the grounded node)
+            auto fvolt = [&](int j, i32 v) { return Xf[static_cast<std::size_t>(j) * nfc + fpos.at(v)]; };
+            for (int j = 0; j < kc; ++j) {
+                const auto& job = jobs[j0 + j];
+                const double vs = fvolt(j, job.first);
+                const double resistance = (fvolt(j, job.second) - vs) - (vs - vs);
+                int times = 0;  // the job's position pairs that are not excluded
+                for (i32 ci : positions[job.first])
+                    for (i32 cj : positions[job.second]) {
+                        if (excluded[static_cast<std::size_t>(ci) * np + cj]) continue;
+                        ++times;
+                        R(ci, cj) = R(cj, ci) = resistance;
+                        if (!shortcut) continue;
                         for (i32 i = 1; i < np; ++i) {
                             const i32 v = points[i];
                             if (v < 0 || g.comp[v] != k) continue;
                             voltmatrix[static_cast<std::size_t>(i) * np + cj] = 1 - ((fvolt(j, v) - vs) / resistance);
                         }
-                        continue;
                     }
-                    if (!o.maps()) continue;
-                    if (o.network) {
-                        // postprocess: the component's node voltages when asked, node and branch currents
+                if (!o.maps() || times == 0) continue;
+                if (raster_maps) {
+                    const double* Xj = A.n > 0 ? dX.p + static_cast<std::size_t>(j) * A.n : nullptr;
+                    if (A.n > 0)
+                        hipLaunchKernelGGL(k_job_voltages, dim3(nblk(count, 256)), dim3(256), 0, cx.s,
+                                           static_cast<int>(count), dmem.p + p0, duidx.p, Xj, vs, dv.p);
+                    else
+                        hipLaunchKernelGGL(k_fill_value, dim3(nblk(g.n, 256)), dim3(256), 0, cx.s,
+                                           static_cast<long long>(g.n), 0.0, dv.p);
+                    graph_currents(G, dmem.p + p0, static_cast<int>(count), dv.p, nullptr, dcur.p, scratch, cx.s);
+                    const bool want_emit = emit && (o.volt_maps || o.cur_maps);
+                    if (want_emit) {
+                        if (o.volt_maps && dvmap.n < static_cast<std::size_t>(ncell)) dvmap.alloc(ncell);
+                        if (dcmap.n < static_cast<std::size_t>(ncell)) dcmap.alloc(ncell);
+                    }
+                    hipLaunchKernelGGL(k_pair_cells, dim3(nblk(ncell, 256)), dim3(256), 0, cx.s, ncell, dnodemap.p,
+                                       dcomp.p, k, dv.p, dcur.p, dzero.p, cell_flags, job.first, job.second, times,
+                                       o.cum ? dcum.p : nullptr, o.max ? dmx.p : nullptr,
+                                       want_emit && o.volt_maps ? dvmap.p : nullptr, want_emit ? dcmap.p : nullptr);
+                    if (!want_emit) continue;
+                    std::vector<double> vmap(o.volt_maps ? ncell : 0), cmap(ncell);
+                    if (o.volt_maps)
+                        LSC_HIP(hipMemcpyAsync(vmap.data(), dvmap.p, ncell * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                    LSC_HIP(hipMemcpyAsync(cmap.data(), dcmap.p, ncell * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                    LSC_HIP(hipStreamSynchronize(cx.s));
+                    for (i32 ci : positions[job.first])
+                        for (i32 cj : positions[job.second])
+                            if (!excluded[static_cast<std::size_t>(ci) * np + cj])
+                                emit(ci, cj, o.volt_maps ? vmap.data() : nullptr, o.cur_maps ? cmap.data() : nullptr, ctx);
+                    continue;
+                }
+                // network: the component's node voltages, node and branch currents per pair, on the host
+                for (i64 q = 0; q < count; ++q) {
+                    const i32 u = uidx[mb[q]];
+                    vcol[q] = (u >= 0 ? X[static_cast<std::size_t>(j) * A.n + u] : 0.0) - vs;
+                    vnode[mb[q]] = vcol[q];
+                }
+                dv.upload(vnode.data(), g.n, cx.s);
+                graph_currents(G, dmem.p + p0, static_cast<int>(count), dv.p, nullptr, dcur.p, scratch, cx.s);
+                LSC_HIP(hipMemcpyAsync(cnode.data(), dcur.p, g.n * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+                LSC_HIP(hipStreamSynchronize(cx.s));
+                branch_currents(g, mb, count, vnode.data(), branch);
+                cur_mem.resize(count);
+                for (i32 ci : positions[job.first])
+                    for (i32 cj : positions[job.second]) {
+                        if (excluded[static_cast<std::size_t>(ci) * np + cj]) continue;
                         for (i64 q = 0; q < count; ++q) {
                             cur_mem[q] = cnode[mb[q]];
                             net->cum[mb[q]] += cnode[mb[q]];
@@ -1296,34 +1397,11 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
                             net->emit(ci, cj, count, mb, o.volt_maps ? vcol.data() : nullptr, cur_mem.data(),
                                       static_cast<i64>(branch.e.size()), branch.r.data(), branch.c.data(),
                                       branch.cur.data(), net->ctx);
-                        continue;
                     }
-                    std::vector<double> vmap, cmap(ncell, 0.0);
-                    if (o.volt_maps) {
-                        vmap.assign(ncell, 0.0);
-                        for (i64 c = 0; c < ncell; ++c) {
-                            const i32 v = g.nodemap[c];
-                            if (v >= 0 && g.comp[v] == k) vmap[c] = vnode[v];
-                            if (o.null_voltages && gmap[c] == 0) vmap[c] = -9999.0;
-                        }
-                    }
-                    for (i64 c = 0; c < ncell; ++c) {
-                        const i32 v = g.nodemap[c];
-                        if (v >= 0 && g.comp[v] == k) {
-                            cmap[c] = cnode[v];
-                            if (o.focal_zero && (v == job.first || v == job.second)) cmap[c] = 0.0;
-                        }
-                        if (o.log_transform) cmap[c] = cmap[c] > 0 ? std::log10(cmap[c]) : -9999.0;
-                        if (o.null_currents && gmap[c] == 0) cmap[c] = -9999.0;
-                    }
-                    if (o.cum)
-                        for (i64 c = 0; c < ncell; ++c) cum[c] += cmap[c];
-                    if (o.max)
-                        for (i64 c = 0; c < ncell; ++c) mx[c] = std::max(mx[c], cmap[c]);
-                    if (emit && (o.volt_maps || o.cur_maps))
-                        emit(ci, cj, o.volt_maps ? vmap.data() : nullptr, o.cur_maps ? cmap.data() : nullptr, ctx);
-                }
+            }
         }
+        progress("pairwise: solved");
+        tally(cx, S);
         if (shortcut) {
             // update_shortcut_resistances! (circuitscape.cpp:708-734)
             const i32 anchor = positions[csub[0]].front();
@@ -1356,10 +1434,14 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
     for (i32 i = 0; i < np; ++i) Rm[static_cast<std::size_t>(i) * np + i] = 0.0;
     std::copy(Rm.begin(), Rm.end(), R_out);
     if (o.network) return;
-    if (o.cum && cum_out)
-        for (i64 c = 0; c < ncell; ++c) cum_out[c] = cum[c] < -9999 ? -9999 : cum[c];
-    if (o.max && max_out)
-        for (i64 c = 0; c < ncell; ++c) max_out[c] = mx[c] < -9999 ? -9999 : mx[c];
+    // the maps at -9999 or above, as the reference writes them
+    auto fetch = [&](const DevBuf<double>& d, double* out) {
+        LSC_HIP(hipMemcpyAsync(out, d.p, ncell * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+        LSC_HIP(hipStreamSynchronize(cx.s));
+        for (i64 c = 0; c < ncell; ++c) out[c] = out[c] < -9999 ? -9999 : out[c];
+    };
+    if (o.cum && cum_out && ncell) fetch(dcum, cum_out);
+    if (o.max && max_out && ncell) fetch(dmx, max_out);
 }
 
 // ---- networks (the reference's network_advanced / network_pairwise,

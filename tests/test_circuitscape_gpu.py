@@ -266,3 +266,43 @@ def test_graph_replay_matches_eager_pcg_sweeps(backend, oracle, monkeypatch):
     np.testing.assert_array_equal(out["1"]["cum"], out["0"]["cum"])
     ref = oracle.cs_pairwise(g, None, cells, ex, False, True, {"cum": True})
     np.testing.assert_allclose(out["1"]["resistances"], ref["resistances"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("network", [False, True])
+def test_pairwise_jobs_in_chunks(backend, oracle, network, monkeypatch):
+    """40 focal points (780 pairs) with the cumulative map: the right-hand sides solved in chunks of 64
+    (LSC_PAIR_CHUNK) give the same resistances and map as one solve of all of them (bit for bit: each
+    right-hand side is solved on its own and the maps add in pair order), and match the oracle."""
+    rng = np.random.default_rng(40)
+    shape = (60, 70)
+    g = np.exp(rng.normal(0, 1, shape))
+    g[rng.random(shape) < 0.05] = 0.0
+    data = np.flatnonzero(g > 0)
+    cells = rng.choice(data, 40, replace=False).astype(np.int64)
+    ex = np.zeros((40, 40), np.uint8)
+    ex[3, 7] = ex[7, 3] = 1
+    backend.settings.precision, backend.settings.tol = "mixed", 1e-12
+    out = {}
+    for chunk in ("0", "64"):
+        if chunk == "0":
+            monkeypatch.delenv("LSC_PAIR_CHUNK", raising=False)
+        else:
+            monkeypatch.setenv("LSC_PAIR_CHUNK", chunk)
+        print(f"pairwise 40 points, chunk {chunk}, network {network}", flush=True)
+        if network:
+            idx = np.arange(g.size).reshape(shape)
+            a = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+            b = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+            w = (g.ravel()[a] + g.ravel()[b]) / 2
+            keep = w > 0
+            out[chunk] = backend.network_pairwise(a[keep].astype(np.int32), b[keep].astype(np.int32), w[keep], g.size,
+                                                  cells.astype(np.int32), ex, {"cum": True})
+        else:
+            out[chunk] = backend.circuitscape_pairwise(g, None, cells, ex, False, True, {"cum": True})
+    np.testing.assert_array_equal(out["0"]["resistances"], out["64"]["resistances"])
+    np.testing.assert_array_equal(out["0"]["cum"], out["64"]["cum"])
+    if not network:
+        ref = oracle.cs_pairwise(g, None, cells, ex, False, True, {"cum": True})
+        np.testing.assert_allclose(out["64"]["resistances"], ref["resistances"], rtol=1e-9)
+        top = ref["cum"].max()
+        np.testing.assert_allclose(out["64"]["cum"], ref["cum"], rtol=1e-9, atol=1e-12 * top)
