@@ -25,6 +25,7 @@
 
 #include "../../include/landschaft_circuit.h"
 #include "common.hpp"
+#include "huge.hpp"
 #include "window.hpp"
 #include "window_kernels.hpp"
 
@@ -66,6 +67,12 @@ template <typename T>
 std::size_t small_lds(int stile) {
     return static_cast<std::size_t>(kSmallWaves) * stile * 256 * sizeof(T);
 }
+// LDS of k_flood_wide: open, present and next-present row masks.
+std::size_t wide_flood_lds(int W) { return 3 * static_cast<std::size_t>(W) * ((W + 63) / 64) * 8; }
+
+// LDS of k_currents: the window's voltages, in HBM (no LDS) beyond kMaxW.
+std::size_t currents_lds(int W) { return W <= kMaxW ? static_cast<std::size_t>(W) * W * sizeof(double) : 0; }
+
 template <typename T>
 std::size_t solve_lds(int lmax) {
     return 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);  // + the 16-byte tail
@@ -99,14 +106,12 @@ void allow_lds(const void* fn, std::size_t bytes) {
 
 }  // namespace
 
-void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
+void DevPlan::upload(Plan p, hipStream_t s) {
     host = std::move(p);
     const Plan& h = host;
     const int H = static_cast<int>(h.lev_ptr.size()) - 1;
-    if (!allow_huge && h.max_front + 1 > kBigTiles * 16)
-        throw std::invalid_argument("a front of " + std::to_string(h.max_front) +
-                                    " rows exceeds the LDS fronts of the batched engine (159 rows)");
     lev_small_host.assign(H, 0);
+    lev_fit_host.assign(H, 0);
     lev_stile.assign(H, 1);
     lev_btile.assign(H, 1);
     lev_lmax.assign(H, 1);
@@ -125,6 +130,9 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
                 huge.push_back(t);
                 continue;
             }
+            // the plan orders a level's fronts by size: the large ones come last
+            if (k - h.lev_ptr[lv] != lev_fit_host[lv]) throw std::logic_error("a level's large fronts are not last");
+            ++lev_fit_host[lv];
             if (m + 1 > kSmallRows) {
                 const int MT = static_cast<int>((m + 1 + 15) / 16);
                 lev_btile[lv] = std::max(lev_btile[lv], MT * (MT + 1) / 2);
@@ -161,7 +169,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool allow_huge) {
     d.first = I(h.first), d.npiv = I(h.npiv), d.nupd = I(h.nupd), d.upd = I(h.upd), d.cb = I(h.cb), d.ce = I(h.ce);
     d.lpos = Lg(lpos), d.uposA = Lg(uposA), d.vpos = Lg(vpos);
     d.kids = I(h.kids), d.rows = I(h.rows), d.relmap = I(h.relmap);
-    d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small_host), d.order = I(h.order);
+    d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small_host), d.lev_fit = I(lev_fit_host), d.order = I(h.order);
     d.fe_ptr = I(h.fe_ptr), d.fe_row = I(h.fe_row), d.fe_col = I(h.fe_col), d.fe_p = I(h.fe_p), d.fe_q = I(h.fe_q);
     {
         std::vector<long long> src(h.fe_src.begin(), h.fe_src.end());
@@ -204,6 +212,12 @@ struct WindowSolver {
     // failed or whose refinement stalled (made on first use)
     std::unique_ptr<WindowSolver<double>> fb;
     int fb_cap = 0;
+    // fronts of more than 159 rows (windows of large radius): per level, in
+    // HBM tiles, for huge_chunk windows at a time
+    std::vector<HugeLevel> lev_huge;
+    long long huge_tiles = 0;  // largest level arena per window (tiles)
+    int huge_rows = 0, huge_nh = 0, huge_chunk = 0;
+    DevBuf<char> HF, HD;       // tile arena and diagonal-block scratch of a chunk
 
     explicit WindowSolver(hipStream_t st) : s(st) {}
 
@@ -211,6 +225,21 @@ struct WindowSolver {
         for (LaunchGraph* g : {&g_flood, &g_factor, &g_sweep0, &g_residual, &g_sweep1}) g->clear();
         const std::size_t used0 = active_arena() ? active_arena()->used : 0;
         plan.upload(std::move(p), s);
+        huge_tiles = build_huge_levels(plan.host, 32, lev_huge, huge_rows, s);
+        huge_nh = 0;
+        for (const HugeLevel& hl : lev_huge) huge_nh = std::max(huge_nh, static_cast<int>(hl.t.size()));
+        if (huge_tiles > 0) {
+            // windows per chunk: launch rows (fronts x windows) within 65535, the arena within 2 GiB
+            // (one window in the static scratch)
+            const long long per = huge_tiles * 256 * static_cast<long long>(sizeof(T));
+            const long long cap = active_arena() ? 1 : std::max<long long>(1, (2ll << 30) / per);
+            huge_chunk = static_cast<int>(std::max<long long>(1, std::min<long long>({cap, 65535 / huge_nh, B})));
+            HF.alloc(static_cast<std::size_t>(huge_chunk) * per);
+            HD.alloc(static_cast<std::size_t>(huge_chunk) * huge_nh * 256 * sizeof(T));
+            const std::size_t hl = (256 + 16 + static_cast<std::size_t>(huge_rows)) * sizeof(double);
+            allow_lds(reinterpret_cast<const void*>(&k_h_forward<T, 1>), hl);
+            allow_lds(reinterpret_cast<const void*>(&k_h_backward<T, 1>), hl);
+        }
         if (active_arena()) plan_bytes = static_cast<long long>(active_arena()->used - used0);
         int smax = 1, lmax = 1;
         for (int v : plan.lev_stile) smax = std::max(smax, v);
@@ -222,6 +251,7 @@ struct WindowSolver {
         allow_lds(reinterpret_cast<const void*>(&k_forward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_backward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
+        if (plan.d.W > kMaxW) allow_lds(reinterpret_cast<const void*>(&k_flood_wide), wide_flood_lds(plan.d.W));
         lane_levels();
         const int W = plan.d.W;
         if (active_arena()) B = arena_batch(B);
@@ -287,7 +317,7 @@ struct WindowSolver {
                                wb.present.p, wb.sc.p, wb.active.p, Ap);
         for (int h = 0; h < P.H; ++h) {
             const int ns = plan.lev_small_host[h];
-            const int nbig = plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h] - ns;
+            const int nbig = plan.lev_fit_host[h] - ns;  // fronts of 65-159 rows
             if (ns > 0)
                 hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kSmallWaves)),
                                    dim3(kSmallWaves * 64), small_lds<T>(plan.lev_stile[h]), s, P, h, nb,
@@ -297,6 +327,47 @@ struct WindowSolver {
                 hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
                                    dim3(kBigWaves * 64), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
                                    P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
+            if (!lev_huge[h].t.empty()) huge_factor(h, nb, Ap, L, Up);
+        }
+    }
+
+    // The level's fronts of more than 159 rows for every window, huge_chunk
+    // windows at a time: tiles in HBM, panel steps with MFMA trailing
+    // updates, the right-hand side row carried (the fused forward
+    // substitution), L, y and the update matrices written as the LDS fronts
+    // write them.
+    void huge_factor(int h, int nb, T* Ap, T* L, T* Up) {
+        const DPlan& P = plan.d;
+        HugeLevel& hl = lev_huge[h];
+        const int nh = static_cast<int>(hl.t.size());
+        T* F = reinterpret_cast<T*>(HF.p);
+        T* D = reinterpret_cast<T*>(HD.p);
+        for (int i0 = 0; i0 < nb; i0 += huge_chunk) {
+            const int ni = std::min(huge_chunk, nb - i0);
+            HugeInst hi;
+            hi.nh = nh, hi.inst0 = i0, hi.ftiles = hl.tiles;
+            hi.lstride = P.lsize, hi.ustride = P.usizeA, hi.astride = P.nent, hi.nstride = P.n;
+            hi.active = wb.active.p, hi.fail = wb.fail.p, hi.fail_stride = 1;
+            const unsigned slots = static_cast<unsigned>(nh * ni);
+            const long long zero = hl.tiles * ni * 256;
+            hipLaunchKernelGGL(k_h_zero<T>, dim3(blocks(zero, 256)), dim3(256), 0, s, F, zero);
+            hipLaunchKernelGGL(k_h_init<T>, dim3(4, slots), dim3(256), 0, s, P, hi, hl.dt.p, hl.doff.p, F,
+                               static_cast<const T*>(Ap), static_cast<const double*>(wb.rs.p));
+            for (std::size_t r = 0; r < hl.chf.size(); ++r)
+                hipLaunchKernelGGL(k_h_extend<T>, dim3(static_cast<unsigned>(hl.cbk[r].back()), ni), dim3(256), 0, s, P,
+                                   hi, static_cast<int>(hl.chf[r].size()), hl.dchf[r].p, hl.dchc[r].p, hl.dcb[r].p,
+                                   hl.dt.p, hl.doff.p, F, static_cast<const T*>(Up));
+            for (int Kt = 0; Kt < hl.maxKt; ++Kt) {
+                hipLaunchKernelGGL(k_h_panel<T>, dim3(hl.rb[Kt], slots), dim3(256), 0, s, P, hi, Kt, hl.dt.p, hl.doff.p,
+                                   F, D);
+                hipLaunchKernelGGL(k_h_diag<T>, dim3(slots), dim3(256), 0, s, P, hi, Kt, hl.dt.p, hl.doff.p, F,
+                                   static_cast<const T*>(D));
+                if (hl.tb[Kt] > 0)
+                    hipLaunchKernelGGL(k_h_trailing<T>, dim3(hl.tb[Kt], slots), dim3(256), 0, s, P, hi, Kt, hl.dt.p,
+                                       hl.doff.p, F);
+            }
+            hipLaunchKernelGGL(k_h_store<T>, dim3(static_cast<unsigned>(std::min<long long>(hl.store_blocks, 16)), slots),
+                               dim3(256), 0, s, P, hi, hl.dt.p, hl.doff.p, static_cast<const T*>(F), L, Up, wb.y.p);
         }
     }
 
@@ -304,7 +375,7 @@ struct WindowSolver {
         const DPlan& P = plan.d;
         const T* L = reinterpret_cast<const T*>(wb.L.p);
         auto grid = [&](int h) {
-            return dim3(static_cast<unsigned>(static_cast<long long>(nb) * (plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h])));
+            return dim3(static_cast<unsigned>(static_cast<long long>(nb) * plan.lev_fit_host[h]));
         };
         auto lanes = [&](int h) {
             return dim3((static_cast<unsigned>(nb) + 63) / 64,
@@ -323,9 +394,16 @@ struct WindowSolver {
                 }
                 const bool small = plan.lev_mmax[h] <= 64;  // rows on lanes, L in registers: the front's vector in LDS
                 auto kern = small ? k_forward_level<T, 1> : k_forward_level<T, 3>;
-                hipLaunchKernelGGL(kern, grid(h), dim3(64), (small ? 64 : 192) * sizeof(double),
-                                   s, P, h, nb, P.lsize,
-                                   wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
+                if (plan.lev_fit_host[h] > 0)
+                    hipLaunchKernelGGL(kern, grid(h), dim3(64), (small ? 64 : 192) * sizeof(double),
+                                       s, P, h, nb, P.lsize,
+                                       wb.active.p, wb.done.p, L, wb.rs.p, wb.y.p, V.p);
+                const HugeLevel& hl = lev_huge[h];
+                if (!hl.t.empty())
+                    hipLaunchKernelGGL((k_h_forward<T, 1>), dim3(static_cast<unsigned>(hl.t.size()), nb), dim3(256),
+                                       (256 + static_cast<std::size_t>(hl.mmax)) * sizeof(double), s, P, hl.dt.p, P.n,
+                                       nb, P.vsize, P.lsize, L, static_cast<const double*>(wb.rs.p), wb.y.p, V.p,
+                                       static_cast<const int*>(wb.done.p), static_cast<const int*>(wb.active.p));
             }
         for (int h = P.H - 1; h >= 0; --h) {
             if (const int M = lane_bwd[h]) {
@@ -337,11 +415,18 @@ struct WindowSolver {
                                    wb.y.p, wb.z.p);
                 continue;
             }
+            const HugeLevel& hl = lev_huge[h];
+            if (!hl.t.empty())
+                hipLaunchKernelGGL((k_h_backward<T, 1>), dim3(static_cast<unsigned>(hl.t.size()), nb), dim3(256),
+                                   (256 + 16 + static_cast<std::size_t>(hl.mmax)) * sizeof(double), s, P, hl.dt.p, P.n,
+                                   nb, P.lsize, L, static_cast<const double*>(wb.y.p), wb.z.p,
+                                   static_cast<const int*>(wb.done.p), static_cast<const int*>(wb.active.p));
             const bool small = plan.lev_mmax[h] <= 64;  // update rows in 64 doubles of LDS
             auto kern = small ? k_backward_level<T, 1> : k_backward_level<T, 3>;
             const std::size_t lds = solve_lds<T>(plan.lev_lmax[h]) - (small ? 128 * sizeof(double) : 0);
-            hipLaunchKernelGGL(kern, grid(h), dim3(64), lds, s, P, h, nb, P.lsize,
-                               wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
+            if (plan.lev_fit_host[h] > 0)
+                hipLaunchKernelGGL(kern, grid(h), dim3(64), lds, s, P, h, nb, P.lsize,
+                                   wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
         }
         hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, static_cast<long long>(P.n),
                            wb.active.p, wb.done.p, wb.sc.p, wb.z.p, wb.x.p);
@@ -363,8 +448,8 @@ struct WindowSolver {
     int arena_batch(int want) const {
         const Arena* a = active_arena();
         long long left = static_cast<long long>(a->size) - static_cast<long long>(arena_round(a->used));
-        if (std::is_same<T, float>::value)
-            left -= plan_bytes + kBufferSlack + kArenaFallback * window_bytes(8) + 4 * 8;
+        if (std::is_same<T, float>::value)  // the fallback's plan, its FP64 large-front arena, its windows
+            left -= 3 * plan_bytes + kBufferSlack + kArenaFallback * window_bytes(8) + 4 * 8;
         const long long fit = (left - kBufferSlack - 4 * static_cast<long long>(sizeof(T))) / window_bytes(sizeof(T));
         if (fit < 1)
             throw std::invalid_argument("one window of radius " + std::to_string(plan.d.D) +
@@ -386,9 +471,14 @@ struct WindowSolver {
             reset(wb.done.p, nb, s);
             reset(wb.iters.p, nb, s);
             reset(wb.fail.p, nb, s);
-            hipLaunchKernelGGL(k_flood, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.src.p, wb.active.p,
-                               wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p,
-                               10 * std::numeric_limits<double>::epsilon());
+            const double shift = 10 * std::numeric_limits<double>::epsilon();
+            if (P.W <= kMaxW)
+                hipLaunchKernelGGL(k_flood, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.src.p, wb.active.p,
+                                   wb.present.p, wb.dg.p, wb.sc.p, wb.b.p, wb.rs.p, wb.x.p, shift);
+            else
+                hipLaunchKernelGGL(k_flood_wide, dim3(nb), dim3(kThreads), wide_flood_lds(P.W), s, P, nb,
+                                   (P.W + 63) / 64, wb.g.p, wb.src.p, wb.active.p, wb.present.p, wb.dg.p, wb.sc.p,
+                                   wb.b.p, wb.rs.p, wb.x.p, shift);
         });
         if (timing) LSC_HIP(hipEventRecord(ev[1], s));
         g_factor.run(s, nb, gr, [&] { factor(nb); });
@@ -568,9 +658,9 @@ std::vector<double> correction(const OmniArgs& a, long long nx, long long ny, hi
     IrStats cst;
     ws.solve(2, std::min(a.tol, 1e-14), std::max(a.maxit, 30), cst, ev, false);
     Lattice Lt{};
-    hipLaunchKernelGGL(k_currents, dim3(2), dim3(kCurThreads), static_cast<std::size_t>(n) * sizeof(double), s,
+    hipLaunchKernelGGL(k_currents, dim3(2), dim3(kCurThreads), currents_lds(static_cast<int>(W)), s,
                        ws.plan.d, Lt, 0ll, 2, ws.wb.g.p, ws.wb.present.p, ws.wb.x.p, ws.wb.active.p,
-                       static_cast<const double*>(nullptr), ws.wb.cur.p);
+                       static_cast<const double*>(nullptr), ws.wb.cur.p, W > kMaxW ? ws.wb.src.p : nullptr);
     std::vector<double> cur(2 * n);
     std::vector<int> act(2);
     LSC_HIP(hipMemcpyAsync(cur.data(), ws.wb.cur.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -695,9 +785,10 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
                 }
                 w.solve(nb, a.tol, a.maxit, part[j], ev, timing);
                 if (timing) LSC_HIP(hipEventRecord(ev[0], sj));
-                hipLaunchKernelGGL(k_currents, dim3(nb), dim3(kCurThreads),
-                                   static_cast<std::size_t>(Lt.W) * Lt.W * sizeof(double), sj, w.plan.d, Lt, k0, nb,
-                                   w.wb.g.p, w.wb.present.p, w.wb.x.p, w.wb.active.p, d_corr, w.wb.cur.p);
+                // wider windows keep their voltages in the batch's source maps (no longer needed)
+                hipLaunchKernelGGL(k_currents, dim3(nb), dim3(kCurThreads), currents_lds(Lt.W), sj, w.plan.d, Lt, k0,
+                                   nb, w.wb.g.p, w.wb.present.p, w.wb.x.p, w.wb.active.p, d_corr, w.wb.cur.p,
+                                   Lt.W > kMaxW ? w.wb.src.p : nullptr);
                 if (timing) LSC_HIP(hipEventRecord(ev[1], sj));
                 const long long ix0 = k0 / Lt.nly, ix1 = (k0 + nb - 1) / Lt.nly;
                 const long long X0 = std::max<long long>(0, Lt.br + ix0 * Lt.bs - Lt.D);
@@ -867,8 +958,8 @@ void omniscape_device(const double* d_res, const double* d_src, long long nx, lo
                       hipStream_t s) {
     if (a.radius < 0 || a.buffer < 0 || a.block < 1 || a.block % 2 == 0)
         throw std::invalid_argument("radius and buffer must be >= 0 and block_size odd");
-    if (2 * (a.radius + a.buffer) + 1 > kMaxW)
-        throw std::invalid_argument("window side 2 (radius + buffer) + 1 exceeds " + std::to_string(kMaxW));
+    if (2 * (a.radius + a.buffer) + 1 > kMaxWideW)
+        throw std::invalid_argument("window side 2 (radius + buffer) + 1 exceeds " + std::to_string(kMaxWideW));
     const Lattice Lt = make_lattice(nx, ny, a);
     const long long ncell = nx * ny;
     const unsigned zb = static_cast<unsigned>((ncell + 255) / 256);

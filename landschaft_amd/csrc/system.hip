@@ -619,7 +619,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
     progress("plan: %d fronts, L %lld entries, largest front %d", p.nfront, static_cast<long long>(p.lsize),
              p.max_front);
-    plan.upload(std::move(p), s, true);
+    plan.upload(std::move(p), s);
     const Plan& h = plan.host;
     const DPlan& P = plan.d;
     const int H = P.H, n = P.n;
@@ -701,7 +701,7 @@ void SystemSolver::Impl::numeric_t(const HostMatrix* m, double& analyze_ms, doub
               static_cast<std::size_t>(kBigTiles * (kBigTiles + 1) / 2) * 256 * sizeof(T));
     for (int lv = 0; lv < H; ++lv) {
         const int ns = plan.lev_small_host[lv];
-        const int nbig = h.lev_ptr[lv + 1] - h.lev_ptr[lv] - ns;
+        const int nbig = plan.lev_fit_host[lv] - ns;  // fronts of 65-159 rows
         if (ns > 0)
             hipLaunchKernelGGL(k_factor_small<T>, dim3(nblocks(ns, kSmallWaves)), dim3(kSmallWaves * 64),
                                static_cast<std::size_t>(kSmallWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
@@ -868,18 +868,20 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
     const T* Lp = reinterpret_cast<const T*>(L.p);
     auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
     auto rgrid = [&](int lv) {
-        return dim3(static_cast<unsigned>(h.lev_ptr[lv + 1] - h.lev_ptr[lv]), static_cast<unsigned>((k + kRhs - 1) / kRhs));
+        return dim3(static_cast<unsigned>(plan.lev_fit_host[lv]), static_cast<unsigned>((k + kRhs - 1) / kRhs));
     };
     if (mode == 1)
         for (int lv = 0; lv < P.H; ++lv) {
             auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_rhs<T, 1> : k_forward_rhs<T, 3>;
-            hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, rs.p, y.p, V.p);
+            if (plan.lev_fit_host[lv] > 0)
+                hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, rs.p, y.p, V.p);
             huge_forward<T>(lev_huge[lv], k);
         }
     for (int lv = P.H - 1; lv >= 0; --lv) {
         huge_backward<T>(lev_huge[lv], k);
         auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_rhs<T, 1> : k_backward_rhs<T, 3>;
-        hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, y.p, z.p);
+        if (plan.lev_fit_host[lv] > 0)
+            hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, y.p, z.p);
     }
 }
 

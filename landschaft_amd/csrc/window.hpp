@@ -26,7 +26,9 @@ static_assert(16 + 48 * kBigWaves >= kBigTiles * 16, "the panel's rows must fit 
 // tolerance has failed (FP32-preconditioned refinement that stalls at the
 // FP64 floor of the residual ends within it): it is solved again in FP64.
 constexpr double kFallbackGap = 1000.0;
-constexpr int kMaxW = 127;         // window side (bit-mask flood fill over 2 x 64-bit words per row)
+constexpr int kMaxW = 127;         // window side of the register flood fill (2 x 64-bit words per row) and of
+                                   // the voltages in LDS for node currents; wider windows: k_flood_wide, HBM
+constexpr int kMaxWideW = 511;     // window side of k_flood_wide (8 words a row: 3 x 511 x 64 B of LDS)
 
 // Device copy of a Plan plus the per-level chunking of the solve kernels.
 struct DPlan {
@@ -35,7 +37,7 @@ struct DPlan {
     const int *first, *npiv, *nupd, *upd, *cb, *ce;
     const long long *lpos, *uposA, *vpos;
     const int *kids, *rows, *relmap;
-    const int *lev_ptr, *lev_small, *order;
+    const int *lev_ptr, *lev_small, *lev_fit, *order;  // per level: fronts, small ones, ones of <= 159 rows
     const int *fe_ptr, *fe_row, *fe_col, *fe_p, *fe_q;  // entries assembled into each front
     const long long* fe_src;
     int nent;
@@ -45,13 +47,14 @@ struct DPlan {
 struct DevPlan {
     Plan host;
     DPlan d{};
+    std::vector<int> lev_fit_host;  // per level: fronts of at most 159 rows (RHS row included), the level's first ones
     std::vector<int> lev_small_host, lev_stile, lev_btile, lev_lmax, lev_mmax;  // per level: small fronts, their
                                                                                 // tiles, the big fronts' tiles,
                                                                                 // largest L block, largest front
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
     std::vector<int> huge;  // fronts of more than 159 rows (RHS row included), in level order
-    void upload(Plan p, hipStream_t s, bool allow_huge = false);
+    void upload(Plan p, hipStream_t s);
 };
 
 // Per-window arrays of a batch (window slot k of B).

@@ -161,6 +161,56 @@ static __global__ __launch_bounds__(kThreads) void k_fill(const double* __restri
     }
 }
 
+// After the flood: the present mask of every cell from the row masks (`words`
+// 64-bit words a row), the window dropped when its component holds no source
+// (omniscape.cpp:101), then the Laplacian diagonal of every unknown (edges to
+// present cells in column-major neighbour order + the 10 eps shift,
+// reference system.cpp:471-485), its Jacobi scale and the right-hand side.
+__device__ void flood_finish(const DPlan& P, int k, const unsigned long long* pm, int words, const double* gw,
+                             const double* sw, int* active, unsigned char* present, double* dg, double* sc,
+                             double* bvec, double* rs, double* x, double shift, double* red) {
+    const int W = P.W, W2 = W * W, D = P.D, n = P.n, nd = P.nd;
+    const int ground = D * W + D;
+    auto bit = [&](int y, int xx) { return ((pm[y * words + (xx >> 6)] >> (xx & 63)) & 1ull) != 0; };
+    unsigned char* pw = present + static_cast<long long>(k) * W2;
+    double part = 0.0;
+    for (int c = threadIdx.x; c < W2; c += kThreads) {
+        const int y = c / W, xx = c - (c / W) * W;
+        const bool p = bit(y, xx);
+        pw[c] = p;
+        if (p && c != ground) part += sw[c];
+    }
+    const double total = block_sum(part, red);
+    if (total == 0) {
+        if (threadIdx.x == 0) active[k] = 0;
+        return;
+    }
+    const long long off = static_cast<long long>(k) * n;
+    for (int p = threadIdx.x; p < n; p += kThreads) {
+        const int c = P.cell[p];
+        const int y = c / W, xx = c - (c / W) * W;
+        double d = 1.0, s = 1.0, b = 0.0;
+        if (bit(y, xx)) {
+            double sum = 0.0;
+            const double gc = gw[c];
+            for (int e = 0; e < nd; ++e) {
+                const int yy = y + ndy(nd, e), x2 = xx + ndx(nd, e);
+                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
+                if (!bit(yy, x2)) continue;
+                sum += edge_weight(gc, gw[yy * W + x2], ndiag(nd, e));
+            }
+            d = sum + shift;
+            s = 1.0 / sqrt(d);
+            b = sw[c];
+        }
+        dg[off + p] = d;
+        sc[off + p] = s;
+        bvec[off + p] = b;
+        rs[off + p] = s * b;
+        x[off + p] = 0.0;
+    }
+}
+
 // The ground's component (cells with g > 0 joined to the centre), by row bit
 // masks: runs are filled along rows with one 128-bit add, then grown between
 // rows, until nothing changes. Then the Laplacian diagonal of every unknown
@@ -255,44 +305,95 @@ static __global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const
         }
     }
     __syncthreads();
-    unsigned char* pw = present + static_cast<long long>(k) * W2;
-    double part = 0.0;
-    for (int c = threadIdx.x; c < W2; c += kThreads) {
-        const int y = c / W, xx = c - (c / W) * W;
-        const bool p = (pm[y][xx >> 6] >> (xx & 63)) & 1ull;
-        pw[c] = p;
-        if (p && c != ground) part += sw[c];
-    }
-    const double total = block_sum(part, red);
-    if (total == 0) {  // no source in the ground's component (omniscape.cpp:101)
+    flood_finish(P, k, &pm[0][0], 2, gw, sw, active, present, dg, sc, bvec, rs, x, shift, red);
+}
+
+// Wider windows (more than 127 cells a side, up to 64 * kWideWords): the
+// same component by row bit masks of `words` 64-bit words in LDS (open,
+// present and the next present mask, W rows each), rows spread over the
+// workgroup's threads: each pass fills along every row (a carry add over the
+// row's words, both directions), then grows every row by its neighbours',
+// until no row changes.
+static __global__ __launch_bounds__(kThreads) void k_flood_wide(DPlan P, int B, int words,
+                                                         const double* __restrict__ g,
+                                                         const double* __restrict__ src, int* __restrict__ active,
+                                                         unsigned char* __restrict__ present, double* __restrict__ dg,
+                                                         double* __restrict__ sc, double* __restrict__ bvec,
+                                                         double* __restrict__ rs, double* __restrict__ x, double shift) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[kWaves];
+    const int k = blockIdx.x;
+    if (k >= B || !active[k]) return;
+    const int W = P.W, D = P.D, nd = P.nd;
+    unsigned long long* om = reinterpret_cast<unsigned long long*>(smem_raw);
+    unsigned long long* pm = om + static_cast<long long>(W) * words;
+    unsigned long long* pn = pm + static_cast<long long>(W) * words;
+    const double* gw = g + static_cast<long long>(k) * W * W;
+    const double* sw = src + static_cast<long long>(k) * W * W;
+    if (!(gw[D * W + D] > 0)) {
         if (threadIdx.x == 0) active[k] = 0;
         return;
     }
-    const long long off = static_cast<long long>(k) * n;
-    for (int p = threadIdx.x; p < n; p += kThreads) {
-        const int c = P.cell[p];
-        const int y = c / W, xx = c - (c / W) * W;
-        const bool pr = (pm[y][xx >> 6] >> (xx & 63)) & 1ull;
-        double d = 1.0, s = 1.0, b = 0.0;
-        if (pr) {
-            double sum = 0.0;
-            const double gc = gw[c];
-            for (int e = 0; e < nd; ++e) {
-                const int yy = y + ndy(nd, e), x2 = xx + ndx(nd, e);
-                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
-                if (!((pm[yy][x2 >> 6] >> (x2 & 63)) & 1ull)) continue;
-                sum += edge_weight(gc, gw[yy * W + x2], ndiag(nd, e));
-            }
-            d = sum + shift;
-            s = 1.0 / sqrt(d);
-            b = sw[c];
+    for (int e = threadIdx.x; e < W * words; e += kThreads) {
+        const int y = e / words, w = e - y * words;
+        unsigned long long m = 0;
+        for (int b = 0; b < 64; ++b) {
+            const int xx = w * 64 + b;
+            if (xx < W && gw[y * W + xx] > 0) m |= 1ull << b;
         }
-        dg[off + p] = d;
-        sc[off + p] = s;
-        bvec[off + p] = b;
-        rs[off + p] = s * b;
-        x[off + p] = 0.0;
+        om[e] = m;
+        pm[e] = (y == D && w == (D >> 6)) ? 1ull << (D & 63) : 0ull;
     }
+    __syncthreads();
+    for (;;) {
+        // along the rows: seeds spread over the open runs they touch, rightwards by a carry add, leftwards
+        // by the same add on the row reversed
+        for (int y = threadIdx.x; y < W; y += kThreads) {
+            unsigned long long* p = pm + static_cast<long long>(y) * words;
+            const unsigned long long* o = om + static_cast<long long>(y) * words;
+            unsigned long long carry = 0;
+            for (int w = 0; w < words; ++w) {
+                const unsigned long long a = o[w] + p[w];
+                const unsigned long long a2 = a + carry;
+                carry = (a < o[w] || a2 < a) ? 1ull : 0ull;
+                p[w] |= (a2 ^ o[w]) & o[w];
+            }
+            carry = 0;
+            for (int r = 0; r < words; ++r) {  // word r of the reversed row is word words-1-r, bits reversed
+                const int w = words - 1 - r;
+                const unsigned long long ro = __builtin_bitreverse64(o[w]), rp = __builtin_bitreverse64(p[w]);
+                const unsigned long long a = ro + rp;
+                const unsigned long long a2 = a + carry;
+                carry = (a < ro || a2 < a) ? 1ull : 0ull;
+                p[w] |= __builtin_bitreverse64((a2 ^ ro) & ro);
+            }
+        }
+        __syncthreads();
+        // across the rows: a cell joins when a neighbour in the row above or below is in
+        int grew = 0;
+        for (int e = threadIdx.x; e < W * words; e += kThreads) {
+            const int y = e / words, w = e - y * words;
+            unsigned long long n0 = 0, lo = 0, hi = 0;
+            for (int dy = -1; dy <= 1; dy += 2) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= W) continue;
+                const unsigned long long* q = pm + static_cast<long long>(yy) * words;
+                n0 |= q[w];
+                if (nd == 8) {
+                    lo |= (q[w] << 1) | (w > 0 ? q[w - 1] >> 63 : 0ull);
+                    hi |= (q[w] >> 1) | (w + 1 < words ? q[w + 1] << 63 : 0ull);
+                }
+            }
+            const unsigned long long next = pm[e] | ((n0 | lo | hi) & om[e]);
+            grew |= next != pm[e];
+            pn[e] = next;
+        }
+        grew = __syncthreads_or(grew);
+        for (int e = threadIdx.x; e < W * words; e += kThreads) pm[e] = pn[e];
+        __syncthreads();
+        if (!grew) break;
+    }
+    flood_finish(P, k, pm, words, gw, sw, active, present, dg, sc, bvec, rs, x, shift, red);
 }
 
 // The off-diagonal entries of S A S (S = diag^-1/2) of every window, in the
@@ -505,11 +606,10 @@ __global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int h, i
     __shared__ int s_rel[kBigTiles * 16 + 1];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int base = P.lev_ptr[h] + P.lev_small[h];
-    const int nb = P.lev_ptr[h + 1] - base;
+    const int nb = P.lev_fit[h] - P.lev_small[h];  // fronts of 65-159 rows (larger ones: huge.hpp)
     const int k = blockIdx.x / nb;
     if (k >= B || !active[k]) return;
     const int t = P.order[base + (blockIdx.x - k * nb)];
-    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: the k_huge_* kernels
     if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
     auto bsync = [] { __syncthreads(); };
@@ -787,11 +887,10 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* fv = reinterpret_cast<double*>(smem_raw);
     const int lane = threadIdx.x;
-    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int nf = P.lev_fit[h];  // the level's fronts of at most 159 rows (larger ones: huge.hpp)
     const int k = blockIdx.x / nf;
     if (k >= B || !active[k] || done[k]) return;
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
-    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_forward
     const long long off = static_cast<long long>(k) * P.n;
     forward_front_direct<T, R, R == 1 ? 16 : 8>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off,
                                                 y + off, Vall + static_cast<long long>(k) * P.vsize);
@@ -806,11 +905,10 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     double* xu = reinterpret_cast<double*>(smem_raw);
     T* Ls = reinterpret_cast<T*>(smem_raw + (R == 1 ? 64 : 192) * sizeof(double));
     const int lane = threadIdx.x;
-    const int nf = P.lev_ptr[h + 1] - P.lev_ptr[h];
+    const int nf = P.lev_fit[h];  // the level's fronts of at most 159 rows (larger ones: huge.hpp)
     const int k = blockIdx.x / nf;
     if (k >= B || !active[k] || done[k]) return;
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
-    if (P.npiv[t] + P.nupd[t] + 1 > kBigTiles * 16) return;  // huge fronts: k_huge_backward
     const long long off = static_cast<long long>(k) * P.n;
     backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
@@ -1034,7 +1132,6 @@ __global__ __launch_bounds__(64) void k_forward_rhs(DPlan P, int h, int k, const
     const int lane = threadIdx.x;
     const int t = P.order[P.lev_ptr[h] + blockIdx.x];
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
     const int cb = P.cb[t], ce = P.ce[t];
     stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
     for (int rr = 0; rr < kRhs; ++rr) {
@@ -1068,7 +1165,6 @@ __global__ __launch_bounds__(64) void k_backward_rhs(DPlan P, int h, int k, cons
     const int lane = threadIdx.x;
     const int t = P.order[P.lev_ptr[h] + blockIdx.x];
     const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
-    if (m + 1 > kBigTiles * 16) return;  // large fronts: their own kernels
     const int* rows = P.rows + P.upd[t];
     stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
     for (int rr = 0; rr < kRhs; ++rr) {
@@ -1274,9 +1370,11 @@ static __global__ __launch_bounds__(kCurThreads) void k_currents(DPlan P, Lattic
                                                        const double* __restrict__ g,
                                                        const unsigned char* __restrict__ present,
                                                        const double* __restrict__ x, const int* __restrict__ active,
-                                                       const double* __restrict__ corr, double* __restrict__ cur) {
+                                                       const double* __restrict__ corr, double* __restrict__ cur,
+                                                       double* __restrict__ vgrid) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* v = reinterpret_cast<double*>(smem_raw);
+    // the window's voltages by cell: in LDS, or (windows wider than kMaxW) in vgrid, W^2 per window
+    double* v = vgrid ? vgrid + static_cast<long long>(blockIdx.x) * P.W * P.W : reinterpret_cast<double*>(smem_raw);
     __shared__ double red[kCurThreads / 64];
     const int k = blockIdx.x;
     if (k >= B || !active[k]) return;

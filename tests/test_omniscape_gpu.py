@@ -187,3 +187,39 @@ def test_fp64_fallback_of_high_contrast_windows(backend, oracle, contrast):
     top = ref["current"].max()
     np.testing.assert_allclose(out["mixed"], out["fp64"], rtol=1e-9, atol=1e-12 * top)
     np.testing.assert_allclose(out["mixed"], ref["current"], rtol=1e-9, atol=1e-10 * top)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("radius,block,four,shape,flow", [(60, 7, False, (100, 120), False),
+                                                          (70, 9, True, (110, 130), True),
+                                                          (100, 11, False, (150, 170), True)])
+def test_large_radius_windows_match_oracle(backend, oracle, radius, block, four, shape, flow):
+    """Windows with fronts of more than 159 rows (radius >= 54: the large-front tiles in HBM, a level at a
+    time for every window) and, at radius 100 (201 cells a side), the wide flood fill and voltages in HBM
+    for the node currents, mixed and FP64, against the oracle."""
+    from landschaft_amd import _native as N
+
+    assert N.window_plan_stats(radius)["max_front"] + 1 > 160
+    rng = np.random.default_rng(radius)
+    r = np.exp(rng.normal(0.0, 0.8, shape))
+    r[rng.random(shape) < 0.06] = np.nan
+    s = np.where(np.isnan(r), 0.0, 1 / np.nan_to_num(r, nan=1.0))
+    ref = oracle.omniscape(r, s, radius, 0, block, four, False, True, flow)
+    top = ref["current"].max()
+    for precision in ("mixed", "fp64"):
+        backend.settings.precision, backend.settings.tol = precision, 1e-12
+        print(f"radius {radius} block {block} four {four} {shape} {precision}", flush=True)
+        out = backend.circuit_omniscape(r, s, radius, 0, block, four, False, True, flow)
+        assert out["n_targets"] == ref["n_targets"] > 0
+        np.testing.assert_allclose(out["current"], ref["current"], rtol=1e-9, atol=1e-12 * top)
+        if flow:
+            np.testing.assert_allclose(out["flow_potential"], ref["flow_potential"], rtol=1e-9,
+                                       atol=1e-12 * ref["flow_potential"].max())
+
+
+def test_large_radius_correction_matches_oracle(backend, oracle):
+    backend.settings.precision, backend.settings.tol = "mixed", 1e-12
+    for args in [(100, 0, 11, False, 300, 300), (64, 4, 7, True, 200, 150)]:
+        print(f"correction {args}", flush=True)
+        np.testing.assert_allclose(backend.circuit_omniscape_correction(*args), oracle.omniscape_correction(*args),
+                                   rtol=1e-9)
