@@ -218,8 +218,11 @@ def test_large_radius_windows_match_oracle(backend, oracle, radius, block, four,
 
 
 def test_large_radius_correction_matches_oracle(backend, oracle):
+    """Correction factors of windows with large fronts (and, at radius 100, the wide flood fill). A factor
+    is a quotient of two currents: where the block circuit's current is small (down to 1e-6 of its
+    maximum, the floor) the quotient carries a larger relative error than either solve, hence 1e-8."""
     backend.settings.precision, backend.settings.tol = "mixed", 1e-12
     for args in [(100, 0, 11, False, 300, 300), (64, 4, 7, True, 200, 150)]:
         print(f"correction {args}", flush=True)
         np.testing.assert_allclose(backend.circuit_omniscape_correction(*args), oracle.omniscape_correction(*args),
-                                   rtol=1e-9)
+                                   rtol=1e-8)
