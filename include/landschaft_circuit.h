@@ -290,6 +290,39 @@ int lsc_ascii_grid_rows(const double* values, int64_t nx, int64_t ny, char* out,
  * figures of the same quantities are internal to frontal_analyze). */
 int lsc_window_plan_stats(int radius, int buffer, int four_neighbors, int disk, double* out);
 
+/* The symbolic analysis of a circuit: its node graph and the multifrontal
+ * plan of every connected component's Laplacian, as the solves build them.
+ * Replaces the analysis half of the reference's factor_component
+ * (src/cpp/circuit/circuitscape.cpp:92-141: the graph from raster_graph,
+ * :145-263, or network_graph, :265-338, then frontal_analyze_graph,
+ * src/cpp/circuit/system.cpp:373-440), exposed on its own so that a caller
+ * can size a job (nodes, factor entries, largest front) before any device
+ * work. Host code: no device is needed. The digests are 64-bit hashes of the
+ * node map, adjacency, weights and component labels (graph_digest) and of
+ * every plan's elimination order, fronts, update rows, child-to-parent row
+ * maps and assembled entries (plan_digest); equal inputs give equal digests
+ * whatever the host thread count (LSC_HOST_THREADS, 1 to 16). */
+typedef struct lsc_analysis {
+    int64_t n_nodes;        /* graph nodes (short-circuit regions merged) */
+    int64_t n_components;
+    int64_t n_entries;      /* adjacency entries, both directions */
+    int64_t n_fronts;       /* fronts of all components' plans */
+    int64_t factor_entries; /* stored entries of L, blocks padded to 4 */
+    int64_t largest_front;  /* rows of the largest front */
+    double factor_flops;    /* sum np^3/3 + np^2 nu + np nu^2 over fronts */
+    double graph_ms;        /* host wall time of the graph and its components */
+    double plan_ms;         /* host wall time of the component matrices and plans */
+    uint64_t graph_digest;
+    uint64_t plan_digest;
+} lsc_analysis;
+
+/* Raster circuit: the arguments of lsc_circuitscape_advanced's graph. */
+int lsc_circuitscape_analyze(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                             int avg_resistances, int four_neighbors, lsc_analysis* out);
+/* Network circuit: the arguments of lsc_network_advanced's graph. */
+int lsc_network_analyze(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                        lsc_analysis* out);
+
 /* Checks the MFMA operand layout the front kernels assume: out[256] receives
  * the 16 x 16 product A (16x4) * B (4x16) of a = [i + 16 k], b = [k + 4 j]
  * computed with v_mfma_f32_16x16x4f32 (precision 0) or the f64 MFMA (1),

@@ -24,7 +24,7 @@ EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_ci
            "lsc_circuitscape_multi_advanced_sum", "lsc_network_advanced", "lsc_network_pairwise",
            "lsc_effective_resistance_grad", "lsc_grid_circuit_create", "lsc_grid_circuit_update",
            "lsc_grid_circuit_solve", "lsc_grid_circuit_adjoint", "lsc_grid_circuit_stats",
-           "lsc_grid_circuit_destroy", "lsc_ascii_grid_rows")
+           "lsc_grid_circuit_destroy", "lsc_ascii_grid_rows", "lsc_circuitscape_analyze", "lsc_network_analyze")
 
 class SolveOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_refine", ctypes.c_int32), ("tol", ctypes.c_double),
@@ -49,6 +49,16 @@ class SolveStats(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class Analysis(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_components", ctypes.c_int64), ("n_entries", ctypes.c_int64),
+                ("n_fronts", ctypes.c_int64), ("factor_entries", ctypes.c_int64), ("largest_front", ctypes.c_int64),
+                ("factor_flops", ctypes.c_double), ("graph_ms", ctypes.c_double), ("plan_ms", ctypes.c_double),
+                ("graph_digest", ctypes.c_uint64), ("plan_digest", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def _signatures() -> dict:
@@ -92,6 +102,8 @@ def _signatures() -> dict:
         "lsc_grid_circuit_stats": (ci, [vp, st]),
         "lsc_grid_circuit_destroy": (None, [vp]),
         "lsc_ascii_grid_rows": (ci, [pf64, i64, i64, c.c_char_p, i64, pi64]),
+        "lsc_circuitscape_analyze": (ci, [pf64, pi64, i64, i64, ci, ci, c.POINTER(Analysis)]),
+        "lsc_network_analyze": (ci, [pi32, pi32, pf64, i64, i32, c.POINTER(Analysis)]),
     }
 
 
@@ -159,6 +171,29 @@ def probe_mfma(precision: int = PRECISION_MIXED) -> np.ndarray:
     out = np.empty(256)
     check(lib().lsc_probe_mfma(int(precision), ptr(out)))
     return out.reshape(16, 16)
+
+
+def circuitscape_analyze(gmap, polymap=None, avg_resistances: bool = False, four_neighbors: bool = False) -> dict:
+    """lsc_circuitscape_analyze: the node graph and every component's plan of a raster circuit (host only)."""
+    g = f64(gmap)
+    poly = None if polymap is None else np.ascontiguousarray(polymap, dtype=np.int64)
+    out = Analysis()
+    check(lib().lsc_circuitscape_analyze(ptr(g), ptr(poly, ctypes.c_int64), g.shape[1], g.shape[0],
+                                         int(bool(avg_resistances)), int(bool(four_neighbors)), ctypes.byref(out)))
+    return out.as_dict()
+
+
+def network_analyze(a, b, w, n_nodes: int) -> dict:
+    """lsc_network_analyze: the node graph and every component's plan of an edge list (host only)."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    w = f64(w)
+    if not (a.shape == b.shape == w.shape):
+        raise ValueError("the edge arrays must have one length")
+    out = Analysis()
+    check(lib().lsc_network_analyze(ptr(a, ctypes.c_int32), ptr(b, ctypes.c_int32), ptr(w), a.size, int(n_nodes),
+                                    ctypes.byref(out)))
+    return out.as_dict()
 
 
 def window_plan_stats(radius: int, buffer: int = 0, four_neighbors: bool = False, disk: bool = True) -> dict:

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <numeric>
 #include <chrono>
 #include <cmath>
@@ -24,6 +26,7 @@
 
 #include "../../include/landschaft_circuit.h"
 #include "common.hpp"
+#include "host.hpp"
 #include "system.hpp"
 
 namespace lsc {
@@ -42,8 +45,9 @@ struct RGraph {
     std::vector<i32> nodemap;   // cell -> node
     std::vector<i64> node_cell; // node -> cell (single-cell nodes), -1 merged
     std::vector<i64> ptr;
-    std::vector<i32> adj;
-    std::vector<double> w, degree;
+    RawVec<i32> adj;
+    RawVec<double> w;
+    std::vector<double> degree;
     std::vector<i32> comp;
     i32 ncomp = 0;
 };
@@ -54,27 +58,66 @@ double average(double x, double y, bool diag, bool res) {
     return diag ? (x + y) / 2.8284271247461903 : (x + y) / 2;
 }
 
-// Components of the graph over edges of nonzero weight (iterative DFS).
+// Components of the graph over edges of nonzero weight, numbered in the
+// order of their smallest node: the numbering of a search started from every
+// unlabelled node in turn. Union-find over host threads: a root is only ever
+// linked under a smaller root, by a compare-and-swap on its own slot, so each
+// tree's root is its component's smallest node whatever the interleaving, and
+// finds halve their paths (every slot keeps pointing at a smaller node of its
+// tree). The adjacency is symmetric, so each edge is taken once (t > v).
 void label_components(RGraph& g) {
-    g.comp.assign(g.n, -1);
+    const i32 n = g.n;
+    g.comp.assign(n, -1);
     g.ncomp = 0;
-    std::vector<i32> st;
-    for (i32 u = 0; u < g.n; ++u) {
-        if (g.comp[u] >= 0) continue;
-        g.comp[u] = g.ncomp;
-        st.push_back(u);
-        while (!st.empty()) {
-            const i32 v = st.back();
-            st.pop_back();
-            for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) {
-                const i32 t = g.adj[k];
-                if (g.w[k] == 0 || g.comp[t] >= 0) continue;
-                g.comp[t] = g.ncomp;
-                st.push_back(t);
-            }
+    if (n == 0) return;
+    std::unique_ptr<std::atomic<i32>[]> up(new std::atomic<i32>[n]);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v) up[v].store(static_cast<i32>(v), std::memory_order_relaxed);
+    });
+    auto find = [&](i32 v) {
+        for (;;) {
+            const i32 p = up[v].load(std::memory_order_acquire);
+            if (p == v) return v;
+            const i32 gp = up[p].load(std::memory_order_acquire);
+            if (gp != p) up[v].store(gp, std::memory_order_release);
+            v = gp;
         }
-        ++g.ncomp;
-    }
+    };
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v)
+            for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) {
+                i32 a = static_cast<i32>(v), b = g.adj[k];
+                if (b <= a || g.w[k] == 0) continue;
+                for (;;) {
+                    a = find(a), b = find(b);
+                    if (a == b) break;
+                    if (a < b) std::swap(a, b);
+                    i32 root = a;
+                    if (up[a].compare_exchange_strong(root, b, std::memory_order_acq_rel)) break;
+                }
+            }
+    });
+    // roots (up[v] == v) in node order get the component numbers
+    const i64 nt = host_chunk_count(n);
+    std::vector<i32> base(static_cast<std::size_t>(nt) + 1, 0);
+    host_chunks(n, [&](i64 t, i64 lo, i64 hi) {
+        i32 c = 0;
+        for (i64 v = lo; v < hi; ++v) c += up[v].load(std::memory_order_relaxed) == v;
+        base[t + 1] = c;
+    });
+    for (i64 t = 0; t < nt; ++t) base[t + 1] += base[t];
+    host_chunks(n, [&](i64 t, i64 lo, i64 hi) {
+        i32 c = base[t];
+        for (i64 v = lo; v < hi; ++v)
+            if (up[v].load(std::memory_order_relaxed) == v) g.comp[v] = c++;
+    });
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v) {
+            const i32 r = find(static_cast<i32>(v));  // roots' own slots are only read here
+            if (r != v) g.comp[v] = g.comp[r];
+        }
+    });
+    g.ncomp = base[nt];
 }
 
 // Without short-circuit regions every node is one cell and nodes are
@@ -83,28 +126,15 @@ void label_components(RGraph& g) {
 // written directly, with the same entries as the sorted edge list (no pair
 // of cells is joined twice, and the edge average is symmetric in its two
 // arguments, bit for bit).
-// f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads (each
-// index writes only its own slots, so the result does not depend on the split).
-template <class F>
-void host_chunks(i64 n, F&& f) {
-    const i64 hw = std::max<i64>(1, std::min<i64>(16, std::thread::hardware_concurrency()));
-    const i64 nt = std::min<i64>(hw, std::max<i64>(1, n / (i64(1) << 16)));
-    std::vector<std::thread> ts;
-    for (i64 t = 1; t < nt; ++t) ts.emplace_back([&, t] { f(t, n * t / nt, n * (t + 1) / nt); });
-    f(i64(0), i64(0), n / nt);
-    for (auto& th : ts) th.join();
-}
-
-RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool four) {
-    RGraph g;
-    g.nx = nx, g.ny = ny;
+// Column-major ranks (x outer, y inner) of the cells with gmap > 0 into
+// rank[c] (-1 elsewhere), and each rank's cell into cell_of when given; the
+// number of such cells. Two row-major passes over blocks of rows: each block
+// counts its cells per column, then numbers them from the column's start
+// plus the blocks above.
+i32 column_ranks(const double* gmap, i64 nx, i64 ny, std::vector<i32>& rank, std::vector<i64>* cell_of) {
     const i64 n = nx * ny;
-    g.nodemap.assign(n, -1);
-    // Column-major labels (x outer, y inner) from two row-major passes over
-    // blocks of rows: each block counts its cells per column, then labels
-    // them from the column's start plus the blocks above.
-    const i64 hw = std::max<i64>(1, std::min<i64>(16, std::thread::hardware_concurrency()));
-    const i64 nb = std::min<i64>(hw, std::max<i64>(1, n / (i64(1) << 16)));
+    rank.resize(n);
+    const i64 nb = host_chunk_count(n);
     std::vector<std::vector<i32>> cnt(nb, std::vector<i32>(nx, 0));
     std::vector<std::thread> ts;
     auto rows = [&](i64 b) { return std::make_pair(ny * b / nb, ny * (b + 1) / nb); };
@@ -125,8 +155,7 @@ RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool
     }
     const i64 total = start[nx];
     if (total >= (i64(1) << 31)) throw std::invalid_argument("too many nodes for 32-bit labels");
-    g.n = static_cast<i32>(total);
-    g.node_cell.resize(g.n);
+    if (cell_of) cell_of->resize(total);
     for (i64 b = 0; b < nb; ++b)
         ts.emplace_back([&, b] {
             const auto [r0, r1] = rows(b);
@@ -137,14 +166,26 @@ RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool
                 next[x] = o;
             }
             for (i64 y = r0; y < r1; ++y)
-                for (i64 x = 0; x < nx; ++x)
-                    if (gmap[y * nx + x] > 0) {
+                for (i64 x = 0; x < nx; ++x) {
+                    const i64 c = y * nx + x;
+                    if (gmap[c] > 0) {
                         const i32 v = static_cast<i32>(next[x]++);
-                        g.nodemap[y * nx + x] = v;
-                        g.node_cell[v] = y * nx + x;
+                        rank[c] = v;
+                        if (cell_of) (*cell_of)[v] = c;
+                    } else {
+                        rank[c] = -1;
                     }
+                }
         });
     for (auto& t : ts) t.join();
+    return static_cast<i32>(total);
+}
+
+RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool four) {
+    RGraph g;
+    g.nx = nx, g.ny = ny;
+    g.n = column_ranks(gmap, nx, ny, g.nodemap, &g.node_cell);
+    progress("raster graph: %d nodes labelled", g.n);
     const i32 cntn = g.n;
     static const int k8[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1}, {0, 1}, {1, -1}, {1, 0}, {1, 1}};
     static const int k4[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
@@ -162,7 +203,7 @@ RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool
             g.ptr[v + 1] = deg;
         }
     });
-    for (i32 v = 0; v < cntn; ++v) g.ptr[v + 1] += g.ptr[v];
+    host_prefix(cntn, g.ptr.data() + 1, g.ptr.data());
     g.adj.resize(g.ptr[cntn]);
     g.w.resize(g.ptr[cntn]);
     g.degree.assign(cntn, 0.0);
@@ -187,122 +228,170 @@ RGraph raster_graph_cells(const double* gmap, i64 nx, i64 ny, bool avg_res, bool
             g.degree[v] = dsum;
         }
     });
+    progress("raster graph: %lld adjacency entries", static_cast<long long>(g.ptr[cntn]));
+    label_components(g);
+    progress("raster graph: %d components", g.ncomp);
+    return g;
+}
+
+// The graph with short-circuit regions (polygon ids != 0): every region with
+// at least one cell of gmap > 0 is one node, numbered at its first such cell
+// in column-major order, and takes all of its cells, NoData ones too; the
+// other nodes keep their cells' column-major order. A node's entries are its
+// cells' branches (right, down and the two right diagonals of every emitting
+// cell, in column-major emitter order), self loops left out, stably sorted by
+// neighbour, repeats summed in that order: per node that is a sort by
+// (neighbour, emitter position, direction), so nodes are built independently
+// over host threads.
+RGraph raster_graph_regions(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny, bool avg_res,
+                            bool four) {
+    RGraph g;
+    g.nx = nx, g.ny = ny;
+    const i64 n = nx * ny;
+    std::vector<i32> rank;
+    std::vector<i64> rank_cell;
+    const i32 nvalid = column_ranks(gmap, nx, ny, rank, &rank_cell);
+    // region cells, row-major
+    std::vector<i64> rcells;
+    {
+        const i64 nt = host_chunk_count(n);
+        std::vector<std::vector<i64>> part(static_cast<std::size_t>(nt));
+        host_chunks(n, [&](i64 t, i64 lo, i64 hi) {
+            for (i64 c = lo; c < hi; ++c)
+                if (poly[c] != 0) part[t].push_back(c);
+        });
+        for (auto& q : part) rcells.insert(rcells.end(), q.begin(), q.end());
+    }
+    // each region's first valid cell (smallest rank); its other valid cells
+    // are merged into it
+    std::unordered_map<std::int64_t, i32> first;
+    for (i64 c : rcells)
+        if (rank[c] >= 0) {
+            auto it = first.emplace(poly[c], rank[c]).first;
+            it->second = std::min(it->second, rank[c]);
+        }
+    std::vector<std::uint8_t> keep(static_cast<std::size_t>(nvalid), 1);
+    for (i64 c : rcells)
+        if (rank[c] >= 0 && rank[c] != first[poly[c]]) keep[rank[c]] = 0;
+    std::vector<i32> id(static_cast<std::size_t>(nvalid) + 1);
+    host_prefix(nvalid, keep.data(), id.data());
+    g.n = id[nvalid];
+    // ranks to nodes; region cells to their region's node
+    g.nodemap.resize(n);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 c = lo; c < hi; ++c) g.nodemap[c] = rank[c] >= 0 ? id[rank[c]] : -1;
+    });
+    std::vector<i32> cells_per(static_cast<std::size_t>(g.n), 1);
+    std::unordered_map<i32, std::vector<i64>> region_cells;  // nodes of more than one cell
+    {
+        for (i64 c : rcells) {
+            auto it = first.find(poly[c]);
+            if (it == first.end()) continue;
+            const i32 v = id[it->second];
+            g.nodemap[c] = v;
+            region_cells[v].push_back(c);
+        }
+        for (auto it = region_cells.begin(); it != region_cells.end();) {
+            if (it->second.size() <= 1) {
+                it = region_cells.erase(it);
+            } else {
+                cells_per[it->first] = static_cast<i32>(it->second.size());
+                ++it;
+            }
+        }
+    }
+    g.node_cell.resize(g.n);
+    host_chunks(nvalid, [&](i64, i64 lo, i64 hi) {
+        for (i64 r = lo; r < hi; ++r)
+            if (keep[r]) g.node_cell[id[r]] = cells_per[id[r]] > 1 ? -1 : rank_cell[r];
+    });
+    std::vector<i32>().swap(rank);
+    struct Entry {
+        i32 nb;
+        i64 key;
+        double w;
+    };
+    // the entries of node v (cells: its cells) appended to out, unsorted
+    auto gather = [&](i32 v, const i64* cells, std::size_t count, std::vector<Entry>& out) {
+        for (std::size_t i = 0; i < count; ++i) {
+            const i64 c = cells[i], x = c % nx, y = c / nx;
+            const auto fwd = [&](i64 d, int dir, bool diag) {
+                const i32 u = g.nodemap[d];
+                if (u >= 0 && u != v) out.push_back({u, (x * ny + y) * 4 + dir, average(gmap[c], gmap[d], diag, avg_res)});
+            };
+            const auto back = [&](i64 e, i64 ex, i64 ey, int dir, bool diag) {
+                const i32 u = g.nodemap[e];
+                if (u >= 0 && u != v) out.push_back({u, (ex * ny + ey) * 4 + dir, average(gmap[e], gmap[c], diag, avg_res)});
+            };
+            if (x + 1 < nx) fwd(c + 1, 0, false);
+            if (y + 1 < ny) fwd(c + nx, 1, false);
+            if (!four && y + 1 < ny && x + 1 < nx) fwd(c + nx + 1, 2, true);
+            if (!four && y > 0 && x + 1 < nx) fwd(c - nx + 1, 3, true);
+            if (x > 0) back(c - 1, x - 1, y, 0, false);
+            if (y > 0) back(c - nx, x, y - 1, 1, false);
+            if (!four && x > 0 && y > 0) back(c - nx - 1, x - 1, y - 1, 2, true);
+            if (!four && x > 0 && y + 1 < ny) back(c + nx - 1, x - 1, y + 1, 3, true);
+        }
+    };
+    // node v's row: entries by (neighbour, emission), repeats summed in order
+    auto row = [&](i32 v, std::vector<Entry>& ent, auto&& put) {
+        ent.clear();
+        auto rc = region_cells.find(v);
+        if (rc != region_cells.end()) gather(v, rc->second.data(), rc->second.size(), ent);
+        else if (g.node_cell[v] >= 0) gather(v, &g.node_cell[v], 1, ent);
+        std::sort(ent.begin(), ent.end(), [](const Entry& a, const Entry& b) {
+            return a.nb != b.nb ? a.nb < b.nb : a.key < b.key;
+        });
+        for (std::size_t k = 0; k < ent.size();) {
+            double w = ent[k].w;
+            std::size_t e = k + 1;
+            for (; e < ent.size() && ent[e].nb == ent[k].nb; ++e) w += ent[e].w;
+            put(ent[k].nb, w);
+            k = e;
+        }
+    };
+    const i32 m = g.n;
+    g.ptr.assign(static_cast<std::size_t>(m) + 1, 0);
+    host_chunks(m, [&](i64, i64 lo, i64 hi) {
+        std::vector<Entry> ent;
+        for (i64 v = lo; v < hi; ++v) {
+            i64 k = 0;
+            row(static_cast<i32>(v), ent, [&](i32, double) { ++k; });
+            g.ptr[v + 1] = k;
+        }
+    });
+    host_prefix(m, g.ptr.data() + 1, g.ptr.data());
+    g.adj.resize(static_cast<std::size_t>(g.ptr[m]));
+    g.w.resize(static_cast<std::size_t>(g.ptr[m]));
+    g.degree.assign(m, 0.0);
+    host_chunks(m, [&](i64, i64 lo, i64 hi) {
+        std::vector<Entry> ent;
+        for (i64 v = lo; v < hi; ++v) {
+            i64 k = g.ptr[v];
+            double d = 0.0;
+            row(static_cast<i32>(v), ent, [&](i32 u, double w) {
+                g.adj[k] = u;
+                g.w[k++] = w;
+                d += w;
+            });
+            g.degree[v] = d;
+        }
+    });
+    progress("raster graph: %d nodes, %lld adjacency entries (regions)", m, static_cast<long long>(g.ptr[m]));
     label_components(g);
     return g;
 }
 
 RGraph raster_graph(const double* gmap, const std::int64_t* poly, i64 nx, i64 ny, bool avg_res, bool four) {
-    bool any_poly = false;
-    if (poly)
-        for (i64 c = 0; c < nx * ny && !any_poly; ++c) any_poly = poly[c] != 0;
-    if (!any_poly) return raster_graph_cells(gmap, nx, ny, avg_res, four);
-    RGraph g;
-    g.nx = nx, g.ny = ny;
     const i64 n = nx * ny;
-    std::vector<i64> label(n, 0);
-    i64 next = 0;
-    for (i64 x = 0; x < nx; ++x)
-        for (i64 y = 0; y < ny; ++y)
-            if (gmap[y * nx + x] > 0) label[y * nx + x] = ++next;
-    if (poly) {
-        std::unordered_map<std::int64_t, i64> first;
-        for (i64 x = 0; x < nx; ++x)
-            for (i64 y = 0; y < ny; ++y) {
-                const i64 c = y * nx + x;
-                if (poly[c] != 0 && gmap[c] > 0) first.emplace(poly[c], label[c]);
-            }
-        for (i64 c = 0; c < n; ++c)
-            if (poly[c] != 0) {
-                auto it = first.find(poly[c]);
-                if (it != first.end()) label[c] = it->second;
-            }
-    }
-    // relabel by rank: labels are column-major positions, so rank order is
-    // the first-cell order
-    std::vector<i64> rank(next + 1, -1);
-    for (i64 c = 0; c < n; ++c)
-        if (label[c] != 0) rank[label[c]] = 0;
-    i32 cnt = 0;
-    for (i64 l = 1; l <= next; ++l)
-        if (rank[l] == 0) rank[l] = cnt++;
-    g.n = cnt;
-    g.nodemap.assign(n, -1);
-    std::vector<i32> cells_per(cnt, 0);
-    g.node_cell.assign(cnt, -1);
-    for (i64 c = 0; c < n; ++c) {
-        if (label[c] == 0) continue;
-        const i32 v = static_cast<i32>(rank[label[c]]);
-        g.nodemap[c] = v;
-        if (cells_per[v]++ == 0) g.node_cell[v] = c;
-    }
-    for (i32 v = 0; v < cnt; ++v)
-        if (cells_per[v] > 1) g.node_cell[v] = -1;
-    // Every branch once in each direction, in column-major cell order (right,
-    // down, and the two right diagonals), self loops of merged nodes left out.
-    auto each_branch = [&](auto&& put) {
-        auto emit = [&](i64 c, i64 d, bool diag) {
-            const i32 a = g.nodemap[c], b = g.nodemap[d];
-            if (a < 0 || b < 0 || a == b) return;
-            put(a, b, c, d, diag);
-            put(b, a, c, d, diag);
-        };
-        for (i64 x = 0; x < nx; ++x)
-            for (i64 y = 0; y < ny; ++y) {
-                const i64 c = y * nx + x;
-                if (g.nodemap[c] < 0) continue;
-                if (x + 1 < nx) emit(c, c + 1, false);
-                if (y + 1 < ny) emit(c, c + nx, false);
-                if (!four) {
-                    if (y + 1 < ny && x + 1 < nx) emit(c, c + nx + 1, true);
-                    if (y > 0 && x + 1 < nx) emit(c, c - nx + 1, true);
-                }
-            }
-    };
-    // bucketed by node in emission order (a stable counting sort), then each
-    // node's entries stably by neighbour: the order of a stable sort of the
-    // whole list by (node, neighbour), so duplicate branches (cells of one
-    // region next to one cell) are summed in emission order
-    std::vector<i64> start(static_cast<std::size_t>(cnt) + 1, 0);
-    each_branch([&](i32 a, i32, i64, i64, bool) { ++start[static_cast<std::size_t>(a) + 1]; });
-    for (i32 v = 0; v < cnt; ++v) start[v + 1] += start[v];
-    std::vector<i32> eb(static_cast<std::size_t>(start[cnt]));
-    std::vector<double> ew(static_cast<std::size_t>(start[cnt]));
-    {
-        std::vector<i64> fill(start.begin(), start.end() - 1);
-        each_branch([&](i32 a, i32 b, i64 c, i64 d, bool diag) {
-            const i64 p = fill[a]++;
-            eb[p] = b;
-            ew[p] = average(gmap[c], gmap[d], diag, avg_res);
+    std::atomic<bool> any_poly{false};
+    if (poly)
+        host_chunks(n, [&](i64, i64 lo, i64 hi) {
+            for (i64 c = lo; c < hi && !any_poly.load(std::memory_order_relaxed); ++c)
+                if (poly[c] != 0) any_poly.store(true, std::memory_order_relaxed);
         });
-    }
-    g.ptr.assign(cnt + 1, 0);
-    g.adj.reserve(eb.size());
-    g.w.reserve(eb.size());
-    std::vector<std::pair<i32, double>> tmp;
-    for (i32 v = 0; v < cnt; ++v) {
-        const i64 lo = start[v], hi = start[v + 1];
-        tmp.assign(hi - lo, {});
-        for (i64 p = lo; p < hi; ++p) tmp[p - lo] = {eb[p], ew[p]};
-        std::stable_sort(tmp.begin(), tmp.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
-        for (std::size_t k = 0; k < tmp.size();) {
-            std::size_t e = k + 1;
-            double w = tmp[k].second;
-            for (; e < tmp.size() && tmp[e].first == tmp[k].first; ++e) w += tmp[e].second;
-            g.adj.push_back(tmp[k].first);
-            g.w.push_back(w);
-            ++g.ptr[v + 1];
-            k = e;
-        }
-    }
-    for (i32 v = 0; v < cnt; ++v) g.ptr[v + 1] += g.ptr[v];
-    g.degree.assign(cnt, 0.0);
-    for (i32 v = 0; v < cnt; ++v) {
-        double d = 0.0;
-        for (i64 k = g.ptr[v]; k < g.ptr[v + 1]; ++k) d += g.w[k];
-        g.degree[v] = d;
-    }
-    label_components(g);
-    return g;
+    return any_poly ? raster_graph_regions(gmap, poly, nx, ny, avg_res, four)
+                    : raster_graph_cells(gmap, nx, ny, avg_res, four);
 }
 
 // A network's graph (reference network_graph, circuitscape.cpp:265-338): the
@@ -391,42 +480,75 @@ template <class Drop>
 HostMatrix component_matrix(const RGraph& g, const i32* mem, i64 count, const double* finite, Drop drop,
                             std::vector<i32>& nodes) {
     HostMatrix M;
-    nodes.clear();
     // node -> unknown, -1 elsewhere; kept between calls and reset after use
-    static thread_local std::vector<i32> local;
+    // (bound to a reference: the host threads below use this thread's copy)
+    static thread_local std::vector<i32> local_of;
+    std::vector<i32>& local = local_of;
     if (local.size() < static_cast<std::size_t>(g.n)) local.assign(g.n, -1);
-    i64 x0 = g.nx, x1 = -1, y0 = g.ny, y1 = -1;
-    for (i64 p = 0; p < count; ++p) {
-        const i32 v = mem[p];
-        if (drop(v)) continue;
-        local[v] = static_cast<i32>(nodes.size());
-        nodes.push_back(v);
-        const i64 c = g.node_cell[v];
-        if (c >= 0) {
-            x0 = std::min(x0, c % g.nx), x1 = std::max(x1, c % g.nx);
-            y0 = std::min(y0, c / g.nx), y1 = std::max(y1, c / g.nx);
+    // unknowns: the kept members in member order, numbered from per-chunk counts
+    const i64 nt = host_chunk_count(count);
+    std::vector<i64> kept(static_cast<std::size_t>(nt) + 1, 0);
+    std::vector<std::array<i64, 4>> box(static_cast<std::size_t>(nt), {g.nx, -1, g.ny, -1});
+    host_chunks(count, [&](i64 t, i64 lo, i64 hi) {
+        i64 c = 0;
+        for (i64 p = lo; p < hi; ++p) c += !drop(mem[p]);
+        kept[t + 1] = c;
+    });
+    for (i64 t = 0; t < nt; ++t) kept[t + 1] += kept[t];
+    nodes.resize(static_cast<std::size_t>(kept[nt]));
+    host_chunks(count, [&](i64 t, i64 lo, i64 hi) {
+        i64 i = kept[t];
+        auto& b = box[t];
+        for (i64 p = lo; p < hi; ++p) {
+            const i32 v = mem[p];
+            if (drop(v)) continue;
+            local[v] = static_cast<i32>(i);
+            nodes[i++] = v;
+            const i64 c = g.node_cell[v];
+            if (c >= 0) {
+                b[0] = std::min(b[0], c % g.nx), b[1] = std::max(b[1], c % g.nx);
+                b[2] = std::min(b[2], c / g.nx), b[3] = std::max(b[3], c / g.nx);
+            }
         }
-    }
-    M.n = static_cast<int>(nodes.size());
+    });
+    i64 x0 = g.nx, x1 = -1, y0 = g.ny, y1 = -1;
+    for (const auto& b : box) x0 = std::min(x0, b[0]), x1 = std::max(x1, b[1]), y0 = std::min(y0, b[2]), y1 = std::max(y1, b[3]);
+    const i64 n = static_cast<i64>(nodes.size());
+    M.n = static_cast<int>(n);
     M.bx = x1 >= x0 ? x1 - x0 + 1 : 1;
     M.by = y1 >= y0 ? y1 - y0 + 1 : 1;
-    M.cell.resize(nodes.size());
-    M.ptr.assign(nodes.size() + 1, 0);
-    M.diag.resize(nodes.size());
-    for (std::size_t i = 0; i < nodes.size(); ++i) {
-        const i32 v = nodes[i];
-        const i64 c = g.node_cell[v];
-        M.cell[i] = c >= 0 ? (c / g.nx - y0) * M.bx + (c % g.nx - x0) : -1;
-        for (i64 e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
-            const i32 u = local[g.adj[e]];
-            if (u < 0) continue;
-            M.adj.push_back(u);
-            M.off.push_back(-g.w[e]);
+    M.cell.resize(n);
+    M.ptr.assign(n + 1, 0);
+    M.diag.resize(n);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 i = lo; i < hi; ++i) {
+            const i32 v = nodes[i];
+            const i64 c = g.node_cell[v];
+            M.cell[i] = c >= 0 ? (c / g.nx - y0) * M.bx + (c % g.nx - x0) : -1;
+            i64 k = 0;
+            for (i64 e = g.ptr[v]; e < g.ptr[v + 1]; ++e) k += local[g.adj[e]] >= 0;
+            M.ptr[i + 1] = k;
+            M.diag[i] = (finite ? g.degree[v] + finite[v] : g.degree[v]) + kShift;
         }
-        M.ptr[i + 1] = static_cast<i64>(M.adj.size());
-        M.diag[i] = (finite ? g.degree[v] + finite[v] : g.degree[v]) + kShift;
-    }
-    for (i32 v : nodes) local[v] = -1;
+    });
+    host_prefix(n, M.ptr.data() + 1, M.ptr.data());
+    M.adj.resize(static_cast<std::size_t>(M.ptr[n]));
+    M.off.resize(static_cast<std::size_t>(M.ptr[n]));
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 i = lo; i < hi; ++i) {
+            const i32 v = nodes[i];
+            i64 k = M.ptr[i];
+            for (i64 e = g.ptr[v]; e < g.ptr[v + 1]; ++e) {
+                const i32 u = local[g.adj[e]];
+                if (u < 0) continue;
+                M.adj[k] = u;
+                M.off[k++] = -g.w[e];
+            }
+        }
+    });
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 i = lo; i < hi; ++i) local[nodes[i]] = -1;
+    });
     return M;
 }
 
@@ -443,7 +565,7 @@ struct DevGraph {
         std::vector<long long> p(g.ptr.begin(), g.ptr.end());
         ptr = to_device(p, s);
         adj = to_device(g.adj, s);
-        w = to_device(g.w.empty() ? std::vector<double>{0.0} : g.w, s);
+        w = g.w.empty() ? to_device(std::vector<double>{0.0}, s) : to_device(g.w, s);
     }
 };
 
@@ -1089,6 +1211,67 @@ void fill_stats(const Ctx& cx, lsc_solve_stats* out) {
     out->resid_mean = cx.st.solved ? cx.st.resid_sum / cx.st.solved : 0.0;
     out->resid_median = cx.st.resid_median();
     out->n_fallback = cx.st.fallback;
+}
+
+// A 64-bit digest of arrays, eight bytes at a time: each word mixed by a
+// multiply and a shift, then folded in with FNV-1a's offset basis and prime;
+// every array's length goes in before its bytes.
+struct Digest {
+    std::uint64_t h = 14695981039346656037ull;
+    void word(std::uint64_t w) {
+        w *= 0x9E3779B97F4A7C15ull;
+        w ^= w >> 29;
+        h = (h ^ w) * 1099511628211ull;
+    }
+    void bytes(const void* p, std::size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        std::size_t i = 0;
+        for (std::uint64_t w; i + 8 <= n; i += 8) {
+            std::memcpy(&w, c + i, 8);
+            word(w);
+        }
+        std::uint64_t w = 0;
+        std::memcpy(&w, c + i, n - i);
+        word(w);
+    }
+    template <class V>
+    void vec(const V& v) {
+        word(v.size());
+        bytes(v.data(), v.size() * sizeof(v[0]));
+    }
+};
+
+// The graph's figures and digest, then every component's matrix (no node
+// dropped) and plan, timed (lsc_circuitscape_analyze).
+void analyze(const RGraph& g, double graph_ms, lsc_analysis* out) {
+    lsc_analysis a{};
+    a.n_nodes = g.n;
+    a.n_components = g.ncomp;
+    a.n_entries = g.ptr.empty() ? 0 : g.ptr.back();
+    a.graph_ms = graph_ms;
+    Digest gd, pd;
+    gd.vec(g.nodemap), gd.vec(g.ptr), gd.vec(g.adj), gd.vec(g.w), gd.vec(g.comp);
+    auto t0 = std::chrono::steady_clock::now();
+    const Members M = members_of(g);
+    progress("analysis: %d components' members", g.ncomp);
+    std::vector<i32> nodes;
+    for (i32 k = 0; k < g.ncomp; ++k) {
+        const HostMatrix A = component_matrix(g, M.mem.data() + M.start[k], M.start[k + 1] - M.start[k], nullptr,
+                                              [](i32) { return false; }, nodes);
+        progress("analysis: component %d matrix of %d unknowns", k, A.n);
+        const Plan p = graph_plan(A.bx, A.by, A.cell, A.ptr, A.adj.data(), kGraphLeaf);
+        a.plan_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        a.n_fronts += p.nfront;
+        a.factor_entries += p.lsize;
+        a.largest_front = std::max<int64_t>(a.largest_front, p.max_front);
+        a.factor_flops += p.flops;
+        pd.vec(p.perm), pd.vec(p.first), pd.vec(p.npiv), pd.vec(p.kids), pd.vec(p.order), pd.vec(p.rows);
+        pd.vec(p.relmap), pd.vec(p.fe_ptr), pd.vec(p.fe_row), pd.vec(p.fe_col), pd.vec(p.fe_src), pd.vec(p.lpos);
+        t0 = std::chrono::steady_clock::now();
+    }
+    a.graph_digest = gd.h;
+    a.plan_digest = pd.h;
+    *out = a;
 }
 
 // ---- pairwise (reference raster_pairwise, circuitscape.cpp:474-743)
@@ -1943,6 +2126,28 @@ int lsc_effective_resistance_grad(const double* resistance, int64_t nx, int64_t 
         lsc::Ctx cx = C->cx;
         for (const auto& S : C->sys) lsc::tally(cx, *S);
         lsc::fill_stats(cx, stats);
+    });
+}
+
+int lsc_circuitscape_analyze(const double* gmap, const int64_t* polymap, int64_t nx, int64_t ny,
+                             int avg_resistances, int four_neighbors, lsc_analysis* out) {
+    return lsc::guarded([&] {
+        if (nx < 0 || ny < 0) throw std::invalid_argument("negative raster size");
+        const auto t0 = std::chrono::steady_clock::now();
+        lsc::RGraph G = lsc::raster_graph(gmap, polymap, nx, ny, avg_resistances != 0, four_neighbors != 0);
+        lsc::analyze(G, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                     out);
+    });
+}
+
+int lsc_network_analyze(const int32_t* a, const int32_t* b, const double* w, int64_t n_edges, int32_t n_nodes,
+                        lsc_analysis* out) {
+    return lsc::guarded([&] {
+        if (n_nodes <= 0 || n_edges < 0) throw std::invalid_argument("a network needs nodes");
+        const auto t0 = std::chrono::steady_clock::now();
+        lsc::RGraph G = lsc::network_graph(a, b, w, n_edges, n_nodes);
+        lsc::analyze(G, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(),
+                     out);
     });
 }
 

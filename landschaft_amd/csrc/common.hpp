@@ -99,8 +99,8 @@ struct DevBuf {
     }
 };
 
-template <typename T>
-DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
+template <typename T, class A>
+DevBuf<T> to_device(const std::vector<T, A>& v, hipStream_t s) {
     DevBuf<T> d(v.size() ? v.size() : 1);
     if (!v.empty()) d.upload(v.data(), v.size(), s);
     return d;

@@ -154,7 +154,7 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     lbufs.clear();
     ibufs.reserve(40);
     lbufs.reserve(8);
-    auto I = [&](const std::vector<int>& v) {
+    auto I = [&](const auto& v) {
         ibufs.push_back(to_device(v, s));
         return static_cast<const int*>(ibufs.back().p);
     };
@@ -171,10 +171,10 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     d.kids = I(h.kids), d.rows = I(h.rows), d.relmap = I(h.relmap);
     d.lev_ptr = I(h.lev_ptr), d.lev_small = I(lev_small_host), d.lev_fit = I(lev_fit_host), d.order = I(h.order);
     d.fe_ptr = I(h.fe_ptr), d.fe_row = I(h.fe_row), d.fe_col = I(h.fe_col), d.fe_p = I(h.fe_p), d.fe_q = I(h.fe_q);
-    {
-        std::vector<long long> src(h.fe_src.begin(), h.fe_src.end());
-        d.fe_src = Lg(src);
-    }
+    static_assert(sizeof(long long) == sizeof(h.fe_src[0]), "64-bit entry sources");
+    lbufs.emplace_back(h.fe_src.empty() ? 1 : h.fe_src.size());
+    if (!h.fe_src.empty()) lbufs.back().upload(reinterpret_cast<const long long*>(h.fe_src.data()), h.fe_src.size(), s);
+    d.fe_src = lbufs.back().p;
     d.nent = static_cast<int>(h.fe_row.size());
     d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
 }

@@ -1,5 +1,7 @@
 #include "plan.hpp"
 
+#include "host.hpp"
+
 #include <algorithm>
 #include <atomic>
 #include <exception>
@@ -12,6 +14,10 @@ namespace lsc {
 namespace {
 
 // Geometric nested dissection (reference: src/cpp/circuit/system.cpp:41-93).
+// The far half of a box of 2^18 cells or more is cut on a host thread of its
+// own (up to the host thread count in all) into a Dissection of its own,
+// whose cells and fronts are then appended after the near half's: the
+// layout of the serial recursion (near half, far half, separator).
 struct Dissection {
     const std::uint8_t* mark;
     i64 stride;
@@ -19,12 +25,27 @@ struct Dissection {
     std::vector<i32> cells;
     std::vector<i32> first, npiv;
     std::vector<std::vector<i32>> kids;
+    std::atomic<int>* helpers = nullptr;  // threads running far halves (shared)
+    int max_helpers = 0;
 
     i32 add(i32 f, std::vector<i32> ch) {
         first.push_back(f);
         npiv.push_back(static_cast<i32>(cells.size()) - f);
         kids.push_back(std::move(ch));
         return static_cast<i32>(first.size()) - 1;
+    }
+
+    // appends the cells and fronts of o; returns o's front index shift
+    i32 append(Dissection& o) {
+        const i32 shift = static_cast<i32>(first.size()), base = static_cast<i32>(cells.size());
+        cells.insert(cells.end(), o.cells.begin(), o.cells.end());
+        for (i32 f : o.first) first.push_back(f + base);
+        npiv.insert(npiv.end(), o.npiv.begin(), o.npiv.end());
+        for (auto& k : o.kids) {
+            for (i32& c : k) c += shift;
+            kids.push_back(std::move(k));
+        }
+        return shift;
     }
 
     i32 cut(i64 r0, i64 r1, i64 c0, i64 c1) {
@@ -53,8 +74,37 @@ struct Dissection {
         ch.reserve(2);
         const bool by_row = h >= w;
         const i64 mid = by_row ? r0 + h / 2 : c0 + w / 2;
-        const i32 a = by_row ? cut(r0, mid, c0, c1) : cut(r0, r1, c0, mid);
-        const i32 b = by_row ? cut(mid + 1, r1, c0, c1) : cut(r0, r1, mid + 1, c1);
+        const auto near = [&] { return by_row ? cut(r0, mid, c0, c1) : cut(r0, r1, c0, mid); };
+        const auto far = [&](Dissection& d) { return by_row ? d.cut(mid + 1, r1, c0, c1) : d.cut(r0, r1, mid + 1, c1); };
+        i32 a, b;
+        if (helpers && h * w >= (i64(1) << 18) && helpers->fetch_add(1) < max_helpers) {
+            Dissection d2{mark, stride, leaf, {}, {}, {}, {}, helpers, max_helpers};
+            i32 rb = -1;
+            std::exception_ptr err;
+            std::thread t([&] {
+                try {
+                    rb = far(d2);
+                } catch (...) {
+                    err = std::current_exception();
+                }
+            });
+            try {
+                a = near();
+            } catch (...) {
+                t.join();
+                helpers->fetch_sub(1);
+                throw;
+            }
+            t.join();
+            helpers->fetch_sub(1);
+            if (err) std::rethrow_exception(err);
+            const i32 shift = append(d2);
+            b = rb >= 0 ? rb + shift : -1;
+        } else {
+            if (helpers && h * w >= (i64(1) << 18)) helpers->fetch_sub(1);
+            a = near();
+            b = far(*this);
+        }
         if (a >= 0) ch.push_back(a);
         if (b >= 0) ch.push_back(b);
         const i32 sep = static_cast<i32>(cells.size());
@@ -86,7 +136,7 @@ struct GraphDissection {
         std::vector<std::vector<i32>> kids;
     };
     const std::vector<i64>& ptr;
-    const std::vector<i32>& adj;
+    const i32* adj;
     std::size_t leaf;
     // stamp[v]: the search that owns v (unique per call, so a search only
     // ever matches its own nodes, also while other threads re-stamp theirs);
@@ -98,10 +148,10 @@ struct GraphDissection {
     int max_helpers;
     Out out;
 
-    GraphDissection(i32 n, const std::vector<i64>& p, const std::vector<i32>& a, int lf)
+    GraphDissection(i32 n, const std::vector<i64>& p, const i32* a, int lf)
         : ptr(p), adj(a), leaf(static_cast<std::size_t>(lf)), stamp(new std::atomic<i32>[n > 0 ? n : 1]),
           level(n, -1),
-          max_helpers(static_cast<int>(std::min<unsigned>(15, std::max(1u, std::thread::hardware_concurrency()) - 1))) {
+          max_helpers(host_threads() - 1) {
         for (i32 v = 0; v < n; ++v) stamp[v].store(-1, std::memory_order_relaxed);
     }
 
@@ -266,71 +316,169 @@ namespace {
 // Fronts, levels and layouts of a plan whose unknowns (elimination order),
 // fronts (first, npiv, children) and neighbour lists (nptr / nidx: unknown ->
 // neighbouring unknowns, nsrc: the source id of each entry) are set.
-void finish_plan(Plan& p, const std::vector<std::vector<i32>>& kids, const std::vector<i64>& nptr,
-                 const std::vector<i32>& nidx, const std::vector<i64>& nsrc) {
+void finish_plan(Plan& p, const std::vector<std::vector<i32>>& kids, const i64* nptr, const i32* nidx,
+                 const i64* nsrc) {
     const i32 n = p.n, nf = p.nfront;
+    progress("plan: fronts of %d unknowns", n);
     p.nupd.assign(nf, 0), p.upd.assign(nf, 0), p.cb.assign(nf, 0), p.ce.assign(nf, 0), p.height.assign(nf, 0);
-    std::vector<i32> stamp(n, -1), loc(n, -1), upd;
-    p.fe_ptr.assign(nf + 1, 0);
-    // every entry (q, r) with r > q is assembled into q's front: at most half
-    // the neighbour lists
-    const std::size_t nent = static_cast<std::size_t>(nptr[n] / 2 + 1);
-    p.fe_row.reserve(nent), p.fe_col.reserve(nent), p.fe_p.reserve(nent), p.fe_q.reserve(nent);
-    p.fe_src.reserve(nent);
     p.kids.reserve(nf);
+    // 1. Update rows, children before parents: the rows past a front's pivots
+    // that its pivots' entries or its children's update rows reach, ascending.
+    // Fronts are in post-order, so the subtree of t is the fronts
+    // t - size[t] + 1 .. t: subtrees of at most `grain` fronts are tasks for
+    // host threads, each writing its fronts' rows into a buffer of its own,
+    // and the fronts above them follow in order on this thread. A stamp per
+    // row skips most repeats; subtrees running at once may stamp one
+    // ancestor row in turn, so each list is also sorted and made unique.
+    std::vector<i32> size(nf, 1), parent(nf, -1);
     for (i32 t = 0; t < nf; ++t) {
         p.cb[t] = static_cast<i32>(p.kids.size());
         i32 h = 0;
         for (i32 c : kids[t]) {
             p.kids.push_back(c);
             h = std::max(h, p.height[c] + 1);
+            size[t] += size[c];
+            parent[c] = t;
         }
         p.height[t] = h;
         p.ce[t] = static_cast<i32>(p.kids.size());
+    }
+    const i32 grain = std::max<i32>(1024, nf / (8 * host_threads()));
+    std::vector<i32> tasks, top;
+    {
+        std::vector<i32> st;
+        for (i32 t = 0; t < nf; ++t)
+            if (parent[t] < 0) st.push_back(t);
+        while (!st.empty()) {
+            const i32 t = st.back();
+            st.pop_back();
+            if (size[t] <= grain) {
+                tasks.push_back(t);
+                continue;
+            }
+            top.push_back(t);
+            for (i32 ci = p.cb[t]; ci < p.ce[t]; ++ci) st.push_back(p.kids[ci]);
+        }
+        std::sort(top.begin(), top.end());
+        // larger subtrees first (ties by index), for balance
+        std::sort(tasks.begin(), tasks.end(), [&](i32 a, i32 b) { return size[a] != size[b] ? size[a] > size[b] : a < b; });
+    }
+    std::unique_ptr<std::atomic<i32>[]> stamp(new std::atomic<i32>[n > 0 ? n : 1]);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 v = lo; v < hi; ++v) stamp[v].store(-1, std::memory_order_relaxed);
+    });
+    std::vector<std::vector<i32>> buf(tasks.size() + 1);
+    std::vector<i32> buf_of(nf, -1);
+    std::vector<i64> off(nf, 0);
+    const auto rows_of = [&](std::vector<i32>& out, i32 b, i32 t, std::vector<i32>& upd) {
         const i32 end = p.first[t] + p.npiv[t];
         upd.clear();
         for (i32 q = p.first[t]; q < end; ++q)
             for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) {
                 const i32 r = nidx[e];
-                if (r >= end && stamp[r] != t) stamp[r] = t, upd.push_back(r);
+                if (r >= end && stamp[r].load(std::memory_order_relaxed) != t) {
+                    stamp[r].store(t, std::memory_order_relaxed);
+                    upd.push_back(r);
+                }
             }
         for (i32 ci = p.cb[t]; ci < p.ce[t]; ++ci) {
             const i32 c = p.kids[ci];
+            const i32* cr = buf[buf_of[c]].data() + off[c];
             for (i32 i = 0; i < p.nupd[c]; ++i) {
-                const i32 r = p.rows[p.upd[c] + i];
-                if (r >= end && stamp[r] != t) stamp[r] = t, upd.push_back(r);
+                const i32 r = cr[i];
+                if (r >= end && stamp[r].load(std::memory_order_relaxed) != t) {
+                    stamp[r].store(t, std::memory_order_relaxed);
+                    upd.push_back(r);
+                }
             }
         }
         std::sort(upd.begin(), upd.end());
-        p.upd[t] = static_cast<i32>(p.rows.size());
+        upd.erase(std::unique(upd.begin(), upd.end()), upd.end());
+        buf_of[t] = b;
+        off[t] = static_cast<i64>(out.size());
         p.nupd[t] = static_cast<i32>(upd.size());
-        if (p.rows.capacity() < p.rows.size() + upd.size()) p.rows.reserve(2 * (p.rows.size() + upd.size()));
-        p.rows.insert(p.rows.end(), upd.begin(), upd.end());
-        if (p.relmap.capacity() < p.rows.size()) p.relmap.reserve(2 * p.rows.size());
-        p.relmap.resize(p.rows.size(), -1);
-        for (i32 q = p.first[t]; q < end; ++q) loc[q] = q - p.first[t];
-        for (i32 i = 0; i < p.nupd[t]; ++i) loc[upd[i]] = p.npiv[t] + i;
-        for (i32 ci = p.cb[t]; ci < p.ce[t]; ++ci) {
-            const i32 c = p.kids[ci];
-            for (i32 i = 0; i < p.nupd[c]; ++i) p.relmap[p.upd[c] + i] = loc[p.rows[p.upd[c] + i]];
-        }
-        // entries (q, r), r > q, of this front's pivots
-        for (i32 q = p.first[t]; q < end; ++q)
-            for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) {
-                const i32 r = nidx[e];
-                if (r <= q) continue;
-                p.fe_row.push_back(loc[r]);
-                p.fe_col.push_back(q - p.first[t]);
-                p.fe_p.push_back(q);
-                p.fe_q.push_back(r);
-                p.fe_src.push_back(nsrc[e]);
-            }
-        p.fe_ptr[t + 1] = static_cast<i32>(p.fe_row.size());
+        out.insert(out.end(), upd.begin(), upd.end());
+    };
+    host_tasks(static_cast<i64>(tasks.size()), [&](i64 k) {
+        std::vector<i32> upd;
+        const i32 r = tasks[k];
+        for (i32 t = r - size[r] + 1; t <= r; ++t) rows_of(buf[k], static_cast<i32>(k), t, upd);
+    });
+    {
+        std::vector<i32> upd;
+        for (i32 t : top) rows_of(buf[tasks.size()], static_cast<i32>(tasks.size()), t, upd);
+    }
+    i64 nrows = 0;
+    for (i32 t = 0; t < nf; ++t) {
+        if (nrows >= (i64(1) << 31)) throw std::invalid_argument("too many update rows for a 32-bit plan");
+        p.upd[t] = static_cast<i32>(nrows);
+        nrows += p.nupd[t];
         const i32 m = p.npiv[t] + p.nupd[t];
         p.max_front = std::max(p.max_front, m);
         const double np = p.npiv[t], nu = p.nupd[t];
         p.flops += np * np * np / 3 + np * np * nu + np * nu * nu;
     }
+    p.rows.resize(static_cast<std::size_t>(nrows));
+    host_chunks(nf, [&](i64, i64 lo, i64 hi) {
+        for (i64 t = lo; t < hi; ++t)
+            std::copy_n(buf[buf_of[t]].data() + off[t], p.nupd[t], p.rows.data() + p.upd[t]);
+    });
+    progress("plan: %zu update rows", p.rows.size());
+    // 2. Over host threads, per front: the rows of its children in its own
+    // front (relmap; every child has one parent) and the entries (q, r),
+    // r > q, of its pivots at their front-local (row, column). A row r of
+    // front t sits at r - first[t] when it is a pivot, else at npiv[t] plus
+    // its place among t's sorted update rows.
+    const auto local_row = [&](i32 t, i32 r) {
+        const i32 end = p.first[t] + p.npiv[t];
+        if (r < end) return r - p.first[t];
+        const i32* u = p.rows.data() + p.upd[t];
+        return p.npiv[t] + static_cast<i32>(std::lower_bound(u, u + p.nupd[t], r) - u);
+    };
+    p.relmap.assign(p.rows.size(), -1);
+    std::vector<i64> fcount(static_cast<std::size_t>(nf), 0);
+    host_chunks(nf, [&](i64, i64 lo, i64 hi) {
+        for (i64 tt = lo; tt < hi; ++tt) {
+            const i32 t = static_cast<i32>(tt), end = p.first[t] + p.npiv[t];
+            for (i32 ci = p.cb[t]; ci < p.ce[t]; ++ci) {
+                const i32 c = p.kids[ci];
+                for (i32 i = 0; i < p.nupd[c]; ++i) p.relmap[p.upd[c] + i] = local_row(t, p.rows[p.upd[c] + i]);
+            }
+            i64 k = 0;
+            for (i32 q = p.first[t]; q < end; ++q)
+                for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) k += nidx[e] > q;
+            fcount[t] = k;
+        }
+    });
+    p.fe_ptr.assign(static_cast<std::size_t>(nf) + 1, 0);
+    {
+        i64 tot = 0;
+        for (i32 t = 0; t < nf; ++t) {
+            tot += fcount[t];
+            if (tot >= (i64(1) << 31)) throw std::invalid_argument("too many matrix entries for a 32-bit plan");
+            p.fe_ptr[t + 1] = static_cast<i32>(tot);
+        }
+    }
+    const std::size_t nent = static_cast<std::size_t>(p.fe_ptr[nf]);
+    p.fe_row.resize(nent), p.fe_col.resize(nent), p.fe_p.resize(nent), p.fe_q.resize(nent), p.fe_src.resize(nent);
+    host_chunks(nf, [&](i64, i64 lo, i64 hi) {
+        for (i64 tt = lo; tt < hi; ++tt) {
+            const i32 t = static_cast<i32>(tt), end = p.first[t] + p.npiv[t];
+            i64 k = p.fe_ptr[t];
+            for (i32 q = p.first[t]; q < end; ++q)
+                for (i64 e = nptr[q]; e < nptr[q + 1]; ++e) {
+                    const i32 r = nidx[e];
+                    if (r <= q) continue;
+                    p.fe_row[k] = local_row(t, r);
+                    p.fe_col[k] = q - p.first[t];
+                    p.fe_p[k] = q;
+                    p.fe_q[k] = r;
+                    p.fe_src[k] = nsrc[e];
+                    ++k;
+                }
+        }
+    });
+    progress("plan: update rows and entries");
     // Levels by height; within a level smaller fronts first (the kernels give
     // small fronts a wavefront each and large ones a workgroup or more).
     i32 H = 0;
@@ -370,7 +518,7 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
     if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
     Plan p;
     p.bx = bx, p.by = by, p.four = four, p.nd = four ? 4 : 8;
-    Dissection d{mark, bx, leaf, {}, {}, {}, {}};
+    Dissection d{mark, bx, leaf, {}, {}, {}, {}, nullptr, 0};
     d.cut(0, by, 0, bx);
     p.cell = std::move(d.cells);
     p.n = static_cast<i32>(p.cell.size());
@@ -397,41 +545,59 @@ Plan make_plan(i64 bx, i64 by, const std::uint8_t* mark, bool four, int leaf) {
         }
         nptr[k + 1] = static_cast<i64>(nidx.size());
     }
-    finish_plan(p, d.kids, nptr, nidx, nsrc);
+    finish_plan(p, d.kids, nptr.data(), nidx.data(), nsrc.data());
     return p;
 }
 
 namespace {
 
-// The rest of a graph plan once the elimination order (perm) and the fronts are set.
+// The rest of a graph plan once the elimination order (perm) and the fronts
+// are set: each unknown's neighbours (its node's adjacency renumbered, in
+// adjacency order), over host threads.
 void finish_graph(Plan& p, i32 n, i64 bx, i64 by, const std::vector<std::vector<i32>>& kids,
-                  const std::vector<i64>& ptr, const std::vector<i32>& adj) {
+                  const std::vector<i64>& ptr, const i32* adj) {
     p.n = n;
     p.nfront = static_cast<i32>(p.first.size());
-    p.iperm.assign(n, -1);
-    for (i32 k = 0; k < n; ++k) p.iperm[p.perm[k]] = k;
+    p.iperm.resize(n);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 k = lo; k < hi; ++k) p.iperm[p.perm[k]] = static_cast<i32>(k);
+    });
     p.unk_of.assign(bx * by, -1);
-    for (i32 k = 0; k < n; ++k)
-        if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = k;
-    std::vector<i64> nptr(n + 1, 0), nsrc;
-    std::vector<i32> nidx;
-    nidx.reserve(static_cast<std::size_t>(ptr[n])), nsrc.reserve(static_cast<std::size_t>(ptr[n]));
-    for (i32 k = 0; k < n; ++k) {
-        const i32 c = p.perm[k];
-        for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
-            if (adj[e] < 0) continue;
-            nidx.push_back(p.iperm[adj[e]]);
-            nsrc.push_back(e);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 k = lo; k < hi; ++k)
+            if (p.cell[k] >= 0) p.unk_of[p.cell[k]] = static_cast<i32>(k);
+    });
+    std::vector<i64> nptr(static_cast<std::size_t>(n) + 1, 0);
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 k = lo; k < hi; ++k) {
+            const i32 c = p.perm[k];
+            i64 m = 0;
+            for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) m += adj[e] >= 0;
+            nptr[k + 1] = m;
         }
-        nptr[k + 1] = static_cast<i64>(nidx.size());
-    }
-    finish_plan(p, kids, nptr, nidx, nsrc);
+    });
+    host_prefix(n, nptr.data() + 1, nptr.data());
+    RawVec<i32> nidx(static_cast<std::size_t>(nptr[n]));
+    RawVec<i64> nsrc(static_cast<std::size_t>(nptr[n]));
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {
+        for (i64 k = lo; k < hi; ++k) {
+            const i32 c = p.perm[k];
+            i64 m = nptr[k];
+            for (i64 e = ptr[c]; e < ptr[c + 1]; ++e) {
+                if (adj[e] < 0) continue;
+                nidx[m] = p.iperm[adj[e]];
+                nsrc[m++] = e;
+            }
+        }
+    });
+    progress("plan: neighbour lists of %lld entries", static_cast<long long>(nptr[n]));
+    finish_plan(p, kids, nptr.data(), nidx.data(), nsrc.data());
 }
 
 }  // namespace
 
-Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
-                const std::vector<i32>& adj, int leaf) {
+Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr, const i32* adj,
+                int leaf) {
     if (bx * by >= (i64(1) << 31)) throw std::invalid_argument("box too large for a 32-bit plan");
     const i32 n = static_cast<i32>(cell.size());
     Plan p;
@@ -454,16 +620,26 @@ Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<
         finish_graph(p, n, bx, by, kids, ptr, adj);
         return p;
     }
-    std::vector<std::uint8_t> mark(bx * by, 0);
-    std::vector<i32> node_of(bx * by, -1);
-    for (i32 c = 0; c < n; ++c)
-        if (cell[c] >= 0) {
-            if (cell[c] >= bx * by) throw std::invalid_argument("graph node outside its box");
-            mark[cell[c]] = 1;
-            node_of[cell[c]] = c;
-        }
-    Dissection d{mark.data(), bx, leaf, {}, {}, {}, {}};
+    if (std::any_of(cell.begin(), cell.end(), [&](i64 c) { return c >= bx * by; }))
+        throw std::invalid_argument("graph node outside its box");
+    RawVec<std::uint8_t> mark(static_cast<std::size_t>(bx * by));
+    RawVec<i32> node_of(static_cast<std::size_t>(bx * by));
+    host_chunks(bx * by, [&](i64, i64 lo, i64 hi) {
+        std::fill(mark.begin() + lo, mark.begin() + hi, 0);
+        std::fill(node_of.begin() + lo, node_of.begin() + hi, -1);
+    });
+    host_chunks(n, [&](i64, i64 lo, i64 hi) {  // cells are distinct: one writer per slot
+        for (i64 c = lo; c < hi; ++c)
+            if (cell[c] >= 0) {
+                mark[cell[c]] = 1;
+                node_of[cell[c]] = static_cast<i32>(c);
+            }
+    });
+    progress("plan: %d cell nodes marked", n);
+    std::atomic<int> helpers{0};
+    Dissection d{mark.data(), bx, leaf, {}, {}, {}, {}, &helpers, host_threads() - 1};
     const i32 root = d.cut(0, by, 0, bx);
+    progress("plan: dissection of %zu fronts", d.first.size());
     p.cell = std::move(d.cells);
     p.first = std::move(d.first);
     p.npiv = std::move(d.npiv);

@@ -20,6 +20,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "host.hpp"
+
 namespace lsc {
 
 using i64 = std::int64_t;
@@ -54,8 +56,9 @@ struct Plan {
     // fe_ptr[t+1] go to front-local (fe_row, fe_col); fe_p / fe_q are the
     // unknowns of the entry, fe_src its source: p * nd + direction (grids) or
     // the caller's adjacency entry (graphs).
-    std::vector<i32> fe_ptr, fe_row, fe_col, fe_p, fe_q;
-    std::vector<i64> fe_src;
+    std::vector<i32> fe_ptr;
+    RawVec<i32> fe_row, fe_col, fe_p, fe_q;
+    RawVec<i64> fe_src;
     // graphs: unknown -> caller's node, and back
     std::vector<i32> perm, iperm;
     // levels: fronts of height h are order[lev_ptr[h] .. lev_ptr[h+1])
@@ -82,7 +85,7 @@ Plan window_plan(int D, bool disk, bool four, int leaf = 16);
 // between cell nodes. A graph with no cell node at all (a network, or a
 // component made only of short-circuit regions) is ordered by a nested
 // dissection of the graph itself instead (breadth-first level structures).
-Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr,
-                const std::vector<i32>& adj, int leaf = 16);
+Plan graph_plan(i64 bx, i64 by, const std::vector<i64>& cell, const std::vector<i64>& ptr, const i32* adj,
+                int leaf = 16);
 
 }  // namespace lsc

@@ -577,7 +577,7 @@ struct SystemSolver::Impl {
     int fallbacks = 0;  // FP64 refactorisations after a failed or stalled FP32 factor
     // the structure the plan was built for (a refactorisation must match it)
     std::vector<i64> ptr_host;
-    std::vector<i32> adj_host;
+    RawVec<i32> adj_host;
     template <typename T>
     void sweep_t(int k, int mode);
     template <typename T>
@@ -616,7 +616,7 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
     g_sweep.clear();
     ++sweep_gen;
     progress("analysis of %d unknowns", m.n);
-    Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj, 32);
+    Plan p = graph_plan(m.bx, m.by, m.cell, m.ptr, m.adj.data(), kGraphLeaf);
     progress("plan: %d fronts, L %lld entries, largest front %d", p.nfront, static_cast<long long>(p.lsize),
              p.max_front);
     plan.upload(std::move(p), s);
@@ -648,7 +648,7 @@ void SystemSolver::Impl::numeric_t(const HostMatrix* m, double& analyze_ms, doub
     const int H = P.H, n = P.n;
     if (m) {  // new values (m == nullptr: the values already on the device)
         diag = to_device(m->diag, s);
-        off = to_device(m->off.empty() ? std::vector<double>{0.0} : m->off, s);
+        off = m->off.empty() ? to_device(std::vector<double>{0.0}, s) : to_device(m->off, s);
     }
     perm = to_device(h.perm, s);
     iperm = to_device(h.iperm, s);

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host.hpp"
 #include "plan.hpp"
 #include "window.hpp"
 
@@ -21,6 +22,9 @@ struct NotPositiveDefinite : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// Nested-dissection leaf size of the graph plans (cells of a leaf front).
+constexpr int kGraphLeaf = 32;
+
 // The matrix in the caller's node order: diagonal, and CSR off-diagonal
 // values (the negated edge conductances) over a symmetric adjacency.
 struct HostMatrix {
@@ -28,8 +32,9 @@ struct HostMatrix {
     i64 bx = 1, by = 1;
     std::vector<i64> cell;   // node -> box cell (row-major bx x by), -1: merged node
     std::vector<i64> ptr;    // CSR
-    std::vector<i32> adj;
-    std::vector<double> diag, off;
+    RawVec<i32> adj;
+    std::vector<double> diag;
+    RawVec<double> off;
 };
 
 class SystemSolver {
