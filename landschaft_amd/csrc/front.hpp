@@ -65,12 +65,16 @@ struct Mfma<float> {
     static __device__ __forceinline__ void store(float* col, int lane, acc v) {
         *reinterpret_cast<acc*>(col + ((lane >> 4) << 2)) = v;
     }
-    // pivot d = sqrt(a) and 1 / d, both correctly rounded: the hardware
-    // reciprocal square root (1 ulp, biased) makes the factor a measurably
-    // worse preconditioner of the FP64 refinement (more refinement steps)
+    // pivot inv = 1 / sqrt(a) and d = a inv: the hardware reciprocal square
+    // root (1 ulp, biased) refined by one Newton step (an unbiased error of
+    // about 1 ulp; the unrefined one makes the factor a measurably worse
+    // preconditioner of the FP64 refinement, and the correctly rounded square
+    // root and division cost four times the instructions)
     static __device__ __forceinline__ void root(float a, float& d, float& inv) {
-        d = sqrtf(a);
-        inv = 1.0f / d;
+        float r = __builtin_amdgcn_rsqf(a);
+        r = fmaf(r, fmaf(-0.5f * a * r, r, 0.5f), r);
+        inv = r;
+        d = a * r;
     }
 };
 template <>
@@ -107,13 +111,15 @@ __device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int k0, int kend,
             Mfma<T>::root(dkk, d, inv);
             const T l = P[kk] * inv;
             P[kk] = i == k0 + kk ? d : l;
+            // every broadcast of the step before its updates: the lane reads
+            // land in distinct scalar registers, so the updates need no wait
+            T lj[16];
 #pragma unroll
-            for (int c = kk + 1; c < 16; ++c) {
-                if (kFull || k0 + c < cend) {
-                    const T ljk = bcast(l, c);
-                    P[c] -= l * ljk;
-                }
-            }
+            for (int c = kk + 1; c < 16; ++c)
+                if (kFull || k0 + c < cend) lj[c] = bcast(l, c);
+#pragma unroll
+            for (int c = kk + 1; c < 16; ++c)
+                if (kFull || k0 + c < cend) P[c] -= l * lj[c];
         }
     }
     return ok;

@@ -106,7 +106,14 @@ void allow_lds(const void* fn, std::size_t bytes) {
 
 }  // namespace
 
-void DevPlan::upload(Plan p, hipStream_t s) {
+namespace {
+
+// Host copy of the device tile layout (front.hpp tix): element (i, j), i >= j.
+int host_tix(int i, int j) { return ((((i >> 4) * ((i >> 4) + 1)) >> 1) + (j >> 4)) * 256 + ((j & 15) << 4) + (i & 15); }
+
+}  // namespace
+
+void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
     host = std::move(p);
     const Plan& h = host;
     const int H = static_cast<int>(h.lev_ptr.size()) - 1;
@@ -176,6 +183,59 @@ void DevPlan::upload(Plan p, hipStream_t s) {
     if (!h.fe_src.empty()) lbufs.back().upload(reinterpret_cast<const long long*>(h.fe_src.data()), h.fe_src.size(), s);
     d.fe_src = lbufs.back().p;
     d.nent = static_cast<int>(h.fe_row.size());
+    {
+        std::vector<FrontDesc> desc(h.nfront > 0 ? h.nfront : 1, FrontDesc{});
+        for (int k = 0; k < h.nfront; ++k) {
+            const int t = h.order[k];
+            FrontDesc& f = desc[k];
+            f.t = t, f.first = h.first[t], f.np = h.npiv[t], f.nu = h.nupd[t];
+            f.fe0 = h.fe_ptr[t], f.fe1 = h.fe_ptr[t + 1], f.cb = h.cb[t], f.ce = h.ce[t];
+            f.lpos = h.lpos[t], f.uposA = uposA[t], f.vpos = h.vpos[t];
+            const int nk = h.ce[t] - h.cb[t];
+            f.nk = nk <= 2 ? nk : -1;
+            for (int c = 0; c < 2; ++c) {
+                const int kid = c < nk ? h.kids[h.cb[t] + c] : -1;
+                f.kid[c] = kid;
+                f.ku[c] = kid >= 0 ? h.nupd[kid] : 0;
+                f.kupd[c] = kid >= 0 ? h.upd[kid] : 0;
+                f.kuposA[c] = kid >= 0 ? uposA[kid] : 0;
+                f.kvpos[c] = kid >= 0 ? h.vpos[kid] : 0;
+            }
+        }
+        dbuf = to_device(desc, s);
+        d.desc = dbuf.p;
+    }
+    d.lidx = d.uidx = d.xidx = nullptr;
+    if (tables) {
+        std::vector<unsigned short> li(static_cast<std::size_t>(h.lsize) + 4, 0), ui(static_cast<std::size_t>(ua) + 4, 0),
+            xi(static_cast<std::size_t>(ua) + 4, 0);
+        for (int t = 0; t < h.nfront; ++t) {
+            const int np = h.npiv[t], nu = h.nupd[t], m = np + nu;
+            if (m + 1 > kBigTiles * 16) continue;  // tiled in HBM (huge.hpp): no tables
+            long long e = h.lpos[t];
+            for (int j = 0; j < np; ++j)
+                for (int i = j; i < m; ++i) li[e++] = static_cast<unsigned short>(host_tix(i, j));
+            e = uposA[t];
+            for (int j = 0; j <= nu; ++j)
+                for (int i = j; i <= nu; ++i) ui[e++] = static_cast<unsigned short>(host_tix(np + i, np + j));
+            // where each entry of this front's update matrix lands in its parent
+            // (columns 0..nu-1; the last row, the right-hand side, goes to the
+            // parent's right-hand side row)
+        }
+        for (int t = 0; t < h.nfront; ++t) {
+            const int m = h.npiv[t] + h.nupd[t];
+            if (m + 1 > kBigTiles * 16) continue;
+            for (int ci = h.cb[t]; ci < h.ce[t]; ++ci) {
+                const int c = h.kids[ci], u = h.nupd[c];
+                auto rel = [&](int i) { return i < u ? h.relmap[h.upd[c] + i] : m; };
+                long long e = uposA[c];
+                for (int j = 0; j < u; ++j)
+                    for (int i = j; i <= u; ++i) xi[e++] = static_cast<unsigned short>(host_tix(rel(i), rel(j)));
+            }
+        }
+        tbuf[0] = to_device(li, s), tbuf[1] = to_device(ui, s), tbuf[2] = to_device(xi, s);
+        d.lidx = tbuf[0].p, d.uidx = tbuf[1].p, d.xidx = tbuf[2].p;
+    }
     d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
 }
 
@@ -224,7 +284,7 @@ struct WindowSolver {
     void set_plan(Plan p, int B) {
         for (LaunchGraph* g : {&g_flood, &g_factor, &g_sweep0, &g_residual, &g_sweep1}) g->clear();
         const std::size_t used0 = active_arena() ? active_arena()->used : 0;
-        plan.upload(std::move(p), s);
+        plan.upload(std::move(p), s, true);
         huge_tiles = build_huge_levels(plan.host, 32, lev_huge, huge_rows, s);
         huge_nh = 0;
         for (const HugeLevel& hl : lev_huge) huge_nh = std::max(huge_nh, static_cast<int>(hl.t.size()));
@@ -320,13 +380,13 @@ struct WindowSolver {
             const int nbig = plan.lev_fit_host[h] - ns;  // fronts of 65-159 rows
             if (ns > 0)
                 hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kSmallWaves)),
-                                   dim3(kSmallWaves * 64), small_lds<T>(plan.lev_stile[h]), s, P, h, nb,
-                                   plan.lev_stile[h],
+                                   dim3(kSmallWaves * 64), small_lds<T>(plan.lev_stile[h]), s, P,
+                                   plan.host.lev_ptr[h], ns, nb, plan.lev_stile[h],
                                    Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
             if (nbig > 0)
                 hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
                                    dim3(kBigWaves * 64), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
-                                   P, h, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
+                                   P, plan.host.lev_ptr[h] + ns, nbig, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
             if (!lev_huge[h].t.empty()) huge_factor(h, nb, Ap, L, Up);
         }
     }

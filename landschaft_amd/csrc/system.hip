@@ -704,12 +704,12 @@ void SystemSolver::Impl::numeric_t(const HostMatrix* m, double& analyze_ms, doub
         const int nbig = plan.lev_fit_host[lv] - ns;  // fronts of 65-159 rows
         if (ns > 0)
             hipLaunchKernelGGL(k_factor_small<T>, dim3(nblocks(ns, kSmallWaves)), dim3(kSmallWaves * 64),
-                               static_cast<std::size_t>(kSmallWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P, lv, 1,
-                               plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
+                               static_cast<std::size_t>(kSmallWaves) * plan.lev_stile[lv] * 256 * sizeof(T), s, P,
+                               plan.host.lev_ptr[lv], ns, 1, plan.lev_stile[lv], Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
         if (nbig > 0)
             hipLaunchKernelGGL(k_factor_big<T>, dim3(nbig), dim3(kBigWaves * 64),
-                               static_cast<std::size_t>(plan.lev_btile[lv]) * 256 * sizeof(T), s, P, lv, 1, Ap, rs0.p,
-                               one.p, Lp, y0.p, Up, fail.p);
+                               static_cast<std::size_t>(plan.lev_btile[lv]) * 256 * sizeof(T), s, P,
+                               plan.host.lev_ptr[lv] + ns, nbig, 1, Ap, rs0.p, one.p, Lp, y0.p, Up, fail.p);
         HugeLevel& hl = lev_huge[lv];
         if (hl.t.empty()) continue;
         const unsigned nh = static_cast<unsigned>(hl.t.size());

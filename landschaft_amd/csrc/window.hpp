@@ -30,6 +30,24 @@ constexpr int kMaxW = 127;         // window side of the register flood fill (2 
                                    // the voltages in LDS for node currents; wider windows: k_flood_wide, HBM
 constexpr int kMaxWideW = 511;     // window side of k_flood_wide (8 words a row: 3 x 511 x 64 B of LDS)
 
+// Everything a factorisation or substitution task reads about its front, one
+// record per level-order position (the plan's `order`), so that a task
+// starts with one load instead of a chain of dependent ones (level start,
+// front id, then its sizes and offsets, then its children's): the front, its
+// pivots, assembled entries and offsets, and its first two children's update
+// sizes and offsets (nk = -1: more than two children, read from the lists).
+struct alignas(16) FrontDesc {
+    int t, first, np, nu;
+    int fe0, fe1, cb, ce;
+    long long lpos, uposA;
+    long long vpos, reserved0;
+    int nk, ku[2], reserved1;
+    int kupd[2], kid[2];
+    long long kuposA[2];
+    long long kvpos[2];
+};
+static_assert(sizeof(FrontDesc) == 128, "front records are two 64-byte scalar loads");
+
 // Device copy of a Plan plus the per-level chunking of the solve kernels.
 struct DPlan {
     int n, nd, W, D, nfront, H;
@@ -40,6 +58,13 @@ struct DPlan {
     const int *lev_ptr, *lev_small, *lev_fit, *order;  // per level: fronts, small ones, ones of <= 159 rows
     const int *fe_ptr, *fe_row, *fe_col, *fe_p, *fe_q;  // entries assembled into each front
     const long long* fe_src;
+    const FrontDesc* desc;  // [nfront] in level order
+    // Window plans: the LDS offset (in the front's tiles) of every stored
+    // entry, so that stores and extend-adds need no index arithmetic:
+    // lidx[lpos[t] + e] for L's packed entry e of front t, uidx[uposA[t] + e]
+    // for its update matrix's, xidx[uposA[c] + e] for where entry e of child
+    // c's update matrix lands in its parent (null for graph plans).
+    const unsigned short *lidx, *uidx, *xidx;
     int nent;
     long long lsize, usizeA, vsize;
 };
@@ -53,8 +78,12 @@ struct DevPlan {
                                                                                 // largest L block, largest front
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
+    DevBuf<FrontDesc> dbuf;
+    DevBuf<unsigned short> tbuf[3];
     std::vector<int> huge;  // fronts of more than 159 rows (RHS row included), in level order
-    void upload(Plan p, hipStream_t s);
+    // `tables`: build lidx / uidx / xidx (window plans, whose fronts share
+    // every window of a batch; graph plans read the lists instead)
+    void upload(Plan p, hipStream_t s, bool tables = false);
 };
 
 // Per-window arrays of a batch (window slot k of B).

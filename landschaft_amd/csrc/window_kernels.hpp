@@ -482,15 +482,121 @@ __device__ __forceinline__ void store_packed(const T* F, int o, int rows, int nc
     }
 }
 
+// A front record, which is the same for the whole wavefront: read with
+// vector loads (the plan is not provably unwritten, so no scalar loads) and
+// moved to scalar registers, so that everything derived from it stays
+// wave-uniform (scalar branches, no vector registers held).
+__device__ __forceinline__ FrontDesc load_desc(const FrontDesc* p) {
+    struct Raw {
+        int v[sizeof(FrontDesc) / 4];
+    } r;
+    const int4* q = reinterpret_cast<const int4*>(p);
+    int4 w[sizeof(FrontDesc) / 16];
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(sizeof(FrontDesc) / 16); ++i) w[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(sizeof(FrontDesc) / 16); ++i) {
+        r.v[4 * i] = __builtin_amdgcn_readfirstlane(w[i].x);
+        r.v[4 * i + 1] = __builtin_amdgcn_readfirstlane(w[i].y);
+        r.v[4 * i + 2] = __builtin_amdgcn_readfirstlane(w[i].z);
+        r.v[4 * i + 3] = __builtin_amdgcn_readfirstlane(w[i].w);
+    }
+    return __builtin_bit_cast(FrontDesc, r);
+}
+
+// The child update matrix Uc (u rows, packed by columns, 16-byte aligned)
+// added into the front at the rows rel (LDS relbuf), streamed in 16-byte
+// chunks, eight in flight per lane, every lane loading (clamped) so the
+// chunks stay in registers.
 template <typename T, typename Sync>
-__device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid, int nt, int wave, int nw, int lane,
-                              int* flag, Sync sync, const T* aval, const double* rsw, T* Lw, double* yw, T* U) {
-    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, ma = m + 1, first = P.first[t];
+__device__ __forceinline__ void extend_add(T* F, int m, int u, const int* rel, const T* Uc, int* relbuf, int tid,
+                                           int nt, Sync sync) {
+    const int u1 = u + 1;
+    for (int i = tid; i < u; i += nt) relbuf[i] = rel[i];
+    if (tid == 0) relbuf[u] = m;
+    sync();
+    constexpr int CH = 16 / static_cast<int>(sizeof(T));
+    const int tot = static_cast<int>(colstart(u, u1));  // columns 0..u-1 of the packed triangle
+    const int nch = (tot + CH - 1) / CH;
+    const uint4* Uv = reinterpret_cast<const uint4*>(Uc);
+    for (int base = tid; base < nch; base += 8 * nt) {
+        uint4 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = base + q * nt;
+            v[q] = Uv[c < nch ? c : nch - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = base + q * nt;
+            if (c < nch) {
+                const int e0 = c * CH;
+                int jj = packed_col(e0, u1);
+                int ii = jj + static_cast<int>(e0 - colstart(jj, u1));
+#pragma unroll
+                for (int r = 0; r < CH; ++r) {
+                    if (e0 + r < tot) F[tix(relbuf[ii], relbuf[jj])] += chunk_elem<T>(v[q], r);
+                    if (++ii == u1) ++jj, ii = jj;
+                }
+            }
+        }
+    }
+    sync();
+}
+
+// out[e] = F[idx[e]] for e < sz: the team walks the entries in order
+// (coalesced stores), four LDS reads in flight per thread.
+template <typename T>
+__device__ __forceinline__ void store_by_table(const T* F, const unsigned short* idx, int sz, T* out, int tid,
+                                               int nt) {
+    for (int e0 = tid; e0 < sz; e0 += 4 * nt) {
+        int o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * nt;
+            o[q] = idx[e < sz ? e : sz - 1];
+        }
+        T v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = F[o[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (e0 + q * nt < sz) out[e0 + q * nt] = v[q];
+    }
+}
+
+// F[idx[e]] += Uc[e] for the tot entries of a child's update matrix: the
+// entries land on distinct front entries, so the team adds without
+// conflicts; four values and offsets in flight per thread.
+template <typename T, typename Sync>
+__device__ __forceinline__ void extend_by_table(T* F, const unsigned short* idx, const T* Uc, int tot, int tid,
+                                                int nt, Sync sync) {
+    for (int e0 = tid; e0 < tot; e0 += 4 * nt) {
+        int o[4];
+        T v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * nt, ec = e < tot ? e : tot - 1;
+            o[q] = idx[ec];
+            v[q] = Uc[ec];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (e0 + q * nt < tot) F[o[q]] += v[q];
+    }
+    sync();
+}
+
+template <typename T, typename Sync>
+__device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* relbuf, int tid, int nt, int wave,
+                              int nw, int lane, int* flag, Sync sync, const T* aval, const double* rsw, T* Lw,
+                              double* yw, T* U) {
+    const int np = D.np, nu = D.nu, m = np + nu, ma = m + 1, first = D.first;
     const int MT = (ma + 15) >> 4;
     const int nz = ((MT * (MT + 1)) >> 1) * 256;
     // the first entries and the pivots' RHS requested before the tiles are
     // cleared (clamped indices: every lane loads, the values stay in registers)
-    const int e0 = P.fe_ptr[t], e1 = P.fe_ptr[t + 1];
+    const int e0 = D.fe0, e1 = D.fe1;
     const int elast = e1 > e0 ? e1 - 1 : 0;  // entry arrays hold at least one element
     int er[4], ec[4];
     T ev[4];
@@ -514,52 +620,36 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
         if (e0 + tid + q * nt < e1) F[tix(er[q], ec[q])] = ev[q];
     for (int e = e0 + tid + 4 * nt; e < e1; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
     sync();
-    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
-        const int c = P.kids[ci];
-        const int u = P.nupd[c], u1 = u + 1;
-        const int* rel = P.relmap + P.upd[c];
-        const T* Uc = U + P.uposA[c];
-        // the child's row map in LDS; its update matrix (columns 0..u-1,
-        // 16-byte aligned) streamed in 16-byte chunks, eight in flight per
-        // lane, every lane loading (clamped) so the chunks stay in registers
-        for (int i = tid; i < u; i += nt) relbuf[i] = rel[i];
-        if (tid == 0) relbuf[u] = m;
-        sync();
-        constexpr int CH = 16 / static_cast<int>(sizeof(T));
-        const int tot = static_cast<int>(colstart(u, u1));  // columns 0..u-1 of the packed triangle
-        const int nch = (tot + CH - 1) / CH;
-        const uint4* Uv = reinterpret_cast<const uint4*>(Uc);
-        for (int base = tid; base < nch; base += 8 * nt) {
-            uint4 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int c = base + q * nt;
-                v[q] = Uv[c < nch ? c : nch - 1];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int c = base + q * nt;
-                if (c < nch) {
-                    const int e0 = c * CH;
-                    int jj = packed_col(e0, u1);
-                    int ii = jj + static_cast<int>(e0 - colstart(jj, u1));
-#pragma unroll
-                    for (int r = 0; r < CH; ++r) {
-                        if (e0 + r < tot) F[tix(relbuf[ii], relbuf[jj])] += chunk_elem<T>(v[q], r);
-                        if (++ii == u1) ++jj, ii = jj;
-                    }
-                }
-            }
+    const int nk = D.nk >= 0 ? D.nk : D.ce - D.cb;
+    for (int ci = 0; ci < nk; ++ci) {
+        int u, upd;
+        long long up;
+        if (D.nk >= 0) {  // the record's children
+            u = ci == 0 ? D.ku[0] : D.ku[1];
+            upd = ci == 0 ? D.kupd[0] : D.kupd[1];
+            up = ci == 0 ? D.kuposA[0] : D.kuposA[1];
+        } else {
+            const int c = P.kids[D.cb + ci];
+            u = P.nupd[c], upd = P.upd[c], up = P.uposA[c];
         }
-        sync();
+        if (P.xidx) {
+            if (u > 0) extend_by_table<T>(F, P.xidx + up, U + up, static_cast<int>(colstart(u, u + 1)), tid, nt, sync);
+        } else {
+            extend_add<T>(F, m, u, P.relmap + upd, U + up, relbuf, tid, nt, sync);
+        }
     }
     factor_front<T>(F, ma, np, wave, nw, lane, flag, sync);
     // L block and update matrix, both packed by columns: the team walks the
     // packed entries in order (coalesced stores), four LDS reads in flight
     // per thread
-    store_packed<T>(F, 0, m, np, Lw + P.lpos[t], tid, nt);
+    if (P.lidx) {
+        store_by_table<T>(F, P.lidx + D.lpos, static_cast<int>(colstart(np, m)), Lw + D.lpos, tid, nt);
+        store_by_table<T>(F, P.uidx + D.uposA, (nu + 1) * (nu + 2) / 2, U + D.uposA, tid, nt);
+    } else {
+        store_packed<T>(F, 0, m, np, Lw + D.lpos, tid, nt);
+        store_packed<T>(F, np, nu + 1, nu + 1, U + D.uposA, tid, nt);
+    }
     for (int j = tid; j < np; j += nt) yw[first + j] = static_cast<double>(F[tix(m, j)]);
-    store_packed<T>(F, np, nu + 1, nu + 1, U + P.uposA[t], tid, nt);
     sync();
 }
 
@@ -571,7 +661,7 @@ __device__ void process_front(const DPlan& P, int t, T* F, int* relbuf, int tid,
 // Small fronts (<= 64 rows with the RHS row): one wavefront each, its own LDS
 // region of `stile` tiles. Large fronts: one workgroup each, up to 55 tiles.
 template <typename T>
-__global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int h, int B, int stile,
+__global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int base, int ns, int B, int stile,
                                                            const T* __restrict__ aval, const double* __restrict__ rs,
                                                            const int* __restrict__ active, T* __restrict__ L,
                                                            double* __restrict__ y, T* __restrict__ Uall,
@@ -580,40 +670,40 @@ __global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int 
     __shared__ int s_flag[kSmallWaves];
     __shared__ int s_rel[kSmallWaves][kSmallRows + 1];
     const int lane = threadIdx.x & 63, wave = wave_id();
-    const int ns = P.lev_small[h];
     const long long task = static_cast<long long>(blockIdx.x) * kSmallWaves + wave;
     const int k = static_cast<int>(task / ns);
     if (k >= B || !active[k]) return;
-    const int t = P.order[P.lev_ptr[h] + static_cast<int>(task - static_cast<long long>(k) * ns)];
+    const FrontDesc D = load_desc(P.desc + base + static_cast<int>(task - static_cast<long long>(k) * ns));
     T* F = reinterpret_cast<T*>(smem_raw) + wave * stile * 256;
     if (lane == 0) s_flag[wave] = 0;
     wave_sync();
     auto wsync = [] { wave_sync(); };
-    process_front<T>(P, t, F, s_rel[wave], lane, 64, 0, 1, lane, &s_flag[wave], wsync,
+    process_front<T>(P, D, F, s_rel[wave], lane, 64, 0, 1, lane, &s_flag[wave], wsync,
                      aval + static_cast<long long>(k) * P.nent,
                      rs + static_cast<long long>(k) * P.n, L + static_cast<long long>(k) * P.lsize,
                      y + static_cast<long long>(k) * P.n, Uall + static_cast<long long>(k) * P.usizeA);
     if (lane == 0 && s_flag[wave]) atomicOr(&fail[k], 1);
 }
 
+// `base`: the level-order position of the level's first front of 65-159
+// rows, `nb` their number.
 template <typename T>
-__global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int h, int B, const T* __restrict__ aval,
-                                                         const double* __restrict__ rs, const int* __restrict__ active,
-                                                         T* __restrict__ L, double* __restrict__ y,
-                                                         T* __restrict__ Uall, int* __restrict__ fail) {
+__global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int base, int nb, int B,
+                                                         const T* __restrict__ aval, const double* __restrict__ rs,
+                                                         const int* __restrict__ active, T* __restrict__ L,
+                                                         double* __restrict__ y, T* __restrict__ Uall,
+                                                         int* __restrict__ fail) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_flag;
     __shared__ int s_rel[kBigTiles * 16 + 1];
     const int lane = threadIdx.x & 63, wave = wave_id();
-    const int base = P.lev_ptr[h] + P.lev_small[h];
-    const int nb = P.lev_fit[h] - P.lev_small[h];  // fronts of 65-159 rows (larger ones: huge.hpp)
     const int k = blockIdx.x / nb;
     if (k >= B || !active[k]) return;
-    const int t = P.order[base + (blockIdx.x - k * nb)];
+    const FrontDesc D = load_desc(P.desc + base + (blockIdx.x - k * nb));
     if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
     auto bsync = [] { __syncthreads(); };
-    process_front<T>(P, t, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kBigWaves * 64, wave, kBigWaves, lane,
+    process_front<T>(P, D, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kBigWaves * 64, wave, kBigWaves, lane,
                      &s_flag, bsync,
                      aval + static_cast<long long>(k) * P.nent, rs + static_cast<long long>(k) * P.n,
                      L + static_cast<long long>(k) * P.lsize, y + static_cast<long long>(k) * P.n,
