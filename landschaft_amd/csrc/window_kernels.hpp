@@ -544,45 +544,67 @@ __device__ __forceinline__ void extend_add(T* F, int m, int u, const int* rel, c
     sync();
 }
 
-// out[e] = F[idx[e]] for e < sz: the team walks the entries in order
-// (coalesced stores), four LDS reads in flight per thread.
+// out[e] = F[idx[e]] for e < sz, 0 on the padding up to a multiple of 4
+// (`out` and `idx` are 4-entry aligned): a thread takes four consecutive
+// entries (one 8-byte index load, one vector store), four such chunks in
+// flight, the team walking the chunks in order (coalesced).
 template <typename T>
 __device__ __forceinline__ void store_by_table(const T* F, const unsigned short* idx, int sz, T* out, int tid,
                                                int nt) {
-    for (int e0 = tid; e0 < sz; e0 += 4 * nt) {
-        int o[4];
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short I4 __attribute__((ext_vector_type(4)));
+    const int nch = (sz + 3) >> 2;
+    const I4* ix = reinterpret_cast<const I4*>(idx);
+    V4* o = reinterpret_cast<V4*>(out);
+    for (int c0 = tid; c0 < nch; c0 += 4 * nt) {
+        I4 id[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int e = e0 + q * nt;
-            o[q] = idx[e < sz ? e : sz - 1];
+            const int c = c0 + q * nt;
+            id[q] = ix[c < nch ? c : nch - 1];
         }
-        T v[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = F[o[q]];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (e0 + q * nt < sz) out[e0 + q * nt] = v[q];
+        for (int q = 0; q < 4; ++q) {
+            const int c = c0 + q * nt, e = 4 * c;
+            V4 v;
+            v.x = e < sz ? F[id[q].x] : T(0);
+            v.y = e + 1 < sz ? F[id[q].y] : T(0);
+            v.z = e + 2 < sz ? F[id[q].z] : T(0);
+            v.w = e + 3 < sz ? F[id[q].w] : T(0);
+            if (c < nch) o[c] = v;
+        }
     }
 }
 
-// F[idx[e]] += Uc[e] for the tot entries of a child's update matrix: the
-// entries land on distinct front entries, so the team adds without
-// conflicts; four values and offsets in flight per thread.
+// F[idx[e]] += Uc[e] for the tot entries of a child's update matrix (4-entry
+// aligned): the entries land on distinct front entries, so the team adds
+// without conflicts; a thread takes four consecutive entries (one vector
+// load of values, one of offsets), eight such chunks in flight.
 template <typename T, typename Sync>
 __device__ __forceinline__ void extend_by_table(T* F, const unsigned short* idx, const T* Uc, int tot, int tid,
                                                 int nt, Sync sync) {
-    for (int e0 = tid; e0 < tot; e0 += 4 * nt) {
-        int o[4];
-        T v[4];
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short I4 __attribute__((ext_vector_type(4)));
+    const int nch = (tot + 3) >> 2;
+    const I4* ix = reinterpret_cast<const I4*>(idx);
+    const V4* uv = reinterpret_cast<const V4*>(Uc);
+    for (int c0 = tid; c0 < nch; c0 += 8 * nt) {
+        I4 id[8];
+        V4 v[8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = e0 + q * nt, ec = e < tot ? e : tot - 1;
-            o[q] = idx[ec];
-            v[q] = Uc[ec];
+        for (int q = 0; q < 8; ++q) {
+            const int c = c0 + q * nt, cc = c < nch ? c : nch - 1;
+            id[q] = ix[cc];
+            v[q] = uv[cc];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (e0 + q * nt < tot) F[o[q]] += v[q];
+        for (int q = 0; q < 8; ++q) {
+            const int e = 4 * (c0 + q * nt);
+            if (e < tot) F[id[q].x] += v[q].x;
+            if (e + 1 < tot) F[id[q].y] += v[q].y;
+            if (e + 2 < tot) F[id[q].z] += v[q].z;
+            if (e + 3 < tot) F[id[q].w] += v[q].w;
+        }
     }
     sync();
 }
