@@ -205,7 +205,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
         dbuf = to_device(desc, s);
         d.desc = dbuf.p;
     }
-    d.lidx = d.uidx = d.xidx = nullptr;
+    d.lidx = d.uidx = d.xidx = d.eidx = nullptr;
     if (tables) {
         std::vector<unsigned short> li(static_cast<std::size_t>(h.lsize) + 4, 0), ui(static_cast<std::size_t>(ua) + 4, 0),
             xi(static_cast<std::size_t>(ua) + 4, 0);
@@ -233,8 +233,11 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
                     for (int i = j; i <= u; ++i) xi[e++] = static_cast<unsigned short>(host_tix(rel(i), rel(j)));
             }
         }
-        tbuf[0] = to_device(li, s), tbuf[1] = to_device(ui, s), tbuf[2] = to_device(xi, s);
-        d.lidx = tbuf[0].p, d.uidx = tbuf[1].p, d.xidx = tbuf[2].p;
+        std::vector<unsigned short> ei(h.fe_row.size() + 1, 0);
+        for (std::size_t e = 0; e < h.fe_row.size(); ++e)
+            ei[e] = static_cast<unsigned short>(host_tix(h.fe_row[e], h.fe_col[e]));
+        tbuf[0] = to_device(li, s), tbuf[1] = to_device(ui, s), tbuf[2] = to_device(xi, s), tbuf[3] = to_device(ei, s);
+        d.lidx = tbuf[0].p, d.uidx = tbuf[1].p, d.xidx = tbuf[2].p, d.eidx = tbuf[3].p;
     }
     d.lsize = h.lsize, d.usizeA = ua, d.vsize = h.vsize;
 }

@@ -63,8 +63,9 @@ struct DPlan {
     // entry, so that stores and extend-adds need no index arithmetic:
     // lidx[lpos[t] + e] for L's packed entry e of front t, uidx[uposA[t] + e]
     // for its update matrix's, xidx[uposA[c] + e] for where entry e of child
-    // c's update matrix lands in its parent (null for graph plans).
-    const unsigned short *lidx, *uidx, *xidx;
+    // c's update matrix lands in its parent, eidx[e] for assembled entry e
+    // (null for graph plans).
+    const unsigned short *lidx, *uidx, *xidx, *eidx;
     int nent;
     long long lsize, usizeA, vsize;
 };
@@ -79,7 +80,7 @@ struct DevPlan {
     std::vector<DevBuf<int>> ibufs;
     std::vector<DevBuf<long long>> lbufs;
     DevBuf<FrontDesc> dbuf;
-    DevBuf<unsigned short> tbuf[3];
+    DevBuf<unsigned short> tbuf[4];
     std::vector<int> huge;  // fronts of more than 159 rows (RHS row included), in level order
     // `tables`: build lidx / uidx / xidx (window plans, whose fronts share
     // every window of a batch; graph plans read the lists instead)

@@ -620,18 +620,22 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
     // cleared (clamped indices: every lane loads, the values stay in registers)
     const int e0 = D.fe0, e1 = D.fe1;
     const int elast = e1 > e0 ? e1 - 1 : 0;  // entry arrays hold at least one element
-    int er[4], ec[4];
+    const bool tab = P.eidx != nullptr;
+    int eo[4];  // the entries' LDS offsets
     T ev[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int e = e0 + tid + q * nt;
         const int eq = e < e1 ? e : elast;
-        er[q] = P.fe_row[eq];
-        ec[q] = P.fe_col[eq];
+        eo[q] = tab ? P.eidx[eq] : tix(P.fe_row[eq], P.fe_col[eq]);
         ev[q] = aval[eq];
     }
     const double rhs0 = np > 0 ? rsw[first + (tid < np ? tid : np - 1)] : 0.0;
-    for (int e = tid; e < nz; e += nt) F[e] = T(0);
+    {  // the tiles cleared 16 bytes at a time (nz is a multiple of 256)
+        uint4* Fz = reinterpret_cast<uint4*>(F);
+        const int nz16 = nz * static_cast<int>(sizeof(T)) / 16;
+        for (int e = tid; e < nz16; e += nt) Fz[e] = uint4{0u, 0u, 0u, 0u};
+    }
     sync();
     for (int j = tid; j < np; j += nt) {
         F[tix(j, j)] = T(1);
@@ -639,8 +643,8 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        if (e0 + tid + q * nt < e1) F[tix(er[q], ec[q])] = ev[q];
-    for (int e = e0 + tid + 4 * nt; e < e1; e += nt) F[tix(P.fe_row[e], P.fe_col[e])] = aval[e];
+        if (e0 + tid + q * nt < e1) F[eo[q]] = ev[q];
+    for (int e = e0 + tid + 4 * nt; e < e1; e += nt) F[tab ? P.eidx[e] : tix(P.fe_row[e], P.fe_col[e])] = aval[e];
     sync();
     const int nk = D.nk >= 0 ? D.nk : D.ce - D.cb;
     for (int ci = 0; ci < nk; ++ci) {
