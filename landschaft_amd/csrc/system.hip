@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_g_keep(int n, const int* __restrict__ k
 // diagonal launch). Every L block is read once per sweep for all right-hand
 // sides, and the results do not depend on the launch's scheduling.
 
-constexpr int kWidePanel = 32;
+constexpr int kWidePanel = 64;  // pivots per panel: one wavefront's lanes
 
 __global__ __launch_bounds__(256) void k_w_fgather(DPlan P, const int* __restrict__ ht,
                                                    const long long* __restrict__ woff, int n, int k,
@@ -346,13 +346,17 @@ __global__ __launch_bounds__(256) void k_w_extend(DPlan P, const int* __restrict
         V[static_cast<long long>(r) * vstride + P.vpos[c] + ii];
 }
 
-// tri[c * 32 + r] = L(j0 + r, j0 + c) of the jb x jb diagonal block (0 elsewhere).
+// tri[c * kWidePanel + r] = L(j0 + r, j0 + c) of the jb x jb diagonal block
+// (0 elsewhere), dinv[c] = 1 / L(j0 + c, j0 + c).
 template <typename T>
-__device__ __forceinline__ void stage_tri32(const T* __restrict__ Lt, int m, int j0, int jb, double* __restrict__ tri) {
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        const int c = e >> 5, r = e & 31;
+__device__ __forceinline__ void stage_tri(const T* __restrict__ Lt, int m, int j0, int jb, double* __restrict__ tri,
+                                          double* __restrict__ dinv) {
+    for (int e = threadIdx.x; e < kWidePanel * kWidePanel; e += blockDim.x) {
+        const int c = e / kWidePanel, r = e - c * kWidePanel;
         tri[e] = (r < jb && c < jb && r >= c) ? static_cast<double>(Lt[hcol(j0 + c, m) + (r - c)]) : 0.0;
     }
+    for (int c = threadIdx.x; c < kWidePanel; c += blockDim.x)
+        dinv[c] = c < jb ? 1.0 / static_cast<double>(Lt[hcol(j0 + c, m)]) : 0.0;
 }
 
 // Forward, panel s: the diagonal block solved for right-hand sides 16 y .. 16 y + 15.
@@ -360,13 +364,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_w_fdiag(DPlan P, const int* __restrict__ ht,
                                                  const long long* __restrict__ woff, int k, int s,
                                                  const T* __restrict__ L, double* __restrict__ W) {
-    __shared__ double tri[1024];
+    __shared__ double tri[kWidePanel * kWidePanel], dinv[kWidePanel];
     const int f = blockIdx.x, t = ht[f];
     const int np = P.npiv[t], m = np + P.nupd[t];
     const int j0 = kWidePanel * s;
     if (j0 >= np) return;
     const int jb = min(kWidePanel, np - j0);
-    stage_tri32<T>(L + P.lpos[t], m, j0, jb, tri);
+    stage_tri<T>(L + P.lpos[t], m, j0, jb, tri, dinv);
     __syncthreads();
     double* Wf = W + woff[f] * k;
     const int lane = threadIdx.x & 63, wave = wave_id();
@@ -375,8 +379,8 @@ __global__ __launch_bounds__(256) void k_w_fdiag(DPlan P, const int* __restrict_
         double* w = Wf + static_cast<long long>(r) * m + j0;
         double v = lane < jb ? w[lane] : 0.0;
         for (int q = 0; q < jb; ++q) {
-            const double yq = bcast(v, q) / tri[q * 33];
-            if (lane > q && lane < jb) v -= tri[q * 32 + lane] * yq;
+            const double yq = bcast(v, q) * dinv[q];
+            if (lane > q && lane < jb) v -= tri[q * kWidePanel + lane] * yq;
             if (lane == q) v = yq;
         }
         if (lane < jb) w[lane] = v;
@@ -404,14 +408,14 @@ __global__ __launch_bounds__(256) void k_w_fupdate(DPlan P, const int* __restric
     const int i = j0 + jb + blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     const T* Lt = L + P.lpos[t];
-    double Lq[kWidePanel];
+    T Lq[kWidePanel];
 #pragma unroll
-    for (int q = 0; q < kWidePanel; ++q) Lq[q] = q < jb ? static_cast<double>(Lt[hcol(j0 + q, m) + i - j0 - q]) : 0.0;
+    for (int q = 0; q < kWidePanel; ++q) Lq[q] = q < jb ? Lt[hcol(j0 + q, m) + i - j0 - q] : T(0);
     for (int r = 0; r < rn; ++r) {
-        double a = 0.0;
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < kWidePanel; ++q) a += Lq[q] * ys[r * kWidePanel + q];
-        Wf[static_cast<long long>(r0 + r) * m + i] -= a;
+        for (int q = 0; q < kWidePanel; ++q) a[q & 3] += static_cast<double>(Lq[q]) * ys[r * kWidePanel + q];
+        Wf[static_cast<long long>(r0 + r) * m + i] -= (a[0] + a[1]) + (a[2] + a[3]);
     }
 }
 
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(256) void k_w_bgather(DPlan P, const int* __restric
 // Backward, step s (the block of pivots s-th from the last): partial dot
 // products of the block's columns with the solved rows below, chunk c of 256
 // rows, right-hand sides 64 z .. 64 z + 63; thread (q, g) takes column q and
-// right-hand sides 8 g .. 8 g + 7.
+// right-hand sides kRg g .. kRg g + kRg - 1.
 template <typename T>
 __global__ __launch_bounds__(256) void k_w_bdots(DPlan P, const int* __restrict__ ht,
                                                  const long long* __restrict__ woff,
@@ -463,13 +467,14 @@ __global__ __launch_bounds__(256) void k_w_bdots(DPlan P, const int* __restrict_
     if (i0 >= m) return;
     const int i1 = min(m, i0 + 256);
     const int r0 = 64 * blockIdx.z, rn = min(64, k - r0);
-    const int q = threadIdx.x & 31, g = threadIdx.x >> 5;
+    constexpr int kRg = 64 * kWidePanel / 256;  // right-hand sides of a thread
+    const int q = threadIdx.x % kWidePanel, g = threadIdx.x / kWidePanel;
     const T* Lt = L + P.lpos[t];
     const long long colbase = hcol(jlo + q, m) - (jlo + q);  // L(i, jlo + q) = Lt[colbase + i], i > jlo + q
     const double* Wf = W + woff[f] * k;
-    double acc[8];
+    double acc[kRg];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+    for (int j = 0; j < kRg; ++j) acc[j] = 0.0;
     for (int s0 = i0; s0 < i1; s0 += 64) {
         const int cnt = min(64, i1 - s0);
         __syncthreads();
@@ -482,13 +487,13 @@ __global__ __launch_bounds__(256) void k_w_bdots(DPlan P, const int* __restrict_
             for (int ii = 0; ii < cnt; ++ii) {
                 const double l = static_cast<double>(Lt[colbase + s0 + ii]);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) acc[j] += l * xs[(g * 8 + j < rn ? g * 8 + j : 0) * 64 + ii];
+                for (int j = 0; j < kRg; ++j) acc[j] += l * xs[(g * kRg + j < rn ? g * kRg + j : 0) * 64 + ii];
             }
     }
     double* pf = part + (cpre[f] + blockIdx.x) * static_cast<long long>(k) * kWidePanel;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int rr = g * 8 + j;
+    for (int j = 0; j < kRg; ++j) {
+        const int rr = g * kRg + j;
         if (rr < rn) pf[static_cast<long long>(r0 + rr) * kWidePanel + q] = acc[j];
     }
 }
@@ -501,14 +506,14 @@ __global__ __launch_bounds__(256) void k_w_bdiag(DPlan P, const int* __restrict_
                                                  const long long* __restrict__ cpre, int k, int s,
                                                  const T* __restrict__ L, const double* __restrict__ part,
                                                  double* __restrict__ W) {
-    __shared__ double tri[1024];
+    __shared__ double tri[kWidePanel * kWidePanel], dinv[kWidePanel];
     const int f = blockIdx.x, t = ht[f];
     const int np = P.npiv[t], m = np + P.nupd[t];
     const int b = (np + kWidePanel - 1) / kWidePanel - 1 - s;
     if (b < 0) return;
     const int jlo = kWidePanel * b, jb = min(kWidePanel, np - jlo);
     const int nch = (m - jlo - jb + 255) / 256;
-    stage_tri32<T>(L + P.lpos[t], m, jlo, jb, tri);
+    stage_tri<T>(L + P.lpos[t], m, jlo, jb, tri, dinv);
     __syncthreads();
     double* Wf = W + woff[f] * k;
     const double* pf = part + cpre[f] * static_cast<long long>(k) * kWidePanel;
@@ -521,8 +526,8 @@ __global__ __launch_bounds__(256) void k_w_bdiag(DPlan P, const int* __restrict_
         double* w = Wf + static_cast<long long>(r) * m + jlo;
         double v = lane < jb ? w[lane] - d : 0.0;
         for (int qq = jb - 1; qq >= 0; --qq) {
-            const double xq = bcast(v, qq) / tri[qq * 33];
-            if (lane < qq) v -= tri[lane * 32 + qq] * xq;
+            const double xq = bcast(v, qq) * dinv[qq];
+            if (lane < qq) v -= tri[lane * kWidePanel + qq] * xq;
             if (lane == qq) v = xq;
         }
         if (lane < jb) w[lane] = v;
@@ -866,20 +871,22 @@ void SystemSolver::Impl::sweep_t(int k, int mode) {
     const DPlan& P = plan.d;
     const Plan& h = plan.host;
     const T* Lp = reinterpret_cast<const T*>(L.p);
-    auto lds = [&](int lv) { return 192 * sizeof(double) + static_cast<std::size_t>(plan.lev_lmax[lv] + 4) * sizeof(T); };
+    auto lds = [&](int lv) {
+        return plan.lev_mmax[lv] <= 64 ? rhs_lds<T, 1>(plan.lev_lmax[lv]) : rows_lds<T>(plan.lev_lmax[lv]);
+    };
     auto rgrid = [&](int lv) {
         return dim3(static_cast<unsigned>(plan.lev_fit_host[lv]), static_cast<unsigned>((k + kRhs - 1) / kRhs));
     };
     if (mode == 1)
         for (int lv = 0; lv < P.H; ++lv) {
-            auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_rhs<T, 1> : k_forward_rhs<T, 3>;
+            auto kern = plan.lev_mmax[lv] <= 64 ? k_forward_rhs<T, 1> : k_forward_rows<T, 3>;
             if (plan.lev_fit_host[lv] > 0)
                 hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, rs.p, y.p, V.p);
             huge_forward<T>(lev_huge[lv], k);
         }
     for (int lv = P.H - 1; lv >= 0; --lv) {
         huge_backward<T>(lev_huge[lv], k);
-        auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_rhs<T, 1> : k_backward_rhs<T, 3>;
+        auto kern = plan.lev_mmax[lv] <= 64 ? k_backward_rhs<T, 1> : k_backward_rows<T, 3>;
         if (plan.lev_fit_host[lv] > 0)
             hipLaunchKernelGGL(kern, rgrid(lv), dim3(64), lds(lv), s, P, lv, k, done.p, Lp, y.p, z.p);
     }
@@ -915,11 +922,10 @@ void SystemSolver::Impl::solve_t(int k, const double* d_b, double* d_x, double t
     }
     int lmax = 1;
     for (int v : plan.lev_lmax) lmax = std::max(lmax, v);
-    const std::size_t lds = 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);
-    sys_allow(reinterpret_cast<const void*>(&k_forward_rhs<T, 1>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_forward_rhs<T, 3>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 1>), lds);
-    sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 3>), lds);
+    sys_allow(reinterpret_cast<const void*>(&k_forward_rhs<T, 1>), rhs_lds<T, 1>(lmax));
+    sys_allow(reinterpret_cast<const void*>(&k_forward_rows<T, 3>), rows_lds<T>(lmax));
+    sys_allow(reinterpret_cast<const void*>(&k_backward_rhs<T, 1>), rhs_lds<T, 1>(lmax));
+    sys_allow(reinterpret_cast<const void*>(&k_backward_rows<T, 3>), rows_lds<T>(lmax));
     const long long kn = static_cast<long long>(k) * n;
     const dim3 g2(nb, k);
     const dim3 gk(nb, cdiv(k, kSpmvRhs));  // the products: kSpmvRhs right-hand sides per thread

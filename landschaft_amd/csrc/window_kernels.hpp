@@ -1122,11 +1122,224 @@ __global__ __launch_bounds__(64) void k_backward_lanes(DPlan P, int h, int B, lo
 }
 
 // Solves of one system for many right-hand sides (graphs: one L shared by
-// every instance): workgroup (f, g) is one wavefront that stages front f's L
-// block in LDS once and solves right-hand sides g kRhs.. against it, so L is
-// read once per kRhs right-hand sides instead of once each.
+// every instance): workgroup (f, g) is one wavefront for front f and the
+// kRhs right-hand sides g kRhs.., the front's L block staged in LDS once.
+// Lane 4 r + q holds rows 16 R q .. 16 R q + 16 R - 1 of right-hand side r's
+// front vector (R = 1: fronts of at most 64 rows; R = 3: at most 192), so a
+// pivot is eliminated for all of them at once: the forward substitution
+// broadcasts y_j from its row's lane to the three others of its quad, the
+// backward sums a pivot's dot product over the quad; each lane updates its
+// own rows. The dependency chain of a front is one pivot step long per
+// pivot, not one per pivot and right-hand side.
 constexpr int kRhs = 16;
 
+// v of lane `src` (LDS crossbar; no LDS memory is touched).
+__device__ __forceinline__ double lane_read(double v, int src) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b >> 32));
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// v summed over the four lanes of its quad (DPP quad permutations), the
+// same sum in every lane of the quad.
+template <int Ctrl>
+__device__ __forceinline__ double quad_perm(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b & 0xffffffffll), Ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), Ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double quad_sum(double v) {
+    v += quad_perm<0xB1>(v);  // lanes (1, 0, 3, 2)
+    return v + quad_perm<0x4E>(v);  // lanes (2, 3, 0, 1)
+}
+
+// The L block of front t into LDS (16-byte chunks; blocks start on 16 bytes).
+template <typename T>
+__device__ __forceinline__ void stage_L(const T* L, long long lpos, int np, int m, int lane, T* Lc) {
+    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
+    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(L + lpos);
+    uint4* dst = reinterpret_cast<uint4*>(Lc);
+#pragma unroll 8
+    for (int e = lane; e < nv; e += 64) dst[e] = src[e];
+}
+
+// LDS of the multi-right-hand-side kernels: kRhs front vectors of 64 R rows
+// (doubles), then the L block with 64 R entries of slack (the row loops read
+// up to that far past a column; those terms are masked).
+template <typename T, int R>
+__host__ __device__ constexpr std::size_t rhs_lds(int lmax) {
+    return static_cast<std::size_t>(kRhs) * 64 * R * sizeof(double) +
+           static_cast<std::size_t>(lmax + 64 * R + 4) * sizeof(T);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_forward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+                                                    const T* __restrict__ L, const double* __restrict__ rs,
+                                                    double* __restrict__ y, double* __restrict__ Vall) {
+    constexpr int MR = 64 * R;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* fs = reinterpret_cast<double*>(smem_raw);  // [kRhs][MR]
+    T* Lc = reinterpret_cast<T*>(smem_raw + kRhs * MR * sizeof(double));
+    const int lane = threadIdx.x, rq = lane >> 2, q = lane & 3;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int g0 = blockIdx.y * kRhs;
+    stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
+    // the front vectors: the pivots' right-hand sides, then the children's
+    // update vectors added at their rows (in child order)
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = g0 + rr;
+        const bool on = inst < k && !done[inst];
+        const double* rsw = rs + static_cast<long long>(on ? inst : 0) * P.n + first;
+        for (int i = lane; i < m; i += 64) fs[rr * MR + i] = on && i < np ? rsw[i] : 0.0;
+    }
+    for (int ci = P.cb[t]; ci < P.ce[t]; ++ci) {
+        const int c = P.kids[ci];
+        const int u = P.nupd[c];
+        const int* rel = P.relmap + P.upd[c];
+        for (int rr = 0; rr < kRhs; ++rr) {
+            const int inst = g0 + rr;
+            if (inst >= k || done[inst]) continue;
+            const double* Vc = Vall + static_cast<long long>(inst) * P.vsize + P.vpos[c];
+            for (int ii = lane; ii < u; ii += 64) fs[rr * MR + rel[ii]] += Vc[ii];
+        }
+    }
+    wave_sync();
+    // rows 64 sb + 16 q + ii of right-hand side rq in f[sb][ii]
+    double f[R][16];
+#pragma unroll
+    for (int sb = 0; sb < R; ++sb)
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int gi = 64 * sb + 16 * q + ii;
+            f[sb][ii] = gi < m ? fs[rq * MR + gi] : 0.0;
+        }
+    const int quad = lane & ~3;
+    for (int j0 = 0; j0 < np; j0 += 16) {
+        const int tb = j0 >> 6, qb = (j0 >> 4) & 3;  // the pivots' slot block and quad
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int j = j0 + s;
+            if (j < np) {
+                const T* col = Lc + colstart(j, m) - j;  // col[i] = L[i][j] for i >= j
+                double fj = f[0][s];
+#pragma unroll
+                for (int sb = 1; sb < R; ++sb) fj = tb == sb ? f[sb][s] : fj;
+                const double yj = lane_read(fj * inv_pivot(col[j]), quad | qb);
+#pragma unroll
+                for (int sb = 0; sb < R; ++sb) {
+                    if (tb == sb && q == qb) f[sb][s] = yj;
+#pragma unroll
+                    for (int ii = 0; ii < 16; ++ii) {
+                        const int gi = 64 * sb + 16 * q + ii;
+                        const double lv = static_cast<double>(col[gi]);
+                        f[sb][ii] = gi > j && gi < m ? fma(-lv, yj, f[sb][ii]) : f[sb][ii];
+                    }
+                }
+            }
+        }
+    }
+    // out through LDS: y of the pivots, the update vector of the rest
+    wave_sync();
+#pragma unroll
+    for (int sb = 0; sb < R; ++sb)
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int gi = 64 * sb + 16 * q + ii;
+            if (gi < m) fs[rq * MR + gi] = f[sb][ii];
+        }
+    wave_sync();
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = g0 + rr;
+        if (inst >= k || done[inst]) continue;
+        double* yw = y + static_cast<long long>(inst) * P.n + first;
+        double* Vt = Vall + static_cast<long long>(inst) * P.vsize + P.vpos[t] - np;
+        for (int i = lane; i < m; i += 64) {
+            const double v = fs[rr * MR + i];
+            if (i < np) yw[i] = v;
+            else Vt[i] = v;
+        }
+    }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64) void k_backward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+                                                     const T* __restrict__ L, const double* __restrict__ y,
+                                                     double* __restrict__ z) {
+    constexpr int MR = 64 * R;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* xs = reinterpret_cast<double*>(smem_raw);  // [kRhs][MR]
+    T* Lc = reinterpret_cast<T*>(smem_raw + kRhs * MR * sizeof(double));
+    const int lane = threadIdx.x, rq = lane >> 2, q = lane & 3;
+    const int t = P.order[P.lev_ptr[h] + blockIdx.x];
+    const int np = P.npiv[t], nu = P.nupd[t], m = np + nu, first = P.first[t];
+    const int* rows = P.rows + P.upd[t];
+    const int g0 = blockIdx.y * kRhs;
+    stage_L<T>(L, P.lpos[t], np, m, lane, Lc);
+    // the pivots' y and the solved update rows' z
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = g0 + rr;
+        const bool on = inst < k && !done[inst];
+        const long long off = static_cast<long long>(on ? inst : 0) * P.n;
+        for (int i = lane; i < m; i += 64)
+            xs[rr * MR + i] = !on ? 0.0 : i < np ? y[off + first + i] : z[off + rows[i - np]];
+    }
+    wave_sync();
+    double x[R][16];
+#pragma unroll
+    for (int sb = 0; sb < R; ++sb)
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int gi = 64 * sb + 16 * q + ii;
+            x[sb][ii] = gi < m ? xs[rq * MR + gi] : 0.0;
+        }
+    for (int j0 = ((np - 1) >> 4) << 4; j0 >= 0; j0 -= 16) {
+        const int tb = j0 >> 6, qb = (j0 >> 4) & 3;
+#pragma unroll
+        for (int s = 15; s >= 0; --s) {
+            const int j = j0 + s;
+            if (j < np) {
+                const T* col = Lc + colstart(j, m) - j;
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int sb = 0; sb < R; ++sb)
+#pragma unroll
+                    for (int ii = 0; ii < 16; ++ii) {
+                        const int gi = 64 * sb + 16 * q + ii;
+                        const double lv = gi > j && gi < m ? static_cast<double>(col[gi]) : 0.0;
+                        acc[ii & 3] = fma(lv, x[sb][ii], acc[ii & 3]);
+                    }
+                const double dot = quad_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+                const double d = inv_pivot(col[j]);
+#pragma unroll
+                for (int sb = 0; sb < R; ++sb)
+                    if (tb == sb && q == qb) x[sb][s] = (x[sb][s] - dot) * d;
+            }
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int sb = 0; sb < R; ++sb)
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int gi = 64 * sb + 16 * q + ii;
+            if (gi < np) xs[rq * MR + gi] = x[sb][ii];
+        }
+    wave_sync();
+    for (int rr = 0; rr < kRhs; ++rr) {
+        const int inst = g0 + rr;
+        if (inst >= k || done[inst]) continue;
+        double* zw = z + static_cast<long long>(inst) * P.n + first;
+        for (int i = lane; i < np; i += 64) zw[i] = xs[rr * MR + i];
+    }
+}
+
+// Fronts of 65 to 192 rows (R = 3): one right-hand side at a time, rows on
+// lanes (three a lane), the pivot broadcast by a lane read; their 48 rows a
+// lane would hold too many registers in the layout above.
 template <typename T, int R>
 __device__ __forceinline__ void forward_compute(const T* Lc, const double* fv, int np, int m, int lane, double* yw,
                                                 double* Vt) {
@@ -1227,19 +1440,15 @@ __device__ __forceinline__ void backward_compute(const T* Lc, const double* xu, 
     }
 }
 
-// The L block of front t into LDS (16-byte chunks; blocks start on 16 bytes).
+// LDS of the rows-on-lanes kernels: one front vector (192 doubles), then
+// the L block.
 template <typename T>
-__device__ __forceinline__ void stage_L(const T* L, long long lpos, int np, int m, int lane, T* Lc) {
-    const long long sz = static_cast<long long>(np) * m - ((static_cast<long long>(np) * (np - 1)) >> 1);
-    const int nv = static_cast<int>((sz * static_cast<long long>(sizeof(T)) + 15) / 16);
-    const uint4* src = reinterpret_cast<const uint4*>(L + lpos);
-    uint4* dst = reinterpret_cast<uint4*>(Lc);
-#pragma unroll 8
-    for (int e = lane; e < nv; e += 64) dst[e] = src[e];
+__host__ __device__ constexpr std::size_t rows_lds(int lmax) {
+    return 192 * sizeof(double) + static_cast<std::size_t>(lmax + 4) * sizeof(T);
 }
 
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_forward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+__global__ __launch_bounds__(64) void k_forward_rows(DPlan P, int h, int k, const int* __restrict__ done,
                                                     const T* __restrict__ L, const double* __restrict__ rs,
                                                     double* __restrict__ y, double* __restrict__ Vall) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1272,7 +1481,7 @@ __global__ __launch_bounds__(64) void k_forward_rhs(DPlan P, int h, int k, const
 }
 
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_backward_rhs(DPlan P, int h, int k, const int* __restrict__ done,
+__global__ __launch_bounds__(64) void k_backward_rows(DPlan P, int h, int k, const int* __restrict__ done,
                                                      const T* __restrict__ L, const double* __restrict__ y,
                                                      double* __restrict__ z) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
