@@ -646,14 +646,18 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
         if (e0 + tid + q * nt < e1) F[eo[q]] = ev[q];
     for (int e = e0 + tid + 4 * nt; e < e1; e += nt) F[tab ? P.eidx[e] : tix(P.fe_row[e], P.fe_col[e])] = aval[e];
     sync();
+    // the children's update matrices. A record's (at most two) children are
+    // told apart by arithmetic on the child number: a select between the two
+    // fields would be turned into an indexed read of the record, which puts
+    // the record in scratch memory.
     const int nk = D.nk >= 0 ? D.nk : D.ce - D.cb;
     for (int ci = 0; ci < nk; ++ci) {
         int u, upd;
         long long up;
-        if (D.nk >= 0) {  // the record's children
-            u = ci == 0 ? D.ku[0] : D.ku[1];
-            upd = ci == 0 ? D.kupd[0] : D.kupd[1];
-            up = ci == 0 ? D.kuposA[0] : D.kuposA[1];
+        if (D.nk >= 0) {
+            u = D.ku[0] + ci * (D.ku[1] - D.ku[0]);
+            upd = D.kupd[0] + ci * (D.kupd[1] - D.kupd[0]);
+            up = D.kuposA[0] + ci * (D.kuposA[1] - D.kuposA[0]);
         } else {
             const int c = P.kids[D.cb + ci];
             u = P.nupd[c], upd = P.upd[c], up = P.uposA[c];
