@@ -39,11 +39,11 @@ HBM_PEAK = 8.0e12          # bytes/s
 # HBM traffic per window, from rocprofv3 --pmc FETCH_SIZE (x2, the reading
 # of a 256 MiB copy: python bench.py --copy-probe) and WRITE_SIZE (x1) over
 # one single-stream run of this workload at 1024 x 1024 (29,147 windows
-# factored, 5.06 substitution sweeps each); profiles/ holds the runs.
+# factored, 5.05 substitution sweeps each); profiles/ holds the runs.
 # One substitution sweep (forward and backward sweeps averaged):
-SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 2160535
+SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 2154205
 # The factorisation (assembly and every front level, forward fused):
-FACTOR_TRAFFIC_PER_WINDOW = 5289138
+FACTOR_TRAFFIC_PER_WINDOW = 5298470
 
 
 def fractal_field(n: int, seed: int, exponent: float = -3.0) -> np.ndarray:
