@@ -491,8 +491,7 @@ struct WindowSolver {
                 hipLaunchKernelGGL(kern, grid(h), dim3(64), lds, s, P, h, nb, P.lsize,
                                    wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
         }
-        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(kThreads), 0, s, P.n, nb, mode, static_cast<long long>(P.n),
-                           wb.active.p, wb.done.p, wb.sc.p, wb.z.p, wb.x.p);
+        // x takes z in the residual that follows every sweep (k_residual)
     }
 
     // Device bytes of one window's batch buffers with L entries of `elem` bytes.
@@ -558,7 +557,7 @@ struct WindowSolver {
             g_residual.run(s, nb, gr, [&] {
                 reset(open.p, 1, s);
                 hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kThreads), 0, s, P, nb, wb.g.p, wb.present.p,
-                                   wb.dg.p, wb.sc.p, wb.b.p, wb.x.p, wb.rs.p, wb.active.p, wb.done.p, wb.rnorm.p,
+                                   wb.dg.p, wb.sc.p, wb.b.p, wb.z.p, wb.x.p, wb.rs.p, wb.active.p, wb.done.p, wb.rnorm.p,
                                    wb.rprev.p, wb.iters.p, tol, open.p);
             });
             if (timing) LSC_HIP(hipEventRecord(ev[1], s));

@@ -1508,22 +1508,6 @@ __global__ __launch_bounds__(64) void k_backward_rows(DPlan P, int h, int k, con
     }
 }
 
-// x = s z (mode 0) or x += s z (mode 1) for the instances not done.
-static __global__ __launch_bounds__(kThreads) void k_update_x(int n, int B, int mode, long long sc_stride,
-                                                              const int* __restrict__ active,
-                                                              const int* __restrict__ done,
-                                                              const double* __restrict__ sc,
-                                                              const double* __restrict__ z, double* __restrict__ x) {
-    const int k = blockIdx.x;
-    if (k >= B || !active[k] || done[k]) return;
-    const long long off = static_cast<long long>(k) * n;
-    const double* sk = sc + static_cast<long long>(k) * sc_stride;
-    for (int p = threadIdx.x; p < n; p += kThreads) {
-        const double d = sk[p] * z[off + p];
-        x[off + p] = mode == 0 ? d : x[off + p] + d;
-    }
-}
-
 // The neighbour offsets of ndy / ndx as arithmetic on a compile-time count,
 // so that unrolled loops fold them into immediates.
 template <int ND>
@@ -1580,15 +1564,19 @@ __device__ __forceinline__ void residual_rows(const DPlan& P, int W, int n, long
     }
 }
 
-// r = b - A x in FP64 from the conductances (the reference's matrix,
-// shifted, system.cpp:458-487); relative residual ||r|| / ||b||; the scaled
-// residual s r for the next refinement step. A window is done when it
-// reaches `tol` or stops improving.
+// The substitutions' solution z (of the scaled system) taken into x first:
+// x = s z after the first solve, x += s z after a refinement step (the
+// window's step count tells which); then r = b - A x in FP64 from the
+// conductances (the reference's matrix, shifted, system.cpp:458-487);
+// relative residual ||r|| / ||b||; the scaled residual s r for the next
+// refinement step. A window is done when it reaches `tol` or stops
+// improving. Every substitution sweep is followed by this kernel, so x is
+// the solution once the last residual is taken.
 static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, const double* __restrict__ g,
                                                        const unsigned char* __restrict__ present,
                                                        const double* __restrict__ dg, const double* __restrict__ sc,
-                                                       const double* __restrict__ bvec,
-                                                       const double* __restrict__ x, double* __restrict__ rs,
+                                                       const double* __restrict__ bvec, const double* __restrict__ z,
+                                                       double* __restrict__ x, double* __restrict__ rs,
                                                        const int* __restrict__ active, int* __restrict__ done,
                                                        double* __restrict__ rnorm, double* __restrict__ rprev,
                                                        int* __restrict__ iters, double tol, int* __restrict__ open) {
@@ -1599,6 +1587,14 @@ static __global__ __launch_bounds__(kThreads) void k_residual(DPlan P, int B, co
     const double* gw = g + static_cast<long long>(k) * W2;
     const unsigned char* pw = present + static_cast<long long>(k) * W2;
     const long long off = static_cast<long long>(k) * n;
+    {
+        const bool first = iters[k] == 0;
+        for (int p = threadIdx.x; p < n; p += kThreads) {
+            const double d = sc[off + p] * z[off + p];
+            x[off + p] = first ? d : x[off + p] + d;
+        }
+        __syncthreads();
+    }
     double rr = 0.0, bb = 0.0;
     if (nd == 8) residual_rows<8>(P, W, n, off, gw, pw, dg, sc, bvec, x, rs, rr, bb);
     else residual_rows<4>(P, W, n, off, gw, pw, dg, sc, bvec, x, rs, rr, bb);
