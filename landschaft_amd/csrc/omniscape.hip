@@ -270,7 +270,7 @@ struct WindowSolver {
     double residual_tol = -1.0;
     // per level: rows M of the one-window-per-lane substitutions (0: the
     // one-front-per-wavefront kernels), forward and backward
-    std::vector<int> lane_fwd, lane_bwd;
+    std::vector<int> quad_fwd, quad_bwd;
     // mixed precision: the FP64 solver of the windows whose FP32 factor
     // failed or whose refinement stalled (made on first use)
     std::unique_ptr<WindowSolver<double>> fb;
@@ -315,7 +315,7 @@ struct WindowSolver {
         allow_lds(reinterpret_cast<const void*>(&k_backward_level<T, 3>), solve_lds<T>(lmax));
         allow_lds(reinterpret_cast<const void*>(&k_currents), static_cast<std::size_t>(kMaxW) * kMaxW * sizeof(double));
         if (plan.d.W > kMaxW) allow_lds(reinterpret_cast<const void*>(&k_flood_wide), wide_flood_lds(plan.d.W));
-        lane_levels();
+        quad_levels();
         const int W = plan.d.W;
         if (active_arena()) B = arena_batch(B);
         Bcap = B;
@@ -326,14 +326,14 @@ struct WindowSolver {
         if (!open.p) open.alloc(4);
     }
 
-    // A level is solved one window per lane when its fronts have at most
+    // A level is solved four lanes per window when its fronts have at most
     // kLaneNp pivots and 64 rows (the forward also needs fronts without
     // children), and every column's reads up to M rows stay inside the
     // window's L. LSC_LANE_LEVELS=0 keeps every level on the per-front kernels.
-    void lane_levels() {
+    void quad_levels() {
         const Plan& h = plan.host;
         const int H = static_cast<int>(h.lev_ptr.size()) - 1;
-        lane_fwd.assign(H, 0), lane_bwd.assign(H, 0);
+        quad_fwd.assign(H, 0), quad_bwd.assign(H, 0);
         const char* e = std::getenv("LSC_LANE_LEVELS");
         if (e && e[0] == '0') return;
         for (int lv = 0; lv < H; ++lv) {
@@ -355,8 +355,8 @@ struct WindowSolver {
                 if (np > 0 && h.lpos[t] + j * m - ((j * (j - 1)) >> 1) - j + M > h.lsize) fits = false;
             }
             if (!fits) continue;
-            lane_bwd[lv] = M;
-            if (leaves) lane_fwd[lv] = M;
+            quad_bwd[lv] = M;
+            if (leaves) quad_fwd[lv] = M;
         }
     }
 
@@ -440,18 +440,18 @@ struct WindowSolver {
         auto grid = [&](int h) {
             return dim3(static_cast<unsigned>(static_cast<long long>(nb) * plan.lev_fit_host[h]));
         };
-        auto lanes = [&](int h) {
-            return dim3((static_cast<unsigned>(nb) + 63) / 64,
+        auto quadg = [&](int h) {
+            return dim3((static_cast<unsigned>(nb) + 15) / 16,
                         static_cast<unsigned>(plan.host.lev_ptr[h + 1] - plan.host.lev_ptr[h]));
         };
         if (mode == 1)
             for (int h = 0; h < P.H; ++h) {
-                if (const int M = lane_fwd[h]) {
-                    auto kern = M == 32   ? k_forward_lanes<T, 32>
-                                : M == 48 ? k_forward_lanes<T, 48>
-                                : M == 56 ? k_forward_lanes<T, 56>
-                                          : k_forward_lanes<T, 64>;
-                    hipLaunchKernelGGL(kern, lanes(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p,
+                if (const int M = quad_fwd[h]) {
+                    auto kern = M == 32   ? k_forward_quads<T, 32>
+                                : M == 48 ? k_forward_quads<T, 48>
+                                : M == 56 ? k_forward_quads<T, 56>
+                                          : k_forward_quads<T, 64>;
+                    hipLaunchKernelGGL(kern, quadg(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p,
                                        L, wb.rs.p, wb.y.p, V.p);
                     continue;
                 }
@@ -469,12 +469,12 @@ struct WindowSolver {
                                        static_cast<const int*>(wb.done.p), static_cast<const int*>(wb.active.p));
             }
         for (int h = P.H - 1; h >= 0; --h) {
-            if (const int M = lane_bwd[h]) {
-                auto kern = M == 32   ? k_backward_lanes<T, 32>
-                            : M == 48 ? k_backward_lanes<T, 48>
-                            : M == 56 ? k_backward_lanes<T, 56>
-                                      : k_backward_lanes<T, 64>;
-                hipLaunchKernelGGL(kern, lanes(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p, L,
+            if (const int M = quad_bwd[h]) {
+                auto kern = M == 32   ? k_backward_quads<T, 32>
+                            : M == 48 ? k_backward_quads<T, 48>
+                            : M == 56 ? k_backward_quads<T, 56>
+                                      : k_backward_quads<T, 64>;
+                hipLaunchKernelGGL(kern, quadg(h), dim3(64), 0, s, P, h, nb, P.lsize, wb.active.p, wb.done.p, L,
                                    wb.y.p, wb.z.p);
                 continue;
             }

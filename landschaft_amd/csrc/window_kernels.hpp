@@ -1033,98 +1033,6 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
 
-// Levels of small fronts solved one window per lane: lane l of workgroup
-// (g, f) substitutes front f of the level for window 64 g + l, the front's
-// vector in registers (rows statically indexed, loops unrolled to kLaneNp
-// pivots and M rows). A lane reads its own window's L column by column,
-// every row up to M at constant offsets from the column's start, so the
-// loads of a column are in flight together and no value crosses lanes. Rows
-// past the front's m read the entries that follow the column in the same
-// window's L (the host checks that they lie inside it); the forward
-// discards them, the backward masks their terms.
-constexpr int kLaneNp = 32;
-
-// Forward substitution of a front without children (reference
-// system.cpp:974-989): y = L11^-1 rs, update vector -L21 y.
-template <typename T, int M>
-__global__ __launch_bounds__(64) void k_forward_lanes(DPlan P, int h, int B, long long lstride,
-                                                      const int* __restrict__ active, const int* __restrict__ done,
-                                                      const T* __restrict__ L, const double* __restrict__ rs,
-                                                      double* __restrict__ y, double* __restrict__ Vall) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
-    if (k >= B || !active[k] || done[k]) return;
-    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
-    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t];
-    const double* rw = rs + static_cast<long long>(k) * P.n + first;
-    double f[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) f[i] = i < np ? rw[i] : 0.0;
-#pragma unroll
-    for (int j = 0; j < kLaneNp; ++j) {
-        if (j < np) {
-            const T* col = Lt + colstart(j, m) - j;  // col[i] = L[i][j] for i >= j
-            T lv[M];
-#pragma unroll
-            for (int i = j; i < M; ++i) lv[i] = col[i];
-            const double yj = f[j] * inv_pivot(lv[j]);
-            f[j] = yj;
-#pragma unroll
-            for (int i = j + 1; i < M; ++i) f[i] -= static_cast<double>(lv[i]) * yj;
-        }
-    }
-    double* yw = y + static_cast<long long>(k) * P.n + first;
-    double* Vt = Vall + static_cast<long long>(k) * P.vsize + P.vpos[t] - np;
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        if (i < np) yw[i] = f[i];
-        else if (i < m) Vt[i] = f[i];
-    }
-}
-
-// Back substitution of a front (reference system.cpp:992-1016): z_piv =
-// L11^-T (y - L21^T z_upd), column j of L dotted with the solved rows below
-// it, from the last pivot up.
-template <typename T, int M>
-__global__ __launch_bounds__(64) void k_backward_lanes(DPlan P, int h, int B, long long lstride,
-                                                       const int* __restrict__ active, const int* __restrict__ done,
-                                                       const T* __restrict__ L, const double* __restrict__ y,
-                                                       double* __restrict__ z) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
-    if (k >= B || !active[k] || done[k]) return;
-    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
-    const int* rows = P.rows + P.upd[t];
-    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t];
-    const long long off = static_cast<long long>(k) * P.n;
-    double x[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        if (i < np) x[i] = y[off + first + i];
-        else if (i < m) x[i] = z[off + rows[i - np]];
-        else x[i] = 0.0;
-    }
-#pragma unroll
-    for (int j = kLaneNp - 1; j >= 0; --j) {
-        if (j < np) {
-            const T* col = Lt + colstart(j, m) - j;
-            T lv[M];
-#pragma unroll
-            for (int i = j; i < M; ++i) lv[i] = col[i];
-            // four partial sums: the dot is not one chain of dependent adds
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int i = j + 1; i < M; ++i)
-                acc[i & 3] = fma(static_cast<double>(i < m ? lv[i] : T(0)), x[i], acc[i & 3]);
-            x[j] = (x[j] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * inv_pivot(lv[j]);
-        }
-    }
-    double* zw = z + off + first;
-#pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < np) zw[i] = x[i];
-}
-
 // Solves of one system for many right-hand sides (graphs: one L shared by
 // every instance): workgroup (f, g) is one wavefront for front f and the
 // kRhs right-hand sides g kRhs.., the front's L block staged in LDS once.
@@ -1157,6 +1065,128 @@ __device__ __forceinline__ double quad_perm(double v) {
 __device__ __forceinline__ double quad_sum(double v) {
     v += quad_perm<0xB1>(v);  // lanes (1, 0, 3, 2)
     return v + quad_perm<0x4E>(v);  // lanes (2, 3, 0, 1)
+}
+
+// Levels of small fronts solved four lanes per window: lane 4 w + q of
+// workgroup (g, f) substitutes front f of the level for window 16 g + w and
+// holds rows q, q + 4, q + 8, .. of the front's vector in registers (rows
+// statically indexed, loops unrolled to kLaneNp pivots and M rows). The four
+// lanes of a window read four neighbouring entries of a column, so one load
+// instruction touches 16 windows' segments of L rather than 64 windows' single
+// entries, and the loads of a column are in flight together. A pivot's value
+// is broadcast over its quad (forward) or its dot product summed over the quad
+// (backward) with DPP; each lane updates only its own rows. A segment that
+// straddles the diagonal reads rows above it (inside the same L block), and
+// rows past the front's m read the entries that follow the column in the same
+// window's L (the host checks that they lie inside it); the forward discards
+// them, the backward masks their terms.
+constexpr int kLaneNp = 32;
+
+// every lane of the quad takes lane r's value (r folds to a constant in the
+// unrolled loops that call this)
+__device__ __forceinline__ double quad_bcast(double v, int r) {
+    switch (r) {
+        case 0: return quad_perm<0x00>(v);
+        case 1: return quad_perm<0x55>(v);
+        case 2: return quad_perm<0xAA>(v);
+        default: return quad_perm<0xFF>(v);
+    }
+}
+
+// Forward substitution of a front without children (reference
+// system.cpp:974-989): y = L11^-1 rs, update vector -L21 y.
+template <typename T, int M>
+__global__ __launch_bounds__(64) void k_forward_quads(DPlan P, int h, int B, long long lstride,
+                                                      const int* __restrict__ active, const int* __restrict__ done,
+                                                      const T* __restrict__ L, const double* __restrict__ rs,
+                                                      double* __restrict__ y, double* __restrict__ Vall) {
+    constexpr int MQ = M / 4;
+    static_assert(M % 4 == 0, "four lanes a window");
+    const int q = threadIdx.x & 3;
+    const int k = blockIdx.x * 16 + (threadIdx.x >> 2);
+    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
+    // the four lanes of a window leave together: DPP reads stay in live quads
+    if (k >= B || !active[k] || done[k]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
+    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t] + q;
+    const double* rw = rs + static_cast<long long>(k) * P.n + first + q;
+    double f[MQ];
+#pragma unroll
+    for (int s = 0; s < MQ; ++s) f[s] = 4 * s + q < np ? rw[4 * s] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kLaneNp; ++j) {
+        if (j < np) {
+            const T* col = Lt + colstart(j, m) - j;  // col[4 s] = L[4 s + q][j]
+            T lv[MQ];
+#pragma unroll
+            for (int s = j >> 2; s < MQ; ++s) lv[s] = col[4 * s];
+            const double yj = quad_bcast(f[j >> 2] * inv_pivot(lv[j >> 2]), j & 3);
+#pragma unroll
+            for (int s = j >> 2; s < MQ; ++s) {
+                const int i = 4 * s + q;
+                if (i > j) f[s] -= static_cast<double>(lv[s]) * yj;
+                else if (i == j) f[s] = yj;
+            }
+        }
+    }
+    double* yw = y + static_cast<long long>(k) * P.n + first + q;
+    double* Vt = Vall + static_cast<long long>(k) * P.vsize + P.vpos[t] - np + q;
+#pragma unroll
+    for (int s = 0; s < MQ; ++s) {
+        const int i = 4 * s + q;
+        if (i < np) yw[4 * s] = f[s];
+        else if (i < m) Vt[4 * s] = f[s];
+    }
+}
+
+// Back substitution of a front (reference system.cpp:992-1016): z_piv =
+// L11^-T (y - L21^T z_upd), column j of L dotted with the solved rows below
+// it, from the last pivot up.
+template <typename T, int M>
+__global__ __launch_bounds__(64) void k_backward_quads(DPlan P, int h, int B, long long lstride,
+                                                       const int* __restrict__ active, const int* __restrict__ done,
+                                                       const T* __restrict__ L, const double* __restrict__ y,
+                                                       double* __restrict__ z) {
+    constexpr int MQ = M / 4;
+    static_assert(M % 4 == 0, "four lanes a window");
+    const int q = threadIdx.x & 3;
+    const int k = blockIdx.x * 16 + (threadIdx.x >> 2);
+    const int t = P.order[P.lev_ptr[h] + blockIdx.y];
+    if (k >= B || !active[k] || done[k]) return;
+    const int np = P.npiv[t], m = np + P.nupd[t], first = P.first[t];
+    const int* rows = P.rows + P.upd[t];
+    const T* Lt = L + static_cast<long long>(k) * lstride + P.lpos[t] + q;
+    const long long off = static_cast<long long>(k) * P.n;
+    double x[MQ];
+#pragma unroll
+    for (int s = 0; s < MQ; ++s) {
+        const int i = 4 * s + q;
+        if (i < np) x[s] = y[off + first + i];
+        else if (i < m) x[s] = z[off + rows[i - np]];
+        else x[s] = 0.0;
+    }
+#pragma unroll
+    for (int j = kLaneNp - 1; j >= 0; --j) {
+        if (j < np) {
+            const T* col = Lt + colstart(j, m) - j;
+            T lv[MQ];
+#pragma unroll
+            for (int s = j >> 2; s < MQ; ++s) lv[s] = col[4 * s];
+            double acc[2] = {0.0, 0.0};
+#pragma unroll
+            for (int s = j >> 2; s < MQ; ++s) {
+                const int i = 4 * s + q;
+                const double l = (i > j && i < m) ? static_cast<double>(lv[s]) : 0.0;
+                acc[s & 1] = fma(l, x[s], acc[s & 1]);
+            }
+            const double dot = quad_sum(acc[0] + acc[1]);
+            if (q == (j & 3)) x[j >> 2] = (x[j >> 2] - dot) * inv_pivot(lv[j >> 2]);
+        }
+    }
+    double* zw = z + off + first + q;
+#pragma unroll
+    for (int s = 0; s < MQ; ++s)
+        if (4 * s + q < np) zw[4 * s] = x[s];
 }
 
 // The L block of front t into LDS (16-byte chunks; blocks start on 16 bytes).

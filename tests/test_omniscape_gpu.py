@@ -142,7 +142,7 @@ def test_graph_replay_matches_eager_launches(backend, oracle, precision, monkeyp
 @pytest.mark.parametrize("four", [False, True])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_lane_substitutions_match_per_front_kernels(backend, oracle, precision, four, monkeypatch):
-    """The small levels' substitutions one window per lane (the default) against the
+    """The small levels' substitutions four lanes per window (the default) against the
     one-front-per-wavefront kernels (LSC_LANE_LEVELS=0), radius 12 windows with 8 and 4 neighbours:
     the maps agree to rounding (the two sum in different orders) and both match the oracle."""
     rng = np.random.default_rng(31)
