@@ -5,8 +5,9 @@
 // of F11, F21, F22: src/cpp/circuit/system.cpp:899-908) runs tile column by
 // tile column:
 //   * the panel (the 16 columns of one tile column, rows k0..m-1) is held in
-//     registers, a row per lane, and factored with lane broadcasts
-//     (v_readlane) - no LDS traffic and no barrier inside the panel;
+//     registers, a row per lane, and factored step by step, each step's
+//     multipliers of the diagonal block broadcast to the wavefront through
+//     a small LDS buffer of its own - no barrier inside the panel;
 //   * the trailing tiles are updated with v_mfma_f32_16x16x4f32 (or the f64
 //     MFMA), C(I,J) -= P_I P_J^T, spread over the waves of the team.
 // Fronts carry one extra row, the right-hand side: eliminating it with the
@@ -99,8 +100,18 @@ struct Mfma<double> {
 };
 
 // The 16 elimination steps of a panel held as P[16] (a row per lane, row i).
+// A step's column of the diagonal block (the multipliers of rows k0..k0+15,
+// lanes 0..15) reaches every lane through the wavefront's 64-entry LDS
+// buffer `bc`: one store of every lane's value, then 16-byte reads of the
+// entries the step needs, the same address on every lane (a broadcast). The
+// update then costs one FMA per later column; a lane read (v_readlane) per
+// column would cost a second vector instruction. A wavefront's LDS
+// operations complete in order, so the next step's store cannot overtake
+// this step's reads.
 template <typename T, bool kFull>
-__device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int k0, int kend, int cend) {
+__device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int lane, int k0, int kend, int cend, T* bc) {
+    constexpr int V = 16 / static_cast<int>(sizeof(T));  // entries per 16-byte read
+    typedef T vec __attribute__((ext_vector_type(V)));
     bool ok = true;
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
@@ -111,15 +122,19 @@ __device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int k0, int kend,
             Mfma<T>::root(dkk, d, inv);
             const T l = P[kk] * inv;
             P[kk] = i == k0 + kk ? d : l;
-            // every broadcast of the step before its updates: the lane reads
-            // land in distinct scalar registers, so the updates need no wait
-            T lj[16];
+            if (kk < 15 && (kFull || k0 + kk + 1 < cend)) {
+                bc[lane] = l;
+                T lj[16];
 #pragma unroll
-            for (int c = kk + 1; c < 16; ++c)
-                if (kFull || k0 + c < cend) lj[c] = bcast(l, c);
+                for (int g = (kk + 1) / V; g < 16 / V; ++g) {
+                    const vec v = *reinterpret_cast<const vec*>(bc + g * V);
 #pragma unroll
-            for (int c = kk + 1; c < 16; ++c)
-                if (kFull || k0 + c < cend) P[c] -= l * lj[c];
+                    for (int e = 0; e < V; ++e) lj[g * V + e] = v[e];
+                }
+#pragma unroll
+                for (int c = kk + 1; c < 16; ++c)
+                    if (kFull || k0 + c < cend) P[c] -= l * lj[c];
+            }
         }
     }
     return ok;
@@ -137,12 +152,14 @@ __device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int k0, int kend,
 // it back. With `diag_out`, the factored diagonal block goes there instead
 // (16 x 16, column-major, zero above the diagonal) and F keeps the
 // unfactored one, for teams spread over several workgroups that cannot be
-// ordered inside one launch.
+// ordered inside one launch. `bc`: the wavefront's 64-entry LDS buffer
+// (16-byte aligned) for the steps' broadcasts.
 // The updates run on every lane without a branch: entries above the
 // diagonal of the diagonal block then hold values that are never read (the
 // broadcasts read rows at or below their column) nor stored.
 template <typename T, typename Sync>
-__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase, T* diag_out, Sync sync) {
+__device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase, T* diag_out, T* bc,
+                      Sync sync) {
     const int cend = min(k0 + 16, m);
     const int i = lane < 16 ? k0 + lane : k0 + 16 + rowbase + 48 * wave + (lane - 16);
     const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0 && diag_out == nullptr));
@@ -156,8 +173,8 @@ __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int row
     }
     sync();
     // a full panel (16 pivots, so 16 columns) without any column test
-    const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, k0, kend, cend)
-                                    : panel_steps<T, false>(P, i, k0, kend, cend);
+    const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, lane, k0, kend, cend, bc)
+                                    : panel_steps<T, false>(P, i, lane, k0, kend, cend, bc);
     if (mine) {
 #pragma unroll
         for (int c = 0; c < 16; ++c)
@@ -233,15 +250,16 @@ __device__ __forceinline__ void next_tile(int& I, int& J, int step, int Tn) {
 
 // Partial Cholesky of the first np columns of a tiled front of m rows by a
 // team of `nw` wavefronts (this one is `wave`); `sync` orders the team. The
-// result flag is returned to every wave through `flag` (team-visible memory).
+// result flag is returned to every wave through `flag` (team-visible memory);
+// `bc` is this wave's 64-entry LDS buffer for the panel's broadcasts.
 // A wave's trailing tiles are pipelined: the next tile's operands are read
 // while the current tile's MFMAs run.
 template <typename T, typename Sync>
-__device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, int* flag, Sync sync) {
+__device__ bool factor_front(T* F, int m, int np, int wave, int nw, int lane, int* flag, T* bc, Sync sync) {
     const int MT = (m + 15) >> 4;
     for (int Kt = 0; Kt * 16 < np; ++Kt) {
         const int k0 = Kt * 16, kend = min(k0 + 16, np);
-        const bool ok = panel<T>(F, m, k0, kend, wave, lane, 0, static_cast<T*>(nullptr), sync);
+        const bool ok = panel<T>(F, m, k0, kend, wave, lane, 0, static_cast<T*>(nullptr), bc, sync);
         if (!ok && wave == 0 && lane == 0) *flag = 1;
         sync();
         const int Tn = MT - Kt - 1;

@@ -121,9 +121,10 @@ __global__ __launch_bounds__(256) void k_h_panel(DPlan P, HugeInst H, int Kt, co
     const int below = ma - k0 - 16;
     if (blockIdx.x > 0 && static_cast<int>(192 * blockIdx.x) >= below) return;
     const int lane = threadIdx.x & 63, wave = wave_id();
+    __shared__ __align__(16) T s_bc[4][64];
     const bool ok = panel<T>(F + (q.il * H.ftiles + hoff[q.i]) * 256, ma, k0, min(k0 + 16, np), wave, lane,
                              192 * blockIdx.x, blockIdx.x == 0 ? D + static_cast<long long>(blockIdx.y) * 256 : nullptr,
-                             [] __device__() {});
+                             s_bc[wave], [] __device__() {});
     if (!ok && lane == 0) atomicOr(H.fail + static_cast<long long>(q.inst) * H.fail_stride, 1);
 }
 

@@ -610,9 +610,9 @@ __device__ __forceinline__ void extend_by_table(T* F, const unsigned short* idx,
 }
 
 template <typename T, typename Sync>
-__device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* relbuf, int tid, int nt, int wave,
-                              int nw, int lane, int* flag, Sync sync, const T* aval, const double* rsw, T* Lw,
-                              double* yw, T* U) {
+__device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* relbuf, T* bc, int tid, int nt,
+                              int wave, int nw, int lane, int* flag, Sync sync, const T* aval, const double* rsw,
+                              T* Lw, double* yw, T* U) {
     const int np = D.np, nu = D.nu, m = np + nu, ma = m + 1, first = D.first;
     const int MT = (ma + 15) >> 4;
     const int nz = ((MT * (MT + 1)) >> 1) * 256;
@@ -668,7 +668,7 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
             extend_add<T>(F, m, u, P.relmap + upd, U + up, relbuf, tid, nt, sync);
         }
     }
-    factor_front<T>(F, ma, np, wave, nw, lane, flag, sync);
+    factor_front<T>(F, ma, np, wave, nw, lane, flag, bc, sync);
     // L block and update matrix, both packed by columns: the team walks the
     // packed entries in order (coalesced stores), four LDS reads in flight
     // per thread
@@ -699,6 +699,7 @@ __global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_flag[kSmallWaves];
     __shared__ int s_rel[kSmallWaves][kSmallRows + 1];
+    __shared__ __align__(16) T s_bc[kSmallWaves][64];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const long long task = static_cast<long long>(blockIdx.x) * kSmallWaves + wave;
     const int k = static_cast<int>(task / ns);
@@ -708,7 +709,7 @@ __global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int 
     if (lane == 0) s_flag[wave] = 0;
     wave_sync();
     auto wsync = [] { wave_sync(); };
-    process_front<T>(P, D, F, s_rel[wave], lane, 64, 0, 1, lane, &s_flag[wave], wsync,
+    process_front<T>(P, D, F, s_rel[wave], s_bc[wave], lane, 64, 0, 1, lane, &s_flag[wave], wsync,
                      aval + static_cast<long long>(k) * P.nent,
                      rs + static_cast<long long>(k) * P.n, L + static_cast<long long>(k) * P.lsize,
                      y + static_cast<long long>(k) * P.n, Uall + static_cast<long long>(k) * P.usizeA);
@@ -726,6 +727,7 @@ __global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int base
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_flag;
     __shared__ int s_rel[kBigTiles * 16 + 1];
+    __shared__ __align__(16) T s_bc[kBigWaves][64];
     const int lane = threadIdx.x & 63, wave = wave_id();
     const int k = blockIdx.x / nb;
     if (k >= B || !active[k]) return;
@@ -733,8 +735,8 @@ __global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int base
     if (threadIdx.x == 0) s_flag = 0;
     __syncthreads();
     auto bsync = [] { __syncthreads(); };
-    process_front<T>(P, D, reinterpret_cast<T*>(smem_raw), s_rel, threadIdx.x, kBigWaves * 64, wave, kBigWaves, lane,
-                     &s_flag, bsync,
+    process_front<T>(P, D, reinterpret_cast<T*>(smem_raw), s_rel, s_bc[wave], threadIdx.x, kBigWaves * 64, wave,
+                     kBigWaves, lane, &s_flag, bsync,
                      aval + static_cast<long long>(k) * P.nent, rs + static_cast<long long>(k) * P.n,
                      L + static_cast<long long>(k) * P.lsize, y + static_cast<long long>(k) * P.n,
                      Uall + static_cast<long long>(k) * P.usizeA);
