@@ -109,32 +109,33 @@ struct Mfma<double> {
 // operations complete in order, so the next step's store cannot overtake
 // this step's reads.
 template <typename T, bool kFull>
-__device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int lane, int k0, int kend, int cend, T* bc) {
+__device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int lane, int k0, int kend, T* bc) {
     constexpr int V = 16 / static_cast<int>(sizeof(T));  // entries per 16-byte read
     typedef T vec __attribute__((ext_vector_type(V)));
     bool ok = true;
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
-        if (kFull || k0 + kk < kend) {
-            const T dkk = bcast(P[kk], kk);
-            ok = ok && (dkk > T(0));
-            T d, inv;
-            Mfma<T>::root(dkk, d, inv);
-            const T l = P[kk] * inv;
-            P[kk] = i == k0 + kk ? d : l;
-            if (kk < 15 && (kFull || k0 + kk + 1 < cend)) {
-                bc[lane] = l;
-                T lj[16];
+        if (!kFull && k0 + kk >= kend) break;  // the tile column's remaining columns are not pivots
+        const T dkk = bcast(P[kk], kk);
+        ok = ok && (dkk > T(0));
+        T d, inv;
+        Mfma<T>::root(dkk, d, inv);
+        const T l = P[kk] * inv;
+        P[kk] = i == k0 + kk ? d : l;
+        if (kk < 15) {
+            bc[lane] = l;
+            T lj[16];
 #pragma unroll
-                for (int g = (kk + 1) / V; g < 16 / V; ++g) {
-                    const vec v = *reinterpret_cast<const vec*>(bc + g * V);
+            for (int g = (kk + 1) / V; g < 16 / V; ++g) {
+                const vec v = *reinterpret_cast<const vec*>(bc + g * V);
 #pragma unroll
-                    for (int e = 0; e < V; ++e) lj[g * V + e] = v[e];
-                }
-#pragma unroll
-                for (int c = kk + 1; c < 16; ++c)
-                    if (kFull || k0 + c < cend) P[c] -= l * lj[c];
+                for (int e = 0; e < V; ++e) lj[g * V + e] = v[e];
             }
+            // every column, without a test: past the front's last row (column
+            // k0 + c >= m) the lanes' rows are zero, so l and lj[c] are zero
+            // there, and those columns are never stored
+#pragma unroll
+            for (int c = kk + 1; c < 16; ++c) P[c] -= l * lj[c];
         }
     }
     return ok;
@@ -173,8 +174,8 @@ __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int row
     }
     sync();
     // a full panel (16 pivots, so 16 columns) without any column test
-    const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, lane, k0, kend, cend, bc)
-                                    : panel_steps<T, false>(P, i, lane, k0, kend, cend, bc);
+    const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, lane, k0, kend, bc)
+                                    : panel_steps<T, false>(P, i, lane, k0, kend, bc);
     if (mine) {
 #pragma unroll
         for (int c = 0; c < 16; ++c)
