@@ -156,18 +156,19 @@ __device__ __forceinline__ bool panel_steps(T (&P)[16], int i, int lane, int k0,
 // ordered inside one launch. `bc`: the wavefront's 64-entry LDS buffer
 // (16-byte aligned) for the steps' broadcasts.
 // The updates run on every lane without a branch: entries above the
-// diagonal of the diagonal block then hold values that are never read (the
-// broadcasts read rows at or below their column) nor stored.
+// diagonal of the diagonal block then hold values that are written back but
+// never read (the broadcasts read rows at or below their column; L, the
+// update matrix and the next panels read the lower triangle only).
 template <typename T, typename Sync>
 __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int rowbase, T* diag_out, T* bc,
                       Sync sync) {
-    const int cend = min(k0 + 16, m);
     const int i = lane < 16 ? k0 + lane : k0 + 16 + rowbase + 48 * wave + (lane - 16);
     const bool mine = i < m && (lane >= 16 || (wave == 0 && rowbase == 0 && diag_out == nullptr));
     T P[16];
     if (i < m) {
+        // columns past the front's last column (k0 + c >= m) hold zeros
 #pragma unroll
-        for (int c = 0; c < 16; ++c) P[c] = k0 + c < cend ? F[tix(i, k0 + c)] : T(0);
+        for (int c = 0; c < 16; ++c) P[c] = F[tix(i, k0 + c)];
     } else {
 #pragma unroll
         for (int c = 0; c < 16; ++c) P[c] = T(0);
@@ -176,10 +177,12 @@ __device__ bool panel(T* F, int m, int k0, int kend, int wave, int lane, int row
     // a full panel (16 pivots, so 16 columns) without any column test
     const bool ok = kend == k0 + 16 ? panel_steps<T, true>(P, i, lane, k0, kend, bc)
                                     : panel_steps<T, false>(P, i, lane, k0, kend, bc);
+    // every column of the lane's row, without a test: above the diagonal
+    // block's diagonal the values are never read, and past the front's last
+    // column they are the zeros that were loaded
     if (mine) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-            if (k0 + c < cend && i >= k0 + c) F[tix(i, k0 + c)] = P[c];
+        for (int c = 0; c < 16; ++c) F[tix(i, k0 + c)] = P[c];
     }
     if (diag_out != nullptr && wave == 0 && lane < 16) {
 #pragma unroll
@@ -213,12 +216,15 @@ struct TileJob {
         acc = Mfma<T>::load(C, lane);
     }
     __device__ __forceinline__ void apply(int nk, int lane) {
+        if (nk >= 16) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (4 * q < nk) {
-                const bool valid = 4 * q + (lane >> 4) < nk;
-                acc = Mfma<T>::run(valid ? -a[q] : T(0), valid ? b[q] : T(0), acc);
-            }
+            for (int q = 0; q < 4; ++q) acc = Mfma<T>::run(-a[q], b[q], acc);
+        } else {
+            // the panel's columns past kend (not pivots) are not multipliers:
+            // their A operand is 0 (B holds finite values of the front)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (4 * q < nk) acc = Mfma<T>::run(4 * q + (lane >> 4) < nk ? -a[q] : T(0), b[q], acc);
         }
         Mfma<T>::store(C, lane, acc);
     }
