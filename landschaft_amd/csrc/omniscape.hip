@@ -231,6 +231,11 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
                 long long e = uposA[c];
                 for (int j = 0; j < u; ++j)
                     for (int i = j; i <= u; ++i) xi[e++] = static_cast<unsigned short>(host_tix(rel(i), rel(j)));
+                // the child's right-hand-side corner and its padding to four
+                // entries go to the parent's corner (m, m), which no result
+                // reads: the extend-add then needs no per-entry test
+                const long long u1 = u + 1, end = uposA[c] + ((u1 * (u1 + 1) / 2 + 3) & ~3ll);
+                while (e < end) xi[e++] = static_cast<unsigned short>(host_tix(m, m));
             }
         }
         std::vector<unsigned short> ei(h.fe_row.size() + 1, 0);

@@ -544,10 +544,13 @@ __device__ __forceinline__ void extend_add(T* F, int m, int u, const int* rel, c
     sync();
 }
 
-// out[e] = F[idx[e]] for e < sz, 0 on the padding up to a multiple of 4
-// (`out` and `idx` are 4-entry aligned): a thread takes four consecutive
-// entries (one 8-byte index load, one vector store), four such chunks in
-// flight, the team walking the chunks in order (coalesced).
+// out[e] = F[idx[e]] for the sz entries and the padding up to a multiple of
+// 4 (`out` and `idx` are 4-entry aligned; a padding index reads some entry
+// of the front, a finite value that no reader of L or of the update matrix
+// uses): a thread takes four consecutive entries (one 8-byte index load, one
+// vector store), four such chunks in flight, the team walking the chunks in
+// order (coalesced). A thread past the last chunk stores the last chunk
+// again, the same values to the same place.
 template <typename T>
 __device__ __forceinline__ void store_by_table(const T* F, const unsigned short* idx, int sz, T* out, int tid,
                                                int nt) {
@@ -557,29 +560,32 @@ __device__ __forceinline__ void store_by_table(const T* F, const unsigned short*
     const I4* ix = reinterpret_cast<const I4*>(idx);
     V4* o = reinterpret_cast<V4*>(out);
     for (int c0 = tid; c0 < nch; c0 += 4 * nt) {
+        int cc[4];
         I4 id[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = c0 + q * nt;
-            id[q] = ix[c < nch ? c : nch - 1];
+            cc[q] = c < nch ? c : nch - 1;
+            id[q] = ix[cc[q]];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = c0 + q * nt, e = 4 * c;
             V4 v;
-            v.x = e < sz ? F[id[q].x] : T(0);
-            v.y = e + 1 < sz ? F[id[q].y] : T(0);
-            v.z = e + 2 < sz ? F[id[q].z] : T(0);
-            v.w = e + 3 < sz ? F[id[q].w] : T(0);
-            if (c < nch) o[c] = v;
+            v.x = F[id[q].x];
+            v.y = F[id[q].y];
+            v.z = F[id[q].z];
+            v.w = F[id[q].w];
+            o[cc[q]] = v;
         }
     }
 }
 
-// F[idx[e]] += Uc[e] for the tot entries of a child's update matrix (4-entry
-// aligned): the entries land on distinct front entries, so the team adds
-// without conflicts; a thread takes four consecutive entries (one vector
-// load of values, one of offsets), eight such chunks in flight.
+// F[idx[e]] += Uc[e] for the tot entries of a child's update matrix, padded
+// to whole 4-entry chunks: the entries land on distinct front entries (the
+// right-hand-side corner and the padding all on the front's unused corner),
+// so the team adds without conflicts and without per-entry tests; a thread
+// takes four consecutive entries (one vector load of values, one of
+// offsets), eight such chunks in flight.
 template <typename T, typename Sync>
 __device__ __forceinline__ void extend_by_table(T* F, const unsigned short* idx, const T* Uc, int tot, int tid,
                                                 int nt, Sync sync) {
@@ -599,11 +605,12 @@ __device__ __forceinline__ void extend_by_table(T* F, const unsigned short* idx,
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int e = 4 * (c0 + q * nt);
-            if (e < tot) F[id[q].x] += v[q].x;
-            if (e + 1 < tot) F[id[q].y] += v[q].y;
-            if (e + 2 < tot) F[id[q].z] += v[q].z;
-            if (e + 3 < tot) F[id[q].w] += v[q].w;
+            if (c0 + q * nt < nch) {
+                F[id[q].x] += v[q].x;
+                F[id[q].y] += v[q].y;
+                F[id[q].z] += v[q].z;
+                F[id[q].w] += v[q].w;
+            }
         }
     }
     sync();
@@ -631,10 +638,13 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
         ev[q] = aval[eq];
     }
     const double rhs0 = np > 0 ? rsw[first + (tid < np ? tid : np - 1)] : 0.0;
-    {  // the tiles cleared 16 bytes at a time (nz is a multiple of 256)
+    {  // the tiles cleared 16 bytes at a time (nz is a multiple of 256), in
+       // rounds counted by a wave-uniform bound
         uint4* Fz = reinterpret_cast<uint4*>(F);
         const int nz16 = nz * static_cast<int>(sizeof(T)) / 16;
-        for (int e = tid; e < nz16; e += nt) Fz[e] = uint4{0u, 0u, 0u, 0u};
+        const int rounds = nz16 / nt;
+        for (int r = 0; r < rounds; ++r) Fz[tid + r * nt] = uint4{0u, 0u, 0u, 0u};
+        if (tid + rounds * nt < nz16) Fz[tid + rounds * nt] = uint4{0u, 0u, 0u, 0u};
     }
     sync();
     for (int j = tid; j < np; j += nt) {
@@ -663,7 +673,7 @@ __device__ void process_front(const DPlan& P, const FrontDesc& D, T* F, int* rel
             u = P.nupd[c], upd = P.upd[c], up = P.uposA[c];
         }
         if (P.xidx) {
-            if (u > 0) extend_by_table<T>(F, P.xidx + up, U + up, static_cast<int>(colstart(u, u + 1)), tid, nt, sync);
+            if (u > 0) extend_by_table<T>(F, P.xidx + up, U + up, (u + 1) * (u + 2) / 2, tid, nt, sync);
         } else {
             extend_add<T>(F, m, u, P.relmap + upd, U + up, relbuf, tid, nt, sync);
         }
