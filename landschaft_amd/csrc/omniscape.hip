@@ -123,6 +123,12 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
     lev_btile.assign(H, 1);
     lev_lmax.assign(H, 1);
     lev_mmax.assign(H, 1);
+    lev_sgroups.assign(H, {});
+    lev_bgroups.assign(H, {});
+    auto group = [](std::vector<FrontGroup>& g, int off, int tiles) {
+        if (!g.empty() && g.back().tiles == tiles && g.back().first + g.back().count == off) ++g.back().count;
+        else g.push_back(FrontGroup{off, 1, tiles});
+    };
     huge.clear();
     for (int lv = 0; lv < H; ++lv)
         for (int k = h.lev_ptr[lv]; k < h.lev_ptr[lv + 1]; ++k) {
@@ -132,6 +138,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
                 ++lev_small_host[lv];
                 const int MT = static_cast<int>((m + 1 + 15) / 16);
                 lev_stile[lv] = std::max(lev_stile[lv], MT * (MT + 1) / 2);
+                group(lev_sgroups[lv], k - h.lev_ptr[lv], MT * (MT + 1) / 2);
             }
             if (m + 1 > kBigTiles * 16) {
                 huge.push_back(t);
@@ -143,6 +150,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
             if (m + 1 > kSmallRows) {
                 const int MT = static_cast<int>((m + 1 + 15) / 16);
                 lev_btile[lv] = std::max(lev_btile[lv], MT * (MT + 1) / 2);
+                group(lev_bgroups[lv], k - h.lev_ptr[lv], MT * (MT + 1) / 2);
             }
             lev_lmax[lv] = std::max(lev_lmax[lv], static_cast<int>(np * m - np * (np - 1) / 2));
             lev_mmax[lv] = std::max(lev_mmax[lv], static_cast<int>(m));
@@ -384,17 +392,16 @@ struct WindowSolver {
             hipLaunchKernelGGL(k_assemble<T>, dim3(blocks(P.nent, kThreads), nb), dim3(kThreads), 0, s, P, nb, wb.g.p,
                                wb.present.p, wb.sc.p, wb.active.p, Ap);
         for (int h = 0; h < P.H; ++h) {
-            const int ns = plan.lev_small_host[h];
-            const int nbig = plan.lev_fit_host[h] - ns;  // fronts of 65-159 rows
-            if (ns > 0)
-                hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * ns, kSmallWaves)),
-                                   dim3(kSmallWaves * 64), small_lds<T>(plan.lev_stile[h]), s, P,
-                                   plan.host.lev_ptr[h], ns, nb, plan.lev_stile[h],
+            for (const FrontGroup& g : plan.lev_sgroups[h])
+                hipLaunchKernelGGL(k_factor_small<T>, dim3(blocks(static_cast<long long>(nb) * g.count, kSmallWaves)),
+                                   dim3(kSmallWaves * 64), small_lds<T>(g.tiles), s, P,
+                                   plan.host.lev_ptr[h] + g.first, g.count, nb, g.tiles,
                                    Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
-            if (nbig > 0)
-                hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * nbig)),
-                                   dim3(kBigWaves * 64), static_cast<std::size_t>(plan.lev_btile[h]) * 256 * sizeof(T), s,
-                                   P, plan.host.lev_ptr[h] + ns, nbig, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p, Up, wb.fail.p);
+            for (const FrontGroup& g : plan.lev_bgroups[h])
+                hipLaunchKernelGGL(k_factor_big<T>, dim3(static_cast<unsigned>(static_cast<long long>(nb) * g.count)),
+                                   dim3(kBigWaves * 64), static_cast<std::size_t>(g.tiles) * 256 * sizeof(T), s,
+                                   P, plan.host.lev_ptr[h] + g.first, g.count, nb, Ap, wb.rs.p, wb.active.p, L, wb.y.p,
+                                   Up, wb.fail.p);
             if (!lev_huge[h].t.empty()) huge_factor(h, nb, Ap, L, Up);
         }
     }

@@ -70,9 +70,20 @@ struct DPlan {
     long long lsize, usizeA, vsize;
 };
 
+// A run of a level's fronts whose LDS tiles are the same (the plan orders a
+// level's fronts by size): `first` is its offset in the level, `count` its
+// fronts, `tiles` the 16 x 16 tiles of each.
+struct FrontGroup {
+    int first, count, tiles;
+};
+
 struct DevPlan {
     Plan host;
     DPlan d{};
+    // per level: the small (<= 64 rows) and large (65-159 rows) fronts in runs
+    // of equal tile counts, each factorised by a launch with that much LDS per
+    // front, so that smaller fronts are not given the level's largest share
+    std::vector<std::vector<FrontGroup>> lev_sgroups, lev_bgroups;
     std::vector<int> lev_fit_host;  // per level: fronts of at most 159 rows (RHS row included), the level's first ones
     std::vector<int> lev_small_host, lev_stile, lev_btile, lev_lmax, lev_mmax;  // per level: small fronts, their
                                                                                 // tiles, the big fronts' tiles,
