@@ -125,6 +125,7 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
     lev_mmax.assign(H, 1);
     lev_sgroups.assign(H, {});
     lev_bgroups.assign(H, {});
+    lev_lgroups.assign(H, {});
     auto group = [](std::vector<FrontGroup>& g, int off, int tiles) {
         if (!g.empty() && g.back().tiles == tiles && g.back().first + g.back().count == off) ++g.back().count;
         else g.push_back(FrontGroup{off, 1, tiles});
@@ -152,8 +153,24 @@ void DevPlan::upload(Plan p, hipStream_t s, bool tables) {
                 lev_btile[lv] = std::max(lev_btile[lv], MT * (MT + 1) / 2);
                 group(lev_bgroups[lv], k - h.lev_ptr[lv], MT * (MT + 1) / 2);
             }
-            lev_lmax[lv] = std::max(lev_lmax[lv], static_cast<int>(np * m - np * (np - 1) / 2));
+            const int lsz = static_cast<int>(np * m - np * (np - 1) / 2);
+            lev_lmax[lv] = std::max(lev_lmax[lv], lsz);
             lev_mmax[lv] = std::max(lev_mmax[lv], static_cast<int>(m));
+            // back-substitution runs: a new run where the L block needs a
+            // larger power-of-two share of LDS (from 4 KB) than the run so far
+            auto cls = [](int e) {
+                int c = 1024;
+                while (c < e) c <<= 1;
+                return c;
+            };
+            std::vector<FrontGroup>& lg = lev_lgroups[lv];
+            const int off = k - h.lev_ptr[lv];
+            if (!lg.empty() && cls(lsz) <= cls(lg.back().tiles)) {
+                ++lg.back().count;
+                lg.back().tiles = std::max(lg.back().tiles, lsz);
+            } else {
+                lg.push_back(FrontGroup{off, 1, lsz});
+            }
         }
     std::vector<long long> uposA(h.nfront, 0);
     long long ua = 0;
@@ -498,10 +515,12 @@ struct WindowSolver {
                                    static_cast<const int*>(wb.done.p), static_cast<const int*>(wb.active.p));
             const bool small = plan.lev_mmax[h] <= 64;  // update rows in 64 doubles of LDS
             auto kern = small ? k_backward_level<T, 1> : k_backward_level<T, 3>;
-            const std::size_t lds = solve_lds<T>(plan.lev_lmax[h]) - (small ? 128 * sizeof(double) : 0);
-            if (plan.lev_fit_host[h] > 0)
-                hipLaunchKernelGGL(kern, grid(h), dim3(64), lds, s, P, h, nb, P.lsize,
-                                   wb.active.p, wb.done.p, L, wb.y.p, wb.z.p);
+            for (const FrontGroup& g : plan.lev_lgroups[h]) {
+                const std::size_t lds = solve_lds<T>(g.tiles) - (small ? 128 * sizeof(double) : 0);
+                hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(static_cast<long long>(nb) * g.count)), dim3(64),
+                                   lds, s, P, h, g.first, g.count, nb, P.lsize, wb.active.p, wb.done.p, L, wb.y.p,
+                                   wb.z.p);
+            }
         }
         // x takes z in the residual that follows every sweep (k_residual)
     }

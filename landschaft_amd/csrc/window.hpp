@@ -84,6 +84,9 @@ struct DevPlan {
     // of equal tile counts, each factorised by a launch with that much LDS per
     // front, so that smaller fronts are not given the level's largest share
     std::vector<std::vector<FrontGroup>> lev_sgroups, lev_bgroups;
+    // per level: the fronts of at most 159 rows in runs for the back
+    // substitution, `tiles` holding the run's largest L block (entries)
+    std::vector<std::vector<FrontGroup>> lev_lgroups;
     std::vector<int> lev_fit_host;  // per level: fronts of at most 159 rows (RHS row included), the level's first ones
     std::vector<int> lev_small_host, lev_stile, lev_btile, lev_lmax, lev_mmax;  // per level: small fronts, their
                                                                                 // tiles, the big fronts' tiles,

@@ -1028,8 +1028,10 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
                                                 y + off, Vall + static_cast<long long>(k) * P.vsize);
 }
 
+// `first`, `nf`: the run of the level's fronts (of at most 159 rows) this
+// launch substitutes, its LDS sized to the run's largest L block.
 template <typename T, int R>
-__global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, long long lstride,
+__global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int first, int nf, int B, long long lstride,
                                                        const int* __restrict__ active,
                                                        const int* __restrict__ done, const T* __restrict__ L,
                                                        const double* __restrict__ y, double* __restrict__ z) {
@@ -1037,10 +1039,9 @@ __global__ __launch_bounds__(64) void k_backward_level(DPlan P, int h, int B, lo
     double* xu = reinterpret_cast<double*>(smem_raw);
     T* Ls = reinterpret_cast<T*>(smem_raw + (R == 1 ? 64 : 192) * sizeof(double));
     const int lane = threadIdx.x;
-    const int nf = P.lev_fit[h];  // the level's fronts of at most 159 rows (larger ones: huge.hpp)
     const int k = blockIdx.x / nf;
     if (k >= B || !active[k] || done[k]) return;
-    const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
+    const int t = P.order[P.lev_ptr[h] + first + (blockIdx.x - k * nf)];
     const long long off = static_cast<long long>(k) * P.n;
     backward_front<T, R>(P, t, L + static_cast<long long>(k) * lstride, Ls, xu, lane, y + off, z + off);
 }
