@@ -411,8 +411,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(DPlan P, int B, const dou
     const int W = P.W, W2 = W * W, nd = P.nd;
     const int p = P.fe_p[e], d = static_cast<int>(P.fe_src[e] - static_cast<long long>(p) * nd);
     const int c = P.cell[p];
-    const int y = c / W, xx = c - (c / W) * W;
-    const int c2 = (y + ndy(nd, d)) * W + xx + ndx(nd, d);
+    const int c2 = c + ndy(nd, d) * W + ndx(nd, d);  // an entry's neighbour lies inside the window
     const unsigned char* pw = present + static_cast<long long>(k) * W2;
     double v = 0.0;
     if (pw[c] && pw[c2]) {
@@ -1582,13 +1581,12 @@ __device__ __forceinline__ void residual_rows(const DPlan& P, int W, int n, long
         const double xp = x[off + p], dgp = dg[off + p], bp = bvec[off + p], scp = sc[off + p];
         const bool pr = pw[c];
         const double gc = gw[c];
-        const int y = c / W, xx = c - y * W;
         double gn[ND], xn[ND];
         bool on[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            const bool in = q[d] >= 0;
-            const int c2 = in ? (y + fixed_dy<ND>(d)) * W + xx + fixed_dx<ND>(d) : c;
+            const bool in = q[d] >= 0;  // the neighbour is a cell of the window: c plus its offset
+            const int c2 = in ? c + fixed_dy<ND>(d) * W + fixed_dx<ND>(d) : c;
             const bool pc = pw[c2];  // loaded for every neighbour (c2 is clamped)
             on[d] = in & pc;
             gn[d] = gw[c2];
