@@ -24,10 +24,13 @@
 extern "C" {
 #endif
 
-enum { LSC_PRECISION_MIXED = 0, LSC_PRECISION_FP64 = 1 };
+enum { LSC_PRECISION_MIXED = 0, LSC_PRECISION_FP64 = 1, LSC_PRECISION_AUTO = 2 };
 
 /* Solver options (NULL: the defaults). precision: LSC_PRECISION_MIXED (FP32
- * MFMA fronts + FP64 iterative refinement, default) or LSC_PRECISION_FP64;
+ * MFMA fronts + FP64 iterative refinement, default), LSC_PRECISION_FP64, or
+ * LSC_PRECISION_AUTO: Omniscape windows mixed; a graph factorisation FP64
+ * when it serves at least 8 right-hand sides (pairwise jobs, Schur columns,
+ * a grid circuit kept for repeated solves), mixed otherwise;
  * tol: relative residual ||b - Ax|| / ||b|| at which refinement stops
  * (default 1e-12); max_refine: refinement steps per circuit (default 30);
  * batch: circuits per batch (default 16384, fewer when HBM is short);

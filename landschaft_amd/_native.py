@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "liblandschaft_circuit.so"
 
 PRECISION_MIXED = 0
 PRECISION_FP64 = 1
+PRECISION_AUTO = 2
 
 # Every symbol declared in include/landschaft_circuit.h.
 EXPORTS = ("lsc_last_error", "lsc_device_info", "lsc_circuit_omniscape", "lsc_circuit_omniscape_device",
@@ -140,7 +141,7 @@ def check(rc: int) -> None:
 
 def options(precision="mixed", tol=None, max_refine=None, batch=None, timing=False, stream=None) -> SolveOptions:
     o = SolveOptions()
-    o.precision = {"mixed": PRECISION_MIXED, "fp64": PRECISION_FP64}[precision]
+    o.precision = {"mixed": PRECISION_MIXED, "fp64": PRECISION_FP64, "auto": PRECISION_AUTO}[precision]
     o.tol = float(tol) if tol else 0.0
     o.max_refine = int(max_refine) if max_refine else 0
     o.batch = int(batch) if batch else 0

@@ -20,11 +20,13 @@ from .. import _native as N
 
 @dataclass
 class SolverSettings:
-    """How circuits are solved: ``precision`` "mixed" (FP32 MFMA fronts + FP64
+    """How circuits are solved: ``precision`` "auto" (the default: Omniscape
+    windows mixed; a graph factorisation FP64 when it serves at least 8
+    right-hand sides, mixed otherwise), "mixed" (FP32 MFMA fronts + FP64
     refinement) or "fp64"; ``tol`` the relative residual refinement reaches;
     ``batch`` circuits per batch."""
 
-    precision: str = "mixed"
+    precision: str = "auto"
     tol: float = 1e-12
     max_refine: int = 30
     batch: int = 16384

@@ -734,6 +734,18 @@ void tally(Ctx& cx, const SystemSolver& S) {
     cx.st.fallback += S.fallbacks();
 }
 
+// The precision of a factorisation that serves `nrhs` right-hand sides.
+// LSC_PRECISION_AUTO: FP64 from kAutoFp64Rhs of them, mixed below. On these
+// graphs an FP32 factor preconditions about seven CG steps per right-hand
+// side (each a forward and back sweep and a residual) where an FP64 factor,
+// about 2.7 times the FP32 one's cost, solves in one sweep pair: at 8192^2
+// the two break even near 7.5 right-hand sides, at 2048^2 near 2.5.
+constexpr long long kAutoFp64Rhs = 8;
+int solver_precision(int p, long long nrhs) {
+    if (p != LSC_PRECISION_AUTO) return p;
+    return nrhs >= kAutoFp64Rhs ? LSC_PRECISION_FP64 : LSC_PRECISION_MIXED;
+}
+
 Ctx make_ctx(const lsc_solve_options* o) {
     Ctx c{};
     c.precision = o ? o->precision : LSC_PRECISION_MIXED;
@@ -895,7 +907,7 @@ std::vector<char> advanced_many(const RGraph& g, const DevGraph& G, const std::v
                                                 [&](i32 v) { return r.at(r.gnd, v) == kInf; }, nodes);
                 std::vector<double> full(n, 0.0);
                 if (A.n > 0) {
-                    SystemSolver S(cx.precision, cx.s);
+                    SystemSolver S(solver_precision(cx.precision, 1), cx.s);
                     S.factor(A);
                     std::vector<double> b(A.n), xv(A.n);
                     for (int i = 0; i < A.n; ++i) b[i] = r.at(r.src, nodes[i]);
@@ -940,7 +952,7 @@ std::vector<char> advanced_many(const RGraph& g, const DevGraph& G, const std::v
         fidx.shrink_to_fit();
         DevBuf<double> Z(na > 0 ? static_cast<std::size_t>(na) * nf : 1);
         if (na > 0) {
-            SystemSolver sys(cx.precision, cx.s);
+            SystemSolver sys(solver_precision(cx.precision, static_cast<long long>(nf)), cx.s);
             sys.factor(A);
             A = HostMatrix{};
             const long long tot = na * static_cast<long long>(nf);
@@ -1457,7 +1469,7 @@ void pairwise(const RGraph& g, const DevGraph& G, const double* gmap, const std:
         const i32* mb = M.mem.data() + p0;
         progress("pairwise: component %d matrix (%lld nodes)", k, static_cast<long long>(count));
         HostMatrix A = component_matrix(g, mb, count, nullptr, [&](i32 v) { return v == ref; }, nodes);
-        SystemSolver S(cx.precision, cx.s);
+        SystemSolver S(solver_precision(cx.precision, static_cast<long long>(jobs.size())), cx.s);
         S.factor(A);
         progress("pairwise: factored; %zu right-hand sides", jobs.size());
         std::vector<i32> uidx(g.n, -1);  // node -> unknown (-1: the grounded node, other components)
@@ -1760,7 +1772,8 @@ void grid_factor(GridCircuit& C, const double* resistance, bool refactor) {
             C.sys[k]->refactor(A);
             continue;
         }
-        auto S = std::make_unique<SystemSolver>(C.cx.precision, C.cx.s);
+        // a grid circuit is kept for repeated solves (the adjoint's pairs)
+        auto S = std::make_unique<SystemSolver>(solver_precision(C.cx.precision, kAutoFp64Rhs), C.cx.s);
         S->factor(A);
         std::vector<long long> cells(nodes.size());
         for (std::size_t i = 0; i < nodes.size(); ++i) cells[i] = C.g.node_cell[nodes[i]];

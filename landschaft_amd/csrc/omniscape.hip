@@ -1137,6 +1137,7 @@ lsc::OmniArgs omni_args(int radius, int buffer, int block_size, int four, int rc
     a.radius = radius, a.buffer = buffer, a.block = block_size, a.four = four, a.rc = rc, a.correct = correct,
     a.fp = fp;
     a.precision = opt ? opt->precision : LSC_PRECISION_MIXED;
+    if (a.precision == LSC_PRECISION_AUTO) a.precision = LSC_PRECISION_MIXED;  // windows: mixed
     a.tol = opt && opt->tol > 0 ? opt->tol : 1e-12;
     a.maxit = opt && opt->max_refine > 0 ? opt->max_refine : 30;
     a.batch = opt && opt->batch > 0 ? opt->batch : 16384;

@@ -137,3 +137,16 @@ def test_c_abi_argument_errors():
         _backend.circuit_omniscape(r, r, -1, 0, 1, False, False, True, False)
     with pytest.raises(ValueError, match="out of range"):
         _backend.effective_resistance_grad(r, [0], [400])
+
+
+def test_auto_precision_option():
+    """"auto" reaches the C-ABI as LSC_PRECISION_AUTO and is the settings' default."""
+    from landschaft_amd import _native as N
+    from landschaft_amd.circuit import _backend
+
+    assert N.options("auto").precision == N.PRECISION_AUTO == 2
+    assert N.options("mixed").precision == N.PRECISION_MIXED
+    assert N.options("fp64").precision == N.PRECISION_FP64
+    assert _backend.SolverSettings().precision == "auto"
+    header = (Path(__file__).resolve().parents[1] / "include" / "landschaft_circuit.h").read_text()
+    assert "LSC_PRECISION_AUTO = 2" in header

@@ -306,3 +306,35 @@ def test_pairwise_jobs_in_chunks(backend, oracle, network, monkeypatch):
         np.testing.assert_allclose(out["64"]["resistances"], ref["resistances"], rtol=1e-9)
         top = ref["cum"].max()
         np.testing.assert_allclose(out["64"]["cum"], ref["cum"], rtol=1e-9, atol=1e-12 * top)
+
+
+def test_auto_precision_by_right_hand_sides(backend, oracle):
+    """precision "auto": the pairwise factorisation of 12 points with the cumulative map (66 jobs) runs
+    in FP64 (the same bits as precision "fp64"), the advanced solve of one right-hand side in mixed
+    precision (the same bits as "mixed"); both match the oracle."""
+    rng = np.random.default_rng(21)
+    shape = (60, 70)
+    g = np.exp(rng.normal(0, 1, shape))
+    cells = rng.choice(shape[0] * shape[1], 12, replace=False).astype(np.int64)
+    ex = np.zeros((12, 12), dtype=np.uint8)
+    s = np.zeros(shape)
+    s[5, 5] = 1.0
+    gr = np.zeros(shape)
+    gr[50, 60] = np.inf
+    runs = {}
+    try:
+        for prec in ("auto", "fp64", "mixed"):
+            backend.settings.precision = prec
+            runs[prec] = (backend.circuitscape_pairwise(g, None, cells, ex, False, False, {"cum": True}),
+                          backend.circuit_advanced(g, s, gr, False))
+    finally:
+        backend.settings.precision = "mixed"
+    for k in ("resistances", "cum"):
+        np.testing.assert_array_equal(runs["auto"][0][k], runs["fp64"][0][k])
+    np.testing.assert_array_equal(runs["auto"][1]["current"], runs["mixed"][1]["current"])
+    ref = oracle.cs_pairwise(g, None, cells, ex, False, False, {"cum": True})
+    np.testing.assert_allclose(runs["auto"][0]["resistances"], ref["resistances"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(runs["auto"][0]["cum"], ref["cum"], rtol=1e-9, atol=1e-12 * np.abs(ref["cum"]).max())
+    ref_adv = oracle.advanced(g, s, gr, False)
+    top = np.abs(ref_adv["current"]).max()
+    np.testing.assert_allclose(runs["auto"][1]["current"], ref_adv["current"], rtol=1e-9, atol=1e-12 * top)
