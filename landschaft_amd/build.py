@@ -23,8 +23,11 @@ ROOT = PKG.parent
 ARCH = os.environ.get("LSC_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "omniscape.hip", "system.hip", "circuitscape.hip", "plan.cpp", "grid_text.cpp"]
+# MFMA accumulators in ordinary vector registers (gfx950 allows either file):
+# no copies through the accumulation registers around every tile's MFMAs
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-         "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-I", str(ROOT / "include")]
+         "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form=true", "-Wall", "-Wno-unused-result", "-I",
+         str(ROOT / "include")]
 
 
 def hipcc() -> str:
