@@ -80,12 +80,14 @@ std::size_t solve_lds(int lmax) {
 
 // Cells of the nested-dissection leaves of the window plans (LSC_ND_LEAF,
 // default 32: half the fronts of 16-cell leaves for 25 % more flops).
-// Batches in flight at once (LSC_BATCH_STREAMS, 1-4; default 4).
+// Batches in flight at once (LSC_BATCH_STREAMS, 1-4; default 3: on the
+// 4096^2 radius-50 map 2, 3 and 4 give 161k, 169-170k and 163-167k
+// circuits/s with the current kernels).
 long long batch_streams() {
     if (active_arena()) return 1;  // the static scratch serves one batch on the caller's stream
     const char* e = std::getenv("LSC_BATCH_STREAMS");
-    const int v = e ? std::atoi(e) : 4;
-    return v >= 1 && v <= 4 ? v : 4;
+    const int v = e ? std::atoi(e) : 3;
+    return v >= 1 && v <= 4 ? v : 3;
 }
 
 int nd_leaf() {
@@ -813,7 +815,7 @@ void lattice_pass(const double* d_res, const double* d_src, const Lattice& Lt, c
     const long long fit = active_arena() ? std::numeric_limits<int>::max()
                                          : std::max<long long>(1, static_cast<long long>(0.5 * static_cast<double>(free_b) / per_window));
     const long long want = std::min<long long>(a.batch > 0 ? a.batch : 16384, Lt.nlat);
-    // Several batches in flight on their own streams (four by default,
+    // Several batches in flight on their own streams (three by default,
     // LSC_BATCH_STREAMS), each driven by a host thread: the window kernels
     // are latency-bound (one front per wavefront, the factor limited by LDS),
     // so other batches fill issue slots one batch leaves idle. The

@@ -87,22 +87,24 @@ def test_correction_matches_oracle(backend, oracle):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_batches_in_flight_give_identical_maps(backend, oracle, precision, monkeypatch):
-    """The same small batches on one stream and on four streams (the accumulation is chained in batch
-    order): the cumulative and flow-potential maps are bit-identical, and match the oracle. Four streams
-    share the one-batch budget, so `batch` 512 over four streams is batches of 128."""
+    """The same small batches on one stream, on three (the default) and on four streams (the
+    accumulation is chained in batch order): the cumulative and flow-potential maps are bit-identical,
+    and match the oracle. The streams share the one-batch budget, so `batch` 384 over three streams and
+    512 over four are batches of 128."""
     rng = np.random.default_rng(21)
     r = np.exp(rng.normal(0.0, 1.0, (70, 90)))
     r[rng.random(r.shape) < 0.08] = np.nan
     s = np.where(np.isnan(r), 0.0, 1 / np.nan_to_num(r, nan=1.0))
     backend.settings.precision = precision
     maps = {}
-    for streams, batch in (("1", 128), ("4", 512)):
+    for streams, batch in (("1", 128), ("3", 384), ("4", 512)):
         monkeypatch.setattr(backend.settings, "batch", batch)
         monkeypatch.setenv("LSC_BATCH_STREAMS", streams)
         print(f"streams {streams} {precision}", flush=True)
         maps[streams] = backend.circuit_omniscape(r, s, 7, 1, 1, False, False, True, True)
         assert backend.last_stats["n_batches"] > 8
     for k in ("current", "flow_potential"):
+        np.testing.assert_array_equal(maps["1"][k], maps["3"][k])
         np.testing.assert_array_equal(maps["1"][k], maps["4"][k])
     ref = oracle.omniscape(r, s, 7, 1, 1, False, False, True, True)
     top = ref["current"].max()
