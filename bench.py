@@ -356,7 +356,7 @@ def tune(a) -> int:
     saved = {k: os.environ.get(k) for k in ("LSC_ND_LEAF", "LSC_BATCH_STREAMS")}
     try:
         for leaf in (24, 32, 40):
-            for streams in (1, 2, 4):
+            for streams in (1, 2, 3, 4):
                 for batch in (8192, 16384):
                     os.environ["LSC_ND_LEAF"], os.environ["LSC_BATCH_STREAMS"] = str(leaf), str(streams)
                     BATCH = batch
