@@ -1023,7 +1023,7 @@ __global__ __launch_bounds__(64) void k_forward_level(DPlan P, int h, int B, lon
     if (k >= B || !active[k] || done[k]) return;
     const int t = P.order[P.lev_ptr[h] + (blockIdx.x - k * nf)];
     const long long off = static_cast<long long>(k) * P.n;
-    forward_front_direct<T, R, R == 1 ? 16 : 8>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off,
+    forward_front_direct<T, R, R == 1 ? 16 : 4>(P, t, L + static_cast<long long>(k) * lstride, fv, lane, rs + off,
                                                 y + off, Vall + static_cast<long long>(k) * P.vsize);
 }
 
