@@ -41,9 +41,9 @@ HBM_PEAK = 8.0e12          # bytes/s
 # one single-stream run of this workload at 1024 x 1024 (29,147 windows
 # factored, 5.05 substitution sweeps each); profiles/ holds the runs.
 # One substitution sweep (forward and backward sweeps averaged):
-SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 1961758
+SOLVE_TRAFFIC_PER_WINDOW_SWEEP = 1961950
 # The factorisation (assembly and every front level, forward fused):
-FACTOR_TRAFFIC_PER_WINDOW = 5294390
+FACTOR_TRAFFIC_PER_WINDOW = 5294388
 
 
 def fractal_field(n: int, seed: int, exponent: float = -3.0) -> np.ndarray:
