@@ -726,9 +726,11 @@ __global__ __launch_bounds__(kSmallWaves * 64) void k_factor_small(DPlan P, int 
 }
 
 // `base`: the level-order position of the level's first front of 65-159
-// rows, `nb` their number.
+// rows, `nb` their number. FP32 fronts are held to 80 registers (six
+// wavefronts per SIMD, no spills), so that fronts whose tiles take at most
+// 26 KB of LDS fit six workgroups per CU.
 template <typename T>
-__global__ __launch_bounds__(kBigWaves * 64) void k_factor_big(DPlan P, int base, int nb, int B,
+__global__ __launch_bounds__(kBigWaves * 64, sizeof(T) == 4 ? 6 : 1) void k_factor_big(DPlan P, int base, int nb, int B,
                                                          const T* __restrict__ aval, const double* __restrict__ rs,
                                                          const int* __restrict__ active, T* __restrict__ L,
                                                          double* __restrict__ y, T* __restrict__ Uall,
