@@ -626,8 +626,6 @@ void SystemSolver::Impl::factor_t(const HostMatrix& m, double& analyze_ms, doubl
              p.max_front);
     plan.upload(std::move(p), s);
     const Plan& h = plan.host;
-    const DPlan& P = plan.d;
-    const int H = P.H, n = P.n;
     huge_tiles = build_huge_levels(h, kWidePanel, lev_huge, huge_rows, s);
     std::vector<long long> ptrl(m.ptr.begin(), m.ptr.end());
     ptr = to_device(ptrl, s);
@@ -869,7 +867,6 @@ void SystemSolver::Impl::huge_backward(const HugeLevel& H, int k) {
 template <typename T>
 void SystemSolver::Impl::sweep_t(int k, int mode) {
     const DPlan& P = plan.d;
-    const Plan& h = plan.host;
     const T* Lp = reinterpret_cast<const T*>(L.p);
     auto lds = [&](int lv) {
         return plan.lev_mmax[lv] <= 64 ? rhs_lds<T, 1>(plan.lev_lmax[lv]) : rows_lds<T>(plan.lev_lmax[lv]);

@@ -226,7 +226,7 @@ static __global__ __launch_bounds__(kThreads) void k_flood(DPlan P, int B, const
     __shared__ int s_ok;
     const int k = blockIdx.x;
     if (k >= B || !active[k]) return;
-    const int W = P.W, W2 = W * W, D = P.D, n = P.n, nd = P.nd;
+    const int W = P.W, W2 = W * W, D = P.D, nd = P.nd;
     const double* gw = g + static_cast<long long>(k) * W2;
     const double* sw = src + static_cast<long long>(k) * W2;
     const int lane = threadIdx.x & 63, wave = wave_id();
