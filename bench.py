@@ -522,7 +522,9 @@ def main() -> int:
         cum64 = tc.cpu().numpy()
         top = np.nanmax(np.abs(cum64))
         fp64 = {"value": st64["n_targets"] / d64, "ms": d64 * 1e3, "resid_max": st64["resid_max"],
-                "cum_max_rel_diff_vs_mixed": float(np.nanmax(np.abs(cum_mixed - cum64)) / top)}
+                "cum_max_rel_diff_vs_mixed": float(np.nanmax(np.abs(cum_mixed - cum64)) / top),
+                "refine_mean": st64["refine_mean"],
+                "phase_ms": {k[3:]: round(v, 2) for k, v in st64.items() if k.startswith("ms_")}}
     log("CPU baseline")
     cpu = None if a.no_cpu else cpu_baseline(r, s, a.radius, a.block, a.cpu_seconds)
     graph = None
