@@ -161,6 +161,28 @@ static __global__ __launch_bounds__(kThreads) void k_fill(const double* __restri
     }
 }
 
+// The summed edge weights from cell c (row y, column xx) to its present
+// neighbours, in the neighbour order; every neighbour's conductance is
+// requested before any is used (outside the window the cell's own, unused).
+template <int ND, typename Bit>
+__device__ __forceinline__ double present_edge_sum(int W, int c, int y, int xx, const double* gw, const Bit& bit) {
+    bool on[ND];
+    double gn[ND];
+#pragma unroll
+    for (int e = 0; e < ND; ++e) {
+        const int yy = y + ndy(ND, e), x2 = xx + ndx(ND, e);
+        const bool in = yy >= 0 && yy < W && x2 >= 0 && x2 < W;
+        on[e] = in && bit(in ? yy : y, in ? x2 : xx);
+        gn[e] = gw[in ? yy * W + x2 : c];
+    }
+    const double gc = gw[c];
+    double sum = 0.0;
+#pragma unroll
+    for (int e = 0; e < ND; ++e)
+        if (on[e]) sum += edge_weight(gc, gn[e], ndiag(ND, e));
+    return sum;
+}
+
 // After the flood: the present mask of every cell from the row masks (`words`
 // 64-bit words a row), the window dropped when its component holds no source
 // (omniscape.cpp:101), then the Laplacian diagonal of every unknown (edges to
@@ -191,14 +213,7 @@ __device__ void flood_finish(const DPlan& P, int k, const unsigned long long* pm
         const int y = c / W, xx = c - (c / W) * W;
         double d = 1.0, s = 1.0, b = 0.0;
         if (bit(y, xx)) {
-            double sum = 0.0;
-            const double gc = gw[c];
-            for (int e = 0; e < nd; ++e) {
-                const int yy = y + ndy(nd, e), x2 = xx + ndx(nd, e);
-                if (yy < 0 || yy >= W || x2 < 0 || x2 >= W) continue;
-                if (!bit(yy, x2)) continue;
-                sum += edge_weight(gc, gw[yy * W + x2], ndiag(nd, e));
-            }
+            const double sum = nd == 8 ? present_edge_sum<8>(W, c, y, xx, gw, bit) : present_edge_sum<4>(W, c, y, xx, gw, bit);
             d = sum + shift;
             s = 1.0 / sqrt(d);
             b = sw[c];
